@@ -1,0 +1,69 @@
+// Shared device/host definitions for the MI355X (gfx950) TDDFT time-step library.
+//
+// Field layout in HBM follows the reference's Fortran arrays byte for byte:
+// a real field f(nx,ny,nz) is stored x-fastest, index ix + nx*(iy + ny*iz);
+// a half spectrum (nx/2+1, ny, nz) likewise with M = nx/2+1 in place of nx
+// (modules.f90, dimen.f90:29-143).  Complex values are interleaved (re, im).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+namespace hetd {
+
+template <class T>
+struct alignas(2 * sizeof(T)) cplx {
+  T x, y;
+};
+
+using c64 = cplx<double>;
+using c32 = cplx<float>;
+
+template <class T>
+__host__ __device__ inline cplx<T> mk(T a, T b) {
+  cplx<T> r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <class T>
+__host__ __device__ inline cplx<T> operator+(cplx<T> a, cplx<T> b) { return mk(a.x + b.x, a.y + b.y); }
+template <class T>
+__host__ __device__ inline cplx<T> operator-(cplx<T> a, cplx<T> b) { return mk(a.x - b.x, a.y - b.y); }
+template <class T>
+__host__ __device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <class T>
+__host__ __device__ inline cplx<T> cmulc(cplx<T> a, cplx<T> b) {  // a * conj(b)
+  return mk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <class T>
+__host__ __device__ inline cplx<T> conj(cplx<T> a) { return mk(a.x, -a.y); }
+template <class T>
+__host__ __device__ inline cplx<T> scale(cplx<T> a, T s) { return mk(a.x * s, a.y * s); }
+
+// Maximum FFT plan depth (radix stages) and supported line length.
+constexpr int kMaxStages = 8;
+constexpr int kMaxN = 256;
+
+// Radix plan of one 1D transform length, passed to kernels by value.
+struct FftPlan {
+  int n;                      // transform length
+  int nstages;                // number of radix stages
+  int radix[kMaxStages];      // radix of each stage (2,3,4,5,8)
+};
+
+// 13-point stencil weights for offsets -6..6, already scaled by 1/h^order.
+struct Stencil13 {
+  double w[13];
+};
+
+struct Grid {
+  int nx, ny, nz;
+  int mx;           // nx/2 + 1
+  long long n;      // nx*ny*nz
+  long long h;      // mx*ny*nz
+};
+
+}  // namespace hetd

@@ -1,0 +1,547 @@
+// Grid-point kernels: 13-point stencils, Hamiltonian application fused with
+// the RK-Gill / Hamming updates of psi, potential assembly, NGP histogram,
+// ironing, trilinear force gather, particle updates and reductions.
+//
+// Floating-point expressions keep the reference's operand order so that the
+// FP64 path reproduces the CPU restatement of the reference to rounding.
+#pragma once
+#include "common.h"
+
+namespace hetd {
+
+constexpr int kBlock = 256;
+
+struct Idx3 {
+  int ix, iy, iz;
+};
+
+__device__ inline Idx3 idx3(long long i, const Grid& g) {
+  Idx3 r;
+  const long long pl = (long long)g.nx * g.ny;
+  r.iz = (int)(i / pl);
+  const long long rem = i - (long long)r.iz * pl;
+  r.iy = (int)(rem / g.nx);
+  r.ix = (int)(rem - (long long)r.iy * g.nx);
+  return r;
+}
+
+// 1D 13-point derivative along one axis with f == 0 outside the box
+// (derivnD with icon=13: init_deriv_parallel.f90:428-535, redef.f:121-170).
+// The sum runs over offsets -6..6 in order; out-of-box terms are absent.
+template <class R>
+__device__ inline double d13(const R* __restrict__ f, long long i, int pos, int n, long long stride,
+                             const Stencil13& w) {
+  double acc = 0.0;
+#pragma unroll
+  for (int m = -6; m <= 6; ++m) {
+    const int p = pos + m;
+    if (m != 0 || w.w[6] != 0.0) {
+      if (p >= 0 && p < n) acc = acc + w.w[m + 6] * (double)f[i + m * stride];
+    }
+  }
+  return acc;
+}
+
+__device__ inline c64 d13c(const c64* __restrict__ f, long long i, int pos, int n, long long stride,
+                           const Stencil13& w) {
+  double ax = 0.0, ay = 0.0;
+#pragma unroll
+  for (int m = -6; m <= 6; ++m) {
+    const int p = pos + m;
+    if (p >= 0 && p < n) {
+      const c64 v = f[i + m * stride];
+      ax = ax + w.w[m + 6] * v.x;
+      ay = ay + w.w[m + 6] * v.y;
+    }
+  }
+  return mk(ax, ay);
+}
+
+struct HamArgs {
+  Grid g;
+  Stencil13 d2x, d2y, d2z;
+  double h2o2m4;
+  const c64* timec_z;  // per-z time constant, (Lambda(1+tanh((z-zm)/zs)), 1)
+};
+
+// Sto4c = timec*((d2x+d2y+d2z)psi*h2o2m4 - pot4*psi)  (steppc.f90:36-46)
+template <class R>
+__device__ inline c64 hpsi_at(const HamArgs& a, const c64* __restrict__ psi, const R* __restrict__ pot4,
+                              long long i, Idx3 p) {
+  const Grid& g = a.g;
+  const c64 sx = d13c(psi, i, p.ix, g.nx, 1, a.d2x);
+  const c64 sy = d13c(psi, i, p.iy, g.ny, g.nx, a.d2y);
+  const c64 sz = d13c(psi, i, p.iz, g.nz, (long long)g.nx * g.ny, a.d2z);
+  const c64 lap = (sx + sy) + sz;
+  const c64 ps = psi[i];
+  const double pv = (double)pot4[i];
+  const c64 t = mk(lap.x * a.h2o2m4 - pv * ps.x, lap.y * a.h2o2m4 - pv * ps.y);
+  return cmul(a.timec_z[p.iz], t);
+}
+
+// ---- block reduction (sum of doubles), fixed tree order ------------------
+__device__ inline double block_sum(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int k = 0; k < nw; ++k) s += sh[k];
+  }
+  return s;  // valid in thread 0
+}
+
+// Sum of nblocks partial sums per slot; one block per slot, fixed order.
+__global__ void k_finish_sums(const double* __restrict__ part, int nblocks, int nslots,
+                              double* __restrict__ out) {
+  __shared__ double sh[16];
+  const int s = blockIdx.x;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) v += part[(long long)s * nblocks + b];
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) out[s] = v;
+}
+
+// ---- psi-side fused kernels -------------------------------------------
+// RK-Gill stage (steprk.f90:56-74)
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_rk_stage(HamArgs a, const c64* __restrict__ psi,
+                                                      const R* __restrict__ pot4, c64* __restrict__ q,
+                                                      c64* __restrict__ psi_out, R* __restrict__ den,
+                                                      c64* __restrict__ h1_out, double ca, double cb,
+                                                      double cc, double dt) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.g.n) return;
+  const Idx3 p = idx3(i, a.g);
+  const c64 s4 = hpsi_at(a, psi, pot4, i, p);
+  const c64 qi = q[i];
+  const c64 s1 = mk(ca * (s4.x - cb * qi.x), ca * (s4.y - cb * qi.y));
+  q[i] = mk((qi.x + 3.0 * s1.x) - cc * s4.x, (qi.y + 3.0 * s1.y) - cc * s4.y);
+  if (h1_out) h1_out[i] = s4;
+  const c64 ps = psi[i];
+  const c64 np = mk(ps.x + dt * s1.x, ps.y + dt * s1.y);
+  psi_out[i] = np;
+  den[i] = (R)(np.x * np.x + np.y * np.y);
+}
+
+// Hamming predictor + modifier (steppc.f90:46-56).  `old3` holds
+// psiold(ioldp(3)) on entry and receives the modified psi; `h2` holds
+// hpsiold(ioldh(2)) on entry and receives Sto4c.
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_pc_predict(HamArgs a, const c64* __restrict__ psi,
+                                                        const R* __restrict__ pot4, c64* old3,
+                                                        const c64* __restrict__ h1, c64* h2,
+                                                        c64* __restrict__ pc, R* __restrict__ den,
+                                                        double c4o3dt, double c112) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.g.n) return;
+  const Idx3 p = idx3(i, a.g);
+  const c64 s4 = hpsi_at(a, psi, pot4, i, p);
+  const c64 o3 = old3[i], a1 = h1[i], a2 = h2[i], pci = pc[i];
+  const c64 s1 = mk(o3.x + c4o3dt * ((2.0 * s4.x - a1.x) + 2.0 * a2.x),
+                    o3.y + c4o3dt * ((2.0 * s4.y - a1.y) + 2.0 * a2.y));
+  const c64 np = mk(s1.x - c112 * pci.x, s1.y - c112 * pci.y);
+  old3[i] = np;
+  pc[i] = s1;
+  h2[i] = s4;
+  den[i] = (R)(np.x * np.x + np.y * np.y);
+}
+
+// Hamming corrector + final value + errHe partial sums (steppc.f90:97-112).
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_pc_correct(HamArgs a, const c64* __restrict__ psi,
+                                                        const R* __restrict__ pot4,
+                                                        const c64* __restrict__ o3,
+                                                        const c64* __restrict__ o2,
+                                                        const c64* __restrict__ h1,
+                                                        const c64* __restrict__ h2, c64* __restrict__ pc,
+                                                        c64* __restrict__ psi_out, R* __restrict__ den,
+                                                        double* __restrict__ part, double dt3, double c9) {
+  __shared__ double sh[16];
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0.0;
+  if (i < a.g.n) {
+    const Idx3 p = idx3(i, a.g);
+    const c64 s4 = hpsi_at(a, psi, pot4, i, p);
+    const c64 v3 = o3[i], v2 = o2[i], a1 = h1[i], a2 = h2[i];
+    const c64 s5 = mk(0.125 * ((9.0 * v3.x - v2.x) + dt3 * ((s4.x + 2.0 * a1.x) - a2.x)),
+                      0.125 * ((9.0 * v3.y - v2.y) + dt3 * ((s4.y + 2.0 * a1.y) - a2.y)));
+    const c64 pci = pc[i];
+    const c64 npc = mk(pci.x - s5.x, pci.y - s5.y);
+    pc[i] = npc;
+    const c64 np = mk(s5.x + c9 * npc.x, s5.y + c9 * npc.y);
+    psi_out[i] = np;
+    den[i] = (R)(np.x * np.x + np.y * np.y);
+    e = hypot(npc.x, npc.y);
+  }
+  const double s = block_sum(e, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// den = |psi|^2 (DFT4He3d.f90:630 with conjg(psi)*psi)
+template <class R>
+__global__ void k_density(const c64* __restrict__ psi, R* __restrict__ den, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const c64 v = psi[i];
+  den[i] = (R)(v.x * v.x + v.y * v.y);
+}
+
+// ---- potential side -------------------------------------------------------
+// grad(den) with the first-derivative stencil (derden.f90:26-28)
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_grad3(Grid g, const R* __restrict__ f, Stencil13 wx,
+                                                   Stencil13 wy, Stencil13 wz, R* __restrict__ fx,
+                                                   R* __restrict__ fy, R* __restrict__ fz) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  const Idx3 p = idx3(i, g);
+  fx[i] = (R)d13(f, i, p.ix, g.nx, 1, wx);
+  fy[i] = (R)d13(f, i, p.iy, g.ny, g.nx, wy);
+  fz[i] = (R)d13(f, i, p.iz, g.nz, (long long)g.nx * g.ny, wz);
+}
+
+struct PotArgs {
+  Grid g;
+  Stencil13 d1x, d1y, d1z;
+  double a0, a1;                 // cp4/2, cpp4/3
+  double den0s, h2o2m4, alphas;
+  double C, beta, den_m;
+  int otc;
+};
+
+// Alpha_s divergence, alpha_s field and final pot4 (term_alfa.f90:85-113,
+// poten.f90:80-124).  falfs = 1 - denalf/den0s is formed at each neighbour.
+template <class R>
+__device__ inline double dfint(const R* __restrict__ denalf, const R* __restrict__ in, long long i, int pos,
+                               int n, long long stride, const Stencil13& w, double den0s) {
+  double acc = 0.0;
+#pragma unroll
+  for (int m = -6; m <= 6; ++m) {
+    if (m == 0) continue;
+    const int pp = pos + m;
+    if (pp >= 0 && pp < n) {
+      const long long j = i + m * stride;
+      const double fal = 1.0 - (double)denalf[j] / den0s;
+      acc = acc + w.w[m + 6] * (fal * (double)in[j]);
+    }
+  }
+  return acc;
+}
+
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_pot_final(PotArgs a, const R* __restrict__ den,
+                                                       const R* __restrict__ dencg,
+                                                       const R* __restrict__ delj4,
+                                                       const R* __restrict__ sto1c,
+                                                       const R* __restrict__ uimp,
+                                                       const R* __restrict__ denalf,
+                                                       const R* __restrict__ intx,
+                                                       const R* __restrict__ inty,
+                                                       const R* __restrict__ intz,
+                                                       const R* __restrict__ uaconv, R* __restrict__ pot4) {
+  const Grid& g = a.g;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  const double dn = (double)den[i];
+  const double dcg = (double)dencg[i];
+  double p4 = ((double)delj4[i] + dcg * dcg * (a.a0 + a.a1 * dcg)) + (double)sto1c[i];
+  if (a.otc) {
+    const Idx3 p = idx3(i, g);
+    const double s4 = dfint(denalf, intx, i, p.ix, g.nx, 1, a.d1x, a.den0s);
+    const double s5 = dfint(denalf, inty, i, p.iy, g.ny, g.nx, a.d1y, a.den0s);
+    const double s6 = dfint(denalf, intz, i, p.iz, g.nz, (long long)g.nx * g.ny, a.d1z, a.den0s);
+    const double ua = (double)uaconv[i] / a.den0s;
+    const double ual = (((ua + s4) + s5) + s6) * a.h2o2m4 * a.alphas;
+    p4 = p4 + ual;
+  }
+  const double t = tanh(a.beta * (dn - a.den_m));
+  const double pen = a.C * (1.0 + t + a.beta * dn * (1.0 - t * t));
+  pot4[i] = (R)((p4 + (double)uimp[i]) + pen);
+}
+
+// ---- impurity side ------------------------------------------------------
+// NGP histogram (denpart.f90:24-32): ix = int(1.5 + (r - x1)/h); counts are
+// exact integers, so the atomic order does not matter.
+struct BoxArgs {
+  Grid g;
+  double xmin, ymin, zmin, hx, hy, hz;
+  int npar;
+};
+
+__global__ void k_hist(BoxArgs b, const double* __restrict__ r, unsigned int* __restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.npar) return;
+  const int ix = (int)(1.5 + (r[i] - b.xmin) / b.hx);
+  const int iy = (int)(1.5 + (r[b.npar + i] - b.ymin) / b.hy);
+  const int iz = (int)(1.5 + (r[2 * b.npar + i] - b.zmin) / b.hz);
+  if (iz >= 1 && iz <= b.g.nz && iy >= 1 && iy <= b.g.ny && ix >= 1 && ix <= b.g.nx) {
+    const long long j = (ix - 1) + (long long)b.g.nx * ((iy - 1) + (long long)b.g.ny * (iz - 1));
+    atomicAdd(&cnt[j], 1u);
+  }
+}
+
+// denx = count/N_par/hxyz (denpart.f90:33); optionally the first ironing
+// pass (ironing.f90:16-28) is applied on the fly.
+template <class R>
+__device__ inline double denx_of(const unsigned int* cnt, const R* src, long long j, double npar,
+                                 double hxyz) {
+  return cnt ? ((double)cnt[j] / npar) / hxyz : (double)src[j];
+}
+
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_iron(Grid g, const unsigned int* __restrict__ cnt,
+                                                  const R* __restrict__ src, R* __restrict__ dst,
+                                                  double npar, double hxyz, int smooth) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  const double c = denx_of(cnt, src, i, npar, hxyz);
+  if (!smooth) {
+    dst[i] = (R)c;
+    return;
+  }
+  const Idx3 p = idx3(i, g);
+  if (p.ix == 0 || p.iy == 0 || p.iz == 0 || p.ix == g.nx - 1 || p.iy == g.ny - 1 || p.iz == g.nz - 1) {
+    dst[i] = (R)c;
+    return;
+  }
+  const long long sy = g.nx, sz = (long long)g.nx * g.ny;
+  double s = denx_of(cnt, src, i - 1, npar, hxyz) + denx_of(cnt, src, i + 1, npar, hxyz);
+  s = s + denx_of(cnt, src, i - sy, npar, hxyz);
+  s = s + denx_of(cnt, src, i + sy, npar, hxyz);
+  s = s + denx_of(cnt, src, i - sz, npar, hxyz);
+  s = s + denx_of(cnt, src, i + sz, npar, hxyz);
+  dst[i] = (R)(8.333333333333333e-2 * s + 0.5 * c);
+}
+
+// psix = sqrt(denx); hpsix = -h2o2mx*(d2x+d2y+d2z)psix (denpart.f90:40-46)
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_psix(Grid g, const R* __restrict__ denx, Stencil13 wx,
+                                                  Stencil13 wy, Stencil13 wz, double h2o2mx,
+                                                  R* __restrict__ psix, R* __restrict__ hpsix) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  const Idx3 p = idx3(i, g);
+  psix[i] = (R)sqrt((double)denx[i]);
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  const long long syy = g.nx, szz = (long long)g.nx * g.ny;
+#pragma unroll
+  for (int m = -6; m <= 6; ++m) {
+    int q = p.ix + m;
+    if (q >= 0 && q < g.nx) sx = sx + wx.w[m + 6] * sqrt((double)denx[i + m]);
+    q = p.iy + m;
+    if (q >= 0 && q < g.ny) sy = sy + wy.w[m + 6] * sqrt((double)denx[i + m * syy]);
+    q = p.iz + m;
+    if (q >= 0 && q < g.nz) sz = sz + wz.w[m + 6] * sqrt((double)denx[i + m * szz]);
+  }
+  hpsix[i] = (R)(-h2o2mx * ((sx + sy) + sz));
+}
+
+struct GatherArgs {
+  BoxArgs b;
+  Stencil13 d1[3];
+  double m_imp;
+};
+
+// F_c at grid point j (0-based) of component c (potenimp.f90:52-69):
+// (fconv + sto7*(d_c psix*hpsix - psix*d_c hpsix))/m, sto7 = 1/denx if > 0.
+template <class R>
+__device__ inline double force_at(const GatherArgs& a, int c, const R* __restrict__ fconv,
+                                  const R* __restrict__ denx, const R* __restrict__ psix,
+                                  const R* __restrict__ hpsix, int ix, int iy, int iz) {
+  const Grid& g = a.b.g;
+  const long long j = ix + (long long)g.nx * (iy + (long long)g.ny * iz);
+  const double dx = (double)denx[j];
+  const double sto7 = dx > 0.0 ? 1.0 / dx : 0.0;
+  const int pos = c == 0 ? ix : (c == 1 ? iy : iz);
+  const int n = c == 0 ? g.nx : (c == 1 ? g.ny : g.nz);
+  const long long st = c == 0 ? 1 : (c == 1 ? (long long)g.nx : (long long)g.nx * g.ny);
+  const double s1 = d13(psix, j, pos, n, st, a.d1[c]);
+  const double s4 = d13(hpsix, j, pos, n, st, a.d1[c]);
+  return ((double)fconv[j] + sto7 * (s1 * (double)hpsix[j] - (double)psix[j] * s4)) / a.m_imp;
+}
+
+// Trilinear gather at the particles (interpola3 + blend_103,
+// potenimp.f90:76-116,278-286); zero outside [1, n-1].
+template <class R>
+__global__ void k_gather(GatherArgs a, const double* __restrict__ r, const R* __restrict__ fcx,
+                         const R* __restrict__ fcy, const R* __restrict__ fcz, const R* __restrict__ denx,
+                         const R* __restrict__ psix, const R* __restrict__ hpsix, double* __restrict__ acc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int np = a.b.npar;
+  if (i >= np) return;
+  const Grid& g = a.b.g;
+  const double ax = (r[i] - a.b.xmin) / a.b.hx + 1;
+  const double ay = (r[np + i] - a.b.ymin) / a.b.hy + 1;
+  const double az = (r[2 * np + i] - a.b.zmin) / a.b.hz + 1;
+  const int ix = (int)ax, iy = (int)ay, iz = (int)az;
+  const double rr = ax - ix, ss = ay - iy, tt = az - iz;
+  if (ix >= g.nx || iy >= g.ny || iz >= g.nz || ix <= 0 || iy <= 0 || iz <= 0) {
+    acc[i] = 0.0;
+    acc[np + i] = 0.0;
+    acc[2 * np + i] = 0.0;
+    return;
+  }
+  const int i0 = ix - 1, j0 = iy - 1, k0 = iz - 1;
+  const R* fc[3] = {fcx, fcy, fcz};
+  for (int c = 0; c < 3; ++c) {
+    const double x000 = force_at(a, c, fc[c], denx, psix, hpsix, i0, j0, k0);
+    const double x001 = force_at(a, c, fc[c], denx, psix, hpsix, i0, j0, k0 + 1);
+    const double x010 = force_at(a, c, fc[c], denx, psix, hpsix, i0, j0 + 1, k0);
+    const double x011 = force_at(a, c, fc[c], denx, psix, hpsix, i0, j0 + 1, k0 + 1);
+    const double x100 = force_at(a, c, fc[c], denx, psix, hpsix, i0 + 1, j0, k0);
+    const double x101 = force_at(a, c, fc[c], denx, psix, hpsix, i0 + 1, j0, k0 + 1);
+    const double x110 = force_at(a, c, fc[c], denx, psix, hpsix, i0 + 1, j0 + 1, k0);
+    const double x111 = force_at(a, c, fc[c], denx, psix, hpsix, i0 + 1, j0 + 1, k0 + 1);
+    const double v = 1.0 * (+x000) + rr * (-x000 + x100) + ss * (-x000 + x010) + tt * (-x000 + x001) +
+                     rr * ss * (+x000 - x100 - x010 + x110) + rr * tt * (+x000 - x100 - x001 + x101) +
+                     ss * tt * (+x000 - x010 - x001 + x011) +
+                     rr * ss * tt * (-x000 + x100 + x010 + x001 - x011 - x101 - x110 + x111);
+    acc[c * np + i] = v;
+  }
+}
+
+// ---- particle integrators (steprk.f90:78-105, steppc.f90:64-87,122-151) --
+// All particle arrays are (N_par,3) column-major: entry c*N_par + i.
+__global__ void k_part_rk(int n3, double* __restrict__ r, double* __restrict__ v,
+                          const double* __restrict__ acc, double* __restrict__ qr, double* __restrict__ qv,
+                          double* __restrict__ r1_out, double* __restrict__ v1_out,
+                          double* __restrict__ a1_out, double ca, double cb, double cc, double dt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n3) return;
+  const double vi = v[i], ai = acc[i];
+  double st = ca * (vi - cb * qr[i]);
+  qr[i] = (qr[i] + 3.0 * st) - cc * vi;
+  if (r1_out) r1_out[i] = r[i];
+  r[i] = r[i] + dt * st;
+  st = ca * (ai - cb * qv[i]);
+  qv[i] = (qv[i] + 3.0 * st) - cc * ai;
+  if (a1_out) a1_out[i] = ai;
+  if (v1_out) v1_out[i] = vi;
+  v[i] = vi + dt * st;
+}
+
+__global__ void k_part_predict(int n3, double* __restrict__ r, const double* __restrict__ v,
+                               const double* __restrict__ acc, double* r3, const double* __restrict__ v1,
+                               const double* __restrict__ v2, double* v3, const double* __restrict__ a1,
+                               double* a2, double* __restrict__ pcr, double* __restrict__ pcv,
+                               double c4o3dt, double c112) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n3) return;
+  double st = r3[i] + c4o3dt * ((2.0 * v[i] - v1[i]) + 2.0 * v2[i]);
+  r3[i] = r[i];
+  r[i] = st - c112 * pcr[i];
+  pcr[i] = st;
+  st = v3[i] + c4o3dt * ((2.0 * acc[i] - a1[i]) + 2.0 * a2[i]);
+  v3[i] = st - c112 * pcv[i];
+  pcv[i] = st;
+  a2[i] = acc[i];
+}
+
+__global__ void k_part_correct(int n3, double* __restrict__ r, double* __restrict__ v,
+                               const double* __restrict__ acc, const double* __restrict__ r3,
+                               const double* __restrict__ r2, const double* __restrict__ v1,
+                               const double* __restrict__ v2, double* v3, const double* __restrict__ a1,
+                               const double* __restrict__ a2, double* __restrict__ pcr,
+                               double* __restrict__ pcv, double* __restrict__ part, double dt3,
+                               double c9) {
+  __shared__ double sh[16];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double er = 0.0, ev = 0.0;
+  if (i < n3) {
+    double st = 0.125 * ((9.0 * r3[i] - r2[i]) + dt3 * ((v3[i] + 2.0 * v[i]) - v1[i]));
+    double p = pcr[i] - st;
+    pcr[i] = p;
+    r[i] = st + c9 * p;
+    er = fabs(c9 * p);
+    st = 0.125 * ((9.0 * v[i] - v2[i]) + dt3 * ((acc[i] + 2.0 * a1[i]) - a2[i]));
+    p = pcv[i] - st;
+    pcv[i] = p;
+    v3[i] = v[i];
+    v[i] = st + c9 * p;
+    ev = fabs(c9 * p);
+  }
+  double s = block_sum(er, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  __syncthreads();
+  s = block_sum(ev, sh);
+  if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = s;
+}
+
+// Per-component sums of r, v and a for the per-step means (DFT4He3d.f90:864-878).
+__global__ void k_part_sums(int np, const double* __restrict__ r, const double* __restrict__ v,
+                            const double* __restrict__ acc, double* __restrict__ part) {
+  __shared__ double sh[16];
+  const int comp = blockIdx.y;  // 0..8
+  const double* src = comp < 3 ? r : (comp < 6 ? v : acc);
+  const int c = comp % 3;
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x)
+    s += src[(long long)c * np + i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[comp * gridDim.x + blockIdx.x] = s;
+}
+
+// ---- energy (energy.f90:32-124) ----------------------------------------
+struct EnergyArgs {
+  Grid g;
+  Stencil13 d1x, d1y, d1z;
+  double aux1, aux2;   // cp4/2, cpp4/3
+  double den0s, beta, den_m;
+  int otc;
+};
+
+// Slots: 0 |grad psi|^2, 1 delj4*den, 2 correlation, 3 alpha_s, 4 solid,
+// 5 psix*hpsix, 6 uimp*den, 7 den (norm).
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_energy(EnergyArgs a, const c64* __restrict__ psi,
+                                                    const R* __restrict__ den, const R* __restrict__ dencg,
+                                                    const R* __restrict__ delj4, const R* __restrict__ denalf,
+                                                    const R* __restrict__ dxd, const R* __restrict__ dyd,
+                                                    const R* __restrict__ dzd, const R* __restrict__ ix,
+                                                    const R* __restrict__ iy, const R* __restrict__ iz,
+                                                    const R* __restrict__ psix, const R* __restrict__ hpsix,
+                                                    const R* __restrict__ uimp, double* __restrict__ part) {
+  __shared__ double sh[16];
+  const Grid& g = a.g;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (i < g.n) {
+    const Idx3 p = idx3(i, g);
+    const c64 gx = d13c(psi, i, p.ix, g.nx, 1, a.d1x);
+    const c64 gy = d13c(psi, i, p.iy, g.ny, g.nx, a.d1y);
+    const c64 gz = d13c(psi, i, p.iz, g.nz, (long long)g.nx * g.ny, a.d1z);
+    const double ax = hypot(gx.x, gx.y), ay = hypot(gy.x, gy.y), az = hypot(gz.x, gz.y);
+    v[0] = ax * ax + ay * ay + az * az;
+    const double dn = (double)den[i], dcg = (double)dencg[i];
+    v[1] = (double)delj4[i] * dn;
+    v[2] = dn * (dcg * dcg) * (a.aux1 + a.aux2 * dcg);
+    if (a.otc) {
+      const double fal = 1.0 - (double)denalf[i] / a.den0s;
+      v[3] = fal * ((double)dxd[i] * (double)ix[i] + (double)dyd[i] * (double)iy[i] +
+                    (double)dzd[i] * (double)iz[i]);
+    }
+    v[4] = dn * (1.0 + tanh(a.beta * (dn - a.den_m)));
+    v[5] = (double)psix[i] * (double)hpsix[i];
+    v[6] = (double)uimp[i] * dn;
+    v[7] = dn;
+  }
+  for (int s = 0; s < 8; ++s) {
+    const double t = block_sum(v[s], sh);
+    if (threadIdx.x == 0) part[(long long)s * gridDim.x + blockIdx.x] = t;
+    __syncthreads();
+  }
+}
+
+__global__ void k_sq_sum(int n, const double* __restrict__ v, double* __restrict__ part) {
+  __shared__ double sh[16];
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += v[i] * v[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+}  // namespace hetd

@@ -1,0 +1,1130 @@
+// MI355X (gfx950) engine behind the hetddft C-ABI (include/hetddft.h).
+//
+// One context = one droplet+impurity instance resident in HBM.  The per-step
+// work of the reference (steprk.f90, steppc.f90, potenimp.f90, poten.f90,
+// term_alfa.f90, energy.f90) is issued as a fixed sequence of hand-written
+// kernels on one HIP stream; history slots rotate by pointer, never by copy.
+#include "../../include/hetddft.h"
+#include "common.h"
+#include "fft_kernels.h"
+#include "field_kernels.h"
+#include "kinit.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+using namespace hetd;
+
+namespace {
+
+const int kValid[] = {1,  2,  3,  4,  5,  6,  8,  9,  10, 12, 15, 16, 18, 20, 24, 25, 27, 30,
+                      32, 36, 40, 45, 48, 50, 54, 60, 64, 72, 75, 80, 81, 90, 96, 100, 108, 120,
+                      125, 128, 135, 144, 150, 160, 162, 180, 192, 200, 216, 225, 240, 243, 250, 256};
+
+bool tstgrid_ok(int n) {
+  for (int v : kValid)
+    if (v == n) return true;
+  return false;
+}
+
+FftPlan make_plan(int n) {
+  FftPlan p{};
+  p.n = n;
+  int m = n;
+  const int order[] = {8, 4, 2, 5, 3};
+  while (m > 1) {
+    int r = 0;
+    for (int c : order)
+      if (m % c == 0) {
+        r = c;
+        break;
+      }
+    if (r == 0 || p.nstages >= kMaxStages) {
+      p.nstages = -1;
+      return p;
+    }
+    p.radix[p.nstages++] = r;
+    m /= r;
+  }
+  return p;
+}
+
+Stencil13 make_stencil(int order, double h) {
+  // Centre rows of the 13-point table (mates.f:827-832 for K=1, :874-879 for
+  // K=2), offsets -6..6, times 1/h^order as Deriv3D_p does.
+  static const long long d1n[6] = {6, -15, 5, -1, 1, -1}, d1d[6] = {7, 56, 63, 56, 385, 5544};
+  static const long long d2n[6] = {12, -15, 10, -1, 2, -1}, d2d[6] = {7, 56, 189, 112, 1925, 16632};
+  Stencil13 s{};
+  const double sto = 1.0 / (order == 1 ? h : h * h);
+  for (int j = 1; j <= 6; ++j) {
+    double c;
+    if (order == 1) {
+      c = (double)d1n[j - 1] / (double)d1d[j - 1];
+      s.w[6 + j] = c * sto;
+      s.w[6 - j] = (-c) * sto;
+    } else {
+      c = (double)d2n[j - 1] / (double)d2d[j - 1];
+      s.w[6 + j] = c * sto;
+      s.w[6 - j] = c * sto;
+    }
+  }
+  s.w[6] = order == 1 ? 0.0 * sto : ((double)(-5369) / 1800.0) * sto;
+  return s;
+}
+
+}  // namespace
+
+struct hetddft_ctx {
+  hetddft_params p{};
+  Grid g{};
+  double hx = 0, hy = 0, hz = 0, dxyz = 0, renor = 0, m_imp = 0, h2o2mx = 0;
+  hipStream_t st = nullptr;
+  std::string err;
+  long long bytes = 0;
+  bool kernels_set = false, started = false;
+  int iter_in_run = 0;
+  bool mixed = false;
+  size_t tsz = 8;  // bytes of the FFT/field real type
+
+  Stencil13 d1x{}, d1y{}, d1z{}, d2x{}, d2y{}, d2z{};
+  FftPlan pl_xh{}, pl_y{}, pl_z{};
+  int rows = 16, tx_y = 16, tx_z = 16, ntile_y = 1, ntile_z = 1;
+  size_t lds_x = 0, lds_y = 0, lds_z = 0;
+
+  // device memory
+  std::vector<void*> allocs;
+  void* tw_xh = nullptr;  // exp(-2 pi i m/(nx/2))
+  void* twh = nullptr;    // exp(-2 pi i k/nx), k = 0..nx/2
+  void* tw_y = nullptr;
+  void* tw_z = nullptr;
+  c64* timec = nullptr;
+  // psi side (always FP64 complex)
+  c64* psi = nullptr;
+  c64* spare = nullptr;
+  c64* old[3] = {nullptr, nullptr, nullptr};
+  c64* hold[2] = {nullptr, nullptr};
+  c64* pc = nullptr;
+  c64* q = nullptr;
+  int ioldp[3] = {0, 1, 2}, ioldh[2] = {0, 1};
+  // potential side (R = double or float)
+  enum {
+    F_DEN, F_POT4, F_UIMP, F_DENCG, F_DELJ4, F_DENALF, F_DX, F_DY, F_DZ, F_IX, F_IY, F_IZ,
+    F_DENXA, F_PSIX, F_HPSIX, F_FCX, F_FCY, F_FCZ, F_STO1, F_UACONV, F_DENXB, F_COUNT
+  };
+  void* fld[F_COUNT] = {};
+  void* denx = nullptr;  // whichever of DENXA/DENXB holds the ironed histogram
+  unsigned int* counts = nullptr;
+  void* spec[12] = {};
+  void* kfvlj4 = nullptr;
+  void* kwcgk = nullptr;
+  void* kkalfs = nullptr;
+  void* kvq = nullptr;
+  // particles
+  double* rimp = nullptr;
+  double* vimp = nullptr;
+  double* aimp = nullptr;
+  double* rold[3] = {};
+  double* vold[3] = {};
+  double* aold[2] = {};
+  double* qr = nullptr;
+  double* qv = nullptr;
+  double* pcr = nullptr;
+  double* pcv = nullptr;
+  int ioldr[3] = {0, 1, 2}, ioldv[3] = {0, 1, 2}, iolda[2] = {0, 1};
+  // reductions
+  double* part = nullptr;
+  long long part_len = 0;
+  double* res = nullptr;       // device results
+  double* hres = nullptr;      // pinned host copy
+  double last_err[3] = {0, 0, 0};
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int fail(hetddft_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                       \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess) return fail(ctx, HETDDFT_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCHCHK(ctx)                                                                            \
+  do {                                                                                            \
+    hipError_t e_ = hipGetLastError();                                                            \
+    if (e_ != hipSuccess) return fail(ctx, HETDDFT_E_HIP, "kernel launch: %s (%s:%d)",            \
+                                      hipGetErrorString(e_), __FILE__, __LINE__);                 \
+  } while (0)
+
+#define RET(expr)             \
+  do {                        \
+    int r_ = (expr);          \
+    if (r_ != HETDDFT_OK) return r_; \
+  } while (0)
+
+int dalloc(hetddft_ctx* c, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(c, HETDDFT_E_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  c->allocs.push_back(*p);
+  c->bytes += (long long)bytes;
+  hipError_t e2 = hipMemset(*p, 0, bytes);
+  if (e2 != hipSuccess) return fail(c, HETDDFT_E_HIP, "hipMemset: %s", hipGetErrorString(e2));
+  return HETDDFT_OK;
+}
+
+template <class T>
+int upload_twiddles(hetddft_ctx* c, void** dst, int n, int count, int denom) {
+  // table[m] = exp(-2 pi i m/denom), m = 0..count-1, built in long double
+  std::vector<cplx<T>> h(count);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int m = 0; m < count; ++m) {
+    const long double a = -2.0L * pi * (long double)m / (long double)denom;
+    h[m].x = (T)cosl(a);
+    h[m].y = (T)sinl(a);
+  }
+  (void)n;
+  RET(dalloc(c, dst, sizeof(cplx<T>) * count));
+  HIPCHK(c, hipMemcpy(*dst, h.data(), sizeof(cplx<T>) * count, hipMemcpyHostToDevice));
+  return HETDDFT_OK;
+}
+
+inline unsigned blocks_for(long long n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
+
+template <class T>
+void set_lds_attr() {
+  hipFuncSetAttribute((const void*)k_fft_xf<T, T>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  hipFuncSetAttribute((const void*)k_fft_xi<T, T>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  hipFuncSetAttribute((const void*)k_fft_y<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  hipFuncSetAttribute((const void*)k_fft_y<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  hipFuncSetAttribute((const void*)k_fft_zconv<T>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+// Line tile sizes: keep each workgroup's LDS near 64 KiB so two workgroups
+// share a CU; tiles of consecutive kx give >= 13*16 B contiguous per row.
+void choose_tiles(hetddft_ctx* c) {
+  const size_t esz = 2 * c->tsz;
+  const size_t budget = 64 * 1024;
+  const int m = c->g.nx / 2;
+  int rows = 16;
+  while (rows > 1 && (size_t)2 * rows * (m + 1) * esz > budget) rows /= 2;
+  c->rows = rows;
+  c->lds_x = (size_t)2 * rows * (m + 1) * esz;
+  auto pick = [&](int n, int nbuf, int& tx, int& ntile, size_t& lds) {
+    int txmax = 16;
+    while (txmax > 1 && (size_t)nbuf * txmax * (n + 1) * esz > budget + budget / 2) --txmax;
+    const int M = c->g.mx;
+    ntile = (M + txmax - 1) / txmax;
+    tx = (M + ntile - 1) / ntile;
+    lds = (size_t)nbuf * tx * (n + 1) * esz;
+  };
+  pick(c->g.ny, 2, c->tx_y, c->ntile_y, c->lds_y);
+  pick(c->g.nz, 3, c->tx_z, c->ntile_z, c->lds_z);
+}
+
+template <class T>
+int fft_alloc_tables(hetddft_ctx* c) {
+  const int m = c->g.nx / 2;
+  RET(upload_twiddles<T>(c, &c->tw_xh, m, m, m));
+  RET(upload_twiddles<T>(c, &c->twh, c->g.nx, m + 1, c->g.nx));
+  RET(upload_twiddles<T>(c, &c->tw_y, c->g.ny, c->g.ny, c->g.ny));
+  RET(upload_twiddles<T>(c, &c->tw_z, c->g.nz, c->g.nz, c->g.nz));
+  set_lds_attr<T>();
+  return HETDDFT_OK;
+}
+
+// ---- one batched convolution: out_j = c2r(r2c(in_i) * sign*kernel) ------
+template <class T>
+struct ConvJob {
+  int input;
+  const void* kern;
+  double sign;
+  void* out;  // real field R*
+};
+
+template <class T>
+int conv_batch(hetddft_ctx* c, const XfArgs<T>& xf_in, int nin, const std::vector<ConvJob<T>>& jobs) {
+  const Grid& g = c->g;
+  XfArgs<T> xa = xf_in;
+  for (int i = 0; i < nin; ++i) xa.out[i] = c->spec[i];
+  const long long nrows = (long long)g.ny * g.nz;
+  dim3 gx((unsigned)((nrows + c->rows - 1) / c->rows), (unsigned)nin);
+  hipLaunchKernelGGL((k_fft_xf<T, T>), gx, dim3(kBlock), c->lds_x, c->st, xa, g, c->pl_xh, c->rows,
+                     (const cplx<T>*)c->tw_xh, (const cplx<T>*)c->twh);
+  LAUNCHCHK(c);
+  YArgs<T> ya{};
+  for (int i = 0; i < nin; ++i) ya.buf[i] = c->spec[i];
+  hipLaunchKernelGGL((k_fft_y<T, false>), dim3(c->ntile_y, g.nz, nin), dim3(kBlock), c->lds_y, c->st, ya, g,
+                     c->pl_y, c->tx_y, (const cplx<T>*)c->tw_y);
+  LAUNCHCHK(c);
+  // z pass: jobs grouped by input
+  ZArgs<T> za{};
+  int nout = 0;
+  for (int i = 0; i < nin; ++i) {
+    za.in[i] = c->spec[i];
+    za.job0[i] = nout;
+    for (const auto& j : jobs)
+      if (j.input == i) {
+        if (nout >= 7) return fail(c, HETDDFT_E_ARG, "conv_batch: more than 7 outputs");
+        za.kern[nout] = (const T*)j.kern;
+        za.ksign[nout] = (T)j.sign;
+        za.out[nout] = c->spec[5 + nout];
+        ++nout;
+      }
+  }
+  za.job0[nin] = nout;
+  hipLaunchKernelGGL((k_fft_zconv<T>), dim3(c->ntile_z, g.ny, nin), dim3(kBlock), c->lds_z, c->st, za, g,
+                     c->pl_z, c->tx_z, (const cplx<T>*)c->tw_z);
+  LAUNCHCHK(c);
+  YArgs<T> yb{};
+  for (int k = 0; k < nout; ++k) yb.buf[k] = c->spec[5 + k];
+  hipLaunchKernelGGL((k_fft_y<T, true>), dim3(c->ntile_y, g.nz, nout), dim3(kBlock), c->lds_y, c->st, yb, g,
+                     c->pl_y, c->tx_y, (const cplx<T>*)c->tw_y);
+  LAUNCHCHK(c);
+  // outputs in job order of the z pass
+  XiArgs<T, T> xi{};
+  xi.renor = c->renor;
+  {
+    int k = 0;
+    for (int i = 0; i < nin; ++i)
+      for (const auto& j : jobs)
+        if (j.input == i) {
+          xi.in[k] = c->spec[5 + k];
+          xi.out[k] = (T*)j.out;
+          ++k;
+        }
+  }
+  dim3 gxi((unsigned)((nrows + c->rows - 1) / c->rows), (unsigned)nout);
+  hipLaunchKernelGGL((k_fft_xi<T, T>), gxi, dim3(kBlock), c->lds_x, c->st, xi, g, c->pl_xh, c->rows,
+                     (const cplx<T>*)c->tw_xh, (const cplx<T>*)c->twh);
+  LAUNCHCHK(c);
+  return HETDDFT_OK;
+}
+
+BoxArgs box_args(const hetddft_ctx* c) {
+  BoxArgs b{};
+  b.g = c->g;
+  b.xmin = -c->p.xmax;
+  b.ymin = -c->p.ymax;
+  b.zmin = -c->p.zmax;
+  b.hx = c->hx;
+  b.hy = c->hy;
+  b.hz = c->hz;
+  b.npar = c->p.n_par;
+  return b;
+}
+
+template <class T>
+__global__ void k_fill_u32(unsigned int* p, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// potenimp(); poten(); forceimp()
+template <class T>
+int veval(hetddft_ctx* c) {
+  const Grid& g = c->g;
+  auto F = [&](int id) { return (T*)c->fld[id]; };
+  // ---- potenimp: denpart + uimp (potenimp.f90:1-20, denpart.f90:5-49)
+  hipLaunchKernelGGL((k_fill_u32<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, c->counts, g.n);
+  LAUNCHCHK(c);
+  BoxArgs b = box_args(c);
+  if (c->p.n_par > 0) {
+    hipLaunchKernelGGL(k_hist, dim3(blocks_for(c->p.n_par)), dim3(kBlock), 0, c->st, b, c->rimp, c->counts);
+    LAUNCHCHK(c);
+  }
+  const double hxyz = c->hx * c->hy * c->hz;
+  const double npar = (double)c->p.n_par;
+  T* cur = F(hetddft_ctx::F_DENXA);
+  T* oth = F(hetddft_ctx::F_DENXB);
+  hipLaunchKernelGGL((k_iron<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, g, c->counts, (const T*)nullptr,
+                     cur, npar, hxyz, c->p.ironx > 0 ? 1 : 0);
+  LAUNCHCHK(c);
+  for (int it = 1; it < c->p.ironx; ++it) {
+    hipLaunchKernelGGL((k_iron<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, g,
+                       (const unsigned int*)nullptr, (const T*)cur, oth, npar, hxyz, 1);
+    LAUNCHCHK(c);
+    std::swap(cur, oth);
+  }
+  c->denx = cur;
+  hipLaunchKernelGGL((k_psix<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, g, (const T*)cur, c->d2x, c->d2y,
+                     c->d2z, c->h2o2mx, F(hetddft_ctx::F_PSIX), F(hetddft_ctx::F_HPSIX));
+  LAUNCHCHK(c);
+  // ---- poten: derden (derden.f90:26-28)
+  hipLaunchKernelGGL((k_grad3<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, g,
+                     (const T*)F(hetddft_ctx::F_DEN), c->d1x, c->d1y, c->d1z, F(hetddft_ctx::F_DX),
+                     F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ));
+  LAUNCHCHK(c);
+  const bool otc = c->p.core_otc != 0;
+  // batch 1: denx*Vq -> uimp; den*{wcgk,fvlj4,kalfs} -> dencg, delj4, denalf;
+  //          -grad(den)*Vq -> force convolutions (potenimp.f90:15-17,39-50; poten.f90:27-50,94-99)
+  {
+    XfArgs<T> xa{};
+    const int ids[5] = {-1, hetddft_ctx::F_DEN, hetddft_ctx::F_DX, hetddft_ctx::F_DY, hetddft_ctx::F_DZ};
+    for (int i = 0; i < 5; ++i) {
+      xa.op[i] = XF_COPY;
+      xa.in[i][0] = i == 0 ? (const T*)cur : (const T*)F(ids[i]);
+    }
+    std::vector<ConvJob<T>> jobs;
+    jobs.push_back({0, c->kvq, 1.0, F(hetddft_ctx::F_UIMP)});
+    jobs.push_back({1, c->kwcgk, 1.0, F(hetddft_ctx::F_DENCG)});
+    jobs.push_back({1, c->kfvlj4, 1.0, F(hetddft_ctx::F_DELJ4)});
+    if (otc) jobs.push_back({1, c->kkalfs, 1.0, F(hetddft_ctx::F_DENALF)});
+    jobs.push_back({2, c->kvq, -1.0, F(hetddft_ctx::F_FCX)});
+    jobs.push_back({3, c->kvq, -1.0, F(hetddft_ctx::F_FCY)});
+    jobs.push_back({4, c->kvq, -1.0, F(hetddft_ctx::F_FCZ)});
+    RET(conv_batch<T>(c, xa, 5, jobs));
+  }
+  // batch 2: correlation term (poten.f90:62-74); alpha_s integrals (term_alfa.f90:28-52)
+  {
+    XfArgs<T> xa{};
+    xa.op[0] = XF_CORREL;
+    xa.in[0][0] = F(hetddft_ctx::F_DEN);
+    xa.in[0][1] = F(hetddft_ctx::F_DENCG);
+    xa.c[0][0] = c->p.cp4;
+    xa.c[0][1] = c->p.cpp4;
+    std::vector<ConvJob<T>> jobs;
+    jobs.push_back({0, c->kwcgk, 1.0, F(hetddft_ctx::F_STO1)});
+    int nin = 1;
+    if (otc) {
+      const int d[3] = {hetddft_ctx::F_DX, hetddft_ctx::F_DY, hetddft_ctx::F_DZ};
+      const int o[3] = {hetddft_ctx::F_IX, hetddft_ctx::F_IY, hetddft_ctx::F_IZ};
+      for (int k = 0; k < 3; ++k) {
+        xa.op[1 + k] = XF_FALFS_D;
+        xa.in[1 + k][0] = F(hetddft_ctx::F_DENALF);
+        xa.in[1 + k][1] = F(d[k]);
+        xa.c[1 + k][0] = c->p.den0s;
+        jobs.push_back({1 + k, c->kkalfs, 1.0, F(o[k])});
+      }
+      nin = 4;
+    }
+    RET(conv_batch<T>(c, xa, nin, jobs));
+  }
+  if (otc) {
+    // batch 3: grad(den).int -> ualphas convolution (term_alfa.f90:65-80)
+    XfArgs<T> xa{};
+    xa.op[0] = XF_GRAD_DOT;
+    xa.in[0][0] = F(hetddft_ctx::F_DX);
+    xa.in[0][1] = F(hetddft_ctx::F_DY);
+    xa.in[0][2] = F(hetddft_ctx::F_DZ);
+    xa.in[0][3] = F(hetddft_ctx::F_IX);
+    xa.in[0][4] = F(hetddft_ctx::F_IY);
+    xa.in[0][5] = F(hetddft_ctx::F_IZ);
+    std::vector<ConvJob<T>> jobs;
+    jobs.push_back({0, c->kkalfs, 1.0, F(hetddft_ctx::F_UACONV)});
+    RET(conv_batch<T>(c, xa, 1, jobs));
+  }
+  // pot4 assembly with the alpha_s divergence (term_alfa.f90:85-113, poten.f90:80-124)
+  PotArgs pa{};
+  pa.g = g;
+  pa.d1x = c->d1x;
+  pa.d1y = c->d1y;
+  pa.d1z = c->d1z;
+  pa.a0 = c->p.cp4 / 2.0;
+  pa.a1 = c->p.cpp4 / 3.0;
+  pa.den0s = c->p.den0s;
+  pa.h2o2m4 = c->p.h2o2m4;
+  pa.alphas = c->p.alphas;
+  pa.C = c->p.C;
+  pa.beta = c->p.beta;
+  pa.den_m = c->p.den_m;
+  pa.otc = otc ? 1 : 0;
+  hipLaunchKernelGGL((k_pot_final<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, pa,
+                     (const T*)F(hetddft_ctx::F_DEN), (const T*)F(hetddft_ctx::F_DENCG),
+                     (const T*)F(hetddft_ctx::F_DELJ4), (const T*)F(hetddft_ctx::F_STO1),
+                     (const T*)F(hetddft_ctx::F_UIMP), (const T*)F(hetddft_ctx::F_DENALF),
+                     (const T*)F(hetddft_ctx::F_IX), (const T*)F(hetddft_ctx::F_IY),
+                     (const T*)F(hetddft_ctx::F_IZ), (const T*)F(hetddft_ctx::F_UACONV), F(hetddft_ctx::F_POT4));
+  LAUNCHCHK(c);
+  // ---- forceimp gather (potenimp.f90:52-71)
+  if (c->p.n_par > 0) {
+    GatherArgs ga{};
+    ga.b = b;
+    ga.d1[0] = c->d1x;
+    ga.d1[1] = c->d1y;
+    ga.d1[2] = c->d1z;
+    ga.m_imp = c->m_imp;
+    hipLaunchKernelGGL((k_gather<T>), dim3(blocks_for(c->p.n_par)), dim3(kBlock), 0, c->st, ga, c->rimp,
+                       (const T*)F(hetddft_ctx::F_FCX), (const T*)F(hetddft_ctx::F_FCY),
+                       (const T*)F(hetddft_ctx::F_FCZ), (const T*)cur, (const T*)F(hetddft_ctx::F_PSIX),
+                       (const T*)F(hetddft_ctx::F_HPSIX), c->aimp);
+    LAUNCHCHK(c);
+  }
+  return HETDDFT_OK;
+}
+
+int run_veval(hetddft_ctx* c) { return c->mixed ? veval<float>(c) : veval<double>(c); }
+
+HamArgs ham_args(const hetddft_ctx* c) {
+  HamArgs a{};
+  a.g = c->g;
+  a.d2x = c->d2x;
+  a.d2y = c->d2y;
+  a.d2z = c->d2z;
+  a.h2o2m4 = c->p.h2o2m4;
+  a.timec_z = c->timec;
+  return a;
+}
+
+template <class T>
+int step_rk_t(hetddft_ctx* c, double dt) {
+  const double s2 = std::sqrt(2.0);
+  const double arun[4] = {0.5, 1.0 - 1.0 / s2, 1.0 + 1.0 / s2, 1.0 / 6.0};
+  const double brun[4] = {2.0, 1.0, 1.0, 2.0};
+  const double crun[4] = {0.5, 1.0 - 1.0 / s2, 1.0 + 1.0 / s2, 0.5};
+  const Grid& g = c->g;
+  const HamArgs ha = ham_args(c);
+  const int n3 = 3 * c->p.n_par;
+  for (int j = 0; j < 4; ++j) {
+    c64* out;
+    c64* h1o = nullptr;
+    double *r1o = nullptr, *v1o = nullptr, *a1o = nullptr;
+    if (j == 0) {
+      // psiold(3)=psiold(2); psiold(2)=psiold(1); psiold(1)=psi; hpsiold(2)=hpsiold(1)
+      c64* freed = c->old[2];
+      c->old[2] = c->old[1];
+      c->old[1] = c->old[0];
+      c->old[0] = c->psi;  // the current psi becomes psiold(1); new psi goes to `freed`
+      out = freed;
+      c64* hf = c->hold[1];
+      c->hold[1] = c->hold[0];
+      c->hold[0] = hf;
+      h1o = hf;
+      double* rf = c->rold[2];
+      c->rold[2] = c->rold[1];
+      c->rold[1] = c->rold[0];
+      c->rold[0] = rf;
+      r1o = rf;
+      double* vf = c->vold[2];
+      c->vold[2] = c->vold[1];
+      c->vold[1] = c->vold[0];
+      c->vold[0] = vf;
+      v1o = vf;
+      double* af = c->aold[1];
+      c->aold[1] = c->aold[0];
+      c->aold[0] = af;
+      a1o = af;
+    } else {
+      out = c->spare;
+    }
+    hipLaunchKernelGGL((k_rk_stage<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, ha, (const c64*)c->psi,
+                       (const T*)c->fld[hetddft_ctx::F_POT4], c->q, out, (T*)c->fld[hetddft_ctx::F_DEN], h1o,
+                       arun[j], brun[j], crun[j], dt);
+    LAUNCHCHK(c);
+    if (j == 0) {
+      c->psi = out;
+    } else {
+      c->spare = c->psi;
+      c->psi = out;
+    }
+    if (n3 > 0) {
+      hipLaunchKernelGGL(k_part_rk, dim3(blocks_for(n3)), dim3(kBlock), 0, c->st, n3, c->rimp, c->vimp,
+                         (const double*)c->aimp, c->qr, c->qv, r1o, v1o, a1o, arun[j], brun[j], crun[j], dt);
+      LAUNCHCHK(c);
+    }
+    if (j <= 2) RET(veval<T>(c));
+  }
+  for (int k = 0; k < 3; ++k) c->ioldp[k] = c->ioldr[k] = c->ioldv[k] = k;
+  c->ioldh[0] = c->iolda[0] = 0;
+  c->ioldh[1] = c->iolda[1] = 1;
+  return HETDDFT_OK;
+}
+
+template <class T>
+int step_pc_t(hetddft_ctx* c, double dt) {
+  const double c112 = 112.0 / 121.0, c9 = 9.0 / 121.0, c4o3 = 4.0 / 3.0;
+  const Grid& g = c->g;
+  const HamArgs ha = ham_args(c);
+  int* P = c->ioldp;
+  int* H = c->ioldh;
+  // predictor + modifier (steppc.f90:36-62)
+  c64* x = c->psi;
+  c64* y = c->old[P[2]];
+  hipLaunchKernelGGL((k_pc_predict<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, ha, (const c64*)x,
+                     (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]], c->hold[H[1]], c->pc,
+                     (T*)c->fld[hetddft_ctx::F_DEN], c4o3 * dt, c112);
+  LAUNCHCHK(c);
+  c->old[P[2]] = x;
+  c->psi = y;
+  std::swap(H[0], H[1]);
+  int* R = c->ioldr;
+  int* V = c->ioldv;
+  int* A = c->iolda;
+  const int n3 = 3 * c->p.n_par;
+  if (n3 > 0) {
+    hipLaunchKernelGGL(k_part_predict, dim3(blocks_for(n3)), dim3(kBlock), 0, c->st, n3, c->rimp,
+                       (const double*)c->vimp, (const double*)c->aimp, c->rold[R[2]], (const double*)c->vold[V[0]],
+                       (const double*)c->vold[V[1]], c->vold[V[2]], (const double*)c->aold[A[0]], c->aold[A[1]],
+                       c->pcr, c->pcv, c4o3 * dt, c112);
+    LAUNCHCHK(c);
+  }
+  std::swap(A[0], A[1]);
+  RET(veval<T>(c));
+  // corrector (steppc.f90:97-151)
+  const unsigned nb = blocks_for(g.n);
+  hipLaunchKernelGGL((k_pc_correct<T>), dim3(nb), dim3(kBlock), 0, c->st, ha, (const c64*)c->psi,
+                     (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
+                     (const c64*)c->hold[H[0]], (const c64*)c->hold[H[1]], c->pc, c->spare,
+                     (T*)c->fld[hetddft_ctx::F_DEN], c->part, 3.0 * dt, c9);
+  LAUNCHCHK(c);
+  std::swap(c->psi, c->spare);
+  hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(kBlock), 0, c->st, (const double*)c->part, (int)nb, 1, c->res);
+  LAUNCHCHK(c);
+  {
+    const int t = P[2];
+    P[2] = P[1];
+    P[1] = P[0];
+    P[0] = t;
+  }
+  if (n3 > 0) {
+    const unsigned pb = blocks_for(n3);
+    double* pp = c->part + nb;  // after the psi partials
+    hipLaunchKernelGGL(k_part_correct, dim3(pb), dim3(kBlock), 0, c->st, n3, c->rimp, c->vimp,
+                       (const double*)c->aimp, (const double*)c->rold[R[2]], (const double*)c->rold[R[1]],
+                       (const double*)c->vold[V[0]], (const double*)c->vold[V[1]], c->vold[V[2]],
+                       (const double*)c->aold[A[0]], (const double*)c->aold[A[1]], c->pcr, c->pcv, pp, 3.0 * dt, c9);
+    LAUNCHCHK(c);
+    hipLaunchKernelGGL(k_finish_sums, dim3(2), dim3(kBlock), 0, c->st, (const double*)pp, (int)pb, 2, c->res + 1);
+    LAUNCHCHK(c);
+  } else {
+    HIPCHK(c, hipMemsetAsync(c->res + 1, 0, 2 * sizeof(double), c->st));
+  }
+  {
+    int t = R[2];
+    R[2] = R[1];
+    R[1] = R[0];
+    R[0] = t;
+    t = V[2];
+    V[2] = V[1];
+    V[1] = V[0];
+    V[0] = t;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->hres, c->res, 3 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  return HETDDFT_OK;
+}
+
+int fetch_errors(hetddft_ctx* c, double err[3]) {
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  const double c9 = 9.0 / 121.0;
+  const double e0 = c9 * c->hres[0] / (double)c->g.n;
+  const double e1 = c->hres[1] * 0.3333333333 / (double)c->p.n_par;
+  const double e2 = c->hres[2] * 0.3333333333 / (double)c->p.n_par;
+  c->last_err[0] = e0;
+  c->last_err[1] = c->p.n_par > 0 ? e1 : 0.0;
+  c->last_err[2] = c->p.n_par > 0 ? e2 : 0.0;
+  if (err) std::memcpy(err, c->last_err, sizeof c->last_err);
+  if (!std::isfinite(e0) || !std::isfinite(c->last_err[1]) || !std::isfinite(c->last_err[2]))
+    return fail(c, HETDDFT_E_NONFINITE, "non-finite PC error estimate (errHe=%g)", e0);
+  return HETDDFT_OK;
+}
+
+template <class T>
+int field_to_host(hetddft_ctx* c, const void* dsrc, double* out) {
+  const size_t n = (size_t)c->g.n;
+  if (sizeof(T) == sizeof(double)) {
+    HIPCHK(c, hipMemcpyAsync(out, dsrc, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+  } else {
+    std::vector<float> tmp(n);
+    HIPCHK(c, hipMemcpyAsync(tmp.data(), dsrc, n * sizeof(float), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (size_t i = 0; i < n; ++i) out[i] = tmp[i];
+  }
+  return HETDDFT_OK;
+}
+
+int upload_real(hetddft_ctx* c, void* dst, const double* src, size_t n) {
+  if (!c->mixed) {
+    HIPCHK(c, hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> tmp(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = (float)src[i];
+    HIPCHK(c, hipMemcpy(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return HETDDFT_OK;
+}
+
+template <class T>
+int energy_t(hetddft_ctx* c, double out[12]) {
+  const Grid& g = c->g;
+  auto F = [&](int id) { return (const T*)c->fld[id]; };
+  EnergyArgs ea{};
+  ea.g = g;
+  ea.d1x = c->d1x;
+  ea.d1y = c->d1y;
+  ea.d1z = c->d1z;
+  ea.aux1 = 0.5 * c->p.cp4;
+  ea.aux2 = (1.0 / 3.0) * c->p.cpp4;
+  ea.den0s = c->p.den0s;
+  ea.beta = c->p.beta;
+  ea.den_m = c->p.den_m;
+  ea.otc = c->p.core_otc;
+  const unsigned nb = blocks_for(g.n);
+  hipLaunchKernelGGL((k_energy<T>), dim3(nb), dim3(kBlock), 0, c->st, ea, (const c64*)c->psi, F(hetddft_ctx::F_DEN),
+                     F(hetddft_ctx::F_DENCG), F(hetddft_ctx::F_DELJ4), F(hetddft_ctx::F_DENALF),
+                     F(hetddft_ctx::F_DX), F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ), F(hetddft_ctx::F_IX),
+                     F(hetddft_ctx::F_IY), F(hetddft_ctx::F_IZ), F(hetddft_ctx::F_PSIX),
+                     F(hetddft_ctx::F_HPSIX), F(hetddft_ctx::F_UIMP), c->part);
+  LAUNCHCHK(c);
+  hipLaunchKernelGGL(k_finish_sums, dim3(8), dim3(kBlock), 0, c->st, (const double*)c->part, (int)nb, 8, c->res + 4);
+  LAUNCHCHK(c);
+  const int n3 = 3 * c->p.n_par;
+  if (n3 > 0) {
+    const unsigned pb = std::min(blocks_for(n3), 1024u);
+    hipLaunchKernelGGL(k_sq_sum, dim3(pb), dim3(kBlock), 0, c->st, n3, (const double*)c->vimp, c->part + 8 * nb);
+    LAUNCHCHK(c);
+    hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(kBlock), 0, c->st, (const double*)(c->part + 8 * nb), (int)pb, 1,
+                       c->res + 12);
+    LAUNCHCHK(c);
+  } else {
+    HIPCHK(c, hipMemsetAsync(c->res + 12, 0, sizeof(double), c->st));
+  }
+  HIPCHK(c, hipMemcpyAsync(c->hres + 4, c->res + 4, 9 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  const double* s = c->hres + 4;
+  const hetddft_params& p = c->p;
+  const double dxyz = c->dxyz;
+  const double ekin4 = p.h2o2m4 * s[0] * dxyz;
+  const double elj4 = 0.5 * s[1] * dxyz;
+  const double ecor4 = s[2] * dxyz;
+  const double ealphas = p.core_otc ? -p.h2o2m4 * 0.5 * p.alphas * s[3] * dxyz : 0.0;
+  const double esolid = p.C * s[4] * dxyz;
+  double etot4 = ekin4 + elj4 + ecor4 + esolid;
+  if (p.core_otc) etot4 = etot4 + ealphas;
+  const double ekinx = p.n_par > 0 ? 0.5 * c->m_imp * s[8] / p.n_par : 0.0;
+  const double ekinqx = s[5] * dxyz;
+  const double ehex = s[6] * dxyz;
+  const double eimpu = ekinx + ehex + ekinqx;
+  const double vals[12] = {etot4 + eimpu, etot4, ekin4, elj4, ealphas, esolid, ecor4, eimpu, ekinx, ekinqx, ehex,
+                           s[7] * dxyz};
+  std::memcpy(out, vals, sizeof vals);
+  return HETDDFT_OK;
+}
+
+}  // namespace
+
+// ============================ C-ABI ======================================
+extern "C" {
+
+int hetddft_default_params(hetddft_params* p) {
+  if (!p) return HETDDFT_E_ARG;
+  std::memset(p, 0, sizeof *p);
+  p->nx = p->ny = p->nz = 64;
+  p->xmax = p->ymax = p->zmax = 16.0;
+  p->n_par = 1000;
+  p->mimpur = 40.0;
+  p->ironx = 3;
+  p->core_otc = 1;
+  p->lambdah = 0.0;
+  p->tzmean = 0.0;
+  p->tzsurf = 1.0;
+  p->cp4 = -2.41186e4;
+  p->cpp4 = 1.85850e6;
+  p->alphas = 54.31;
+  p->l = 1.0;
+  p->den0s = 0.04;
+  p->h2o2m4 = 6.05969638298;
+  p->C = 3.1577504e4;
+  p->beta = 40.0;
+  p->den_m = 0.37;
+  p->h4 = 2.190323;
+  p->eps4 = 10.22;
+  p->sigma4 = 2.556;
+  p->precision = HETDDFT_FP64;
+  return HETDDFT_OK;
+}
+
+const char* hetddft_last_error(const hetddft_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+long long hetddft_device_bytes(const hetddft_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+int hetddft_destroy(hetddft_ctx* c) {
+  if (!c) return HETDDFT_OK;
+  if (c->st) hipStreamSynchronize(c->st);
+  for (void* p : c->allocs) hipFree(p);
+  if (c->hres) hipHostFree(c->hres);
+  if (c->st) hipStreamDestroy(c->st);
+  delete c;
+  return HETDDFT_OK;
+}
+
+int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
+  if (!p || !out) return HETDDFT_E_ARG;
+  *out = nullptr;
+  hetddft_ctx* c = new hetddft_ctx();
+  c->p = *p;
+  auto bad = [&](int code) {
+    g_create_error = c->err;  // readable through hetddft_last_error(NULL)
+    hetddft_destroy(c);
+    return code;
+  };
+  if (!tstgrid_ok(p->nx) || !tstgrid_ok(p->ny) || !tstgrid_ok(p->nz) || (p->nx % 2) != 0 || p->nx < 4 ||
+      p->ny < 2 || p->nz < 2) {
+    fail(c, HETDDFT_E_ARG, "grid %dx%dx%d not supported (tstgrid list, nx even)", p->nx, p->ny, p->nz);
+    return bad(HETDDFT_E_ARG);
+  }
+  if (p->n_par < 0 || p->ironx < 0 || p->mimpur <= 0 || !(p->xmax > 0) || !(p->ymax > 0) || !(p->zmax > 0) ||
+      (p->precision != HETDDFT_FP64 && p->precision != HETDDFT_MIXED) || p->tzsurf == 0.0) {
+    fail(c, HETDDFT_E_ARG, "invalid parameters");
+    return bad(HETDDFT_E_ARG);
+  }
+  c->mixed = p->precision == HETDDFT_MIXED;
+  c->tsz = c->mixed ? 4 : 8;
+  Grid& g = c->g;
+  g.nx = p->nx;
+  g.ny = p->ny;
+  g.nz = p->nz;
+  g.mx = p->nx / 2 + 1;
+  g.n = (long long)p->nx * p->ny * p->nz;
+  g.h = (long long)g.mx * p->ny * p->nz;
+  c->hx = 2.0 * std::fabs(p->xmax) / p->nx;
+  c->hy = 2.0 * std::fabs(p->ymax) / p->ny;
+  c->hz = 2.0 * std::fabs(p->zmax) / p->nz;
+  c->dxyz = c->hx * c->hy * c->hz;
+  c->renor = 1.0 / ((double)p->nx * p->ny * p->nz);
+  c->m_imp = p->mimpur * 0.0206149591055;
+  c->h2o2mx = p->h2o2m4 * (4.002602 / p->mimpur);
+  c->d1x = make_stencil(1, c->hx);
+  c->d1y = make_stencil(1, c->hy);
+  c->d1z = make_stencil(1, c->hz);
+  c->d2x = make_stencil(2, c->hx);
+  c->d2y = make_stencil(2, c->hy);
+  c->d2z = make_stencil(2, c->hz);
+  c->pl_xh = make_plan(p->nx / 2);
+  c->pl_y = make_plan(p->ny);
+  c->pl_z = make_plan(p->nz);
+  if (c->pl_xh.nstages < 0 || c->pl_y.nstages < 0 || c->pl_z.nstages < 0) {
+    fail(c, HETDDFT_E_ARG, "no radix plan for grid %dx%dx%d", p->nx, p->ny, p->nz);
+    return bad(HETDDFT_E_ARG);
+  }
+  if (hipStreamCreate(&c->st) != hipSuccess) {
+    fail(c, HETDDFT_E_HIP, "hipStreamCreate failed");
+    return bad(HETDDFT_E_HIP);
+  }
+  choose_tiles(c);
+  int rc = c->mixed ? fft_alloc_tables<float>(c) : fft_alloc_tables<double>(c);
+  if (rc) return bad(rc);
+  const size_t n = (size_t)g.n, h = (size_t)g.h;
+  auto A = [&](void** ptr, size_t bytes) { return dalloc(c, ptr, bytes); };
+  std::vector<std::pair<void**, size_t>> want;
+  want.push_back({(void**)&c->psi, n * 16});
+  want.push_back({(void**)&c->spare, n * 16});
+  for (int k = 0; k < 3; ++k) want.push_back({(void**)&c->old[k], n * 16});
+  for (int k = 0; k < 2; ++k) want.push_back({(void**)&c->hold[k], n * 16});
+  want.push_back({(void**)&c->pc, n * 16});
+  want.push_back({(void**)&c->q, n * 16});
+  for (int k = 0; k < hetddft_ctx::F_COUNT; ++k) want.push_back({&c->fld[k], n * c->tsz});
+  want.push_back({(void**)&c->counts, n * 4});
+  for (int k = 0; k < 12; ++k) want.push_back({&c->spec[k], h * 2 * c->tsz});
+  want.push_back({&c->kfvlj4, h * c->tsz});
+  want.push_back({&c->kwcgk, h * c->tsz});
+  want.push_back({&c->kkalfs, h * c->tsz});
+  want.push_back({&c->kvq, h * c->tsz});
+  const size_t np3 = (size_t)3 * std::max(p->n_par, 1);
+  for (double** pp : {&c->rimp, &c->vimp, &c->aimp, &c->qr, &c->qv, &c->pcr, &c->pcv})
+    want.push_back({(void**)pp, np3 * 8});
+  for (int k = 0; k < 3; ++k) want.push_back({(void**)&c->rold[k], np3 * 8});
+  for (int k = 0; k < 3; ++k) want.push_back({(void**)&c->vold[k], np3 * 8});
+  for (int k = 0; k < 2; ++k) want.push_back({(void**)&c->aold[k], np3 * 8});
+  c->part_len = 8LL * blocks_for(g.n) + 2LL * blocks_for((long long)np3) + 2048;
+  want.push_back({(void**)&c->part, (size_t)c->part_len * 8});
+  want.push_back({(void**)&c->res, 64 * 8});
+  want.push_back({(void**)&c->timec, (size_t)g.nz * 16});
+  for (auto& w : want) {
+    rc = A(w.first, w.second);
+    if (rc) return bad(rc);
+  }
+  if (hipHostMalloc((void**)&c->hres, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    fail(c, HETDDFT_E_NOMEM, "hipHostMalloc failed");
+    return bad(HETDDFT_E_NOMEM);
+  }
+  std::memset(c->hres, 0, 64 * sizeof(double));
+  c->denx = c->fld[hetddft_ctx::F_DENXA];
+  // timec = cmplx(Lambdah*(1+tanh((z-tzmean)/tzsurf)), 1) with the real part
+  // rounded to REAL(4) by the KIND-less cmplx (DFT4He3d.f90:773-782)
+  std::vector<c64> tc(g.nz);
+  for (int iz = 0; iz < g.nz; ++iz) {
+    const double z = -p->zmax + c->hz * iz;
+    const double re = p->lambdah * (1.0 + std::tanh((z - p->tzmean) / p->tzsurf));
+    tc[iz].x = (double)(float)re;
+    tc[iz].y = 1.0;
+  }
+  if (hipMemcpy(c->timec, tc.data(), g.nz * sizeof(c64), hipMemcpyHostToDevice) != hipSuccess) {
+    fail(c, HETDDFT_E_HIP, "timec upload failed");
+    return bad(HETDDFT_E_HIP);
+  }
+  *out = c;
+  return HETDDFT_OK;
+}
+
+int hetddft_make_kernels(const hetddft_params* p, const hetddft_morse* mo, double* fvlj4, double* wcgk,
+                         double* kalfs, double* vq) {
+  if (!p || p->nx <= 0 || p->ny <= 0 || p->nz <= 0) return HETDDFT_E_ARG;
+  using namespace kinit;
+  const int nx = p->nx, ny = p->ny, nz = p->nz, mx = nx / 2 + 1;
+  const double hx = 2.0 * std::fabs(p->xmax) / nx, hy = 2.0 * std::fabs(p->ymax) / ny,
+               hz = 2.0 * std::fabs(p->zmax) / nz;
+  const double pi = 4.0 * std::atan(1.0), twopi = 2.0 * pi;
+  std::vector<double> px(mx), py(ny), pz(nz);
+  for (int i = 0; i < mx; ++i) px[i] = pfreq(i, nx, hx);
+  for (int i = 0; i < ny; ++i) py[i] = pfreq(i, ny, hy);
+  for (int i = 0; i < nz; ++i) pz[i] = pfreq(i, nz, hz);
+  const size_t H = (size_t)mx * ny * nz;
+  std::vector<double> pm(H);
+  for (int iz = 0; iz < nz; ++iz)
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < mx; ++ix)
+        pm[ix + (size_t)mx * (iy + (size_t)ny * iz)] = std::sqrt((px[ix] * px[ix] + py[iy] * py[iy]) + pz[iz] * pz[iz]);
+  const double h4 = p->h4;
+  if (wcgk)
+    for (size_t i = 0; i < H; ++i) {
+      const double pp = pm[i];
+      if (pp > 0.0) {
+        const double arg = twopi * pp * h4;
+        const double aux = std::sin(arg) - arg * std::cos(arg);
+        wcgk[i] = std::fabs(aux) < 1e-10 ? 1.0 : 3.0 * aux / (arg * arg * arg);
+      } else {
+        wcgk[i] = 1.0;
+      }
+    }
+  if (kalfs)
+    for (size_t i = 0; i < H; ++i) {
+      const double t = pi * p->l * pm[i];
+      kalfs[i] = std::exp(-(t * t));
+    }
+  if (fvlj4) {
+    const double eps = p->eps4, sigma = p->sigma4;
+    const double sigma6 = std::pow(sigma, 6), aux1 = 8.0 * eps;
+    const double pi4 = 4.0 * pi, eps4 = 4.0 * eps, s12 = sigma6 * sigma6;
+    const double b0 = pi4 * eps4 * s12 / (9.0 * std::pow(h4, 9)) + (-pi4 * eps4 * sigma6 / (3.0 * std::pow(h4, 3)));
+    for (size_t i = 0; i < H; ++i) {
+      const double pp = pm[i];
+      if (std::fabs(pp) < 1e-10) {
+        fvlj4[i] = b0;
+      } else {
+        const double a2 = twopi * pp;
+        fvlj4[i] = (aux1 * sigma6 * (sigma6 * sinx11(a2, h4) - sinx5(a2, h4))) / pp;
+      }
+    }
+  }
+  if (vq) {
+    MorsePot pot{};
+    pot.D = mo ? mo->D : 20.0;
+    pot.re = mo ? mo->re : 5.0;
+    pot.alpha = mo ? mo->alpha : 0.8;
+    pot.rc = mo ? mo->rc : 2.0;
+    pot.umax = 0.0;
+    {
+      const double e = std::exp(-pot.alpha * (pot.rc - pot.re));
+      pot.umax = (mo && mo->umax != 0.0) ? mo->umax : pot.D * (e * e - 2.0 * e);
+    }
+    const int nq = (mo && mo->nq > 1) ? mo->nq : 1500;
+    const double pmaxx = 1.0 / (2.0 * hx), pmaxy = 1.0 / (2.0 * hy), pmaxz = 1.0 / (2.0 * hz);
+    const double qmax = std::sqrt(pmaxx * pmaxx + pmaxy * pmaxy + pmaxz * pmaxz);
+    const double hq = qmax / (nq - 1);
+    FtvSpline ft(pot, qmax, hq);
+    for (size_t i = 0; i < H; ++i) vq[i] = (double)(float)ft(pm[i]);
+  }
+  return HETDDFT_OK;
+}
+
+int hetddft_set_kernels(hetddft_ctx* c, const double* fvlj4, const double* wcgk, const double* kalfs,
+                        const double* vq) {
+  if (!c || !fvlj4 || !wcgk || !kalfs || !vq) return fail(c, HETDDFT_E_ARG, "null kernel array");
+  const size_t h = (size_t)c->g.h;
+  RET(upload_real(c, c->kfvlj4, fvlj4, h));
+  RET(upload_real(c, c->kwcgk, wcgk, h));
+  RET(upload_real(c, c->kkalfs, kalfs, h));
+  RET(upload_real(c, c->kvq, vq, h));
+  c->kernels_set = true;
+  return HETDDFT_OK;
+}
+
+int hetddft_set_state(hetddft_ctx* c, const double* psi, const double* rimp, const double* vimp) {
+  if (!c) return HETDDFT_E_ARG;
+  if (psi) HIPCHK(c, hipMemcpy(c->psi, psi, (size_t)c->g.n * 16, hipMemcpyHostToDevice));
+  const size_t np3 = (size_t)3 * c->p.n_par;
+  if (rimp && np3) HIPCHK(c, hipMemcpy(c->rimp, rimp, np3 * 8, hipMemcpyHostToDevice));
+  if (vimp && np3) HIPCHK(c, hipMemcpy(c->vimp, vimp, np3 * 8, hipMemcpyHostToDevice));
+  c->started = false;
+  return HETDDFT_OK;
+}
+
+int hetddft_start(hetddft_ctx* c) {
+  if (!c) return HETDDFT_E_ARG;
+  if (!c->kernels_set) return fail(c, HETDDFT_E_STATE, "hetddft_set_kernels not called");
+  const Grid& g = c->g;
+  if (c->mixed)
+    hipLaunchKernelGGL((k_density<float>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, (const c64*)c->psi,
+                       (float*)c->fld[hetddft_ctx::F_DEN], g.n);
+  else
+    hipLaunchKernelGGL((k_density<double>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, (const c64*)c->psi,
+                       (double*)c->fld[hetddft_ctx::F_DEN], g.n);
+  LAUNCHCHK(c);
+  RET(run_veval(c));
+  const size_t np3 = (size_t)3 * std::max(c->p.n_par, 1);
+  for (double* pp : {c->qr, c->qv, c->rold[0], c->rold[1], c->rold[2], c->vold[0], c->vold[1], c->vold[2],
+                     c->aold[0], c->aold[1]})
+    HIPCHK(c, hipMemsetAsync(pp, 0, np3 * 8, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->started = true;
+  c->iter_in_run = 0;
+  return HETDDFT_OK;
+}
+
+int hetddft_potential(hetddft_ctx* c) {
+  if (!c) return HETDDFT_E_ARG;
+  if (!c->kernels_set) return fail(c, HETDDFT_E_STATE, "hetddft_set_kernels not called");
+  RET(run_veval(c));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return HETDDFT_OK;
+}
+
+int hetddft_step_rk(hetddft_ctx* c, double dt) {
+  if (!c) return HETDDFT_E_ARG;
+  if (!c->started) return fail(c, HETDDFT_E_STATE, "hetddft_start not called");
+  RET(c->mixed ? step_rk_t<float>(c, dt) : step_rk_t<double>(c, dt));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return HETDDFT_OK;
+}
+
+int hetddft_step_pc(hetddft_ctx* c, double dt, double err[3]) {
+  if (!c) return HETDDFT_E_ARG;
+  if (!c->started) return fail(c, HETDDFT_E_STATE, "hetddft_start not called");
+  RET(c->mixed ? step_pc_t<float>(c, dt) : step_pc_t<double>(c, dt));
+  if (err) return fetch_errors(c, err);
+  return HETDDFT_OK;
+}
+
+int hetddft_last_errors(hetddft_ctx* c, double err[3]) {
+  if (!c || !err) return HETDDFT_E_ARG;
+  return fetch_errors(c, err);
+}
+
+int hetddft_iterate(hetddft_ctx* c, double dt, double err[3]) {
+  if (!c) return HETDDFT_E_ARG;
+  if (!c->started) return fail(c, HETDDFT_E_STATE, "hetddft_start not called");
+  if (c->iter_in_run < 3) {
+    RET(c->mixed ? step_rk_t<float>(c, dt) : step_rk_t<double>(c, dt));
+    RET(run_veval(c));
+    if (err) {
+      err[0] = err[1] = err[2] = 0.0;
+      HIPCHK(c, hipStreamSynchronize(c->st));
+    }
+  } else {
+    RET(c->mixed ? step_pc_t<float>(c, dt) : step_pc_t<double>(c, dt));
+    RET(run_veval(c));
+    if (err) RET(fetch_errors(c, err));
+  }
+  c->iter_in_run++;
+  return HETDDFT_OK;
+}
+
+int hetddft_particle_means(hetddft_ctx* c, double out[9]) {
+  if (!c || !out) return HETDDFT_E_ARG;
+  const int np = c->p.n_par;
+  if (np == 0) {
+    std::memset(out, 0, 9 * sizeof(double));
+    return HETDDFT_OK;
+  }
+  const unsigned nb = std::min(blocks_for(np), 64u);
+  hipLaunchKernelGGL(k_part_sums, dim3(nb, 9), dim3(kBlock), 0, c->st, np, (const double*)c->rimp,
+                     (const double*)c->vimp, (const double*)c->aimp, c->part);
+  LAUNCHCHK(c);
+  hipLaunchKernelGGL(k_finish_sums, dim3(9), dim3(kBlock), 0, c->st, (const double*)c->part, (int)nb, 9,
+                     c->res + 16);
+  LAUNCHCHK(c);
+  HIPCHK(c, hipMemcpyAsync(c->hres + 16, c->res + 16, 9 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  const double pstok = 1.0 / 7.63822291;
+  for (int k = 0; k < 9; ++k) {
+    double v = c->hres[16 + k] / np;
+    if (k >= 3 && k < 6) v = v * pstok;
+    if (k >= 6) v = v * pstok * pstok;
+    out[k] = v;
+  }
+  return HETDDFT_OK;
+}
+
+int hetddft_energy(hetddft_ctx* c, double out[12]) {
+  if (!c || !out) return HETDDFT_E_ARG;
+  if (!c->started) return fail(c, HETDDFT_E_STATE, "hetddft_start not called");
+  return c->mixed ? energy_t<float>(c, out) : energy_t<double>(c, out);
+}
+
+int hetddft_get_state(hetddft_ctx* c, double* psi, double* rimp, double* vimp, double* aimp) {
+  if (!c) return HETDDFT_E_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  const size_t np3 = (size_t)3 * c->p.n_par;
+  if (psi) HIPCHK(c, hipMemcpy(psi, c->psi, (size_t)c->g.n * 16, hipMemcpyDeviceToHost));
+  if (rimp && np3) HIPCHK(c, hipMemcpy(rimp, c->rimp, np3 * 8, hipMemcpyDeviceToHost));
+  if (vimp && np3) HIPCHK(c, hipMemcpy(vimp, c->vimp, np3 * 8, hipMemcpyDeviceToHost));
+  if (aimp && np3) HIPCHK(c, hipMemcpy(aimp, c->aimp, np3 * 8, hipMemcpyDeviceToHost));
+  return HETDDFT_OK;
+}
+
+int hetddft_get_field(hetddft_ctx* c, int which, double* out) {
+  if (!c || !out) return HETDDFT_E_ARG;
+  static const int map[20] = {hetddft_ctx::F_DEN,    hetddft_ctx::F_POT4,  hetddft_ctx::F_UIMP,
+                              hetddft_ctx::F_DENCG,  hetddft_ctx::F_DELJ4, hetddft_ctx::F_DENALF,
+                              hetddft_ctx::F_DX,     hetddft_ctx::F_DY,    hetddft_ctx::F_DZ,
+                              hetddft_ctx::F_IX,     hetddft_ctx::F_IY,    hetddft_ctx::F_IZ,
+                              -1,                    hetddft_ctx::F_PSIX,  hetddft_ctx::F_HPSIX,
+                              hetddft_ctx::F_FCX,    hetddft_ctx::F_FCY,   hetddft_ctx::F_FCZ,
+                              hetddft_ctx::F_STO1,   hetddft_ctx::F_UACONV};
+  if (which < 0 || which >= 20) return fail(c, HETDDFT_E_ARG, "unknown field id %d", which);
+  const void* src = which == HETDDFT_F_DENX ? c->denx : c->fld[map[which]];
+  return c->mixed ? field_to_host<float>(c, src, out) : field_to_host<double>(c, src, out);
+}
+
+int hetddft_convolve(hetddft_ctx* c, int nf, const double* const* in, const double* kern, double* const* out) {
+  if (!c || nf < 1 || nf > 5 || !in || !kern || !out) return fail(c, HETDDFT_E_ARG, "convolve: bad arguments");
+  const size_t n = (size_t)c->g.n;
+  // stage inputs in the force-convolution fields, kernel in the alpha_s slot
+  const int stage[5] = {hetddft_ctx::F_FCX, hetddft_ctx::F_FCY, hetddft_ctx::F_FCZ, hetddft_ctx::F_STO1,
+                        hetddft_ctx::F_UACONV};
+  const int dst[5] = {hetddft_ctx::F_IX, hetddft_ctx::F_IY, hetddft_ctx::F_IZ, hetddft_ctx::F_DENCG,
+                      hetddft_ctx::F_DELJ4};
+  for (int i = 0; i < nf; ++i) RET(upload_real(c, c->fld[stage[i]], in[i], n));
+  RET(upload_real(c, c->kkalfs, kern, (size_t)c->g.h));
+  c->kernels_set = false;
+  auto run = [&](auto tag) -> int {
+    using T = decltype(tag);
+    XfArgs<T> xa{};
+    std::vector<ConvJob<T>> jobs;
+    for (int i = 0; i < nf; ++i) {
+      xa.op[i] = XF_COPY;
+      xa.in[i][0] = (const T*)c->fld[stage[i]];
+      jobs.push_back({i, c->kkalfs, 1.0, c->fld[dst[i]]});
+    }
+    RET(conv_batch<T>(c, xa, nf, jobs));
+    for (int i = 0; i < nf; ++i) RET(field_to_host<T>(c, c->fld[dst[i]], out[i]));
+    return HETDDFT_OK;
+  };
+  return c->mixed ? run(float{}) : run(double{});
+}
+
+int hetddft_sync(hetddft_ctx* c) {
+  if (!c) return HETDDFT_E_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return HETDDFT_OK;
+}
+
+}  // extern "C"

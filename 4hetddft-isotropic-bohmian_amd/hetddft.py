@@ -1,0 +1,274 @@
+"""Host-side mirror of the reference's per-step interface over the HIP C-ABI.
+
+The reference Fortran driver calls ``steprk(deltat)``, ``steppc(deltat, errHe,
+errimp, errvimp)``, ``potenimp(); poten(); forceimp()`` and ``energy()``
+(DFT4He3d.f90:632-634,653,849-863,882) on module-global arrays.  ``Instance``
+exposes the same operations by the same names on one GPU-resident instance;
+every call goes through ``libhetddft.so`` (include/hetddft.h).  There is no
+CPU fallback: if the library is missing, loading fails with an error.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_NAME = "libhetddft.so"
+
+FP64 = 0
+MIXED = 1
+
+FIELDS = {
+    "den": 0, "pot4": 1, "uimp": 2, "dencg": 3, "delj4": 4, "denalf": 5,
+    "dxden": 6, "dyden": 7, "dzden": 8, "intxalf": 9, "intyalf": 10, "intzalf": 11,
+    "denx": 12, "psix": 13, "hpsix": 14, "fxconv": 15, "fyconv": 16, "fzconv": 17,
+    "correl": 18, "ualphas_conv": 19,
+}
+
+ENERGY_KEYS = ("etot", "etot4", "ekin4", "elj4", "ealphas", "esolid", "ecor4", "eimpu",
+               "ekinx", "ekinqx", "ehex", "norm")
+
+KTOPS = 7.63822291          # DFT4He3d.f90:110
+PSTOK = 1.0 / 7.63822291
+
+
+class HetddftError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    """Mirror of ``hetddft_params`` (include/hetddft.h)."""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("xmax", ctypes.c_double), ("ymax", ctypes.c_double), ("zmax", ctypes.c_double),
+        ("n_par", ctypes.c_int), ("mimpur", ctypes.c_double), ("ironx", ctypes.c_int),
+        ("core_otc", ctypes.c_int),
+        ("lambdah", ctypes.c_double), ("tzmean", ctypes.c_double), ("tzsurf", ctypes.c_double),
+        ("cp4", ctypes.c_double), ("cpp4", ctypes.c_double), ("alphas", ctypes.c_double),
+        ("l", ctypes.c_double), ("den0s", ctypes.c_double), ("h2o2m4", ctypes.c_double),
+        ("C", ctypes.c_double), ("beta", ctypes.c_double), ("den_m", ctypes.c_double),
+        ("h4", ctypes.c_double), ("eps4", ctypes.c_double), ("sigma4", ctypes.c_double),
+        ("precision", ctypes.c_int),
+    ]
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    @property
+    def hshape(self):
+        return (self.nz, self.ny, self.nx // 2 + 1)
+
+
+class Morse(ctypes.Structure):
+    """Mirror of ``hetddft_morse``."""
+    _fields_ = [("D", ctypes.c_double), ("re", ctypes.c_double), ("alpha", ctypes.c_double),
+                ("rc", ctypes.c_double), ("umax", ctypes.c_double), ("nq", ctypes.c_int)]
+
+
+_lib = None
+
+_DP = ctypes.POINTER(ctypes.c_double)
+_VP = ctypes.c_void_p
+
+
+def lib_path():
+    return _HERE / LIB_NAME
+
+
+def load_library():
+    """Load libhetddft.so from the package directory (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = lib_path()
+    if not path.exists():
+        raise HetddftError(f"{path} not found: build it first (__graft_entry__.build())")
+    lib = ctypes.CDLL(str(path))
+    P = ctypes.POINTER(Params)
+    C = ctypes.c_void_p
+    sig = {
+        "hetddft_default_params": ([P], ctypes.c_int),
+        "hetddft_create": ([P, ctypes.POINTER(C)], ctypes.c_int),
+        "hetddft_destroy": ([C], ctypes.c_int),
+        "hetddft_last_error": ([C], ctypes.c_char_p),
+        "hetddft_make_kernels": ([P, ctypes.POINTER(Morse), _VP, _VP, _VP, _VP], ctypes.c_int),
+        "hetddft_set_kernels": ([C, _VP, _VP, _VP, _VP], ctypes.c_int),
+        "hetddft_set_state": ([C, _VP, _VP, _VP], ctypes.c_int),
+        "hetddft_start": ([C], ctypes.c_int),
+        "hetddft_potential": ([C], ctypes.c_int),
+        "hetddft_step_rk": ([C, ctypes.c_double], ctypes.c_int),
+        "hetddft_step_pc": ([C, ctypes.c_double, _VP], ctypes.c_int),
+        "hetddft_last_errors": ([C, _VP], ctypes.c_int),
+        "hetddft_iterate": ([C, ctypes.c_double, _VP], ctypes.c_int),
+        "hetddft_particle_means": ([C, _VP], ctypes.c_int),
+        "hetddft_energy": ([C, _VP], ctypes.c_int),
+        "hetddft_get_state": ([C, _VP, _VP, _VP, _VP], ctypes.c_int),
+        "hetddft_get_field": ([C, ctypes.c_int, _VP], ctypes.c_int),
+        "hetddft_convolve": ([C, ctypes.c_int, _VP, _VP, _VP], ctypes.c_int),
+        "hetddft_sync": ([C], ctypes.c_int),
+        "hetddft_device_bytes": ([C], ctypes.c_longlong),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def default_params(**kw) -> Params:
+    lib = load_library()
+    p = Params()
+    lib.hetddft_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def make_kernels(p: Params, morse: Morse | None = None) -> dict:
+    """fvlj4, wcgk, kalfs, vq on the half grid (hetddft_make_kernels)."""
+    lib = load_library()
+    out = {k: np.zeros(p.hshape) for k in ("fvlj4", "wcgk", "kalfs", "vq")}
+    rc = lib.hetddft_make_kernels(ctypes.byref(p), ctypes.byref(morse) if morse else None,
+                                  _ptr(out["fvlj4"]), _ptr(out["wcgk"]), _ptr(out["kalfs"]),
+                                  _ptr(out["vq"]))
+    if rc != 0:
+        raise HetddftError(f"hetddft_make_kernels failed ({rc})")
+    return out
+
+
+class Instance:
+    """One droplet + impurity instance resident on the GPU."""
+
+    def __init__(self, p: Params):
+        self.lib = load_library()
+        self.p = p
+        self._c = ctypes.c_void_p()
+        rc = self.lib.hetddft_create(ctypes.byref(p), ctypes.byref(self._c))
+        if rc != 0:
+            raise HetddftError(f"hetddft_create failed ({rc}): "
+                               f"{self.lib.hetddft_last_error(None).decode()}")
+
+    # -- plumbing --
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.hetddft_last_error(self._c).decode()
+            raise HetddftError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if self._c:
+            self.lib.hetddft_destroy(self._c)
+            self._c = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def device_bytes(self):
+        return int(self.lib.hetddft_device_bytes(self._c))
+
+    # -- state --
+    def set_kernels(self, k: dict):
+        arrs = [np.ascontiguousarray(k[n], dtype=np.float64) for n in ("fvlj4", "wcgk", "kalfs", "vq")]
+        for a in arrs:
+            assert a.shape == self.p.hshape, (a.shape, self.p.hshape)
+        self._check(self.lib.hetddft_set_kernels(self._c, *[_ptr(a) for a in arrs]), "set_kernels")
+
+    def set_state(self, psi, rimp, vimp):
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        assert psi.shape == self.p.shape
+        rimp = np.ascontiguousarray(rimp, dtype=np.float64)
+        vimp = np.ascontiguousarray(vimp, dtype=np.float64)
+        assert rimp.shape == (3, self.p.n_par) and vimp.shape == (3, self.p.n_par)
+        self._check(self.lib.hetddft_set_state(self._c, _ptr(psi), _ptr(rimp), _ptr(vimp)), "set_state")
+
+    def start(self):
+        self._check(self.lib.hetddft_start(self._c), "start")
+
+    # -- the reference's per-step operations --
+    def potential(self):
+        """potenimp(); poten(); forceimp()"""
+        self._check(self.lib.hetddft_potential(self._c), "potential")
+
+    def steprk(self, deltat):
+        self._check(self.lib.hetddft_step_rk(self._c, float(deltat)), "steprk")
+
+    def steppc(self, deltat, want_errors=True):
+        err = np.zeros(3) if want_errors else None
+        self._check(self.lib.hetddft_step_pc(self._c, float(deltat), _ptr(err)), "steppc")
+        return tuple(err) if want_errors else None
+
+    def iterate(self, deltat, want_errors=True):
+        err = np.zeros(3) if want_errors else None
+        self._check(self.lib.hetddft_iterate(self._c, float(deltat), _ptr(err)), "iterate")
+        return tuple(err) if want_errors else None
+
+    def last_errors(self):
+        err = np.zeros(3)
+        self._check(self.lib.hetddft_last_errors(self._c, _ptr(err)), "last_errors")
+        return tuple(err)
+
+    def particle_means(self):
+        out = np.zeros(9)
+        self._check(self.lib.hetddft_particle_means(self._c, _ptr(out)), "particle_means")
+        return out
+
+    def energy(self):
+        out = np.zeros(12)
+        self._check(self.lib.hetddft_energy(self._c, _ptr(out)), "energy")
+        return dict(zip(ENERGY_KEYS, out))
+
+    def get_state(self):
+        psi = np.zeros(self.p.shape, np.complex128)
+        n = self.p.n_par
+        r, v, a = np.zeros((3, n)), np.zeros((3, n)), np.zeros((3, n))
+        self._check(self.lib.hetddft_get_state(self._c, _ptr(psi), _ptr(r), _ptr(v), _ptr(a)),
+                    "get_state")
+        return psi, r, v, a
+
+    def get_field(self, name):
+        out = np.zeros(self.p.shape)
+        self._check(self.lib.hetddft_get_field(self._c, FIELDS[name], _ptr(out)), f"get_field({name})")
+        return out
+
+    def convolve(self, fields, kern):
+        """c2r(r2c(f) * kern) * renor for up to five real fields."""
+        ins = [np.ascontiguousarray(f, dtype=np.float64) for f in fields]
+        outs = [np.zeros(self.p.shape) for _ in ins]
+        kern = np.ascontiguousarray(kern, dtype=np.float64)
+        inp = (ctypes.c_void_p * len(ins))(*[a.ctypes.data for a in ins])
+        outp = (ctypes.c_void_p * len(outs))(*[a.ctypes.data for a in outs])
+        self._check(self.lib.hetddft_convolve(self._c, len(ins), ctypes.cast(inp, ctypes.c_void_p),
+                                              _ptr(kern), ctypes.cast(outp, ctypes.c_void_p)),
+                    "convolve")
+        return outs
+
+    def sync(self):
+        self._check(self.lib.hetddft_sync(self._c), "sync")
+
+
+def header_functions(header=None):
+    """Names of the functions declared in include/hetddft.h."""
+    import re
+    header = Path(header) if header else _HERE.parent / "include" / "hetddft.h"
+    text = header.read_text()
+    return sorted(set(re.findall(r"\b(hetddft_[a-z0-9_]+)\s*\(", text)))

@@ -1,0 +1,165 @@
+/*
+ * hetddft.h -- C-ABI of the MI355X (gfx950) time-step library for the
+ * Orsay-Trento He-4 TDDFT solver with Bohmian impurity test particles.
+ *
+ * Drop-in boundary for the per-step subroutines of the reference Fortran code
+ * (bcntls2016/4hetddft-isotropic-bohmian).  The reference talks through module
+ * globals; here the same state lives on the GPU inside an opaque context and
+ * crosses the boundary only in hetddft_set_* / hetddft_get_* calls.
+ *
+ * Array conventions (identical to the Fortran arrays, so a Fortran host can
+ * pass its arrays through ISO_C_BINDING unchanged):
+ *   real field      f(nx,ny,nz)        double[nz][ny][nx]         (x fastest)
+ *   complex field   psi(nx,ny,nz)      double[nz][ny][nx][2]      (re,im)
+ *   half spectrum   k(nx/2+1,ny,nz)    double[nz][ny][nx/2+1]     (real kernels)
+ *   particles       rimp(N_par,3)      double[3][N_par]           (x comps first)
+ *
+ * Every function returns 0 on success or a negative HETDDFT_E_* code; the
+ * message of the last failure is available from hetddft_last_error().
+ * Calls on one context are synchronous with respect to their host outputs
+ * and must not overlap.  One context per host thread.
+ */
+#ifndef HETDDFT_H
+#define HETDDFT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HETDDFT_OK 0
+#define HETDDFT_E_ARG -1        /* bad argument or unsupported grid size   */
+#define HETDDFT_E_HIP -2        /* HIP runtime failure                     */
+#define HETDDFT_E_NOMEM -3      /* device allocation failed                */
+#define HETDDFT_E_STATE -4      /* call out of order (e.g. no kernels set) */
+#define HETDDFT_E_NONFINITE -5  /* non-finite error estimate in a PC step  */
+
+#define HETDDFT_FP64 0          /* every field FP64                         */
+#define HETDDFT_MIXED 1         /* potential fields and FFTs FP32; psi,
+                                   reductions and particles FP64            */
+
+typedef struct hetddft_ctx hetddft_ctx;
+
+/* Parameters of one droplet + impurity instance.  Field names follow the
+ * reference namelist /input/ (DFT4He3d.f90:147-165) and modules.f90. */
+typedef struct hetddft_params {
+  int nx, ny, nz;            /* grid; each in the tstgrid list (tstgrid.f90:8-12), nx even */
+  double xmax, ymax, zmax;   /* box is [-xmax, xmax) etc., hx = 2 xmax/nx */
+  int n_par;                 /* number of Bohmian test particles */
+  double mimpur;             /* impurity mass in u */
+  int ironx;                 /* smoothing passes of the particle histogram */
+  int core_otc;              /* 1: core4='OTC' (alpha_s term on), 0: 'OT ' */
+  double lambdah, tzmean, tzsurf;  /* time constant timec (DFT4He3d.f90:773-782) */
+  /* Orsay-Trento functional (modules.f90:122-141, 201-213) */
+  double cp4, cpp4, alphas, l, den0s, h2o2m4, C, beta, den_m, h4, eps4, sigma4;
+  int precision;             /* HETDDFT_FP64 or HETDDFT_MIXED */
+} hetddft_params;
+
+/* Cored Morse He-impurity pair potential V(r)=D[e^{-2a(r-re)}-2e^{-a(r-re)}],
+ * V = umax for r < rc (Select_Pot core convention, V_impur.f90:179-189). */
+typedef struct hetddft_morse {
+  double D, re, alpha, rc, umax;
+  int nq;                    /* momenta of the spline transform (Nq) */
+} hetddft_morse;
+
+/* Fills the Orsay-Trento defaults and a 64^3 box of +-16 A. */
+int hetddft_default_params(hetddft_params* p);
+
+/* Allocates device state.  Replaces dimen() + fftini() (dimen.f90:8-146,
+ * fft.f90:5-76). */
+int hetddft_create(const hetddft_params* p, hetddft_ctx** out);
+
+int hetddft_destroy(hetddft_ctx* ctx);
+
+/* Message of the last failure on ctx; with ctx == NULL, the message of the
+ * last failed hetddft_create on this thread. */
+const char* hetddft_last_error(const hetddft_ctx* ctx);
+
+/* Host-side generator of the k-space kernels on the half grid:
+ * fvlj4 (fforma.f90:11-61), wcgk (initcg.f90:5-51), kalfs
+ * (DFT4He3d.f90:566-570) and the FP32-rounded Vq of the cored Morse potential
+ * (FT_V_spline.f90:14-276 via DFT4He3d.f90:527-536).  Any pointer may be NULL. */
+int hetddft_make_kernels(const hetddft_params* p, const hetddft_morse* morse, double* fvlj4,
+                         double* wcgk, double* kalfs, double* vq);
+
+/* Uploads the k-space kernels (the arrays fforma/initcg/FT_V_spline leave in
+ * fvlj4, wcgk, kalfs and real(vq)). */
+int hetddft_set_kernels(hetddft_ctx* ctx, const double* fvlj4, const double* wcgk,
+                        const double* kalfs, const double* vq);
+
+/* Uploads psi (complex, interleaved), rimp and vimp (Fortran (N_par,3)). */
+int hetddft_set_state(hetddft_ctx* ctx, const double* psi, const double* rimp, const double* vimp);
+
+/* den = conjg(psi)*psi, first potential evaluation, zeroed particle history
+ * (DFT4He3d.f90:630-634, 841-845). */
+int hetddft_start(hetddft_ctx* ctx);
+
+/* potenimp(); poten(); forceimp()  (potenimp.f90:1-73, poten.f90:5-129). */
+int hetddft_potential(hetddft_ctx* ctx);
+
+/* steprk(deltat)  (steprk.f90:1-128); deltat in K^-1. */
+int hetddft_step_rk(hetddft_ctx* ctx, double deltat);
+
+/* steppc(deltat, errHe, errimp, errvimp)  (steppc.f90:1-155).  err may be
+ * NULL: the step then runs without a host synchronisation and the error
+ * estimates stay on the device (hetddft_last_errors fetches them). */
+int hetddft_step_pc(hetddft_ctx* ctx, double deltat, double err[3]);
+
+int hetddft_last_errors(hetddft_ctx* ctx, double err[3]);
+
+/* One iteration of the reference time loop (DFT4He3d.f90:849-863):
+ * steprk for the first three calls after hetddft_start, steppc after, then
+ * potenimp/poten/forceimp.  err (may be NULL) gets the PC errors, or zeros
+ * after an RK step. */
+int hetddft_iterate(hetddft_ctx* ctx, double deltat, double err[3]);
+
+/* Means of rimp (A), vimp (A/ps) and aimp (A/ps^2) (DFT4He3d.f90:864-878). */
+int hetddft_particle_means(hetddft_ctx* ctx, double out[9]);
+
+/* energy() (energy.f90:12-129) plus the norm sum(den)*dxyz:
+ * out = {etot, etot4, ekin4, elj4, ealphas, esolid, ecor4, eimpu, ekinx,
+ *        ekinqx, eHeX, norm}. */
+int hetddft_energy(hetddft_ctx* ctx, double out[12]);
+
+/* Downloads psi, rimp, vimp, aimp (any pointer may be NULL). */
+int hetddft_get_state(hetddft_ctx* ctx, double* psi, double* rimp, double* vimp, double* aimp);
+
+/* Downloads one real field (as double) by id. */
+#define HETDDFT_F_DEN 0
+#define HETDDFT_F_POT4 1
+#define HETDDFT_F_UIMP 2
+#define HETDDFT_F_DENCG 3
+#define HETDDFT_F_DELJ4 4
+#define HETDDFT_F_DENALF 5
+#define HETDDFT_F_DXDEN 6
+#define HETDDFT_F_DYDEN 7
+#define HETDDFT_F_DZDEN 8
+#define HETDDFT_F_INTXALF 9
+#define HETDDFT_F_INTYALF 10
+#define HETDDFT_F_INTZALF 11
+#define HETDDFT_F_DENX 12
+#define HETDDFT_F_PSIX 13
+#define HETDDFT_F_HPSIX 14
+#define HETDDFT_F_FXCONV 15
+#define HETDDFT_F_FYCONV 16
+#define HETDDFT_F_FZCONV 17
+#define HETDDFT_F_CORREL 18
+#define HETDDFT_F_UALPHAS_CONV 19
+int hetddft_get_field(hetddft_ctx* ctx, int which, double* out);
+
+/* Standalone batched 3D r2c + c2r convolution of nf real fields with one
+ * real k-space kernel: out_i = c2r(r2c(in_i) * kern) * renor.  Exposes the
+ * transform layer (fft.f90) for testing and for hosts that only offload the
+ * FFTs.  Arrays are host pointers in the layouts above. */
+int hetddft_convolve(hetddft_ctx* ctx, int nf, const double* const* in, const double* kern,
+                     double* const* out);
+
+/* Waits for all work queued on the context's stream. */
+int hetddft_sync(hetddft_ctx* ctx);
+
+/* Device bytes allocated by the context. */
+long long hetddft_device_bytes(const hetddft_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,108 @@
+"""FP64 parity of the HIP time step with the CPU oracle, per step (steprk/steppc + Veval)."""
+import numpy as np
+import pytest
+
+from conftest import oracle_params, rel_max
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10   # FP64 parity bar: relative to the field's max magnitude
+
+
+def _setup(hip, name, gkw=None, okw=None):
+    import inputs
+    from oracle import hetddft_oracle as orc
+    wl = inputs.WORKLOADS[name]
+    p = inputs.params_for(wl, **(gkw or {}))
+    k = hip.make_kernels(p)
+    psi, r, v = inputs.initial_state(wl)
+    o = orc.Oracle(oracle_params(wl, **(okw or {})), k)
+    o.set_state(psi, r, v)
+    g = hip.Instance(p)
+    g.set_kernels(k)
+    g.set_state(psi, r, v)
+    return wl, o, g
+
+
+FIELD_MAP = {"den": "den", "uimp": "uimp", "dencg": "dencg", "delj4": "delj4", "denalf": "denalf",
+             "dxden": "dxden", "dyden": "dyden", "dzden": "dzden", "intxalf": "intxalf",
+             "intyalf": "intyalf", "intzalf": "intzalf", "denx": "denx", "psix": "psix",
+             "hpsix": "hpsix", "pot4": "pot4"}
+
+
+@pytest.mark.parametrize("name", ["t24", "t32"])
+def test_potential_fields(hip, name):
+    wl, o, g = _setup(hip, name)
+    print(f"potential fields {name}", flush=True)
+    o.start()
+    g.start()
+    for gname, oname in FIELD_MAP.items():
+        a = g.get_field(gname)
+        b = getattr(o, oname)
+        assert rel_max(a, b) < TOL, (gname, rel_max(a, b))
+    _, _, _, acc = g.get_state()
+    assert rel_max(acc, o.aimp) < TOL
+    g.close()
+
+
+@pytest.mark.parametrize("name", ["t24", "t32"])
+def test_trajectory(hip, name):
+    wl, o, g = _setup(hip, name)
+    print(f"trajectory {name}", flush=True)
+    o.start()
+    g.start()
+    for it in range(3 + wl.n_pc):
+        eo = o.iterate(it, wl.deltat)
+        eg = g.iterate(wl.deltat)
+        psi, r, v, a = g.get_state()
+        assert rel_max(psi, o.psi) < TOL, (it, rel_max(psi, o.psi))
+        assert rel_max(r, o.rimp) < TOL, it
+        assert rel_max(v, o.vimp) < TOL, it
+        assert rel_max(a, o.aimp) < TOL, it
+        assert rel_max(g.get_field("pot4"), o.pot4) < TOL, it
+        if eo is not None:
+            np.testing.assert_allclose(eg, eo, rtol=1e-9, atol=1e-300)
+        np.testing.assert_allclose(g.particle_means(), o.particle_means(), rtol=1e-10, atol=1e-12)
+    g.close()
+
+
+def test_energy(hip):
+    wl, o, g = _setup(hip, "t32")
+    o.start()
+    g.start()
+    for it in range(4):
+        o.iterate(it, wl.deltat)
+        g.iterate(wl.deltat)
+    eo = o.energy()
+    eg = g.energy()
+    for key, val in eo.items():
+        assert abs(eg[key] - val) <= 1e-10 * max(1.0, abs(val)), (key, eg[key], val)
+    g.close()
+
+
+def test_alpha_off_core_ot(hip):
+    """core4='OT ' (no alpha_s term, poten.f90:94-105 skipped)."""
+    wl, o, g = _setup(hip, "t24", gkw=dict(core_otc=0), okw=dict(core="OT"))
+    o.start()
+    g.start()
+    assert rel_max(g.get_field("pot4"), o.pot4) < TOL
+    for it in range(4):
+        o.iterate(it, wl.deltat)
+        g.iterate(wl.deltat)
+    psi, r, v, a = g.get_state()
+    assert rel_max(psi, o.psi) < TOL
+    g.close()
+
+
+def test_timec_lambda(hip):
+    """Non-zero absorbing time constant timec = (Lambda(1+tanh((z-zm)/zs)), 1) (DFT4He3d.f90:773-782)."""
+    tc = dict(lambdah=0.3, tzmean=2.0, tzsurf=1.5)
+    wl, o, g = _setup(hip, "t24", gkw=tc, okw=tc)
+    o.start()
+    g.start()
+    for it in range(5):
+        o.iterate(it, wl.deltat)
+        g.iterate(wl.deltat)
+    psi, r, v, a = g.get_state()
+    assert rel_max(psi, o.psi) < TOL
+    g.close()
