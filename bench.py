@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
+                    help="time one precision only (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this path has no multi-GPU mode "
@@ -84,6 +86,11 @@ def main():
         sys.exit(2)
     import inputs
     wl = inputs.WORKLOADS[args.workload]
+    if args.only:
+        res = run_gpu(wl, 0 if args.only == "fp64" else 1, args.steps, args.warmup)
+        print(json.dumps({"only": args.only, "steps_per_s": res["steps_per_s"],
+                          "ms_per_step": 1000.0 / res["steps_per_s"]}))
+        return
     res64 = run_gpu(wl, 0, args.steps, args.warmup)
     res32 = run_gpu(wl, 1, args.steps, args.warmup)
     e64 = res64["energy1"]["etot"]

@@ -192,7 +192,7 @@ def r2c(f):
 
 
 def c2r(F, shape, renor):
-    return np.fft.irfftn(F, s=shape, norm="forward") * renor
+    return np.fft.irfftn(F, s=shape, axes=(0, 1, 2), norm="forward") * renor
 
 
 # --------------------------------------------------------------------------

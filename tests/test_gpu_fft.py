@@ -10,9 +10,19 @@ SHAPES = [(24, 24, 24), (32, 32, 32), (48, 48, 48), (64, 64, 64), (40, 24, 30), 
           (80, 80, 80), (96, 96, 96), (128, 128, 128)]
 
 
+def _hermitian_kernel(rng, shape):
+    """Random real kernel with K(kx,ky,kz) = K(kx,-ky,-kz), as every kernel of
+    the path (a function of |p|) has; c2r of the product is then well defined."""
+    nz, ny, nx = shape
+    a = rng.standard_normal(nx // 2 + 1)
+    b = rng.standard_normal(ny // 2 + 1)[np.minimum(np.arange(ny), ny - np.arange(ny))]
+    c = rng.standard_normal(nz // 2 + 1)[np.minimum(np.arange(nz), nz - np.arange(nz))]
+    return c[:, None, None] * b[None, :, None] * a[None, None, :]
+
+
 def _conv_ref(f, k):
     n = f.size
-    return np.fft.irfftn(np.fft.rfftn(f) * k, s=f.shape, norm="forward") * (1.0 / n)
+    return np.fft.irfftn(np.fft.rfftn(f) * k, s=f.shape, axes=(0, 1, 2), norm="forward") * (1.0 / n)
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -24,7 +34,7 @@ def test_convolve_random(hip, shape, prec):
     print(f"convolve shape={shape} prec={prec}", flush=True)
     rng = np.random.default_rng(3)
     fields = [rng.standard_normal(shape) for _ in range(3)]
-    kern = rng.standard_normal((nz, ny, nx // 2 + 1))
+    kern = _hermitian_kernel(rng, shape)
     with hip.Instance(p) as g:
         outs = g.convolve(fields, kern)
     tol = 1e-12 if prec == 0 else 2e-5
