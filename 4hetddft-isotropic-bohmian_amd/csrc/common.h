@@ -62,8 +62,9 @@ struct Stencil13 {
 struct Grid {
   int nx, ny, nz;
   int mx;           // nx/2 + 1
+  int mp;           // padded spectrum row length (>= mx, 128-byte multiple)
   long long n;      // nx*ny*nz
-  long long h;      // mx*ny*nz
+  long long h;      // mp*ny*nz (padded half spectrum)
 };
 
 }  // namespace hetd

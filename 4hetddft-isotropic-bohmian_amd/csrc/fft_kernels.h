@@ -4,14 +4,24 @@
 // (poten.f90, term_alfa.f90, potenimp.f90; plans in fft.f90:43-65).  Here a
 // 3D convolution is five batched passes over HBM:
 //   XF  real rows -> half-length complex FFT + Hermitian split (r2c along x),
-//       with the pointwise producer of the transformed field fused on load;
-//   YF  complex FFT along y on tiles of consecutive kx;
+//       with the pointwise producer of the transformed field fused on load,
+//       written transposed so that ky is the fastest index;
+//   YF  complex FFT along y on contiguous lines;
 //   Z   forward FFT along z, multiplication by one or more real k-space
 //       kernels, inverse FFT along z, one output spectrum per kernel;
-//   YI  inverse complex FFT along y;
-//   XI  Hermitian merge + half-length inverse FFT (c2r along x), times
-//       renor = 1/(nx ny nz) (fft.f90:125-207).
+//   YI  inverse complex FFT along y on contiguous lines;
+//   XI  transposed read, Hermitian merge + half-length inverse FFT (c2r
+//       along x), times renor = 1/(nx ny nz) (fft.f90:125-207).
 // Every launch carries several fields so one launch fills the chip.
+//
+// Internal half spectra are stored S[kz][kx][ky] (ky fastest, kx < nx/2+1);
+// k-space kernels use the same order.  Every global access moves rows of at
+// least 16 consecutive complex values.
+//
+// Kernels come in a compile-time-length form (N > 0: radix plan, tiles and
+// index arithmetic fold to constants; radix stages run in place through
+// registers, one LDS buffer) and a runtime-plan form (N == 0, two LDS
+// buffers) for every other size of the tstgrid list.
 #pragma once
 #include "fft_device.h"
 
@@ -19,13 +29,105 @@ namespace hetd {
 
 constexpr int kMaxBatch = 8;
 constexpr int kMaxJobs = 12;
+constexpr int kThreads = 256;
+constexpr int kRowsX = 16;   // rows (consecutive y) per x-pass workgroup
+constexpr int kLinesY = 16;  // contiguous lines per y-pass workgroup
+
+// Consecutive ky per z-pass workgroup: 256-byte rows up to nz = 128, half
+// that for longer (or runtime-length) lines to bound the LDS tile.
+template <class T, int N>
+__host__ __device__ constexpr int tile_z() {
+  return (N > 0 && N <= 128) ? (sizeof(T) == 8 ? 16 : 32) : (sizeof(T) == 8 ? 8 : 16);
+}
+
+// ---- compile-time radix plan ---------------------------------------------
+__host__ __device__ constexpr int pick_radix(int m) {
+  return (m % 8 == 0) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 5 == 0) ? 5 : (m % 3 == 0) ? 3 : m;
+}
+
+// One in-place Stockham stage over L lines: every thread loads its
+// butterflies into registers, the block synchronises, then stores.
+template <class T, bool INV, int R, int N, int NS, int L>
+__device__ inline void stage_ip(cplx<T>* buf, int ls, const cplx<T>* __restrict__ tw) {
+  constexpr int NB = N / R;
+  constexpr int TWS = N / (NS * R);
+  constexpr int TOT = L * NB;
+  constexpr int PER = (TOT + kThreads - 1) / kThreads;
+  cplx<T> v[PER][R];
+  int dst[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = threadIdx.x + q * kThreads;
+    dst[q] = -1;
+    if (TOT % kThreads == 0 || b < TOT) {
+      const int l = b / NB;
+      const int j = b - l * NB;
+      const cplx<T>* s = buf + l * ls;
+      const int k = j % NS;
+#pragma unroll
+      for (int t = 0; t < R; ++t) v[q][t] = s[j + t * NB];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int t = 1; t < R; ++t) {
+          const cplx<T> w = tw[t * k * TWS];
+          v[q][t] = INV ? cmulc(v[q][t], w) : cmul(v[q][t], w);
+        }
+      }
+      if constexpr (R == 2) bfly2<T, INV>(v[q]);
+      if constexpr (R == 3) bfly3<T, INV>(v[q]);
+      if constexpr (R == 4) bfly4<T, INV>(v[q]);
+      if constexpr (R == 5) bfly5<T, INV>(v[q]);
+      if constexpr (R == 8) bfly8<T, INV>(v[q]);
+      dst[q] = l * ls + (j / NS) * NS * R + k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    if (TOT % kThreads == 0 || dst[q] >= 0) {
+#pragma unroll
+      for (int t = 0; t < R; ++t) buf[dst[q] + t * NS] = v[q][t];
+    }
+  }
+  __syncthreads();
+}
+
+template <class T, bool INV, int N, int NS, int REM, int L>
+__device__ inline void stages_ip(cplx<T>* buf, int ls, const cplx<T>* __restrict__ tw) {
+  if constexpr (REM > 1) {
+    constexpr int R = pick_radix(REM);
+    stage_ip<T, INV, R, N, NS, L>(buf, ls, tw);
+    stages_ip<T, INV, N, NS * R, REM / R, L>(buf, ls, tw);
+  }
+}
+
+// Transform L lines held in b0.  Compile-time N runs in place and returns
+// b0; N == 0 ping-pongs through b1 and returns whichever holds the result.
+// Callers must synchronise before the first call (data in LDS).
+template <class T, bool INV, int N, int L>
+__device__ inline cplx<T>* fft_tile(cplx<T>* b0, cplx<T>* b1, int ls, const FftPlan& pl,
+                                    const cplx<T>* __restrict__ tw) {
+  if constexpr (N > 0) {
+    stages_ip<T, INV, N, 1, N, L>(b0, ls, tw);
+    return b0;
+  } else {
+    const int r = fft_lines<T, INV>(b0, b1, L, ls, pl, tw);
+    return r ? b1 : b0;
+  }
+}
+
+// LDS bytes of a tile of L lines of length n (line stride n+1).
+template <class T>
+__host__ __device__ constexpr size_t tile_bytes(int L, int n) {
+  return (size_t)L * (n + 1) * sizeof(cplx<T>);
+}
 
 // Pointwise producers fused into the forward x pass (the array the reference
 // hands to dfftw_execute is formed on the fly from its inputs).
 enum XfOp : int {
   XF_COPY = 0,      // f = a
   XF_CORREL = 1,    // f = den*dencg*(cp4+cpp4*dencg)           (poten.f90:62-64)
-  XF_FALFS_D = 2,   // f = (1 - denalf/den0s)*d                 (term_alfa.f90:28-36)
+  XF_MUL = 2,       // f = a*b, e.g. falfs*d                    (term_alfa.f90:32-36)
   XF_GRAD_DOT = 3,  // f = dx*ix + dy*iy + dz*iz                (term_alfa.f90:65-69)
 };
 
@@ -34,7 +136,7 @@ struct XfArgs {
   const R* in[kMaxBatch][6];
   int op[kMaxBatch];
   void* out[kMaxBatch];     // cplx<T>* half spectra
-  double c[kMaxBatch][2];   // per-field op constants: (cp4, cpp4) or (den0s, -)
+  double c[kMaxBatch][2];   // per-field op constants: (cp4, cpp4)
 };
 
 template <class R>
@@ -44,10 +146,8 @@ __device__ inline double xf_value(const XfArgs<R>& a, int f, long long i) {
       double den = (double)a.in[f][0][i], dcg = (double)a.in[f][1][i];
       return den * dcg * (a.c[f][0] + a.c[f][1] * dcg);
     }
-    case XF_FALFS_D: {
-      double fal = 1.0 - (double)a.in[f][0][i] / a.c[f][0];
-      return fal * (double)a.in[f][1][i];
-    }
+    case XF_MUL:
+      return (double)a.in[f][0][i] * (double)a.in[f][1][i];
     case XF_GRAD_DOT: {
       double s = (double)a.in[f][0][i] * (double)a.in[f][3][i] +
                  (double)a.in[f][1][i] * (double)a.in[f][4][i];
@@ -58,188 +158,221 @@ __device__ inline double xf_value(const XfArgs<R>& a, int f, long long i) {
   }
 }
 
-// r2c along x.  grid = (ceil(ny*nz/rows), nfields); dynamic LDS holds two
-// buffers of rows x (m+1) complex, m = nx/2.
-template <class T, class R>
-__global__ void __launch_bounds__(256) k_fft_xf(XfArgs<R> a, Grid g, FftPlan pl, int rows,
-                                                const cplx<T>* __restrict__ tw,
-                                                const cplx<T>* __restrict__ twh) {
+// r2c along x for kRowsX consecutive y rows of one z plane.
+// grid = (ceil(ny/kRowsX), nz, nfields).  MH > 0: compile-time nx/2.
+template <class T, class R, int MH>
+__global__ void __launch_bounds__(kThreads) k_fft_xf(XfArgs<R> a, Grid g, FftPlan pl,
+                                                     const cplx<T>* __restrict__ tw,
+                                                     const cplx<T>* __restrict__ twh) {
   extern __shared__ unsigned char smem_raw[];
-  const int m = g.nx / 2;
+  constexpr int ROWS = kRowsX;
+  const int m = MH > 0 ? MH : g.nx / 2;
+  const int nx = 2 * m;
   const int ls = m + 1;
   cplx<T>* b0 = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* b1 = b0 + rows * ls;
-  const int f = blockIdx.y;
-  const long long nrows = (long long)g.ny * g.nz;
-  const long long row0 = (long long)blockIdx.x * rows;
-  const int nrow = (int)((nrows - row0) < rows ? (nrows - row0) : rows);
-  // load: x[2j] -> re, x[2j+1] -> im of complex j
+  cplx<T>* b1 = b0 + ROWS * ls;
+  const int f = blockIdx.z;
+  const int y0 = blockIdx.x * ROWS;
+  const int iz = blockIdx.y;
+  const int nrow = (g.ny - y0) < ROWS ? (g.ny - y0) : ROWS;
   T* b0r = reinterpret_cast<T*>(b0);
-  for (int e = threadIdx.x; e < rows * g.nx; e += blockDim.x) {
-    const int r = e / g.nx, c = e - r * g.nx;
+  const long long src0 = ((long long)iz * g.ny + y0) * nx;
+  for (int e = threadIdx.x; e < ROWS * nx; e += kThreads) {
+    const int r = e / nx, c = e - r * nx;
     T v = T(0);
-    if (r < nrow) v = (T)xf_value(a, f, (row0 + r) * g.nx + c);
+    if (r < nrow) v = (T)xf_value(a, f, src0 + e);
     b0r[2 * (r * ls) + c] = v;
   }
   __syncthreads();
-  const int res = fft_lines<T, false>(b0, b1, rows, ls, pl, tw);
-  const cplx<T>* Z = res ? b1 : b0;
-  cplx<T>* out = reinterpret_cast<cplx<T>*>(a.out[f]);
+  cplx<T>* Z = fft_tile<T, false, MH, ROWS>(b0, b1, ls, pl, tw);
+  // Hermitian split in place, one thread per (k, m-k) pair:
+  // E = (Z[k] + conj Z[m-k])/2, O = (Z[k] - conj Z[m-k])/(2i), X[k] = E + W^k O
+  const int half = m / 2 + 1;
+  for (int e = threadIdx.x; e < ROWS * half; e += kThreads) {
+    const int r = e / half, k = e - r * half;
+    cplx<T>* z = Z + r * ls;
+    const int k2 = m - k;  // == m for k == 0
+    const cplx<T> za = z[k];
+    const cplx<T> zb = z[k == 0 ? 0 : k2];
+    {
+      const cplx<T> zc = conj(zb);
+      const cplx<T> E = scale(za + zc, T(0.5));
+      const cplx<T> D = za - zc;
+      const cplx<T> O = mk(T(0.5) * D.y, T(-0.5) * D.x);
+      z[k] = E + cmul(twh[k], O);
+    }
+    if (k2 != k) {
+      const cplx<T> zc = conj(za);
+      const cplx<T> E = scale(zb + zc, T(0.5));
+      const cplx<T> D = zb - zc;
+      const cplx<T> O = mk(T(0.5) * D.y, T(-0.5) * D.x);
+      z[k2] = E + cmul(twh[k2], O);
+    }
+  }
+  __syncthreads();
+  // transposed store: S[iz][kx][y0 + r]
+  cplx<T>* out = reinterpret_cast<cplx<T>*>(a.out[f]) + (long long)iz * g.mx * g.ny + y0;
   const int M = m + 1;
-  for (int e = threadIdx.x; e < nrow * M; e += blockDim.x) {
-    const int r = e / M, k = e - r * M;
-    const cplx<T>* z = Z + r * ls;
-    const cplx<T> zk = z[k == m ? 0 : k];
-    const cplx<T> zc = conj(z[k == 0 ? 0 : m - k]);
-    // E = (Z[k] + conj Z[m-k])/2, O = (Z[k] - conj Z[m-k])/(2i), X = E + W^k O
-    const cplx<T> E = scale(zk + zc, T(0.5));
-    const cplx<T> D = zk - zc;
-    const cplx<T> O = mk(T(0.5) * D.y, T(-0.5) * D.x);
-    const cplx<T> X = E + cmul(twh[k], O);
-    out[(row0 + r) * M + k] = X;
+  for (int e = threadIdx.x; e < ROWS * M; e += kThreads) {
+    const int kx = e / ROWS, r = e - kx * ROWS;
+    if (r < nrow) out[(long long)kx * g.ny + r] = Z[r * ls + kx];
   }
 }
 
-// c2r along x: half spectrum rows -> real rows, times renor.
+// c2r along x: transposed read of kRowsX consecutive ky columns, real rows
+// out, times renor.  grid = (ceil(ny/kRowsX), nz, nfields).
+// Epilogues of the inverse x pass: the c2r result v = z*renor is stored as
+// is, or as 1 - v/c (falfs = 1 - denalf/den0s, term_alfa.f90:28-30).
+enum XiEpi : int { XI_STORE = 0, XI_ONE_MINUS_DIV = 1 };
+
 template <class T, class R>
 struct XiArgs {
   const void* in[kMaxBatch];   // cplx<T>* half spectra
   R* out[kMaxBatch];
+  int epi[kMaxBatch];
+  double ec[kMaxBatch];
   double renor;
 };
 
-template <class T, class R>
-__global__ void __launch_bounds__(256) k_fft_xi(XiArgs<T, R> a, Grid g, FftPlan pl, int rows,
-                                                const cplx<T>* __restrict__ tw,
-                                                const cplx<T>* __restrict__ twh) {
+template <class T, class R, int MH>
+__global__ void __launch_bounds__(kThreads) k_fft_xi(XiArgs<T, R> a, Grid g, FftPlan pl,
+                                                     const cplx<T>* __restrict__ tw,
+                                                     const cplx<T>* __restrict__ twh) {
   extern __shared__ unsigned char smem_raw[];
-  const int m = g.nx / 2;
+  constexpr int ROWS = kRowsX;
+  const int m = MH > 0 ? MH : g.nx / 2;
+  const int nx = 2 * m;
   const int ls = m + 1;
   const int M = m + 1;
   cplx<T>* b0 = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* b1 = b0 + rows * ls;
-  const int f = blockIdx.y;
-  const long long nrows = (long long)g.ny * g.nz;
-  const long long row0 = (long long)blockIdx.x * rows;
-  const int nrow = (int)((nrows - row0) < rows ? (nrows - row0) : rows);
-  const cplx<T>* in = reinterpret_cast<const cplx<T>*>(a.in[f]);
-  for (int e = threadIdx.x; e < rows * m; e += blockDim.x) {
-    const int r = e / m, k = e - r * m;
-    cplx<T> y = mk(T(0), T(0));
-    if (r < nrow) {
-      const cplx<T>* x = in + (row0 + r) * M;
-      const cplx<T> xk = x[k];
-      const cplx<T> xc = conj(x[m - k]);
-      // Y[k] = (X[k] + conj X[m-k]) + i (X[k] - conj X[m-k]) W^{-k}
-      const cplx<T> s = xk + xc;
-      const cplx<T> d = cmulc(xk - xc, twh[k]);
-      y = mk(s.x - d.y, s.y + d.x);
-    }
-    b0[r * ls + k] = y;
+  cplx<T>* b1 = b0 + ROWS * ls;
+  const int f = blockIdx.z;
+  const int y0 = blockIdx.x * ROWS;
+  const int iz = blockIdx.y;
+  const int nrow = (g.ny - y0) < ROWS ? (g.ny - y0) : ROWS;
+  const cplx<T>* in = reinterpret_cast<const cplx<T>*>(a.in[f]) + (long long)iz * g.mx * g.ny + y0;
+  for (int e = threadIdx.x; e < ROWS * M; e += kThreads) {
+    const int kx = e / ROWS, r = e - kx * ROWS;
+    b0[r * ls + kx] = (r < nrow) ? in[(long long)kx * g.ny + r] : mk(T(0), T(0));
   }
   __syncthreads();
-  const int res = fft_lines<T, true>(b0, b1, rows, ls, pl, tw);
-  const T* zr = reinterpret_cast<const T*>(res ? b1 : b0);
-  R* out = a.out[f];
+  // Hermitian merge in place:
+  // Y[k] = (X[k] + conj X[m-k]) + i (X[k] - conj X[m-k]) W^{-k}
+  const int half = m / 2 + 1;
+  for (int e = threadIdx.x; e < ROWS * half; e += kThreads) {
+    const int r = e / half, k = e - r * half;
+    cplx<T>* x = b0 + r * ls;
+    const int k2 = m - k;
+    const cplx<T> xa = x[k], xb = x[k2];
+    {
+      const cplx<T> xc = conj(xb);
+      const cplx<T> s = xa + xc;
+      const cplx<T> d = cmulc(xa - xc, twh[k]);
+      x[k] = mk(s.x - d.y, s.y + d.x);
+    }
+    if (k != 0 && k2 != k) {
+      const cplx<T> xc = conj(xa);
+      const cplx<T> s = xb + xc;
+      const cplx<T> d = cmulc(xb - xc, twh[k2]);
+      x[k2] = mk(s.x - d.y, s.y + d.x);
+    }
+  }
+  __syncthreads();
+  const cplx<T>* Zc = fft_tile<T, true, MH, ROWS>(b0, b1, ls, pl, tw);
+  const T* zr = reinterpret_cast<const T*>(Zc);
+  R* out = a.out[f] + ((long long)iz * g.ny + y0) * nx;
   const T renor = (T)a.renor;
-  for (int e = threadIdx.x; e < nrow * g.nx; e += blockDim.x) {
-    const int r = e / g.nx, c = e - r * g.nx;
-    out[(row0 + r) * g.nx + c] = (R)(zr[2 * (r * ls) + c] * renor);
+  const int epi = a.epi[f];
+  const double ec = a.ec[f];
+  for (int e = threadIdx.x; e < nrow * nx; e += kThreads) {
+    const int r = e / nx, c = e - r * nx;
+    const T v = zr[2 * (r * ls) + c] * renor;
+    out[e] = epi == XI_ONE_MINUS_DIV ? (R)(1.0 - (double)(R)v / ec) : (R)v;
   }
 }
 
-// Complex FFT along y (stride M) for tiles of tx consecutive kx at fixed kz.
-// grid = (ntiles, nz, nfields).  In place.
+// Complex FFT along y on contiguous lines (line q = kz*mx + kx, length ny).
+// grid = (ceil(mx*nz/kLinesY), 1, nfields).  In place.
 template <class T>
 struct YArgs {
   void* buf[kMaxBatch];
 };
 
-template <class T, bool INV>
-__global__ void __launch_bounds__(256) k_fft_y(YArgs<T> a, Grid g, FftPlan pl, int tx,
-                                               const cplx<T>* __restrict__ tw) {
+template <class T, bool INV, int N>
+__global__ void __launch_bounds__(kThreads) k_fft_y(YArgs<T> a, Grid g, FftPlan pl, const cplx<T>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
-  const int n = g.ny;
+  constexpr int L = kLinesY;
+  const int n = N > 0 ? N : g.ny;
   const int ls = n + 1;
   cplx<T>* b0 = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* b1 = b0 + tx * ls;
-  const int M = g.mx;
-  const int kx0 = blockIdx.x * tx;
-  const int ntx = (M - kx0) < tx ? (M - kx0) : tx;
-  cplx<T>* base = reinterpret_cast<cplx<T>*>(a.buf[blockIdx.z]) + (long long)blockIdx.y * n * M + kx0;
-  for (int e = threadIdx.x; e < tx * n; e += blockDim.x) {
-    const int j = e / tx, l = e - j * tx;
-    b0[l * ls + j] = (l < ntx) ? base[(long long)j * M + l] : mk(T(0), T(0));
+  cplx<T>* b1 = b0 + L * ls;
+  const int nlines = g.mx * g.nz;
+  const int q0 = blockIdx.x * L;
+  const int nl = (nlines - q0) < L ? (nlines - q0) : L;
+  cplx<T>* base = reinterpret_cast<cplx<T>*>(a.buf[blockIdx.z]) + (long long)q0 * n;
+  for (int e = threadIdx.x; e < L * n; e += kThreads) {
+    const int l = e / n, j = e - l * n;
+    b0[l * ls + j] = (l < nl) ? base[e] : mk(T(0), T(0));
   }
   __syncthreads();
-  const int res = fft_lines<T, INV>(b0, b1, tx, ls, pl, tw);
-  const cplx<T>* Y = res ? b1 : b0;
-  for (int e = threadIdx.x; e < tx * n; e += blockDim.x) {
-    const int j = e / tx, l = e - j * tx;
-    if (l < ntx) base[(long long)j * M + l] = Y[l * ls + j];
+  const cplx<T>* Y = fft_tile<T, INV, N, L>(b0, b1, ls, pl, tw);
+  for (int e = threadIdx.x; e < nl * n; e += kThreads) {
+    const int l = e / n, j = e - l * n;
+    base[e] = Y[l * ls + j];
   }
 }
 
-// z pass of a convolution: forward FFT along z of input i, then for each
-// job of that input: multiply by sign*kernel(kx,ky,kz), inverse FFT along z,
-// store to the job's output spectrum.  grid = (ntiles, ny, ninputs).
+// z pass of a convolution, one job per workgroup: forward FFT along z of
+// the job's input, multiplication by sign*kernel(kx,ky,kz), inverse FFT along
+// z, store to the job's output spectrum.  Tile = TY consecutive ky at fixed
+// kx.  grid = (ceil(ny/TY), mx, njobs).
 template <class T>
 struct ZArgs {
-  const void* in[kMaxBatch];
-  int job0[kMaxBatch + 1];
+  const void* in[kMaxJobs];
   const T* kern[kMaxJobs];
   T ksign[kMaxJobs];
   void* out[kMaxJobs];
 };
 
-template <class T>
-__global__ void __launch_bounds__(256) k_fft_zconv(ZArgs<T> a, Grid g, FftPlan pl, int tx,
-                                                   const cplx<T>* __restrict__ tw) {
+template <class T, int N, int TY>
+__global__ void __launch_bounds__(kThreads) k_fft_zconv(ZArgs<T> a, Grid g, FftPlan pl,
+                                                        const cplx<T>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
-  const int n = g.nz;
+  const int n = N > 0 ? N : g.nz;
   const int ls = n + 1;
-  cplx<T>* bA = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* bB = bA + tx * ls;
-  cplx<T>* bC = bB + tx * ls;
-  const int M = g.mx;
-  const long long pstride = (long long)M * g.ny;  // stride between kz planes
-  const int kx0 = blockIdx.x * tx;
-  const int ntx = (M - kx0) < tx ? (M - kx0) : tx;
-  const int ky = blockIdx.y;
-  const int ii = blockIdx.z;
-  const long long off = (long long)ky * M + kx0;
-  const cplx<T>* src = reinterpret_cast<const cplx<T>*>(a.in[ii]) + off;
-  for (int e = threadIdx.x; e < tx * n; e += blockDim.x) {
-    const int j = e / tx, l = e - j * tx;
-    bA[l * ls + j] = (l < ntx) ? src[(long long)j * pstride + l] : mk(T(0), T(0));
+  cplx<T>* b0 = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* b1 = b0 + TY * ls;                             // runtime-plan scratch
+  const long long pstride = (long long)g.mx * g.ny;      // stride between kz planes
+  const int ky0 = blockIdx.x * TY;
+  const int nty = (g.ny - ky0) < TY ? (g.ny - ky0) : TY;
+  const int kx = blockIdx.y;
+  const int jb = blockIdx.z;
+  const long long off = (long long)kx * g.ny + ky0;
+  const cplx<T>* src = reinterpret_cast<const cplx<T>*>(a.in[jb]) + off;
+  for (int e = threadIdx.x; e < TY * n; e += kThreads) {
+    const int j = e / TY, l = e - j * TY;
+    b0[l * ls + j] = (l < nty) ? src[(long long)j * pstride + l] : mk(T(0), T(0));
   }
   __syncthreads();
-  int res = fft_lines<T, false>(bA, bB, tx, ls, pl, tw);
-  cplx<T>* F = res ? bB : bA;     // forward spectrum, kept
-  cplx<T>* W0 = res ? bA : bB;    // free buffers for the inverse
-  cplx<T>* W1 = bC;
-  for (int jb = a.job0[ii]; jb < a.job0[ii + 1]; ++jb) {
-    const T* kern = a.kern[jb] + off;
-    const T sg = a.ksign[jb];
-    for (int e = threadIdx.x; e < tx * n; e += blockDim.x) {
-      const int j = e / tx, l = e - j * tx;
-      cplx<T> v = F[l * ls + j];
-      if (l < ntx) {
-        const T kv = kern[(long long)j * pstride + l];
-        v = mk(v.x * kv, v.y * kv);
-        if (sg < T(0)) v = mk(-v.x, -v.y);
-      }
-      W0[l * ls + j] = v;
+  cplx<T>* F = fft_tile<T, false, N, TY>(b0, b1, ls, pl, tw);
+  const T* kern = a.kern[jb] + off;
+  const T sg = a.ksign[jb];
+  for (int e = threadIdx.x; e < TY * n; e += kThreads) {
+    const int j = e / TY, l = e - j * TY;
+    if (l < nty) {
+      const cplx<T> v = F[l * ls + j];
+      const T kv = kern[(long long)j * pstride + l];
+      const cplx<T> w = mk(v.x * kv, v.y * kv);
+      F[l * ls + j] = sg < T(0) ? mk(-w.x, -w.y) : w;
     }
-    __syncthreads();
-    const int r2 = fft_lines<T, true>(W0, W1, tx, ls, pl, tw);
-    const cplx<T>* Y = r2 ? W1 : W0;
-    cplx<T>* dst = reinterpret_cast<cplx<T>*>(a.out[jb]) + off;
-    for (int e = threadIdx.x; e < tx * n; e += blockDim.x) {
-      const int j = e / tx, l = e - j * tx;
-      if (l < ntx) dst[(long long)j * pstride + l] = Y[l * ls + j];
-    }
-    __syncthreads();
+  }
+  __syncthreads();
+  cplx<T>* other = (F == b0) ? b1 : b0;
+  const cplx<T>* Y = fft_tile<T, true, N, TY>(F, other, ls, pl, tw);
+  cplx<T>* dst = reinterpret_cast<cplx<T>*>(a.out[jb]) + off;
+  for (int e = threadIdx.x; e < TY * n; e += kThreads) {
+    const int j = e / TY, l = e - j * TY;
+    if (l < nty) dst[(long long)j * pstride + l] = Y[l * ls + j];
   }
 }
 

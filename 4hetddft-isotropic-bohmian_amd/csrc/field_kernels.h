@@ -16,43 +16,54 @@ struct Idx3 {
 };
 
 __device__ inline Idx3 idx3(long long i, const Grid& g) {
+  // 32-bit unsigned index arithmetic: grids hold at most 256^3 points
   Idx3 r;
-  const long long pl = (long long)g.nx * g.ny;
-  r.iz = (int)(i / pl);
-  const long long rem = i - (long long)r.iz * pl;
-  r.iy = (int)(rem / g.nx);
-  r.ix = (int)(rem - (long long)r.iy * g.nx);
+  const unsigned u = (unsigned)i;
+  const unsigned pl = (unsigned)g.nx * (unsigned)g.ny;
+  const unsigned iz = u / pl;
+  const unsigned rem = u - iz * pl;
+  const unsigned iy = rem / (unsigned)g.nx;
+  r.iz = (int)iz;
+  r.iy = (int)iy;
+  r.ix = (int)(rem - iy * (unsigned)g.nx);
   return r;
 }
 
 // 1D 13-point derivative along one axis with f == 0 outside the box
 // (derivnD with icon=13: init_deriv_parallel.f90:428-535, redef.f:121-170).
-// The sum runs over offsets -6..6 in order; out-of-box terms are absent.
+// The sum runs over offsets -6..6 in order, adding w*0 for out-of-box
+// points.  Every load is issued unconditionally from an in-box address (the
+// centre for out-of-box offsets) so all 13 can be in flight together.
 template <class R>
 __device__ inline double d13(const R* __restrict__ f, long long i, int pos, int n, long long stride,
                              const Stencil13& w) {
-  double acc = 0.0;
+  double v[13];
 #pragma unroll
   for (int m = -6; m <= 6; ++m) {
-    const int p = pos + m;
-    if (m != 0 || w.w[6] != 0.0) {
-      if (p >= 0 && p < n) acc = acc + w.w[m + 6] * (double)f[i + m * stride];
-    }
+    const bool inb = (unsigned)(pos + m) < (unsigned)n;
+    const double t = (double)f[i + (inb ? m * stride : 0)];
+    v[m + 6] = inb ? t : 0.0;
   }
+  double acc = 0.0;
+#pragma unroll
+  for (int m = 0; m < 13; ++m) acc = acc + w.w[m] * v[m];
   return acc;
 }
 
 __device__ inline c64 d13c(const c64* __restrict__ f, long long i, int pos, int n, long long stride,
                            const Stencil13& w) {
-  double ax = 0.0, ay = 0.0;
+  c64 v[13];
 #pragma unroll
   for (int m = -6; m <= 6; ++m) {
-    const int p = pos + m;
-    if (p >= 0 && p < n) {
-      const c64 v = f[i + m * stride];
-      ax = ax + w.w[m + 6] * v.x;
-      ay = ay + w.w[m + 6] * v.y;
-    }
+    const bool inb = (unsigned)(pos + m) < (unsigned)n;
+    const c64 t = f[i + (inb ? m * stride : 0)];
+    v[m + 6] = inb ? t : mk(0.0, 0.0);
+  }
+  double ax = 0.0, ay = 0.0;
+#pragma unroll
+  for (int m = 0; m < 13; ++m) {
+    ax = ax + w.w[m] * v[m].x;
+    ay = ay + w.w[m] * v[m].y;
   }
   return mk(ax, ay);
 }
@@ -214,21 +225,21 @@ struct PotArgs {
 };
 
 // Alpha_s divergence, alpha_s field and final pot4 (term_alfa.f90:85-113,
-// poten.f90:80-124).  falfs = 1 - denalf/den0s is formed at each neighbour.
+// poten.f90:80-124).  The products falfs*int are formed at each neighbour.
 template <class R>
-__device__ inline double dfint(const R* __restrict__ denalf, const R* __restrict__ in, long long i, int pos,
-                               int n, long long stride, const Stencil13& w, double den0s) {
-  double acc = 0.0;
+__device__ inline double dfint(const R* __restrict__ falfs, const R* __restrict__ in, long long i, int pos,
+                               int n, long long stride, const Stencil13& w) {
+  double v[13];
 #pragma unroll
   for (int m = -6; m <= 6; ++m) {
-    if (m == 0) continue;
-    const int pp = pos + m;
-    if (pp >= 0 && pp < n) {
-      const long long j = i + m * stride;
-      const double fal = 1.0 - (double)denalf[j] / den0s;
-      acc = acc + w.w[m + 6] * (fal * (double)in[j]);
-    }
+    const bool inb = (unsigned)(pos + m) < (unsigned)n;
+    const long long j = i + (inb ? m * stride : 0);
+    const double t = (double)falfs[j] * (double)in[j];
+    v[m + 6] = inb ? t : 0.0;
   }
+  double acc = 0.0;
+#pragma unroll
+  for (int m = 0; m < 13; ++m) acc = acc + w.w[m] * v[m];
   return acc;
 }
 
@@ -238,7 +249,7 @@ __global__ void __launch_bounds__(kBlock) k_pot_final(PotArgs a, const R* __rest
                                                        const R* __restrict__ delj4,
                                                        const R* __restrict__ sto1c,
                                                        const R* __restrict__ uimp,
-                                                       const R* __restrict__ denalf,
+                                                       const R* __restrict__ falfs,
                                                        const R* __restrict__ intx,
                                                        const R* __restrict__ inty,
                                                        const R* __restrict__ intz,
@@ -251,9 +262,9 @@ __global__ void __launch_bounds__(kBlock) k_pot_final(PotArgs a, const R* __rest
   double p4 = ((double)delj4[i] + dcg * dcg * (a.a0 + a.a1 * dcg)) + (double)sto1c[i];
   if (a.otc) {
     const Idx3 p = idx3(i, g);
-    const double s4 = dfint(denalf, intx, i, p.ix, g.nx, 1, a.d1x, a.den0s);
-    const double s5 = dfint(denalf, inty, i, p.iy, g.ny, g.nx, a.d1y, a.den0s);
-    const double s6 = dfint(denalf, intz, i, p.iz, g.nz, (long long)g.nx * g.ny, a.d1z, a.den0s);
+    const double s4 = dfint(falfs, intx, i, p.ix, g.nx, 1, a.d1x);
+    const double s5 = dfint(falfs, inty, i, p.iy, g.ny, g.nx, a.d1y);
+    const double s6 = dfint(falfs, intz, i, p.iz, g.nz, (long long)g.nx * g.ny, a.d1z);
     const double ua = (double)uaconv[i] / a.den0s;
     const double ual = (((ua + s4) + s5) + s6) * a.h2o2m4 * a.alphas;
     p4 = p4 + ual;
@@ -265,22 +276,105 @@ __global__ void __launch_bounds__(kBlock) k_pot_final(PotArgs a, const R* __rest
 
 // ---- impurity side ------------------------------------------------------
 // NGP histogram (denpart.f90:24-32): ix = int(1.5 + (r - x1)/h); counts are
-// exact integers, so the atomic order does not matter.
+// exact integers, so the atomic order does not matter.  The histogram also
+// records the bounding box of the occupied cells in reg[0..5].
 struct BoxArgs {
   Grid g;
   double xmin, ymin, zmin, hx, hy, hz;
   int npar;
 };
 
-__global__ void k_hist(BoxArgs b, const double* __restrict__ r, unsigned int* __restrict__ cnt) {
+// reg layout: [0..5] bbox of occupied cells (imin,jmin,kmin,imax,jmax,kmax);
+// [6..11] region that may hold non-zero denx/psix/hpsix; [12..17] region
+// recomputed by this evaluation.  Regions are inclusive 0-based boxes.
+constexpr int kRegBox = 0, kRegLive = 6, kRegWork = 12, kRegLen = 18;
+
+__global__ void k_hist(BoxArgs b, const double* __restrict__ r, unsigned int* __restrict__ cnt,
+                       int* __restrict__ reg) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= b.npar) return;
-  const int ix = (int)(1.5 + (r[i] - b.xmin) / b.hx);
-  const int iy = (int)(1.5 + (r[b.npar + i] - b.ymin) / b.hy);
-  const int iz = (int)(1.5 + (r[2 * b.npar + i] - b.zmin) / b.hz);
-  if (iz >= 1 && iz <= b.g.nz && iy >= 1 && iy <= b.g.ny && ix >= 1 && ix <= b.g.nx) {
+  int ix = 0, iy = 0, iz = 0;
+  if (i < b.npar) {
+    ix = (int)(1.5 + (r[i] - b.xmin) / b.hx);
+    iy = (int)(1.5 + (r[b.npar + i] - b.ymin) / b.hy);
+    iz = (int)(1.5 + (r[2 * b.npar + i] - b.zmin) / b.hz);
+  }
+  const bool in = iz >= 1 && iz <= b.g.nz && iy >= 1 && iy <= b.g.ny && ix >= 1 && ix <= b.g.nx;
+  if (in) {
     const long long j = (ix - 1) + (long long)b.g.nx * ((iy - 1) + (long long)b.g.ny * (iz - 1));
     atomicAdd(&cnt[j], 1u);
+  }
+  // bounding box: reduce across the wavefront first, one atomic per wave
+  int v[6] = {in ? ix - 1 : 0x7fffffff, in ? iy - 1 : 0x7fffffff, in ? iz - 1 : 0x7fffffff,
+              in ? ix - 1 : -1, in ? iy - 1 : -1, in ? iz - 1 : -1};
+  for (int off = 32; off > 0; off >>= 1)
+    for (int d = 0; d < 6; ++d) {
+      const int o = __shfl_down(v[d], off, 64);
+      v[d] = d < 3 ? min(v[d], o) : max(v[d], o);
+    }
+  if ((threadIdx.x & 63) == 0 && v[3] >= 0) {
+    for (int d = 0; d < 3; ++d) atomicMin(&reg[d], v[d]);
+    for (int d = 3; d < 6; ++d) atomicMax(&reg[d], v[d]);
+  }
+}
+
+// work = live U expand(bbox, margin); live := expand(bbox, margin).
+__global__ void k_reg_union(Grid g, int* __restrict__ reg, int margin) {
+  if (threadIdx.x != 0) return;
+  const int n[3] = {g.nx, g.ny, g.nz};
+  int e[6];
+  const bool empty = reg[3] < 0;
+  for (int d = 0; d < 3; ++d) {
+    e[d] = empty ? 0x7fffffff : max(reg[d] - margin, 0);
+    e[3 + d] = empty ? -1 : min(reg[3 + d] + margin, n[d] - 1);
+  }
+  for (int d = 0; d < 3; ++d) {
+    const bool lempty = reg[kRegLive + 3] < 0;
+    const int lo = lempty ? e[d] : min(reg[kRegLive + d], e[d]);
+    const int hi = lempty ? e[3 + d] : max(reg[kRegLive + 3 + d], e[3 + d]);
+    reg[kRegWork + d] = lo;
+    reg[kRegWork + 3 + d] = hi;
+  }
+  for (int d = 0; d < 6; ++d) reg[kRegLive + d] = e[d];
+}
+
+// Grid-stride iteration over an inclusive region box.
+struct RegionIter {
+  int x0, y0, z0, nxr, nyr, vol;
+};
+
+__device__ inline RegionIter region_iter(const int* reg) {
+  RegionIter it;
+  it.x0 = reg[0];
+  it.y0 = reg[1];
+  it.z0 = reg[2];
+  it.nxr = reg[3] - reg[0] + 1;
+  it.nyr = reg[4] - reg[1] + 1;
+  const int nzr = reg[5] - reg[2] + 1;
+  it.vol = (reg[3] < 0 || it.nxr <= 0 || it.nyr <= 0 || nzr <= 0) ? 0 : it.nxr * it.nyr * nzr;
+  return it;
+}
+
+__device__ inline Idx3 region_point(const RegionIter& it, int v) {
+  Idx3 p;
+  const int a = v / it.nxr;
+  p.ix = it.x0 + (v - a * it.nxr);
+  const int b = a / it.nyr;
+  p.iy = it.y0 + (a - b * it.nyr);
+  p.iz = it.z0 + b;
+  return p;
+}
+
+// Clears the counts of the live region and resets the occupied-cell box
+// (reg[0..5]) for the histogram that follows.
+__global__ void k_clear_counts(Grid g, int* __restrict__ reg, unsigned int* __restrict__ cnt) {
+  const RegionIter it = region_iter(reg + kRegLive);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    reg[0] = reg[1] = reg[2] = 0x7fffffff;
+    reg[3] = reg[4] = reg[5] = -1;
+  }
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < it.vol; v += gridDim.x * blockDim.x) {
+    const Idx3 p = region_point(it, v);
+    cnt[p.ix + (long long)g.nx * (p.iy + (long long)g.ny * p.iz)] = 0u;
   }
 }
 
@@ -292,52 +386,50 @@ __device__ inline double denx_of(const unsigned int* cnt, const R* src, long lon
   return cnt ? ((double)cnt[j] / npar) / hxyz : (double)src[j];
 }
 
+// When psix_out is given, the pass also writes psix = sqrt(denx)
+// (denpart.f90:40) for the value it produces.
 template <class R>
-__global__ void __launch_bounds__(kBlock) k_iron(Grid g, const unsigned int* __restrict__ cnt,
+__global__ void __launch_bounds__(kBlock) k_iron(Grid g, const int* __restrict__ reg,
+                                                  const unsigned int* __restrict__ cnt,
                                                   const R* __restrict__ src, R* __restrict__ dst,
-                                                  double npar, double hxyz, int smooth) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g.n) return;
+                                                  R* __restrict__ psix_out, double npar, double hxyz,
+                                                  int smooth) {
+  const RegionIter it = region_iter(reg + kRegWork);
+  for (int vv = blockIdx.x * blockDim.x + threadIdx.x; vv < it.vol; vv += gridDim.x * blockDim.x) {
+  const Idx3 p = region_point(it, vv);
+  const long long i = p.ix + (long long)g.nx * (p.iy + (long long)g.ny * p.iz);
   const double c = denx_of(cnt, src, i, npar, hxyz);
-  if (!smooth) {
-    dst[i] = (R)c;
-    return;
+  double v = c;
+  if (smooth) {
+    if (!(p.ix == 0 || p.iy == 0 || p.iz == 0 || p.ix == g.nx - 1 || p.iy == g.ny - 1 || p.iz == g.nz - 1)) {
+      const long long sy = g.nx, sz = (long long)g.nx * g.ny;
+      double s = denx_of(cnt, src, i - 1, npar, hxyz) + denx_of(cnt, src, i + 1, npar, hxyz);
+      s = s + denx_of(cnt, src, i - sy, npar, hxyz);
+      s = s + denx_of(cnt, src, i + sy, npar, hxyz);
+      s = s + denx_of(cnt, src, i - sz, npar, hxyz);
+      s = s + denx_of(cnt, src, i + sz, npar, hxyz);
+      v = 8.333333333333333e-2 * s + 0.5 * c;
+    }
   }
-  const Idx3 p = idx3(i, g);
-  if (p.ix == 0 || p.iy == 0 || p.iz == 0 || p.ix == g.nx - 1 || p.iy == g.ny - 1 || p.iz == g.nz - 1) {
-    dst[i] = (R)c;
-    return;
+  dst[i] = (R)v;
+  if (psix_out) psix_out[i] = (R)sqrt((double)(R)v);
   }
-  const long long sy = g.nx, sz = (long long)g.nx * g.ny;
-  double s = denx_of(cnt, src, i - 1, npar, hxyz) + denx_of(cnt, src, i + 1, npar, hxyz);
-  s = s + denx_of(cnt, src, i - sy, npar, hxyz);
-  s = s + denx_of(cnt, src, i + sy, npar, hxyz);
-  s = s + denx_of(cnt, src, i - sz, npar, hxyz);
-  s = s + denx_of(cnt, src, i + sz, npar, hxyz);
-  dst[i] = (R)(8.333333333333333e-2 * s + 0.5 * c);
 }
 
-// psix = sqrt(denx); hpsix = -h2o2mx*(d2x+d2y+d2z)psix (denpart.f90:40-46)
+// hpsix = -h2o2mx*(d2x+d2y+d2z)psix (denpart.f90:42-46)
 template <class R>
-__global__ void __launch_bounds__(kBlock) k_psix(Grid g, const R* __restrict__ denx, Stencil13 wx,
-                                                  Stencil13 wy, Stencil13 wz, double h2o2mx,
-                                                  R* __restrict__ psix, R* __restrict__ hpsix) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g.n) return;
-  const Idx3 p = idx3(i, g);
-  psix[i] = (R)sqrt((double)denx[i]);
-  double sx = 0.0, sy = 0.0, sz = 0.0;
-  const long long syy = g.nx, szz = (long long)g.nx * g.ny;
-#pragma unroll
-  for (int m = -6; m <= 6; ++m) {
-    int q = p.ix + m;
-    if (q >= 0 && q < g.nx) sx = sx + wx.w[m + 6] * sqrt((double)denx[i + m]);
-    q = p.iy + m;
-    if (q >= 0 && q < g.ny) sy = sy + wy.w[m + 6] * sqrt((double)denx[i + m * syy]);
-    q = p.iz + m;
-    if (q >= 0 && q < g.nz) sz = sz + wz.w[m + 6] * sqrt((double)denx[i + m * szz]);
-  }
+__global__ void __launch_bounds__(kBlock) k_hpsix(Grid g, const int* __restrict__ reg, const R* __restrict__ psix,
+                                                   Stencil13 wx, Stencil13 wy, Stencil13 wz, double h2o2mx,
+                                                   R* __restrict__ hpsix) {
+  const RegionIter it = region_iter(reg + kRegWork);
+  for (int vv = blockIdx.x * blockDim.x + threadIdx.x; vv < it.vol; vv += gridDim.x * blockDim.x) {
+  const Idx3 p = region_point(it, vv);
+  const long long i = p.ix + (long long)g.nx * (p.iy + (long long)g.ny * p.iz);
+  const double sx = d13(psix, i, p.ix, g.nx, 1, wx);
+  const double sy = d13(psix, i, p.iy, g.ny, g.nx, wy);
+  const double sz = d13(psix, i, p.iz, g.nz, (long long)g.nx * g.ny, wz);
   hpsix[i] = (R)(-h2o2mx * ((sx + sy) + sz));
+  }
 }
 
 struct GatherArgs {
@@ -364,44 +456,78 @@ __device__ inline double force_at(const GatherArgs& a, int c, const R* __restric
   return ((double)fconv[j] + sto7 * (s1 * (double)hpsix[j] - (double)psix[j] * s4)) / a.m_imp;
 }
 
-// Trilinear gather at the particles (interpola3 + blend_103,
-// potenimp.f90:76-116,278-286); zero outside [1, n-1].
+// Cell of a particle for the trilinear gather (interpola3,
+// potenimp.f90:91-106): ix = int((r - x1)/h + 1); valid when 1 <= ix < n.
+struct GatherCell {
+  int ix, iy, iz;
+  double r, s, t;
+  bool ok;
+};
+
+__device__ inline GatherCell gather_cell(const BoxArgs& b, const double* __restrict__ rp, int i) {
+  GatherCell q;
+  const int np = b.npar;
+  const double ax = (rp[i] - b.xmin) / b.hx + 1;
+  const double ay = (rp[np + i] - b.ymin) / b.hy + 1;
+  const double az = (rp[2 * np + i] - b.zmin) / b.hz + 1;
+  q.ix = (int)ax;
+  q.iy = (int)ay;
+  q.iz = (int)az;
+  q.r = ax - q.ix;
+  q.s = ay - q.iy;
+  q.t = az - q.iz;
+  q.ok = !(q.ix >= b.g.nx || q.iy >= b.g.ny || q.iz >= b.g.nz || q.ix <= 0 || q.iy <= 0 || q.iz <= 0);
+  return q;
+}
+
+// Force component c at the 8 corners of each particle's cell: one thread per
+// (particle, corner, component); corner bits (dx,dy,dz) = (bit2,bit1,bit0)
+// in the order x000,x001,...,x111 of blend_103.  out[(c*8+corner)*N_par+i].
 template <class R>
-__global__ void k_gather(GatherArgs a, const double* __restrict__ r, const R* __restrict__ fcx,
-                         const R* __restrict__ fcy, const R* __restrict__ fcz, const R* __restrict__ denx,
-                         const R* __restrict__ psix, const R* __restrict__ hpsix, double* __restrict__ acc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kBlock) k_force_corners(GatherArgs a, const double* __restrict__ rp,
+                                                           const R* __restrict__ fcx, const R* __restrict__ fcy,
+                                                           const R* __restrict__ fcz, const R* __restrict__ denx,
+                                                           const R* __restrict__ psix,
+                                                           const R* __restrict__ hpsix,
+                                                           double* __restrict__ out) {
   const int np = a.b.npar;
-  if (i >= np) return;
-  const Grid& g = a.b.g;
-  const double ax = (r[i] - a.b.xmin) / a.b.hx + 1;
-  const double ay = (r[np + i] - a.b.ymin) / a.b.hy + 1;
-  const double az = (r[2 * np + i] - a.b.zmin) / a.b.hz + 1;
-  const int ix = (int)ax, iy = (int)ay, iz = (int)az;
-  const double rr = ax - ix, ss = ay - iy, tt = az - iz;
-  if (ix >= g.nx || iy >= g.ny || iz >= g.nz || ix <= 0 || iy <= 0 || iz <= 0) {
-    acc[i] = 0.0;
-    acc[np + i] = 0.0;
-    acc[2 * np + i] = 0.0;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 24 * np) return;
+  const int slot = t / np;
+  const int i = t - slot * np;
+  const int c = slot >> 3, corner = slot & 7;
+  const GatherCell q = gather_cell(a.b, rp, i);
+  double v = 0.0;
+  if (q.ok) {
+    const R* fc = c == 0 ? fcx : (c == 1 ? fcy : fcz);
+    v = force_at(a, c, fc, denx, psix, hpsix, q.ix - 1 + ((corner >> 2) & 1), q.iy - 1 + ((corner >> 1) & 1),
+                 q.iz - 1 + (corner & 1));
+  }
+  out[t] = v;
+}
+
+// Trilinear blend at the particles (blend_103, potenimp.f90:278-286); zero
+// outside [1, n-1] (potenimp.f90:104-105).  One thread per (particle, comp).
+__global__ void __launch_bounds__(kBlock) k_blend(BoxArgs b, const double* __restrict__ rp,
+                                                   const double* __restrict__ fcorner, double* __restrict__ acc) {
+  const int np = b.npar;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * np) return;
+  const int c = t / np;
+  const int i = t - c * np;
+  const GatherCell q = gather_cell(b, rp, i);
+  if (!q.ok) {
+    acc[t] = 0.0;
     return;
   }
-  const int i0 = ix - 1, j0 = iy - 1, k0 = iz - 1;
-  const R* fc[3] = {fcx, fcy, fcz};
-  for (int c = 0; c < 3; ++c) {
-    const double x000 = force_at(a, c, fc[c], denx, psix, hpsix, i0, j0, k0);
-    const double x001 = force_at(a, c, fc[c], denx, psix, hpsix, i0, j0, k0 + 1);
-    const double x010 = force_at(a, c, fc[c], denx, psix, hpsix, i0, j0 + 1, k0);
-    const double x011 = force_at(a, c, fc[c], denx, psix, hpsix, i0, j0 + 1, k0 + 1);
-    const double x100 = force_at(a, c, fc[c], denx, psix, hpsix, i0 + 1, j0, k0);
-    const double x101 = force_at(a, c, fc[c], denx, psix, hpsix, i0 + 1, j0, k0 + 1);
-    const double x110 = force_at(a, c, fc[c], denx, psix, hpsix, i0 + 1, j0 + 1, k0);
-    const double x111 = force_at(a, c, fc[c], denx, psix, hpsix, i0 + 1, j0 + 1, k0 + 1);
-    const double v = 1.0 * (+x000) + rr * (-x000 + x100) + ss * (-x000 + x010) + tt * (-x000 + x001) +
-                     rr * ss * (+x000 - x100 - x010 + x110) + rr * tt * (+x000 - x100 - x001 + x101) +
-                     ss * tt * (+x000 - x010 - x001 + x011) +
-                     rr * ss * tt * (-x000 + x100 + x010 + x001 - x011 - x101 - x110 + x111);
-    acc[c * np + i] = v;
-  }
+  const double* f = fcorner + (long long)c * 8 * np + i;
+  const double x000 = f[0], x001 = f[np], x010 = f[2 * np], x011 = f[3 * np];
+  const double x100 = f[4 * np], x101 = f[5 * np], x110 = f[6 * np], x111 = f[7 * np];
+  const double rr = q.r, ss = q.s, tt = q.t;
+  acc[t] = 1.0 * (+x000) + rr * (-x000 + x100) + ss * (-x000 + x010) + tt * (-x000 + x001) +
+           rr * ss * (+x000 - x100 - x010 + x110) + rr * tt * (+x000 - x100 - x001 + x101) +
+           ss * tt * (+x000 - x010 - x001 + x011) +
+           rr * ss * tt * (-x000 + x100 + x010 + x001 - x011 - x101 - x110 + x111);
 }
 
 // ---- particle integrators (steprk.f90:78-105, steppc.f90:64-87,122-151) --
@@ -499,7 +625,7 @@ struct EnergyArgs {
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_energy(EnergyArgs a, const c64* __restrict__ psi,
                                                     const R* __restrict__ den, const R* __restrict__ dencg,
-                                                    const R* __restrict__ delj4, const R* __restrict__ denalf,
+                                                    const R* __restrict__ delj4, const R* __restrict__ falfs,
                                                     const R* __restrict__ dxd, const R* __restrict__ dyd,
                                                     const R* __restrict__ dzd, const R* __restrict__ ix,
                                                     const R* __restrict__ iy, const R* __restrict__ iz,
@@ -520,7 +646,7 @@ __global__ void __launch_bounds__(kBlock) k_energy(EnergyArgs a, const c64* __re
     v[1] = (double)delj4[i] * dn;
     v[2] = dn * (dcg * dcg) * (a.aux1 + a.aux2 * dcg);
     if (a.otc) {
-      const double fal = 1.0 - (double)denalf[i] / a.den0s;
+      const double fal = (double)falfs[i];
       v[3] = fal * ((double)dxd[i] * (double)ix[i] + (double)dyd[i] * (double)iy[i] +
                     (double)dzd[i] * (double)iz[i]);
     }

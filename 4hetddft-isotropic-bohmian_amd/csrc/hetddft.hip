@@ -9,10 +9,12 @@
 #include "fft_kernels.h"
 #include "field_kernels.h"
 #include "kinit.h"
+#include "stencil25.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -80,6 +82,20 @@ Stencil13 make_stencil(int order, double h) {
 
 }  // namespace
 
+// FFT kernel tables (filled in by select_fft below)
+template <class T>
+struct FftKernelsT {
+  void (*xf)(XfArgs<T>, Grid, FftPlan, const cplx<T>*, const cplx<T>*) = nullptr;
+  void (*xi)(XiArgs<T, T>, Grid, FftPlan, const cplx<T>*, const cplx<T>*) = nullptr;
+  void (*yf)(YArgs<T>, Grid, FftPlan, const cplx<T>*) = nullptr;
+  void (*yi)(YArgs<T>, Grid, FftPlan, const cplx<T>*) = nullptr;
+  void (*zc)(ZArgs<T>, Grid, FftPlan, const cplx<T>*) = nullptr;
+  int tyz = 16;        // z-pass tile width (consecutive ky)
+  size_t lds_x = 0, lds_y = 0, lds_z = 0;
+};
+using FftKernels64 = FftKernelsT<double>;
+using FftKernels32 = FftKernelsT<float>;
+
 struct hetddft_ctx {
   hetddft_params p{};
   Grid g{};
@@ -96,6 +112,8 @@ struct hetddft_ctx {
   FftPlan pl_xh{}, pl_y{}, pl_z{};
   int rows = 16, tx_y = 16, tx_z = 16, ntile_y = 1, ntile_z = 1;
   size_t lds_x = 0, lds_y = 0, lds_z = 0;
+  int zc = 16;  // z planes per workgroup of the stencil engine
+  bool st25 = true;  // z-marching stencil engine (false: direct neighbour loads)
 
   // device memory
   std::vector<void*> allocs;
@@ -114,12 +132,14 @@ struct hetddft_ctx {
   int ioldp[3] = {0, 1, 2}, ioldh[2] = {0, 1};
   // potential side (R = double or float)
   enum {
-    F_DEN, F_POT4, F_UIMP, F_DENCG, F_DELJ4, F_DENALF, F_DX, F_DY, F_DZ, F_IX, F_IY, F_IZ,
+    F_DEN, F_POT4, F_UIMP, F_DENCG, F_DELJ4, F_FALFS, F_DX, F_DY, F_DZ, F_IX, F_IY, F_IZ,
     F_DENXA, F_PSIX, F_HPSIX, F_FCX, F_FCY, F_FCZ, F_STO1, F_UACONV, F_DENXB, F_COUNT
   };
   void* fld[F_COUNT] = {};
   void* denx = nullptr;  // whichever of DENXA/DENXB holds the ironed histogram
   unsigned int* counts = nullptr;
+  int* reg = nullptr;           // occupied-region bookkeeping (field_kernels.h)
+  double* fcorner = nullptr;    // force at the 8 cell corners of each particle
   void* spec[12] = {};
   void* kfvlj4 = nullptr;
   void* kwcgk = nullptr;
@@ -143,6 +163,8 @@ struct hetddft_ctx {
   double* res = nullptr;       // device results
   double* hres = nullptr;      // pinned host copy
   double last_err[3] = {0, 0, 0};
+  FftKernels64 fft64;
+  FftKernels32 fft32;
 };
 
 namespace {
@@ -208,35 +230,79 @@ int upload_twiddles(hetddft_ctx* c, void** dst, int n, int count, int denom) {
 
 inline unsigned blocks_for(long long n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
 
+// ---- FFT kernel selection: compile-time lengths for the common grid sizes,
+// runtime radix plans for the rest of the tstgrid list.
 template <class T>
-void set_lds_attr() {
-  hipFuncSetAttribute((const void*)k_fft_xf<T, T>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  hipFuncSetAttribute((const void*)k_fft_xi<T, T>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  hipFuncSetAttribute((const void*)k_fft_y<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  hipFuncSetAttribute((const void*)k_fft_y<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  hipFuncSetAttribute((const void*)k_fft_zconv<T>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+using FftKernels = FftKernelsT<T>;
+
+template <class T, int MH>
+void pick_x(FftKernels<T>& k, int m) {
+  k.xf = k_fft_xf<T, T, MH>;
+  k.xi = k_fft_xi<T, T, MH>;
+  k.lds_x = (MH > 0 ? 1 : 2) * tile_bytes<T>(kRowsX, m);
+}
+template <class T, int N>
+void pick_y(FftKernels<T>& k, int n) {
+  k.yf = k_fft_y<T, false, N>;
+  k.yi = k_fft_y<T, true, N>;
+  k.lds_y = (N > 0 ? 1 : 2) * tile_bytes<T>(kLinesY, n);
+}
+template <class T, int N>
+void pick_z(FftKernels<T>& k, int n) {
+  constexpr int TY = tile_z<T, N>();
+  k.zc = k_fft_zconv<T, N, TY>;
+  k.tyz = TY;
+  if constexpr (N == 128) {
+    // alternative tile widths (HETDDFT_TYZ) for measurement
+    const char* e = std::getenv("HETDDFT_TYZ");
+    const int want = e ? std::atoi(e) : 0;
+    if (want == TY / 2) {
+      k.zc = k_fft_zconv<T, N, TY / 2>;
+      k.tyz = TY / 2;
+    } else if (want == TY * 2) {
+      k.zc = k_fft_zconv<T, N, TY * 2>;
+      k.tyz = TY * 2;
+    }
+  }
+  k.lds_z = (N > 0 ? 1 : 2) * tile_bytes<T>(k.tyz, n);
 }
 
-// Line tile sizes: keep each workgroup's LDS near 64 KiB so two workgroups
-// share a CU; tiles of consecutive kx give >= 13*16 B contiguous per row.
-void choose_tiles(hetddft_ctx* c) {
-  const size_t esz = 2 * c->tsz;
-  const size_t budget = 64 * 1024;
-  const int m = c->g.nx / 2;
-  int rows = 16;
-  while (rows > 1 && (size_t)2 * rows * (m + 1) * esz > budget) rows /= 2;
-  c->rows = rows;
-  c->lds_x = (size_t)2 * rows * (m + 1) * esz;
-  auto pick = [&](int n, int nbuf, int& tx, int& ntile, size_t& lds) {
-    int txmax = 16;
-    while (txmax > 1 && (size_t)nbuf * txmax * (n + 1) * esz > budget + budget / 2) --txmax;
-    const int M = c->g.mx;
-    ntile = (M + txmax - 1) / txmax;
-    tx = (M + ntile - 1) / ntile;
-    lds = (size_t)nbuf * tx * (n + 1) * esz;
-  };
-  pick(c->g.ny, 2, c->tx_y, c->ntile_y, c->lds_y);
-  pick(c->g.nz, 3, c->tx_z, c->ntile_z, c->lds_z);
+template <class T>
+FftKernels<T> select_fft(int m, int ny, int nz) {
+  FftKernels<T> k;
+#define HETD_X(M) \
+  case M: pick_x<T, M>(k, m); break;
+  switch (m) {
+    HETD_X(12) HETD_X(16) HETD_X(20) HETD_X(24) HETD_X(32) HETD_X(40) HETD_X(48) HETD_X(64) HETD_X(128)
+    default: pick_x<T, 0>(k, m); break;
+  }
+#undef HETD_X
+#define HETD_YZ(N) \
+  case N: pick_y<T, N>(k, ny); break;
+  switch (ny) {
+    HETD_YZ(24) HETD_YZ(32) HETD_YZ(48) HETD_YZ(64) HETD_YZ(80) HETD_YZ(96) HETD_YZ(128) HETD_YZ(256)
+    default: pick_y<T, 0>(k, ny); break;
+  }
+#undef HETD_YZ
+#define HETD_YZ(N) \
+  case N: pick_z<T, N>(k, nz); break;
+  switch (nz) {
+    HETD_YZ(24) HETD_YZ(32) HETD_YZ(48) HETD_YZ(64) HETD_YZ(80) HETD_YZ(96) HETD_YZ(128) HETD_YZ(256)
+    default: pick_z<T, 0>(k, nz); break;
+  }
+#undef HETD_YZ
+  return k;
+}
+
+template <class T>
+FftKernels<T>& fft_of(hetddft_ctx* c);
+template <>
+FftKernels<double>& fft_of<double>(hetddft_ctx* c) {
+  return c->fft64;
+}
+template <>
+FftKernels<float>& fft_of<float>(hetddft_ctx* c) {
+  return c->fft32;
 }
 
 template <class T>
@@ -246,7 +312,19 @@ int fft_alloc_tables(hetddft_ctx* c) {
   RET(upload_twiddles<T>(c, &c->twh, c->g.nx, m + 1, c->g.nx));
   RET(upload_twiddles<T>(c, &c->tw_y, c->g.ny, c->g.ny, c->g.ny));
   RET(upload_twiddles<T>(c, &c->tw_z, c->g.nz, c->g.nz, c->g.nz));
-  set_lds_attr<T>();
+  FftKernels<T>& k = fft_of<T>(c);
+  k = select_fft<T>(m, c->g.ny, c->g.nz);
+  c->lds_x = k.lds_x;
+  c->lds_y = k.lds_y;
+  c->lds_z = k.lds_z;
+  const int lim = 160 * 1024;
+  if (c->lds_x > (size_t)lim || c->lds_y > (size_t)lim || c->lds_z > (size_t)lim)
+    return fail(c, HETDDFT_E_ARG, "FFT tiles exceed LDS");
+  HIPCHK(c, hipFuncSetAttribute((const void*)k.xf, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  HIPCHK(c, hipFuncSetAttribute((const void*)k.xi, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  HIPCHK(c, hipFuncSetAttribute((const void*)k.yf, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  HIPCHK(c, hipFuncSetAttribute((const void*)k.yi, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  HIPCHK(c, hipFuncSetAttribute((const void*)k.zc, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   return HETDDFT_OK;
 }
 
@@ -256,66 +334,71 @@ struct ConvJob {
   int input;
   const void* kern;
   double sign;
-  void* out;  // real field R*
+  void* out;        // real field T*
+  int epi = XI_STORE;
+  double ec = 0.0;
 };
 
+// Five passes over HBM (see fft_kernels.h): inputs i < nin are formed by
+// the x-pass producers in xf_in; every job reads one input spectrum.
 template <class T>
 int conv_batch(hetddft_ctx* c, const XfArgs<T>& xf_in, int nin, const std::vector<ConvJob<T>>& jobs) {
   const Grid& g = c->g;
+  const int nout = (int)jobs.size();
+  if (nin > 5 || nout > 7 || nin < 1 || nout < 1) return fail(c, HETDDFT_E_ARG, "conv_batch: %d in / %d out", nin, nout);
   XfArgs<T> xa = xf_in;
   for (int i = 0; i < nin; ++i) xa.out[i] = c->spec[i];
-  const long long nrows = (long long)g.ny * g.nz;
-  dim3 gx((unsigned)((nrows + c->rows - 1) / c->rows), (unsigned)nin);
-  hipLaunchKernelGGL((k_fft_xf<T, T>), gx, dim3(kBlock), c->lds_x, c->st, xa, g, c->pl_xh, c->rows,
-                     (const cplx<T>*)c->tw_xh, (const cplx<T>*)c->twh);
+  FftKernels<T>& K = fft_of<T>(c);
+  const dim3 gx((unsigned)((g.ny + kRowsX - 1) / kRowsX), (unsigned)g.nz, (unsigned)nin);
+  const unsigned ylines = (unsigned)(((long long)g.mx * g.nz + kLinesY - 1) / kLinesY);
+  K.xf<<<gx, dim3(kThreads), c->lds_x, c->st>>>(xa, g, c->pl_xh, (const cplx<T>*)c->tw_xh, (const cplx<T>*)c->twh);
   LAUNCHCHK(c);
   YArgs<T> ya{};
   for (int i = 0; i < nin; ++i) ya.buf[i] = c->spec[i];
-  hipLaunchKernelGGL((k_fft_y<T, false>), dim3(c->ntile_y, g.nz, nin), dim3(kBlock), c->lds_y, c->st, ya, g,
-                     c->pl_y, c->tx_y, (const cplx<T>*)c->tw_y);
+  K.yf<<<dim3(ylines, 1, nin), dim3(kThreads), c->lds_y, c->st>>>(ya, g, c->pl_y, (const cplx<T>*)c->tw_y);
   LAUNCHCHK(c);
-  // z pass: jobs grouped by input
   ZArgs<T> za{};
-  int nout = 0;
-  for (int i = 0; i < nin; ++i) {
-    za.in[i] = c->spec[i];
-    za.job0[i] = nout;
-    for (const auto& j : jobs)
-      if (j.input == i) {
-        if (nout >= 7) return fail(c, HETDDFT_E_ARG, "conv_batch: more than 7 outputs");
-        za.kern[nout] = (const T*)j.kern;
-        za.ksign[nout] = (T)j.sign;
-        za.out[nout] = c->spec[5 + nout];
-        ++nout;
-      }
-  }
-  za.job0[nin] = nout;
-  hipLaunchKernelGGL((k_fft_zconv<T>), dim3(c->ntile_z, g.ny, nin), dim3(kBlock), c->lds_z, c->st, za, g,
-                     c->pl_z, c->tx_z, (const cplx<T>*)c->tw_z);
-  LAUNCHCHK(c);
-  YArgs<T> yb{};
-  for (int k = 0; k < nout; ++k) yb.buf[k] = c->spec[5 + k];
-  hipLaunchKernelGGL((k_fft_y<T, true>), dim3(c->ntile_y, g.nz, nout), dim3(kBlock), c->lds_y, c->st, yb, g,
-                     c->pl_y, c->tx_y, (const cplx<T>*)c->tw_y);
-  LAUNCHCHK(c);
-  // outputs in job order of the z pass
   XiArgs<T, T> xi{};
+  YArgs<T> yb{};
   xi.renor = c->renor;
-  {
-    int k = 0;
-    for (int i = 0; i < nin; ++i)
-      for (const auto& j : jobs)
-        if (j.input == i) {
-          xi.in[k] = c->spec[5 + k];
-          xi.out[k] = (T*)j.out;
-          ++k;
-        }
+  for (int k = 0; k < nout; ++k) {
+    const ConvJob<T>& j = jobs[k];
+    za.in[k] = c->spec[j.input];
+    za.kern[k] = (const T*)j.kern;
+    za.ksign[k] = (T)j.sign;
+    za.out[k] = c->spec[5 + k];
+    yb.buf[k] = c->spec[5 + k];
+    xi.in[k] = c->spec[5 + k];
+    xi.out[k] = (T*)j.out;
+    xi.epi[k] = j.epi;
+    xi.ec[k] = j.ec;
   }
-  dim3 gxi((unsigned)((nrows + c->rows - 1) / c->rows), (unsigned)nout);
-  hipLaunchKernelGGL((k_fft_xi<T, T>), gxi, dim3(kBlock), c->lds_x, c->st, xi, g, c->pl_xh, c->rows,
-                     (const cplx<T>*)c->tw_xh, (const cplx<T>*)c->twh);
+  K.zc<<<dim3((g.ny + K.tyz - 1) / K.tyz, g.mx, nout), dim3(kThreads), c->lds_z, c->st>>>(
+      za, g, c->pl_z, (const cplx<T>*)c->tw_z);
+  LAUNCHCHK(c);
+  K.yi<<<dim3(ylines, 1, nout), dim3(kThreads), c->lds_y, c->st>>>(yb, g, c->pl_y, (const cplx<T>*)c->tw_y);
+  LAUNCHCHK(c);
+  const dim3 gxi((unsigned)((g.ny + kRowsX - 1) / kRowsX), (unsigned)g.nz, (unsigned)nout);
+  K.xi<<<gxi, dim3(kThreads), c->lds_x, c->st>>>(xi, g, c->pl_xh, (const cplx<T>*)c->tw_xh, (const cplx<T>*)c->twh);
   LAUNCHCHK(c);
   return HETDDFT_OK;
+}
+
+HamArgs ham_direct(const Ham25& h) {
+  HamArgs a{};
+  a.g = h.g;
+  a.d2x = h.d2x;
+  a.d2y = h.d2y;
+  a.d2z = h.d2z;
+  a.h2o2m4 = h.h2o2m4;
+  a.timec_z = h.timec_z;
+  return a;
+}
+
+// Workgroups of the z-marching stencil engine: S_TX x S_TY columns, c->zc planes.
+dim3 grid25(const hetddft_ctx* c) {
+  return dim3((unsigned)((c->g.nx + S_TX - 1) / S_TX), (unsigned)((c->g.ny + S_TY - 1) / S_TY),
+              (unsigned)((c->g.nz + c->zc - 1) / c->zc));
 }
 
 BoxArgs box_args(const hetddft_ctx* c) {
@@ -331,68 +414,74 @@ BoxArgs box_args(const hetddft_ctx* c) {
   return b;
 }
 
-template <class T>
-__global__ void k_fill_u32(unsigned int* p, long long n) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0u;
-}
+// Grid-stride launches over a device-side region: enough blocks to cover a
+// 40^3 box in one sweep; larger regions loop.
+constexpr unsigned kRegionBlocks = 256;
 
 // potenimp(); poten(); forceimp()
 template <class T>
 int veval(hetddft_ctx* c) {
   const Grid& g = c->g;
   auto F = [&](int id) { return (T*)c->fld[id]; };
-  // ---- potenimp: denpart + uimp (potenimp.f90:1-20, denpart.f90:5-49)
-  hipLaunchKernelGGL((k_fill_u32<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, c->counts, g.n);
-  LAUNCHCHK(c);
+  const bool otc = c->p.core_otc != 0;
+  // ---- potenimp: denpart (denpart.f90:5-49) on the occupied region only
   BoxArgs b = box_args(c);
+  hipLaunchKernelGGL(k_clear_counts, dim3(kRegionBlocks), dim3(kBlock), 0, c->st, g, c->reg, c->counts);
+  LAUNCHCHK(c);
   if (c->p.n_par > 0) {
-    hipLaunchKernelGGL(k_hist, dim3(blocks_for(c->p.n_par)), dim3(kBlock), 0, c->st, b, c->rimp, c->counts);
+    hipLaunchKernelGGL(k_hist, dim3(blocks_for(c->p.n_par)), dim3(kBlock), 0, c->st, b, c->rimp, c->counts, c->reg);
     LAUNCHCHK(c);
   }
+  hipLaunchKernelGGL(k_reg_union, dim3(1), dim3(64), 0, c->st, g, c->reg, c->p.ironx + 6);
+  LAUNCHCHK(c);
   const double hxyz = c->hx * c->hy * c->hz;
   const double npar = (double)c->p.n_par;
   T* cur = F(hetddft_ctx::F_DENXA);
   T* oth = F(hetddft_ctx::F_DENXB);
-  hipLaunchKernelGGL((k_iron<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, g, c->counts, (const T*)nullptr,
-                     cur, npar, hxyz, c->p.ironx > 0 ? 1 : 0);
+  T* psix = F(hetddft_ctx::F_PSIX);
+  const int npass = c->p.ironx > 0 ? c->p.ironx : 1;
+  hipLaunchKernelGGL((k_iron<T>), dim3(kRegionBlocks), dim3(kBlock), 0, c->st, g, (const int*)c->reg, c->counts,
+                     (const T*)nullptr, cur, npass == 1 ? psix : (T*)nullptr, npar, hxyz, c->p.ironx > 0 ? 1 : 0);
   LAUNCHCHK(c);
-  for (int it = 1; it < c->p.ironx; ++it) {
-    hipLaunchKernelGGL((k_iron<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, g,
-                       (const unsigned int*)nullptr, (const T*)cur, oth, npar, hxyz, 1);
+  for (int it = 1; it < npass; ++it) {
+    hipLaunchKernelGGL((k_iron<T>), dim3(kRegionBlocks), dim3(kBlock), 0, c->st, g, (const int*)c->reg,
+                       (const unsigned int*)nullptr, (const T*)cur, oth, it == npass - 1 ? psix : (T*)nullptr, npar,
+                       hxyz, 1);
     LAUNCHCHK(c);
     std::swap(cur, oth);
   }
   c->denx = cur;
-  hipLaunchKernelGGL((k_psix<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, g, (const T*)cur, c->d2x, c->d2y,
-                     c->d2z, c->h2o2mx, F(hetddft_ctx::F_PSIX), F(hetddft_ctx::F_HPSIX));
+  hipLaunchKernelGGL((k_hpsix<T>), dim3(kRegionBlocks), dim3(kBlock), 0, c->st, g, (const int*)c->reg,
+                     (const T*)psix, c->d2x, c->d2y, c->d2z, c->h2o2mx, F(hetddft_ctx::F_HPSIX));
   LAUNCHCHK(c);
   // ---- poten: derden (derden.f90:26-28)
-  hipLaunchKernelGGL((k_grad3<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, g,
-                     (const T*)F(hetddft_ctx::F_DEN), c->d1x, c->d1y, c->d1z, F(hetddft_ctx::F_DX),
-                     F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ));
+  if (c->st25)
+    hipLaunchKernelGGL((k_grad3_25<T>), grid25(c), dim3(S_TX, S_TY), 0, c->st, g, c->zc,
+                       (const T*)F(hetddft_ctx::F_DEN), c->d1x, c->d1y, c->d1z, F(hetddft_ctx::F_DX),
+                       F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ));
+  else
+    hipLaunchKernelGGL((k_grad3<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, g,
+                       (const T*)F(hetddft_ctx::F_DEN), c->d1x, c->d1y, c->d1z, F(hetddft_ctx::F_DX),
+                       F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ));
   LAUNCHCHK(c);
-  const bool otc = c->p.core_otc != 0;
-  // batch 1: denx*Vq -> uimp; den*{wcgk,fvlj4,kalfs} -> dencg, delj4, denalf;
-  //          -grad(den)*Vq -> force convolutions (potenimp.f90:15-17,39-50; poten.f90:27-50,94-99)
+  // batch 1: denx*Vq -> uimp (potenimp.f90:15-17); den*{wcgk, fvlj4, kalfs}
+  //          -> dencg, delj4, falfs = 1 - denalf/den0s (poten.f90:27-50,94-99;
+  //          term_alfa.f90:28-30)
   {
     XfArgs<T> xa{};
-    const int ids[5] = {-1, hetddft_ctx::F_DEN, hetddft_ctx::F_DX, hetddft_ctx::F_DY, hetddft_ctx::F_DZ};
-    for (int i = 0; i < 5; ++i) {
-      xa.op[i] = XF_COPY;
-      xa.in[i][0] = i == 0 ? (const T*)cur : (const T*)F(ids[i]);
-    }
+    xa.op[0] = XF_COPY;
+    xa.in[0][0] = cur;
+    xa.op[1] = XF_COPY;
+    xa.in[1][0] = F(hetddft_ctx::F_DEN);
     std::vector<ConvJob<T>> jobs;
     jobs.push_back({0, c->kvq, 1.0, F(hetddft_ctx::F_UIMP)});
     jobs.push_back({1, c->kwcgk, 1.0, F(hetddft_ctx::F_DENCG)});
     jobs.push_back({1, c->kfvlj4, 1.0, F(hetddft_ctx::F_DELJ4)});
-    if (otc) jobs.push_back({1, c->kkalfs, 1.0, F(hetddft_ctx::F_DENALF)});
-    jobs.push_back({2, c->kvq, -1.0, F(hetddft_ctx::F_FCX)});
-    jobs.push_back({3, c->kvq, -1.0, F(hetddft_ctx::F_FCY)});
-    jobs.push_back({4, c->kvq, -1.0, F(hetddft_ctx::F_FCZ)});
-    RET(conv_batch<T>(c, xa, 5, jobs));
+    if (otc) jobs.push_back({1, c->kkalfs, 1.0, F(hetddft_ctx::F_FALFS), XI_ONE_MINUS_DIV, c->p.den0s});
+    RET(conv_batch<T>(c, xa, 2, jobs));
   }
-  // batch 2: correlation term (poten.f90:62-74); alpha_s integrals (term_alfa.f90:28-52)
+  // batch 2: correlation term (poten.f90:62-74); alpha_s integrals
+  //          int_c = G*(falfs d_c den) (term_alfa.f90:32-52)
   {
     XfArgs<T> xa{};
     xa.op[0] = XF_CORREL;
@@ -407,29 +496,41 @@ int veval(hetddft_ctx* c) {
       const int d[3] = {hetddft_ctx::F_DX, hetddft_ctx::F_DY, hetddft_ctx::F_DZ};
       const int o[3] = {hetddft_ctx::F_IX, hetddft_ctx::F_IY, hetddft_ctx::F_IZ};
       for (int k = 0; k < 3; ++k) {
-        xa.op[1 + k] = XF_FALFS_D;
-        xa.in[1 + k][0] = F(hetddft_ctx::F_DENALF);
+        xa.op[1 + k] = XF_MUL;
+        xa.in[1 + k][0] = F(hetddft_ctx::F_FALFS);
         xa.in[1 + k][1] = F(d[k]);
-        xa.c[1 + k][0] = c->p.den0s;
         jobs.push_back({1 + k, c->kkalfs, 1.0, F(o[k])});
       }
       nin = 4;
     }
     RET(conv_batch<T>(c, xa, nin, jobs));
   }
-  if (otc) {
-    // batch 3: grad(den).int -> ualphas convolution (term_alfa.f90:65-80)
+  // batch 3: grad(den).int -> ualphas convolution (term_alfa.f90:65-80);
+  //          -Vq*FFT(d_c den) -> force convolutions (potenimp.f90:39-50)
+  {
     XfArgs<T> xa{};
-    xa.op[0] = XF_GRAD_DOT;
-    xa.in[0][0] = F(hetddft_ctx::F_DX);
-    xa.in[0][1] = F(hetddft_ctx::F_DY);
-    xa.in[0][2] = F(hetddft_ctx::F_DZ);
-    xa.in[0][3] = F(hetddft_ctx::F_IX);
-    xa.in[0][4] = F(hetddft_ctx::F_IY);
-    xa.in[0][5] = F(hetddft_ctx::F_IZ);
     std::vector<ConvJob<T>> jobs;
-    jobs.push_back({0, c->kkalfs, 1.0, F(hetddft_ctx::F_UACONV)});
-    RET(conv_batch<T>(c, xa, 1, jobs));
+    int nin = 0;
+    if (otc) {
+      xa.op[0] = XF_GRAD_DOT;
+      xa.in[0][0] = F(hetddft_ctx::F_DX);
+      xa.in[0][1] = F(hetddft_ctx::F_DY);
+      xa.in[0][2] = F(hetddft_ctx::F_DZ);
+      xa.in[0][3] = F(hetddft_ctx::F_IX);
+      xa.in[0][4] = F(hetddft_ctx::F_IY);
+      xa.in[0][5] = F(hetddft_ctx::F_IZ);
+      jobs.push_back({0, c->kkalfs, 1.0, F(hetddft_ctx::F_UACONV)});
+      nin = 1;
+    }
+    const int d[3] = {hetddft_ctx::F_DX, hetddft_ctx::F_DY, hetddft_ctx::F_DZ};
+    const int o[3] = {hetddft_ctx::F_FCX, hetddft_ctx::F_FCY, hetddft_ctx::F_FCZ};
+    for (int k = 0; k < 3; ++k) {
+      xa.op[nin] = XF_COPY;
+      xa.in[nin][0] = F(d[k]);
+      jobs.push_back({nin, c->kvq, -1.0, F(o[k])});
+      ++nin;
+    }
+    RET(conv_batch<T>(c, xa, nin, jobs));
   }
   // pot4 assembly with the alpha_s divergence (term_alfa.f90:85-113, poten.f90:80-124)
   PotArgs pa{};
@@ -446,12 +547,36 @@ int veval(hetddft_ctx* c) {
   pa.beta = c->p.beta;
   pa.den_m = c->p.den_m;
   pa.otc = otc ? 1 : 0;
-  hipLaunchKernelGGL((k_pot_final<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, pa,
-                     (const T*)F(hetddft_ctx::F_DEN), (const T*)F(hetddft_ctx::F_DENCG),
-                     (const T*)F(hetddft_ctx::F_DELJ4), (const T*)F(hetddft_ctx::F_STO1),
-                     (const T*)F(hetddft_ctx::F_UIMP), (const T*)F(hetddft_ctx::F_DENALF),
-                     (const T*)F(hetddft_ctx::F_IX), (const T*)F(hetddft_ctx::F_IY),
-                     (const T*)F(hetddft_ctx::F_IZ), (const T*)F(hetddft_ctx::F_UACONV), F(hetddft_ctx::F_POT4));
+  if (c->st25) {
+    Pot25 p5{};
+    p5.g = g;
+    p5.d1x = c->d1x;
+    p5.d1y = c->d1y;
+    p5.d1z = c->d1z;
+    p5.a0 = pa.a0;
+    p5.a1 = pa.a1;
+    p5.den0s = pa.den0s;
+    p5.h2o2m4 = pa.h2o2m4;
+    p5.alphas = pa.alphas;
+    p5.C = pa.C;
+    p5.beta = pa.beta;
+    p5.den_m = pa.den_m;
+    p5.otc = pa.otc;
+    p5.zc = c->zc;
+    hipLaunchKernelGGL((k_pot_final25<T>), grid25(c), dim3(S_TX, S_TY), 0, c->st, p5,
+                       (const T*)F(hetddft_ctx::F_DEN), (const T*)F(hetddft_ctx::F_DENCG),
+                       (const T*)F(hetddft_ctx::F_DELJ4), (const T*)F(hetddft_ctx::F_STO1),
+                       (const T*)F(hetddft_ctx::F_UIMP), (const T*)F(hetddft_ctx::F_FALFS),
+                       (const T*)F(hetddft_ctx::F_IX), (const T*)F(hetddft_ctx::F_IY),
+                       (const T*)F(hetddft_ctx::F_IZ), (const T*)F(hetddft_ctx::F_UACONV), F(hetddft_ctx::F_POT4));
+  } else {
+    hipLaunchKernelGGL((k_pot_final<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, pa,
+                       (const T*)F(hetddft_ctx::F_DEN), (const T*)F(hetddft_ctx::F_DENCG),
+                       (const T*)F(hetddft_ctx::F_DELJ4), (const T*)F(hetddft_ctx::F_STO1),
+                       (const T*)F(hetddft_ctx::F_UIMP), (const T*)F(hetddft_ctx::F_FALFS),
+                       (const T*)F(hetddft_ctx::F_IX), (const T*)F(hetddft_ctx::F_IY),
+                       (const T*)F(hetddft_ctx::F_IZ), (const T*)F(hetddft_ctx::F_UACONV), F(hetddft_ctx::F_POT4));
+  }
   LAUNCHCHK(c);
   // ---- forceimp gather (potenimp.f90:52-71)
   if (c->p.n_par > 0) {
@@ -461,10 +586,13 @@ int veval(hetddft_ctx* c) {
     ga.d1[1] = c->d1y;
     ga.d1[2] = c->d1z;
     ga.m_imp = c->m_imp;
-    hipLaunchKernelGGL((k_gather<T>), dim3(blocks_for(c->p.n_par)), dim3(kBlock), 0, c->st, ga, c->rimp,
-                       (const T*)F(hetddft_ctx::F_FCX), (const T*)F(hetddft_ctx::F_FCY),
-                       (const T*)F(hetddft_ctx::F_FCZ), (const T*)cur, (const T*)F(hetddft_ctx::F_PSIX),
-                       (const T*)F(hetddft_ctx::F_HPSIX), c->aimp);
+    hipLaunchKernelGGL((k_force_corners<T>), dim3(blocks_for(24LL * c->p.n_par)), dim3(kBlock), 0, c->st, ga,
+                       c->rimp, (const T*)F(hetddft_ctx::F_FCX), (const T*)F(hetddft_ctx::F_FCY),
+                       (const T*)F(hetddft_ctx::F_FCZ), (const T*)cur, (const T*)psix,
+                       (const T*)F(hetddft_ctx::F_HPSIX), c->fcorner);
+    LAUNCHCHK(c);
+    hipLaunchKernelGGL(k_blend, dim3(blocks_for(3LL * c->p.n_par)), dim3(kBlock), 0, c->st, b, c->rimp,
+                       (const double*)c->fcorner, c->aimp);
     LAUNCHCHK(c);
   }
   return HETDDFT_OK;
@@ -472,14 +600,15 @@ int veval(hetddft_ctx* c) {
 
 int run_veval(hetddft_ctx* c) { return c->mixed ? veval<float>(c) : veval<double>(c); }
 
-HamArgs ham_args(const hetddft_ctx* c) {
-  HamArgs a{};
+Ham25 ham_args(const hetddft_ctx* c) {
+  Ham25 a{};
   a.g = c->g;
   a.d2x = c->d2x;
   a.d2y = c->d2y;
   a.d2z = c->d2z;
   a.h2o2m4 = c->p.h2o2m4;
   a.timec_z = c->timec;
+  a.zc = c->zc;
   return a;
 }
 
@@ -490,7 +619,7 @@ int step_rk_t(hetddft_ctx* c, double dt) {
   const double brun[4] = {2.0, 1.0, 1.0, 2.0};
   const double crun[4] = {0.5, 1.0 - 1.0 / s2, 1.0 + 1.0 / s2, 0.5};
   const Grid& g = c->g;
-  const HamArgs ha = ham_args(c);
+  const Ham25 ha = ham_args(c);
   const int n3 = 3 * c->p.n_par;
   for (int j = 0; j < 4; ++j) {
     c64* out;
@@ -524,9 +653,14 @@ int step_rk_t(hetddft_ctx* c, double dt) {
     } else {
       out = c->spare;
     }
-    hipLaunchKernelGGL((k_rk_stage<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, ha, (const c64*)c->psi,
-                       (const T*)c->fld[hetddft_ctx::F_POT4], c->q, out, (T*)c->fld[hetddft_ctx::F_DEN], h1o,
-                       arun[j], brun[j], crun[j], dt);
+    if (c->st25)
+      hipLaunchKernelGGL((k_rk_stage25<T>), grid25(c), dim3(S_TX, S_TY), 0, c->st, ha, (const c64*)c->psi,
+                         (const T*)c->fld[hetddft_ctx::F_POT4], c->q, out, (T*)c->fld[hetddft_ctx::F_DEN], h1o,
+                         arun[j], brun[j], crun[j], dt);
+    else
+      hipLaunchKernelGGL((k_rk_stage<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, ham_direct(ha),
+                         (const c64*)c->psi, (const T*)c->fld[hetddft_ctx::F_POT4], c->q, out,
+                         (T*)c->fld[hetddft_ctx::F_DEN], h1o, arun[j], brun[j], crun[j], dt);
     LAUNCHCHK(c);
     if (j == 0) {
       c->psi = out;
@@ -551,15 +685,20 @@ template <class T>
 int step_pc_t(hetddft_ctx* c, double dt) {
   const double c112 = 112.0 / 121.0, c9 = 9.0 / 121.0, c4o3 = 4.0 / 3.0;
   const Grid& g = c->g;
-  const HamArgs ha = ham_args(c);
+  const Ham25 ha = ham_args(c);
   int* P = c->ioldp;
   int* H = c->ioldh;
   // predictor + modifier (steppc.f90:36-62)
   c64* x = c->psi;
   c64* y = c->old[P[2]];
-  hipLaunchKernelGGL((k_pc_predict<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, ha, (const c64*)x,
-                     (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]], c->hold[H[1]], c->pc,
-                     (T*)c->fld[hetddft_ctx::F_DEN], c4o3 * dt, c112);
+  if (c->st25)
+    hipLaunchKernelGGL((k_pc_predict25<T>), grid25(c), dim3(S_TX, S_TY), 0, c->st, ha, (const c64*)x,
+                       (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]], c->hold[H[1]], c->pc,
+                       (T*)c->fld[hetddft_ctx::F_DEN], c4o3 * dt, c112);
+  else
+    hipLaunchKernelGGL((k_pc_predict<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, ham_direct(ha),
+                       (const c64*)x, (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]],
+                       c->hold[H[1]], c->pc, (T*)c->fld[hetddft_ctx::F_DEN], c4o3 * dt, c112);
   LAUNCHCHK(c);
   c->old[P[2]] = x;
   c->psi = y;
@@ -578,11 +717,18 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   std::swap(A[0], A[1]);
   RET(veval<T>(c));
   // corrector (steppc.f90:97-151)
-  const unsigned nb = blocks_for(g.n);
-  hipLaunchKernelGGL((k_pc_correct<T>), dim3(nb), dim3(kBlock), 0, c->st, ha, (const c64*)c->psi,
-                     (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
-                     (const c64*)c->hold[H[0]], (const c64*)c->hold[H[1]], c->pc, c->spare,
-                     (T*)c->fld[hetddft_ctx::F_DEN], c->part, 3.0 * dt, c9);
+  const dim3 g25 = grid25(c);
+  const unsigned nb = c->st25 ? g25.x * g25.y * g25.z : blocks_for(g.n);
+  if (c->st25)
+    hipLaunchKernelGGL((k_pc_correct25<T>), g25, dim3(S_TX, S_TY), 0, c->st, ha, (const c64*)c->psi,
+                       (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
+                       (const c64*)c->hold[H[0]], (const c64*)c->hold[H[1]], c->pc, c->spare,
+                       (T*)c->fld[hetddft_ctx::F_DEN], c->part, 3.0 * dt, c9);
+  else
+    hipLaunchKernelGGL((k_pc_correct<T>), dim3(nb), dim3(kBlock), 0, c->st, ham_direct(ha), (const c64*)c->psi,
+                       (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
+                       (const c64*)c->hold[H[0]], (const c64*)c->hold[H[1]], c->pc, c->spare,
+                       (T*)c->fld[hetddft_ctx::F_DEN], c->part, 3.0 * dt, c9);
   LAUNCHCHK(c);
   std::swap(c->psi, c->spare);
   hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(kBlock), 0, c->st, (const double*)c->part, (int)nb, 1, c->res);
@@ -661,6 +807,18 @@ int upload_real(hetddft_ctx* c, void* dst, const double* src, size_t n) {
   return HETDDFT_OK;
 }
 
+// k-space kernel from the Fortran (mx,ny,nz) layout into the internal
+// spectrum order [kz][kx][ky]
+int upload_kernel(hetddft_ctx* c, void* dst, const double* src) {
+  const Grid& g = c->g;
+  std::vector<double> t((size_t)g.h);
+  for (int iz = 0; iz < g.nz; ++iz)
+    for (int iy = 0; iy < g.ny; ++iy)
+      for (int ix = 0; ix < g.mx; ++ix)
+        t[((size_t)iz * g.mx + ix) * g.ny + iy] = src[((size_t)iz * g.ny + iy) * g.mx + ix];
+  return upload_real(c, dst, t.data(), t.size());
+}
+
 template <class T>
 int energy_t(hetddft_ctx* c, double out[12]) {
   const Grid& g = c->g;
@@ -678,7 +836,7 @@ int energy_t(hetddft_ctx* c, double out[12]) {
   ea.otc = c->p.core_otc;
   const unsigned nb = blocks_for(g.n);
   hipLaunchKernelGGL((k_energy<T>), dim3(nb), dim3(kBlock), 0, c->st, ea, (const c64*)c->psi, F(hetddft_ctx::F_DEN),
-                     F(hetddft_ctx::F_DENCG), F(hetddft_ctx::F_DELJ4), F(hetddft_ctx::F_DENALF),
+                     F(hetddft_ctx::F_DENCG), F(hetddft_ctx::F_DELJ4), F(hetddft_ctx::F_FALFS),
                      F(hetddft_ctx::F_DX), F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ), F(hetddft_ctx::F_IX),
                      F(hetddft_ctx::F_IY), F(hetddft_ctx::F_IZ), F(hetddft_ctx::F_PSIX),
                      F(hetddft_ctx::F_HPSIX), F(hetddft_ctx::F_UIMP), c->part);
@@ -793,6 +951,7 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
   g.nz = p->nz;
   g.mx = p->nx / 2 + 1;
   g.n = (long long)p->nx * p->ny * p->nz;
+  g.mp = g.mx;
   g.h = (long long)g.mx * p->ny * p->nz;
   c->hx = 2.0 * std::fabs(p->xmax) / p->nx;
   c->hy = 2.0 * std::fabs(p->ymax) / p->ny;
@@ -807,6 +966,16 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
   c->d2x = make_stencil(2, c->hx);
   c->d2y = make_stencil(2, c->hy);
   c->d2z = make_stencil(2, c->hz);
+  {
+    // z chunk of the stencil engine: about 512 workgroups, 4..32 planes each
+    const long long cols = (long long)((p->nx + S_TX - 1) / S_TX) * ((p->ny + S_TY - 1) / S_TY);
+    long long zc = cols * p->nz / 1024;
+    c->zc = (int)std::max(4LL, std::min(32LL, zc));
+    const char* e = std::getenv("HETDDFT_STENCIL");
+    if (e && std::strcmp(e, "direct") == 0) c->st25 = false;
+    const char* z = std::getenv("HETDDFT_ZC");
+    if (z && std::atoi(z) > 0) c->zc = std::atoi(z);
+  }
   c->pl_xh = make_plan(p->nx / 2);
   c->pl_y = make_plan(p->ny);
   c->pl_z = make_plan(p->nz);
@@ -818,7 +987,6 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
     fail(c, HETDDFT_E_HIP, "hipStreamCreate failed");
     return bad(HETDDFT_E_HIP);
   }
-  choose_tiles(c);
   int rc = c->mixed ? fft_alloc_tables<float>(c) : fft_alloc_tables<double>(c);
   if (rc) return bad(rc);
   const size_t n = (size_t)g.n, h = (size_t)g.h;
@@ -832,6 +1000,8 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
   want.push_back({(void**)&c->q, n * 16});
   for (int k = 0; k < hetddft_ctx::F_COUNT; ++k) want.push_back({&c->fld[k], n * c->tsz});
   want.push_back({(void**)&c->counts, n * 4});
+  want.push_back({(void**)&c->reg, kRegLen * sizeof(int)});
+  want.push_back({(void**)&c->fcorner, (size_t)24 * std::max(p->n_par, 1) * 8});
   for (int k = 0; k < 12; ++k) want.push_back({&c->spec[k], h * 2 * c->tsz});
   want.push_back({&c->kfvlj4, h * c->tsz});
   want.push_back({&c->kwcgk, h * c->tsz});
@@ -857,6 +1027,15 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
   }
   std::memset(c->hres, 0, 64 * sizeof(double));
   c->denx = c->fld[hetddft_ctx::F_DENXA];
+  {
+    // occupied region: nothing live yet
+    const int r0[kRegLen] = {0x7fffffff, 0x7fffffff, 0x7fffffff, -1, -1, -1, 0x7fffffff, 0x7fffffff, 0x7fffffff,
+                             -1, -1, -1, 0x7fffffff, 0x7fffffff, 0x7fffffff, -1, -1, -1};
+    if (hipMemcpy(c->reg, r0, sizeof r0, hipMemcpyHostToDevice) != hipSuccess) {
+      fail(c, HETDDFT_E_HIP, "region init failed");
+      return bad(HETDDFT_E_HIP);
+    }
+  }
   // timec = cmplx(Lambdah*(1+tanh((z-tzmean)/tzsurf)), 1) with the real part
   // rounded to REAL(4) by the KIND-less cmplx (DFT4He3d.f90:773-782)
   std::vector<c64> tc(g.nz);
@@ -949,10 +1128,11 @@ int hetddft_set_kernels(hetddft_ctx* c, const double* fvlj4, const double* wcgk,
                         const double* vq) {
   if (!c || !fvlj4 || !wcgk || !kalfs || !vq) return fail(c, HETDDFT_E_ARG, "null kernel array");
   const size_t h = (size_t)c->g.h;
-  RET(upload_real(c, c->kfvlj4, fvlj4, h));
-  RET(upload_real(c, c->kwcgk, wcgk, h));
-  RET(upload_real(c, c->kkalfs, kalfs, h));
-  RET(upload_real(c, c->kvq, vq, h));
+  (void)h;
+  RET(upload_kernel(c, c->kfvlj4, fvlj4));
+  RET(upload_kernel(c, c->kwcgk, wcgk));
+  RET(upload_kernel(c, c->kkalfs, kalfs));
+  RET(upload_kernel(c, c->kvq, vq));
   c->kernels_set = true;
   return HETDDFT_OK;
 }
@@ -1083,7 +1263,7 @@ int hetddft_get_state(hetddft_ctx* c, double* psi, double* rimp, double* vimp, d
 int hetddft_get_field(hetddft_ctx* c, int which, double* out) {
   if (!c || !out) return HETDDFT_E_ARG;
   static const int map[20] = {hetddft_ctx::F_DEN,    hetddft_ctx::F_POT4,  hetddft_ctx::F_UIMP,
-                              hetddft_ctx::F_DENCG,  hetddft_ctx::F_DELJ4, hetddft_ctx::F_DENALF,
+                              hetddft_ctx::F_DENCG,  hetddft_ctx::F_DELJ4, hetddft_ctx::F_FALFS,
                               hetddft_ctx::F_DX,     hetddft_ctx::F_DY,    hetddft_ctx::F_DZ,
                               hetddft_ctx::F_IX,     hetddft_ctx::F_IY,    hetddft_ctx::F_IZ,
                               -1,                    hetddft_ctx::F_PSIX,  hetddft_ctx::F_HPSIX,
@@ -1103,7 +1283,7 @@ int hetddft_convolve(hetddft_ctx* c, int nf, const double* const* in, const doub
   const int dst[5] = {hetddft_ctx::F_IX, hetddft_ctx::F_IY, hetddft_ctx::F_IZ, hetddft_ctx::F_DENCG,
                       hetddft_ctx::F_DELJ4};
   for (int i = 0; i < nf; ++i) RET(upload_real(c, c->fld[stage[i]], in[i], n));
-  RET(upload_real(c, c->kkalfs, kern, (size_t)c->g.h));
+  RET(upload_kernel(c, c->kkalfs, kern));
   c->kernels_set = false;
   auto run = [&](auto tag) -> int {
     using T = decltype(tag);

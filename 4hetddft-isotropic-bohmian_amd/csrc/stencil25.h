@@ -1,0 +1,432 @@
+// z-marching ("2.5D") engine for the 13-point star stencils of derivnD
+// (icon=13, zero outside the box: init_deriv_parallel.f90:428-535).
+//
+// A workgroup of S_TX x S_TY threads owns one (x, y) column tile and walks
+// a chunk of z planes.  Each thread keeps the 13 z-neighbours of its column
+// in a register queue; the x and y neighbours come from an LDS copy of the
+// current plane with a 6-point halo.  Per output point this costs one fresh
+// HBM read plus the halo reads, instead of 37 scattered loads.
+//
+// Sums run over the offsets -6..6 in order with zeros outside the box, the
+// same sequence of operations as the direct form (d13/d13c).
+#pragma once
+#include "common.h"
+
+namespace hetd {
+
+constexpr int S_TX = 64;
+constexpr int S_TY = 4;
+constexpr int S_LX = S_TX + 12;
+constexpr int S_LY = S_TY + 12;
+constexpr int S_THREADS = S_TX * S_TY;
+
+__device__ inline double vzero(double) { return 0.0; }
+__device__ inline c64 vzero(c64) { return mk(0.0, 0.0); }
+__device__ inline double vmadd(double acc, double w, double v) { return acc + w * v; }
+__device__ inline c64 vmadd(c64 acc, double w, c64 v) { return mk(acc.x + w * v.x, acc.y + w * v.y); }
+
+template <class V, class R>
+__device__ inline V vload(const R* f, long long i) {
+  return (V)f[i];
+}
+template <>
+__device__ inline c64 vload<c64, c64>(const c64* f, long long i) {
+  return f[i];
+}
+
+struct Col {
+  int x, y, tx, ty, tid;
+  bool in;
+};
+
+__device__ inline Col col_of(const Grid& g) {
+  Col c;
+  c.tx = threadIdx.x;
+  c.ty = threadIdx.y;
+  c.tid = c.tx + S_TX * c.ty;
+  c.x = blockIdx.x * S_TX + c.tx;
+  c.y = blockIdx.y * S_TY + c.ty;
+  c.in = c.x < g.nx && c.y < g.ny;
+  return c;
+}
+
+__device__ inline long long gidx(const Grid& g, int x, int y, int z) {
+  return x + (long long)g.nx * (y + (long long)g.ny * z);
+}
+
+// Queue of the 13 z-neighbours of column (x, y): q[k] = f(z - 6 + k).
+template <class V, class R>
+struct ZQueue {
+  V q[13];
+  __device__ inline void init(const R* f, const Grid& g, const Col& c, int z0) {
+    const int xc = min(c.x, g.nx - 1), yc = min(c.y, g.ny - 1);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      const int z = z0 - 6 + k;
+      const V t = vload<V, R>(f, gidx(g, xc, yc, min(max(z, 0), g.nz - 1)));
+      q[k] = (c.in && z >= 0 && z < g.nz) ? t : vzero(V());
+    }
+  }
+  __device__ inline void fetch(const R* f, const Grid& g, const Col& c, int z) {
+    const int zz = z + 6;
+    const int xc = min(c.x, g.nx - 1), yc = min(c.y, g.ny - 1);
+    const V t = vload<V, R>(f, gidx(g, xc, yc, min(zz, g.nz - 1)));
+    q[12] = (c.in && zz < g.nz) ? t : vzero(V());
+  }
+  __device__ inline void shift() {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) q[k] = q[k + 1];
+  }
+  __device__ inline V centre() const { return q[6]; }
+  __device__ inline V dz(const Stencil13& w) const {
+    V acc = vzero(V());
+#pragma unroll
+    for (int m = 0; m < 13; ++m) acc = vmadd(acc, w.w[m], q[m]);
+    return acc;
+  }
+};
+
+// Fill the LDS plane tile (S_LY x S_LX) of plane z: centre from the queue,
+// halo from memory (zero outside the box).
+template <class V, class R>
+__device__ inline void fill_plane(V* tile, const R* f, const Grid& g, const Col& c, int z, V centre) {
+  tile[(c.ty + 6) * S_LX + c.tx + 6] = centre;
+  const int x0 = blockIdx.x * S_TX - 6, y0 = blockIdx.y * S_TY - 6;
+  constexpr int NE = (S_LX * S_LY + S_THREADS - 1) / S_THREADS;
+  V v[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int e = c.tid + k * S_THREADS;
+    const int ly = e / S_LX, lx = e - ly * S_LX;
+    const int x = x0 + lx, y = y0 + ly;
+    const bool inb = x >= 0 && x < g.nx && y >= 0 && y < g.ny;
+    const int xc = min(max(x, 0), g.nx - 1), yc = min(max(y, 0), g.ny - 1);
+    const V t = vload<V, R>(f, gidx(g, xc, yc, z));
+    v[k] = inb ? t : vzero(V());
+  }
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int e = c.tid + k * S_THREADS;
+    const int ly = e / S_LX, lx = e - ly * S_LX;
+    const bool centre_region = ly >= 6 && ly < 6 + S_TY && lx >= 6 && lx < 6 + S_TX;
+    if (e < S_LX * S_LY && !centre_region) tile[e] = v[k];
+  }
+}
+
+template <class V>
+__device__ inline V tile_dx(const V* tile, const Col& c, const Stencil13& w) {
+  const V* row = tile + (c.ty + 6) * S_LX + c.tx;
+  V acc = vzero(V());
+#pragma unroll
+  for (int m = 0; m < 13; ++m) acc = vmadd(acc, w.w[m], row[m]);
+  return acc;
+}
+
+template <class V>
+__device__ inline V tile_dy(const V* tile, const Col& c, const Stencil13& w) {
+  const V* colp = tile + c.ty * S_LX + c.tx + 6;
+  V acc = vzero(V());
+#pragma unroll
+  for (int m = 0; m < 13; ++m) acc = vmadd(acc, w.w[m], colp[m * S_LX]);
+  return acc;
+}
+
+// Block sum for the 2D engine blocks (S_THREADS threads).
+__device__ inline double block_sum25(double v, double* sh, int tid) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();
+  if ((tid & 63) == 0) sh[tid >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (tid == 0)
+    for (int k = 0; k < S_THREADS / 64; ++k) s += sh[k];
+  return s;
+}
+
+// ---- psi-side kernels on the engine ------------------------------------
+struct Ham25 {
+  Grid g;
+  Stencil13 d2x, d2y, d2z;
+  double h2o2m4;
+  const c64* timec_z;
+  int zc;  // z planes per workgroup
+};
+
+// Hpsi at the column's current plane (steppc.f90:36-46).
+__device__ inline c64 ham25(const Ham25& a, const c64* tile, const ZQueue<c64, c64>& q, const Col& c,
+                            double pv, int z) {
+  const c64 sx = tile_dx(tile, c, a.d2x);
+  const c64 sy = tile_dy(tile, c, a.d2y);
+  const c64 sz = q.dz(a.d2z);
+  const c64 lap = (sx + sy) + sz;
+  const c64 ps = q.centre();
+  const c64 t = mk(lap.x * a.h2o2m4 - pv * ps.x, lap.y * a.h2o2m4 - pv * ps.y);
+  return cmul(a.timec_z[z], t);
+}
+
+// Hamming predictor + modifier (steppc.f90:46-56).
+template <class R>
+__global__ void __launch_bounds__(S_THREADS) k_pc_predict25(Ham25 a, const c64* __restrict__ psi,
+                                                             const R* __restrict__ pot4, c64* old3,
+                                                             const c64* __restrict__ h1, c64* h2,
+                                                             c64* __restrict__ pc, R* __restrict__ den,
+                                                             double c4o3dt, double c112) {
+  __shared__ c64 tile[S_LX * S_LY];
+  const Grid& g = a.g;
+  const Col c = col_of(g);
+  const int z0 = blockIdx.z * a.zc;
+  const int z1 = min(z0 + a.zc, g.nz);
+  ZQueue<c64, c64> q;
+  q.init(psi, g, c, z0);
+  for (int z = z0; z < z1; ++z) {
+    q.fetch(psi, g, c, z);
+    fill_plane(tile, psi, g, c, z, q.q[6]);
+    __syncthreads();
+    if (c.in) {
+      const long long i = gidx(g, c.x, c.y, z);
+      const c64 s4 = ham25(a, tile, q, c, (double)pot4[i], z);
+      const c64 o3 = old3[i], a1 = h1[i], a2 = h2[i], pci = pc[i];
+      const c64 s1 = mk(o3.x + c4o3dt * ((2.0 * s4.x - a1.x) + 2.0 * a2.x),
+                        o3.y + c4o3dt * ((2.0 * s4.y - a1.y) + 2.0 * a2.y));
+      const c64 np = mk(s1.x - c112 * pci.x, s1.y - c112 * pci.y);
+      old3[i] = np;
+      pc[i] = s1;
+      h2[i] = s4;
+      den[i] = (R)(np.x * np.x + np.y * np.y);
+    }
+    __syncthreads();
+    q.shift();
+  }
+}
+
+// Hamming corrector + final value + errHe partial sums (steppc.f90:97-112).
+template <class R>
+__global__ void __launch_bounds__(S_THREADS) k_pc_correct25(Ham25 a, const c64* __restrict__ psi,
+                                                             const R* __restrict__ pot4,
+                                                             const c64* __restrict__ o3p,
+                                                             const c64* __restrict__ o2p,
+                                                             const c64* __restrict__ h1p,
+                                                             const c64* __restrict__ h2p,
+                                                             c64* __restrict__ pc, c64* __restrict__ psi_out,
+                                                             R* __restrict__ den, double* __restrict__ part,
+                                                             double dt3, double c9) {
+  __shared__ c64 tile[S_LX * S_LY];
+  __shared__ double sh[S_THREADS / 64];
+  const Grid& g = a.g;
+  const Col c = col_of(g);
+  const int z0 = blockIdx.z * a.zc;
+  const int z1 = min(z0 + a.zc, g.nz);
+  ZQueue<c64, c64> q;
+  q.init(psi, g, c, z0);
+  double e = 0.0;
+  for (int z = z0; z < z1; ++z) {
+    q.fetch(psi, g, c, z);
+    fill_plane(tile, psi, g, c, z, q.q[6]);
+    __syncthreads();
+    if (c.in) {
+      const long long i = gidx(g, c.x, c.y, z);
+      const c64 s4 = ham25(a, tile, q, c, (double)pot4[i], z);
+      const c64 v3 = o3p[i], v2 = o2p[i], a1 = h1p[i], a2 = h2p[i];
+      const c64 s5 = mk(0.125 * ((9.0 * v3.x - v2.x) + dt3 * ((s4.x + 2.0 * a1.x) - a2.x)),
+                        0.125 * ((9.0 * v3.y - v2.y) + dt3 * ((s4.y + 2.0 * a1.y) - a2.y)));
+      const c64 pci = pc[i];
+      const c64 npc = mk(pci.x - s5.x, pci.y - s5.y);
+      pc[i] = npc;
+      const c64 np = mk(s5.x + c9 * npc.x, s5.y + c9 * npc.y);
+      psi_out[i] = np;
+      den[i] = (R)(np.x * np.x + np.y * np.y);
+      e += hypot(npc.x, npc.y);
+    }
+    __syncthreads();
+    q.shift();
+  }
+  const double s = block_sum25(e, sh, c.tid);
+  if (c.tid == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+}
+
+// RK-Gill stage (steprk.f90:56-74).
+template <class R>
+__global__ void __launch_bounds__(S_THREADS) k_rk_stage25(Ham25 a, const c64* __restrict__ psi,
+                                                           const R* __restrict__ pot4, c64* __restrict__ qacc,
+                                                           c64* __restrict__ psi_out, R* __restrict__ den,
+                                                           c64* __restrict__ h1_out, double ca, double cb,
+                                                           double cc, double dt) {
+  __shared__ c64 tile[S_LX * S_LY];
+  const Grid& g = a.g;
+  const Col c = col_of(g);
+  const int z0 = blockIdx.z * a.zc;
+  const int z1 = min(z0 + a.zc, g.nz);
+  ZQueue<c64, c64> q;
+  q.init(psi, g, c, z0);
+  for (int z = z0; z < z1; ++z) {
+    q.fetch(psi, g, c, z);
+    fill_plane(tile, psi, g, c, z, q.q[6]);
+    __syncthreads();
+    if (c.in) {
+      const long long i = gidx(g, c.x, c.y, z);
+      const c64 s4 = ham25(a, tile, q, c, (double)pot4[i], z);
+      const c64 qi = qacc[i];
+      const c64 s1 = mk(ca * (s4.x - cb * qi.x), ca * (s4.y - cb * qi.y));
+      qacc[i] = mk((qi.x + 3.0 * s1.x) - cc * s4.x, (qi.y + 3.0 * s1.y) - cc * s4.y);
+      if (h1_out) h1_out[i] = s4;
+      const c64 ps = q.centre();
+      const c64 np = mk(ps.x + dt * s1.x, ps.y + dt * s1.y);
+      psi_out[i] = np;
+      den[i] = (R)(np.x * np.x + np.y * np.y);
+    }
+    __syncthreads();
+    q.shift();
+  }
+}
+
+// grad(den) with the first-derivative stencil (derden.f90:26-28).
+template <class R>
+__global__ void __launch_bounds__(S_THREADS) k_grad3_25(Grid g, int zc, const R* __restrict__ f, Stencil13 wx,
+                                                         Stencil13 wy, Stencil13 wz, R* __restrict__ fx,
+                                                         R* __restrict__ fy, R* __restrict__ fz) {
+  __shared__ double tile[S_LX * S_LY];
+  const Col c = col_of(g);
+  const int z0 = blockIdx.z * zc;
+  const int z1 = min(z0 + zc, g.nz);
+  ZQueue<double, R> q;
+  q.init(f, g, c, z0);
+  for (int z = z0; z < z1; ++z) {
+    q.fetch(f, g, c, z);
+    fill_plane(tile, f, g, c, z, q.q[6]);
+    __syncthreads();
+    if (c.in) {
+      const long long i = gidx(g, c.x, c.y, z);
+      fx[i] = (R)tile_dx(tile, c, wx);
+      fy[i] = (R)tile_dy(tile, c, wy);
+      fz[i] = (R)q.dz(wz);
+    }
+    __syncthreads();
+    q.shift();
+  }
+}
+
+}  // namespace hetd
+
+namespace hetd {
+
+// ---- alpha_s divergence + pot4 on the engine ---------------------------
+// term_alfa.f90:85-113 and poten.f90:80-124: each product g_c = falfs*int_c
+// is differentiated along its own axis only, so g_x needs an x-halo row
+// tile, g_y a y-halo column tile and g_z the z queue.
+struct Pot25 {
+  Grid g;
+  Stencil13 d1x, d1y, d1z;
+  double a0, a1;                 // cp4/2, cpp4/3
+  double den0s, h2o2m4, alphas;
+  double C, beta, den_m;
+  int otc;
+  int zc;
+};
+
+template <class R>
+__device__ inline double prod_at(const R* __restrict__ a, const R* __restrict__ b, const Grid& g, int x, int y,
+                                 int z) {
+  const long long j = gidx(g, x, y, z);
+  return (double)a[j] * (double)b[j];
+}
+
+template <class R>
+__global__ void __launch_bounds__(S_THREADS) k_pot_final25(Pot25 a, const R* __restrict__ den,
+                                                            const R* __restrict__ dencg,
+                                                            const R* __restrict__ delj4,
+                                                            const R* __restrict__ sto1c,
+                                                            const R* __restrict__ uimp,
+                                                            const R* __restrict__ falfs,
+                                                            const R* __restrict__ intx,
+                                                            const R* __restrict__ inty,
+                                                            const R* __restrict__ intz,
+                                                            const R* __restrict__ uaconv, R* __restrict__ pot4) {
+  __shared__ double tx_tile[S_TY * S_LX];    // g_x, rows of this tile with x halo
+  __shared__ double ty_tile[S_LY * S_TX];    // g_y, columns of this tile with y halo
+  const Grid& g = a.g;
+  const Col c = col_of(g);
+  const int z0 = blockIdx.z * a.zc;
+  const int z1 = min(z0 + a.zc, g.nz);
+  const int xc = min(c.x, g.nx - 1), yc = min(c.y, g.ny - 1);
+  double q[13];
+  if (a.otc) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      const int z = z0 - 6 + k;
+      const double t = prod_at(falfs, intz, g, xc, yc, min(max(z, 0), g.nz - 1));
+      q[k] = (c.in && z >= 0 && z < g.nz) ? t : 0.0;
+    }
+  }
+  const int x0 = blockIdx.x * S_TX - 6, y0 = blockIdx.y * S_TY - 6;
+  for (int z = z0; z < z1; ++z) {
+    if (a.otc) {
+      const int zz = z + 6;
+      const double t = prod_at(falfs, intz, g, xc, yc, min(zz, g.nz - 1));
+      q[12] = (c.in && zz < g.nz) ? t : 0.0;
+      // g_x rows: S_TY x S_LX
+      constexpr int NX = (S_TY * S_LX + S_THREADS - 1) / S_THREADS;
+      double vx[NX];
+#pragma unroll
+      for (int k = 0; k < NX; ++k) {
+        const int e = c.tid + k * S_THREADS;
+        const int ly = e / S_LX, lx = e - ly * S_LX;
+        const int x = x0 + lx, y = blockIdx.y * S_TY + ly;
+        const bool inb = x >= 0 && x < g.nx && y < g.ny;
+        const double v = prod_at(falfs, intx, g, min(max(x, 0), g.nx - 1), min(y, g.ny - 1), z);
+        vx[k] = inb ? v : 0.0;
+      }
+      constexpr int NY = (S_LY * S_TX + S_THREADS - 1) / S_THREADS;
+      double vy[NY];
+#pragma unroll
+      for (int k = 0; k < NY; ++k) {
+        const int e = c.tid + k * S_THREADS;
+        const int ly = e / S_TX, lx = e - ly * S_TX;
+        const int x = blockIdx.x * S_TX + lx, y = y0 + ly;
+        const bool inb = x < g.nx && y >= 0 && y < g.ny;
+        const double v = prod_at(falfs, inty, g, min(x, g.nx - 1), min(max(y, 0), g.ny - 1), z);
+        vy[k] = inb ? v : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < NX; ++k) {
+        const int e = c.tid + k * S_THREADS;
+        if (e < S_TY * S_LX) tx_tile[e] = vx[k];
+      }
+#pragma unroll
+      for (int k = 0; k < NY; ++k) {
+        const int e = c.tid + k * S_THREADS;
+        if (e < S_LY * S_TX) ty_tile[e] = vy[k];
+      }
+      __syncthreads();
+    }
+    if (c.in) {
+      const long long i = gidx(g, c.x, c.y, z);
+      const double dn = (double)den[i];
+      const double dcg = (double)dencg[i];
+      double p4 = ((double)delj4[i] + dcg * dcg * (a.a0 + a.a1 * dcg)) + (double)sto1c[i];
+      if (a.otc) {
+        double s4 = 0.0, s5 = 0.0, s6 = 0.0;
+        const double* rx = tx_tile + c.ty * S_LX + c.tx;
+        const double* cy = ty_tile + c.tx + c.ty * S_TX;
+#pragma unroll
+        for (int m = 0; m < 13; ++m) s4 = s4 + a.d1x.w[m] * rx[m];
+#pragma unroll
+        for (int m = 0; m < 13; ++m) s5 = s5 + a.d1y.w[m] * cy[m * S_TX];
+#pragma unroll
+        for (int m = 0; m < 13; ++m) s6 = s6 + a.d1z.w[m] * q[m];
+        const double ua = (double)uaconv[i] / a.den0s;
+        const double ual = (((ua + s4) + s5) + s6) * a.h2o2m4 * a.alphas;
+        p4 = p4 + ual;
+      }
+      const double t = tanh(a.beta * (dn - a.den_m));
+      const double pen = a.C * (1.0 + t + a.beta * dn * (1.0 - t * t));
+      pot4[i] = (R)((p4 + (double)uimp[i]) + pen);
+    }
+    if (a.otc) {
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 12; ++k) q[k] = q[k + 1];
+    }
+  }
+}
+
+}  // namespace hetd

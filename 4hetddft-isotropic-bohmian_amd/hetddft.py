@@ -22,7 +22,7 @@ FP64 = 0
 MIXED = 1
 
 FIELDS = {
-    "den": 0, "pot4": 1, "uimp": 2, "dencg": 3, "delj4": 4, "denalf": 5,
+    "den": 0, "pot4": 1, "uimp": 2, "dencg": 3, "delj4": 4, "falfs": 5,
     "dxden": 6, "dyden": 7, "dzden": 8, "intxalf": 9, "intyalf": 10, "intzalf": 11,
     "denx": 12, "psix": 13, "hpsix": 14, "fxconv": 15, "fyconv": 16, "fzconv": 17,
     "correl": 18, "ualphas_conv": 19,

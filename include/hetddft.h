@@ -129,7 +129,7 @@ int hetddft_get_state(hetddft_ctx* ctx, double* psi, double* rimp, double* vimp,
 #define HETDDFT_F_UIMP 2
 #define HETDDFT_F_DENCG 3
 #define HETDDFT_F_DELJ4 4
-#define HETDDFT_F_DENALF 5
+#define HETDDFT_F_FALFS 5       /* 1 - denalf/den0s (term_alfa.f90:28-30) */
 #define HETDDFT_F_DXDEN 6
 #define HETDDFT_F_DYDEN 7
 #define HETDDFT_F_DZDEN 8

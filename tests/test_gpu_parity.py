@@ -24,7 +24,7 @@ def _setup(hip, name, gkw=None, okw=None):
     return wl, o, g
 
 
-FIELD_MAP = {"den": "den", "uimp": "uimp", "dencg": "dencg", "delj4": "delj4", "denalf": "denalf",
+FIELD_MAP = {"den": "den", "uimp": "uimp", "dencg": "dencg", "delj4": "delj4", "falfs": "falfs",
              "dxden": "dxden", "dyden": "dyden", "dzden": "dzden", "intxalf": "intxalf",
              "intyalf": "intyalf", "intzalf": "intzalf", "denx": "denx", "psix": "psix",
              "hpsix": "hpsix", "pot4": "pot4"}
