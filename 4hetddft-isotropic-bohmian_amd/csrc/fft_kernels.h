@@ -37,7 +37,7 @@ constexpr int kLinesY = 16;  // contiguous lines per y-pass workgroup
 // that for longer (or runtime-length) lines to bound the LDS tile.
 template <class T, int N>
 __host__ __device__ constexpr int tile_z() {
-  return (N > 0 && N <= 128) ? (sizeof(T) == 8 ? 16 : 32) : (sizeof(T) == 8 ? 8 : 16);
+  return (N > 0 && N <= 128) ? (sizeof(T) == 8 ? 8 : 16) : (sizeof(T) == 8 ? 8 : 16);
 }
 
 // ---- compile-time radix plan ---------------------------------------------
@@ -373,6 +373,226 @@ __global__ void __launch_bounds__(kThreads) k_fft_zconv(ZArgs<T> a, Grid g, FftP
   for (int e = threadIdx.x; e < TY * n; e += kThreads) {
     const int j = e / TY, l = e - j * TY;
     if (l < nty) dst[(long long)j * pstride + l] = Y[l * ls + j];
+  }
+}
+
+}  // namespace hetd
+
+namespace hetd {
+
+// ---- whole-plane kernels ---------------------------------------------------
+// When one z plane of the half spectrum (nx/2+1 by ny complex) fits in LDS,
+// the x and y transforms run in one pass: XYF reads the real plane once and
+// writes the plane's (kx, ky) spectrum as one contiguous block; XYI does the
+// reverse.  This drops the separate y passes (two full spectrum sweeps per
+// convolution).
+
+constexpr int kPlaneThreads = 1024;
+
+// In-place Stockham stage with compile-time line stride LS and element
+// stride ES over L lines, THR threads.
+template <class T, bool INV, int R, int N, int NS, int L, int THR, int LS, int ES>
+__device__ inline void stage_g(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
+  constexpr int NB = N / R;
+  constexpr int TWS = N / (NS * R);
+  constexpr int TOT = L * NB;
+  constexpr int PER = (TOT + THR - 1) / THR;
+  cplx<T> v[PER][R];
+  int dst[PER];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = tid + q * THR;
+    dst[q] = -1;
+    if (TOT % THR == 0 || b < TOT) {
+      const int l = b / NB;
+      const int j = b - l * NB;
+      const cplx<T>* s = buf + l * LS;
+      const int k = j % NS;
+#pragma unroll
+      for (int t = 0; t < R; ++t) v[q][t] = s[(j + t * NB) * ES];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int t = 1; t < R; ++t) {
+          const cplx<T> w = tw[t * k * TWS];
+          v[q][t] = INV ? cmulc(v[q][t], w) : cmul(v[q][t], w);
+        }
+      }
+      if constexpr (R == 2) bfly2<T, INV>(v[q]);
+      if constexpr (R == 3) bfly3<T, INV>(v[q]);
+      if constexpr (R == 4) bfly4<T, INV>(v[q]);
+      if constexpr (R == 5) bfly5<T, INV>(v[q]);
+      if constexpr (R == 8) bfly8<T, INV>(v[q]);
+      dst[q] = l * LS + ((j / NS) * NS * R + k) * ES;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    if (TOT % THR == 0 || dst[q] >= 0) {
+#pragma unroll
+      for (int t = 0; t < R; ++t) buf[dst[q] + t * NS * ES] = v[q][t];
+    }
+  }
+  __syncthreads();
+}
+
+template <class T, bool INV, int N, int NS, int REM, int L, int THR, int LS, int ES>
+__device__ inline void stages_g(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
+  if constexpr (REM > 1) {
+    constexpr int R = pick_radix(REM);
+    stage_g<T, INV, R, N, NS, L, THR, LS, ES>(buf, tw);
+    stages_g<T, INV, N, NS * R, REM / R, L, THR, LS, ES>(buf, tw);
+  }
+}
+
+template <class T>
+__host__ __device__ constexpr size_t plane_bytes(int mh, int ny) {
+  return (size_t)(mh + 1) * ny * sizeof(cplx<T>);
+}
+
+// r2c in x and y of one z plane.  grid = (nz, nfields), kPlaneThreads.
+// LDS P[ky][kx], row length M = MH+1 (real rows hold 2*MH values first).
+template <class T, class R, int MH, int NY>
+__global__ void __launch_bounds__(kPlaneThreads) k_fft_xyf(XfArgs<R> a, Grid g, const cplx<T>* __restrict__ twx,
+                                                           const cplx<T>* __restrict__ twh,
+                                                           const cplx<T>* __restrict__ twy) {
+  extern __shared__ unsigned char smem_raw[];
+  constexpr int M = MH + 1;
+  constexpr int NX = 2 * MH;
+  constexpr int THR = kPlaneThreads;
+  cplx<T>* P = reinterpret_cast<cplx<T>*>(smem_raw);
+  T* Pr = reinterpret_cast<T*>(smem_raw);
+  const int iz = blockIdx.x;
+  const int f = blockIdx.y;
+  const long long src0 = (long long)iz * NY * NX;
+  for (int e = threadIdx.x; e < NY * NX; e += THR) {
+    const int y = e / NX, c = e - y * NX;
+    Pr[2 * M * y + c] = (T)xf_value(a, f, src0 + e);
+  }
+  __syncthreads();
+  stages_g<T, false, MH, 1, MH, NY, THR, M, 1>(P, twx);
+  // Hermitian split per row (pairs k, MH-k)
+  constexpr int HALF = MH / 2 + 1;
+  for (int e = threadIdx.x; e < NY * HALF; e += THR) {
+    const int y = e / HALF, k = e - y * HALF;
+    cplx<T>* z = P + y * M;
+    const int k2 = MH - k;
+    const cplx<T> za = z[k];
+    const cplx<T> zb = z[k == 0 ? 0 : k2];
+    {
+      const cplx<T> zc = conj(zb);
+      const cplx<T> E = scale(za + zc, T(0.5));
+      const cplx<T> D = za - zc;
+      const cplx<T> O = mk(T(0.5) * D.y, T(-0.5) * D.x);
+      z[k] = E + cmul(twh[k], O);
+    }
+    if (k2 != k) {
+      const cplx<T> zc = conj(za);
+      const cplx<T> E = scale(zb + zc, T(0.5));
+      const cplx<T> D = zb - zc;
+      const cplx<T> O = mk(T(0.5) * D.y, T(-0.5) * D.x);
+      z[k2] = E + cmul(twh[k2], O);
+    }
+  }
+  __syncthreads();
+  stages_g<T, false, NY, 1, NY, M, THR, 1, M>(P, twy);
+  cplx<T>* out = reinterpret_cast<cplx<T>*>(a.out[f]) + (long long)iz * M * NY;
+  for (int e = threadIdx.x; e < M * NY; e += THR) {
+    const int kx = e / NY, ky = e - kx * NY;
+    out[e] = P[ky * M + kx];
+  }
+}
+
+// c2r in y and x of one z plane, times renor, with the XI epilogues.
+template <class T, class R, int MH, int NY>
+__global__ void __launch_bounds__(kPlaneThreads) k_fft_xyi(XiArgs<T, R> a, Grid g, const cplx<T>* __restrict__ twx,
+                                                           const cplx<T>* __restrict__ twh,
+                                                           const cplx<T>* __restrict__ twy) {
+  extern __shared__ unsigned char smem_raw[];
+  constexpr int M = MH + 1;
+  constexpr int NX = 2 * MH;
+  constexpr int THR = kPlaneThreads;
+  cplx<T>* P = reinterpret_cast<cplx<T>*>(smem_raw);
+  const T* Pr = reinterpret_cast<const T*>(smem_raw);
+  const int iz = blockIdx.x;
+  const int f = blockIdx.y;
+  const cplx<T>* in = reinterpret_cast<const cplx<T>*>(a.in[f]) + (long long)iz * M * NY;
+  for (int e = threadIdx.x; e < M * NY; e += THR) {
+    const int kx = e / NY, ky = e - kx * NY;
+    P[ky * M + kx] = in[e];
+  }
+  __syncthreads();
+  stages_g<T, true, NY, 1, NY, M, THR, 1, M>(P, twy);
+  constexpr int HALF = MH / 2 + 1;
+  for (int e = threadIdx.x; e < NY * HALF; e += THR) {
+    const int y = e / HALF, k = e - y * HALF;
+    cplx<T>* x = P + y * M;
+    const int k2 = MH - k;
+    const cplx<T> xa = x[k], xb = x[k2];
+    {
+      const cplx<T> xc = conj(xb);
+      const cplx<T> s = xa + xc;
+      const cplx<T> d = cmulc(xa - xc, twh[k]);
+      x[k] = mk(s.x - d.y, s.y + d.x);
+    }
+    if (k != 0 && k2 != k) {
+      const cplx<T> xc = conj(xa);
+      const cplx<T> s = xb + xc;
+      const cplx<T> d = cmulc(xb - xc, twh[k2]);
+      x[k2] = mk(s.x - d.y, s.y + d.x);
+    }
+  }
+  __syncthreads();
+  stages_g<T, true, MH, 1, MH, NY, THR, M, 1>(P, twx);
+  R* out = a.out[f] + (long long)iz * NY * NX;
+  const T renor = (T)a.renor;
+  const int epi = a.epi[f];
+  const double ec = a.ec[f];
+  for (int e = threadIdx.x; e < NY * NX; e += THR) {
+    const int y = e / NX, c = e - y * NX;
+    const T v = Pr[2 * M * y + c] * renor;
+    out[e] = epi == XI_ONE_MINUS_DIV ? (R)(1.0 - (double)(R)v / ec) : (R)v;
+  }
+}
+
+// z pass with a wide tile: TY consecutive ky (rows of TY complex values),
+// THR threads, one job per workgroup.  grid = (ceil(ny/TY), mx, njobs).
+template <class T, int N, int TY, int THR>
+__global__ void __launch_bounds__(THR) k_fft_zwide(ZArgs<T> a, Grid g, const cplx<T>* __restrict__ tw) {
+  extern __shared__ unsigned char smem_raw[];
+  constexpr int LS = N + 1;
+  cplx<T>* b0 = reinterpret_cast<cplx<T>*>(smem_raw);
+  const long long pstride = (long long)g.mx * g.ny;
+  const int ky0 = blockIdx.x * TY;
+  const int nty = (g.ny - ky0) < TY ? (g.ny - ky0) : TY;
+  const int kx = blockIdx.y;
+  const int jb = blockIdx.z;
+  const long long off = (long long)kx * g.ny + ky0;
+  const cplx<T>* src = reinterpret_cast<const cplx<T>*>(a.in[jb]) + off;
+  for (int e = threadIdx.x; e < TY * N; e += THR) {
+    const int j = e / TY, l = e - j * TY;
+    b0[l * LS + j] = (l < nty) ? src[(long long)j * pstride + l] : mk(T(0), T(0));
+  }
+  __syncthreads();
+  stages_g<T, false, N, 1, N, TY, THR, LS, 1>(b0, tw);
+  const T* kern = a.kern[jb] + off;
+  const T sg = a.ksign[jb];
+  for (int e = threadIdx.x; e < TY * N; e += THR) {
+    const int j = e / TY, l = e - j * TY;
+    if (l < nty) {
+      const cplx<T> v = b0[l * LS + j];
+      const T kv = kern[(long long)j * pstride + l];
+      const cplx<T> w = mk(v.x * kv, v.y * kv);
+      b0[l * LS + j] = sg < T(0) ? mk(-w.x, -w.y) : w;
+    }
+  }
+  __syncthreads();
+  stages_g<T, true, N, 1, N, TY, THR, LS, 1>(b0, tw);
+  cplx<T>* dst = reinterpret_cast<cplx<T>*>(a.out[jb]) + off;
+  for (int e = threadIdx.x; e < TY * N; e += THR) {
+    const int j = e / TY, l = e - j * TY;
+    if (l < nty) dst[(long long)j * pstride + l] = b0[l * LS + j];
   }
 }
 

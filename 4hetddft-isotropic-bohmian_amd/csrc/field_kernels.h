@@ -624,13 +624,14 @@ struct EnergyArgs {
 // 5 psix*hpsix, 6 uimp*den, 7 den (norm).
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_energy(EnergyArgs a, const c64* __restrict__ psi,
-                                                    const R* __restrict__ den, const R* __restrict__ dencg,
-                                                    const R* __restrict__ delj4, const R* __restrict__ falfs,
+                                                    const double* __restrict__ den,
+                                                    const double* __restrict__ dencg,
+                                                    const double* __restrict__ delj4, const R* __restrict__ falfs,
                                                     const R* __restrict__ dxd, const R* __restrict__ dyd,
                                                     const R* __restrict__ dzd, const R* __restrict__ ix,
                                                     const R* __restrict__ iy, const R* __restrict__ iz,
                                                     const R* __restrict__ psix, const R* __restrict__ hpsix,
-                                                    const R* __restrict__ uimp, double* __restrict__ part) {
+                                                    const double* __restrict__ uimp, double* __restrict__ part) {
   __shared__ double sh[16];
   const Grid& g = a.g;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -92,9 +92,32 @@ struct FftKernelsT {
   void (*zc)(ZArgs<T>, Grid, FftPlan, const cplx<T>*) = nullptr;
   int tyz = 16;        // z-pass tile width (consecutive ky)
   size_t lds_x = 0, lds_y = 0, lds_z = 0;
+  // whole-plane x+y kernels (when a plane fits in LDS) and the wide z pass
+  void (*xyf)(XfArgs<T>, Grid, const cplx<T>*, const cplx<T>*, const cplx<T>*) = nullptr;
+  void (*xyi)(XiArgs<T, T>, Grid, const cplx<T>*, const cplx<T>*, const cplx<T>*) = nullptr;
+  void (*zw)(ZArgs<T>, Grid, const cplx<T>*) = nullptr;
+  size_t lds_xy = 0, lds_zw = 0;
+  int tyw = 0, thrw = 0;
 };
 using FftKernels64 = FftKernelsT<double>;
 using FftKernels32 = FftKernelsT<float>;
+
+// Device resources of one transform precision: twiddle tables, spectrum
+// work buffers (internal order [kz][kx][ky]) and the k-space kernels.
+constexpr int kSpectra = 8;
+struct FftSet {
+  void* tw_xh = nullptr;  // exp(-2 pi i m/(nx/2))
+  void* twh = nullptr;    // exp(-2 pi i k/nx), k = 0..nx/2
+  void* tw_y = nullptr;
+  void* tw_z = nullptr;
+  void* spec[kSpectra] = {};
+  void* kfvlj4 = nullptr;
+  void* kwcgk = nullptr;
+  void* kkalfs = nullptr;
+  void* kvq = nullptr;
+  size_t lds_x = 0, lds_y = 0, lds_z = 0;
+  bool ready = false;
+};
 
 struct hetddft_ctx {
   hetddft_params p{};
@@ -110,17 +133,14 @@ struct hetddft_ctx {
 
   Stencil13 d1x{}, d1y{}, d1z{}, d2x{}, d2y{}, d2z{};
   FftPlan pl_xh{}, pl_y{}, pl_z{};
-  int rows = 16, tx_y = 16, tx_z = 16, ntile_y = 1, ntile_z = 1;
-  size_t lds_x = 0, lds_y = 0, lds_z = 0;
   int zc = 16;  // z planes per workgroup of the stencil engine
   bool st25 = true;  // z-marching stencil engine (false: direct neighbour loads)
 
   // device memory
   std::vector<void*> allocs;
-  void* tw_xh = nullptr;  // exp(-2 pi i m/(nx/2))
-  void* twh = nullptr;    // exp(-2 pi i k/nx), k = 0..nx/2
-  void* tw_y = nullptr;
-  void* tw_z = nullptr;
+  FftSet fs;       // transforms of the time step (FP64 or FP32)
+  FftSet fs64e;    // FP64 transforms for energy() in mixed mode
+  void* e64[5] = {};  // FP64 den, denx, dencg, delj4, uimp for energy() in mixed mode
   c64* timec = nullptr;
   // psi side (always FP64 complex)
   c64* psi = nullptr;
@@ -140,11 +160,6 @@ struct hetddft_ctx {
   unsigned int* counts = nullptr;
   int* reg = nullptr;           // occupied-region bookkeeping (field_kernels.h)
   double* fcorner = nullptr;    // force at the 8 cell corners of each particle
-  void* spec[12] = {};
-  void* kfvlj4 = nullptr;
-  void* kwcgk = nullptr;
-  void* kkalfs = nullptr;
-  void* kvq = nullptr;
   // particles
   double* rimp = nullptr;
   double* vimp = nullptr;
@@ -163,6 +178,18 @@ struct hetddft_ctx {
   double* res = nullptr;       // device results
   double* hres = nullptr;      // pinned host copy
   double last_err[3] = {0, 0, 0};
+  // per-kernel-class timing with HIP events on st (hetddft_profile)
+  struct ProfRec {
+    hipEvent_t a, b;
+    int cls;
+    double bytes;
+  };
+  bool prof_on = false;
+  std::vector<ProfRec> prof_pending;
+  std::vector<hipEvent_t> prof_pool;
+  double prof_ms[HETDDFT_PROF_N] = {};
+  long long prof_n[HETDDFT_PROF_N] = {};
+  double prof_bytes[HETDDFT_PROF_N] = {};
   FftKernels64 fft64;
   FftKernels32 fft32;
 };
@@ -170,6 +197,35 @@ struct hetddft_ctx {
 namespace {
 
 thread_local std::string g_create_error;
+
+hipEvent_t prof_event(hetddft_ctx* c) {
+  if (!c->prof_pool.empty()) {
+    hipEvent_t e = c->prof_pool.back();
+    c->prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one launch of kernel class `cls` with events when profiling.
+struct ProfScope {
+  hetddft_ctx* c;
+  int cls;
+  double bytes;
+  hipEvent_t a = nullptr;
+  ProfScope(hetddft_ctx* ctx, int k, double b) : c(ctx), cls(k), bytes(b) {
+    if (c->prof_on && (a = prof_event(c))) hipEventRecord(a, c->st);
+  }
+  ~ProfScope() {
+    if (!a) return;
+    hipEvent_t b = prof_event(c);
+    if (!b) return;
+    hipEventRecord(b, c->st);
+    c->prof_pending.push_back({a, b, cls, bytes});
+  }
+};
 
 int fail(hetddft_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -267,9 +323,54 @@ void pick_z(FftKernels<T>& k, int n) {
   k.lds_z = (N > 0 ? 1 : 2) * tile_bytes<T>(k.tyz, n);
 }
 
+template <class T, int MH, int NY>
+void pick_xy(FftKernels<T>& k) {
+  constexpr size_t b = plane_bytes<T>(MH, NY);
+  if constexpr (b <= 160 * 1024) {
+    k.xyf = k_fft_xyf<T, T, MH, NY>;
+    k.xyi = k_fft_xyi<T, T, MH, NY>;
+    k.lds_xy = b;
+  }
+}
+template <class T, int N>
+void pick_zw(FftKernels<T>& k) {
+  constexpr int TY = 64, THR = 1024;
+  k.zw = k_fft_zwide<T, N, TY, THR>;
+  k.lds_zw = tile_bytes<T>(TY, N);
+  k.tyw = TY;
+  k.thrw = THR;
+}
+
 template <class T>
 FftKernels<T> select_fft(int m, int ny, int nz) {
   FftKernels<T> k;
+  if (m * 2 == ny) {
+    switch (m) {
+      case 12: pick_xy<T, 12, 24>(k); break;
+      case 16: pick_xy<T, 16, 32>(k); break;
+      case 24: pick_xy<T, 24, 48>(k); break;
+      case 32: pick_xy<T, 32, 64>(k); break;
+      case 40: pick_xy<T, 40, 80>(k); break;
+      case 48: pick_xy<T, 48, 96>(k); break;
+      case 64: pick_xy<T, 64, 128>(k); break;
+      default: break;
+    }
+  }
+  switch (nz) {
+    case 32: pick_zw<T, 32>(k); break;
+    case 48: pick_zw<T, 48>(k); break;
+    case 64: pick_zw<T, 64>(k); break;
+    case 80: pick_zw<T, 80>(k); break;
+    case 96: pick_zw<T, 96>(k); break;
+    case 128: pick_zw<T, 128>(k); break;
+    default: break;
+  }
+  {
+    const char* e = std::getenv("HETDDFT_XY");
+    if (e && std::atoi(e) == 0) k.xyf = nullptr, k.xyi = nullptr;
+    const char* w = std::getenv("HETDDFT_ZWIDE");
+    if (!(w && std::atoi(w) == 1)) k.zw = nullptr;
+  }
 #define HETD_X(M) \
   case M: pick_x<T, M>(k, m); break;
   switch (m) {
@@ -306,25 +407,30 @@ FftKernels<float>& fft_of<float>(hetddft_ctx* c) {
 }
 
 template <class T>
-int fft_alloc_tables(hetddft_ctx* c) {
+int fft_alloc_tables(hetddft_ctx* c, FftSet& fs) {
   const int m = c->g.nx / 2;
-  RET(upload_twiddles<T>(c, &c->tw_xh, m, m, m));
-  RET(upload_twiddles<T>(c, &c->twh, c->g.nx, m + 1, c->g.nx));
-  RET(upload_twiddles<T>(c, &c->tw_y, c->g.ny, c->g.ny, c->g.ny));
-  RET(upload_twiddles<T>(c, &c->tw_z, c->g.nz, c->g.nz, c->g.nz));
+  RET(upload_twiddles<T>(c, &fs.tw_xh, m, m, m));
+  RET(upload_twiddles<T>(c, &fs.twh, c->g.nx, m + 1, c->g.nx));
+  RET(upload_twiddles<T>(c, &fs.tw_y, c->g.ny, c->g.ny, c->g.ny));
+  RET(upload_twiddles<T>(c, &fs.tw_z, c->g.nz, c->g.nz, c->g.nz));
   FftKernels<T>& k = fft_of<T>(c);
   k = select_fft<T>(m, c->g.ny, c->g.nz);
-  c->lds_x = k.lds_x;
-  c->lds_y = k.lds_y;
-  c->lds_z = k.lds_z;
+  fs.lds_x = k.lds_x;
+  fs.lds_y = k.lds_y;
+  fs.lds_z = k.lds_z;
   const int lim = 160 * 1024;
-  if (c->lds_x > (size_t)lim || c->lds_y > (size_t)lim || c->lds_z > (size_t)lim)
+  if (fs.lds_x > (size_t)lim || fs.lds_y > (size_t)lim || fs.lds_z > (size_t)lim)
     return fail(c, HETDDFT_E_ARG, "FFT tiles exceed LDS");
   HIPCHK(c, hipFuncSetAttribute((const void*)k.xf, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   HIPCHK(c, hipFuncSetAttribute((const void*)k.xi, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   HIPCHK(c, hipFuncSetAttribute((const void*)k.yf, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   HIPCHK(c, hipFuncSetAttribute((const void*)k.yi, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   HIPCHK(c, hipFuncSetAttribute((const void*)k.zc, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  if (k.xyf) {
+    HIPCHK(c, hipFuncSetAttribute((const void*)k.xyf, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(c, hipFuncSetAttribute((const void*)k.xyi, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  }
+  if (k.zw) HIPCHK(c, hipFuncSetAttribute((const void*)k.zw, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   return HETDDFT_OK;
 }
 
@@ -342,45 +448,85 @@ struct ConvJob {
 // Five passes over HBM (see fft_kernels.h): inputs i < nin are formed by
 // the x-pass producers in xf_in; every job reads one input spectrum.
 template <class T>
-int conv_batch(hetddft_ctx* c, const XfArgs<T>& xf_in, int nin, const std::vector<ConvJob<T>>& jobs) {
+int conv_batch(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin,
+               const std::vector<ConvJob<T>>& jobs) {
   const Grid& g = c->g;
   const int nout = (int)jobs.size();
-  if (nin > 5 || nout > 7 || nin < 1 || nout < 1) return fail(c, HETDDFT_E_ARG, "conv_batch: %d in / %d out", nin, nout);
+  if (nin < 1 || nout < 1 || nin > kMaxBatch || nout > kMaxBatch || nin + nout > kSpectra) return fail(c, HETDDFT_E_ARG, "conv_batch: %d in / %d out", nin, nout);
   XfArgs<T> xa = xf_in;
-  for (int i = 0; i < nin; ++i) xa.out[i] = c->spec[i];
+  for (int i = 0; i < nin; ++i) xa.out[i] = fs.spec[i];
   FftKernels<T>& K = fft_of<T>(c);
   const dim3 gx((unsigned)((g.ny + kRowsX - 1) / kRowsX), (unsigned)g.nz, (unsigned)nin);
   const unsigned ylines = (unsigned)(((long long)g.mx * g.nz + kLinesY - 1) / kLinesY);
-  K.xf<<<gx, dim3(kThreads), c->lds_x, c->st>>>(xa, g, c->pl_xh, (const cplx<T>*)c->tw_xh, (const cplx<T>*)c->twh);
-  LAUNCHCHK(c);
-  YArgs<T> ya{};
-  for (int i = 0; i < nin; ++i) ya.buf[i] = c->spec[i];
-  K.yf<<<dim3(ylines, 1, nin), dim3(kThreads), c->lds_y, c->st>>>(ya, g, c->pl_y, (const cplx<T>*)c->tw_y);
-  LAUNCHCHK(c);
+  const double Rb = (double)g.n * sizeof(T), Sb = 2.0 * (double)g.h * sizeof(T), Kb = (double)g.h * sizeof(T);
+  double in_bytes = 0.0;
+  for (int i = 0; i < nin; ++i) {
+    int nsrc = 1;
+    if (xa.op[i] == XF_CORREL || xa.op[i] == XF_MUL) nsrc = 2;
+    if (xa.op[i] == XF_GRAD_DOT) nsrc = 6;
+    in_bytes += nsrc * Rb;
+  }
+  if (K.xyf) {
+    ProfScope ps(c, HETDDFT_PROF_XYF, in_bytes + nin * Sb);
+    K.xyf<<<dim3(g.nz, nin), dim3(kPlaneThreads), K.lds_xy, c->st>>>(
+        xa, g, (const cplx<T>*)fs.tw_xh, (const cplx<T>*)fs.twh, (const cplx<T>*)fs.tw_y);
+    LAUNCHCHK(c);
+  } else {
+    {
+      ProfScope ps(c, HETDDFT_PROF_XF, in_bytes + nin * Sb);
+      K.xf<<<gx, dim3(kThreads), fs.lds_x, c->st>>>(xa, g, c->pl_xh, (const cplx<T>*)fs.tw_xh,
+                                                    (const cplx<T>*)fs.twh);
+    }
+    LAUNCHCHK(c);
+    YArgs<T> ya{};
+    for (int i = 0; i < nin; ++i) ya.buf[i] = fs.spec[i];
+    ProfScope ps(c, HETDDFT_PROF_YF, nin * 2.0 * Sb);
+    K.yf<<<dim3(ylines, 1, nin), dim3(kThreads), fs.lds_y, c->st>>>(ya, g, c->pl_y, (const cplx<T>*)fs.tw_y);
+    LAUNCHCHK(c);
+  }
   ZArgs<T> za{};
   XiArgs<T, T> xi{};
   YArgs<T> yb{};
   xi.renor = c->renor;
   for (int k = 0; k < nout; ++k) {
     const ConvJob<T>& j = jobs[k];
-    za.in[k] = c->spec[j.input];
+    za.in[k] = fs.spec[j.input];
     za.kern[k] = (const T*)j.kern;
     za.ksign[k] = (T)j.sign;
-    za.out[k] = c->spec[5 + k];
-    yb.buf[k] = c->spec[5 + k];
-    xi.in[k] = c->spec[5 + k];
+    za.out[k] = fs.spec[nin + k];
+    yb.buf[k] = fs.spec[nin + k];
+    xi.in[k] = fs.spec[nin + k];
     xi.out[k] = (T*)j.out;
     xi.epi[k] = j.epi;
     xi.ec[k] = j.ec;
   }
-  K.zc<<<dim3((g.ny + K.tyz - 1) / K.tyz, g.mx, nout), dim3(kThreads), c->lds_z, c->st>>>(
-      za, g, c->pl_z, (const cplx<T>*)c->tw_z);
+  if (K.zw) {
+    ProfScope ps(c, HETDDFT_PROF_Z, nout * (2.0 * Sb + Kb));
+    K.zw<<<dim3((g.ny + K.tyw - 1) / K.tyw, g.mx, nout), dim3(K.thrw), K.lds_zw, c->st>>>(za, g,
+                                                                                         (const cplx<T>*)fs.tw_z);
+  } else {
+    ProfScope ps(c, HETDDFT_PROF_Z, nout * (2.0 * Sb + Kb));
+    K.zc<<<dim3((g.ny + K.tyz - 1) / K.tyz, g.mx, nout), dim3(kThreads), fs.lds_z, c->st>>>(
+        za, g, c->pl_z, (const cplx<T>*)fs.tw_z);
+  }
   LAUNCHCHK(c);
-  K.yi<<<dim3(ylines, 1, nout), dim3(kThreads), c->lds_y, c->st>>>(yb, g, c->pl_y, (const cplx<T>*)c->tw_y);
-  LAUNCHCHK(c);
-  const dim3 gxi((unsigned)((g.ny + kRowsX - 1) / kRowsX), (unsigned)g.nz, (unsigned)nout);
-  K.xi<<<gxi, dim3(kThreads), c->lds_x, c->st>>>(xi, g, c->pl_xh, (const cplx<T>*)c->tw_xh, (const cplx<T>*)c->twh);
-  LAUNCHCHK(c);
+  if (K.xyi) {
+    ProfScope ps(c, HETDDFT_PROF_XYI, nout * (Sb + Rb));
+    K.xyi<<<dim3(g.nz, nout), dim3(kPlaneThreads), K.lds_xy, c->st>>>(
+        xi, g, (const cplx<T>*)fs.tw_xh, (const cplx<T>*)fs.twh, (const cplx<T>*)fs.tw_y);
+    LAUNCHCHK(c);
+  } else {
+    {
+      ProfScope ps(c, HETDDFT_PROF_YI, nout * 2.0 * Sb);
+      K.yi<<<dim3(ylines, 1, nout), dim3(kThreads), fs.lds_y, c->st>>>(yb, g, c->pl_y, (const cplx<T>*)fs.tw_y);
+    }
+    LAUNCHCHK(c);
+    ProfScope ps(c, HETDDFT_PROF_XI, nout * (Sb + Rb));
+    const dim3 gxi((unsigned)((g.ny + kRowsX - 1) / kRowsX), (unsigned)g.nz, (unsigned)nout);
+    K.xi<<<gxi, dim3(kThreads), fs.lds_x, c->st>>>(xi, g, c->pl_xh, (const cplx<T>*)fs.tw_xh,
+                                                   (const cplx<T>*)fs.twh);
+    LAUNCHCHK(c);
+  }
   return HETDDFT_OK;
 }
 
@@ -455,6 +601,8 @@ int veval(hetddft_ctx* c) {
                      (const T*)psix, c->d2x, c->d2y, c->d2z, c->h2o2mx, F(hetddft_ctx::F_HPSIX));
   LAUNCHCHK(c);
   // ---- poten: derden (derden.f90:26-28)
+  {
+  ProfScope psg(c, HETDDFT_PROF_GRAD, 4.0 * (double)g.n * sizeof(T));
   if (c->st25)
     hipLaunchKernelGGL((k_grad3_25<T>), grid25(c), dim3(S_TX, S_TY), 0, c->st, g, c->zc,
                        (const T*)F(hetddft_ctx::F_DEN), c->d1x, c->d1y, c->d1z, F(hetddft_ctx::F_DX),
@@ -463,6 +611,7 @@ int veval(hetddft_ctx* c) {
     hipLaunchKernelGGL((k_grad3<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, g,
                        (const T*)F(hetddft_ctx::F_DEN), c->d1x, c->d1y, c->d1z, F(hetddft_ctx::F_DX),
                        F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ));
+  }
   LAUNCHCHK(c);
   // batch 1: denx*Vq -> uimp (potenimp.f90:15-17); den*{wcgk, fvlj4, kalfs}
   //          -> dencg, delj4, falfs = 1 - denalf/den0s (poten.f90:27-50,94-99;
@@ -474,11 +623,11 @@ int veval(hetddft_ctx* c) {
     xa.op[1] = XF_COPY;
     xa.in[1][0] = F(hetddft_ctx::F_DEN);
     std::vector<ConvJob<T>> jobs;
-    jobs.push_back({0, c->kvq, 1.0, F(hetddft_ctx::F_UIMP)});
-    jobs.push_back({1, c->kwcgk, 1.0, F(hetddft_ctx::F_DENCG)});
-    jobs.push_back({1, c->kfvlj4, 1.0, F(hetddft_ctx::F_DELJ4)});
-    if (otc) jobs.push_back({1, c->kkalfs, 1.0, F(hetddft_ctx::F_FALFS), XI_ONE_MINUS_DIV, c->p.den0s});
-    RET(conv_batch<T>(c, xa, 2, jobs));
+    jobs.push_back({0, c->fs.kvq, 1.0, F(hetddft_ctx::F_UIMP)});
+    jobs.push_back({1, c->fs.kwcgk, 1.0, F(hetddft_ctx::F_DENCG)});
+    jobs.push_back({1, c->fs.kfvlj4, 1.0, F(hetddft_ctx::F_DELJ4)});
+    if (otc) jobs.push_back({1, c->fs.kkalfs, 1.0, F(hetddft_ctx::F_FALFS), XI_ONE_MINUS_DIV, c->p.den0s});
+    RET(conv_batch<T>(c, c->fs, xa, 2, jobs));
   }
   // batch 2: correlation term (poten.f90:62-74); alpha_s integrals
   //          int_c = G*(falfs d_c den) (term_alfa.f90:32-52)
@@ -490,7 +639,7 @@ int veval(hetddft_ctx* c) {
     xa.c[0][0] = c->p.cp4;
     xa.c[0][1] = c->p.cpp4;
     std::vector<ConvJob<T>> jobs;
-    jobs.push_back({0, c->kwcgk, 1.0, F(hetddft_ctx::F_STO1)});
+    jobs.push_back({0, c->fs.kwcgk, 1.0, F(hetddft_ctx::F_STO1)});
     int nin = 1;
     if (otc) {
       const int d[3] = {hetddft_ctx::F_DX, hetddft_ctx::F_DY, hetddft_ctx::F_DZ};
@@ -499,11 +648,11 @@ int veval(hetddft_ctx* c) {
         xa.op[1 + k] = XF_MUL;
         xa.in[1 + k][0] = F(hetddft_ctx::F_FALFS);
         xa.in[1 + k][1] = F(d[k]);
-        jobs.push_back({1 + k, c->kkalfs, 1.0, F(o[k])});
+        jobs.push_back({1 + k, c->fs.kkalfs, 1.0, F(o[k])});
       }
       nin = 4;
     }
-    RET(conv_batch<T>(c, xa, nin, jobs));
+    RET(conv_batch<T>(c, c->fs, xa, nin, jobs));
   }
   // batch 3: grad(den).int -> ualphas convolution (term_alfa.f90:65-80);
   //          -Vq*FFT(d_c den) -> force convolutions (potenimp.f90:39-50)
@@ -519,7 +668,7 @@ int veval(hetddft_ctx* c) {
       xa.in[0][3] = F(hetddft_ctx::F_IX);
       xa.in[0][4] = F(hetddft_ctx::F_IY);
       xa.in[0][5] = F(hetddft_ctx::F_IZ);
-      jobs.push_back({0, c->kkalfs, 1.0, F(hetddft_ctx::F_UACONV)});
+      jobs.push_back({0, c->fs.kkalfs, 1.0, F(hetddft_ctx::F_UACONV)});
       nin = 1;
     }
     const int d[3] = {hetddft_ctx::F_DX, hetddft_ctx::F_DY, hetddft_ctx::F_DZ};
@@ -527,10 +676,10 @@ int veval(hetddft_ctx* c) {
     for (int k = 0; k < 3; ++k) {
       xa.op[nin] = XF_COPY;
       xa.in[nin][0] = F(d[k]);
-      jobs.push_back({nin, c->kvq, -1.0, F(o[k])});
+      jobs.push_back({nin, c->fs.kvq, -1.0, F(o[k])});
       ++nin;
     }
-    RET(conv_batch<T>(c, xa, nin, jobs));
+    RET(conv_batch<T>(c, c->fs, xa, nin, jobs));
   }
   // pot4 assembly with the alpha_s divergence (term_alfa.f90:85-113, poten.f90:80-124)
   PotArgs pa{};
@@ -563,6 +712,7 @@ int veval(hetddft_ctx* c) {
     p5.den_m = pa.den_m;
     p5.otc = pa.otc;
     p5.zc = c->zc;
+    ProfScope ps(c, HETDDFT_PROF_POT, (otc ? 11.0 : 6.0) * (double)g.n * sizeof(T));
     hipLaunchKernelGGL((k_pot_final25<T>), grid25(c), dim3(S_TX, S_TY), 0, c->st, p5,
                        (const T*)F(hetddft_ctx::F_DEN), (const T*)F(hetddft_ctx::F_DENCG),
                        (const T*)F(hetddft_ctx::F_DELJ4), (const T*)F(hetddft_ctx::F_STO1),
@@ -691,6 +841,8 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   // predictor + modifier (steppc.f90:36-62)
   c64* x = c->psi;
   c64* y = c->old[P[2]];
+  {
+  ProfScope pspred(c, HETDDFT_PROF_PCP, (double)g.n * (16.0 * 8 + 2.0 * sizeof(T)));
   if (c->st25)
     hipLaunchKernelGGL((k_pc_predict25<T>), grid25(c), dim3(S_TX, S_TY), 0, c->st, ha, (const c64*)x,
                        (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]], c->hold[H[1]], c->pc,
@@ -699,6 +851,7 @@ int step_pc_t(hetddft_ctx* c, double dt) {
     hipLaunchKernelGGL((k_pc_predict<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, ham_direct(ha),
                        (const c64*)x, (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]],
                        c->hold[H[1]], c->pc, (T*)c->fld[hetddft_ctx::F_DEN], c4o3 * dt, c112);
+  }
   LAUNCHCHK(c);
   c->old[P[2]] = x;
   c->psi = y;
@@ -719,6 +872,8 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   // corrector (steppc.f90:97-151)
   const dim3 g25 = grid25(c);
   const unsigned nb = c->st25 ? g25.x * g25.y * g25.z : blocks_for(g.n);
+  {
+  ProfScope pscorr(c, HETDDFT_PROF_PCC, (double)g.n * (16.0 * 10 + 2.0 * sizeof(T)));
   if (c->st25)
     hipLaunchKernelGGL((k_pc_correct25<T>), g25, dim3(S_TX, S_TY), 0, c->st, ha, (const c64*)c->psi,
                        (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
@@ -729,6 +884,7 @@ int step_pc_t(hetddft_ctx* c, double dt) {
                        (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
                        (const c64*)c->hold[H[0]], (const c64*)c->hold[H[1]], c->pc, c->spare,
                        (T*)c->fld[hetddft_ctx::F_DEN], c->part, 3.0 * dt, c9);
+  }
   LAUNCHCHK(c);
   std::swap(c->psi, c->spare);
   hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(kBlock), 0, c->st, (const double*)c->part, (int)nb, 1, c->res);
@@ -796,8 +952,8 @@ int field_to_host(hetddft_ctx* c, const void* dsrc, double* out) {
   return HETDDFT_OK;
 }
 
-int upload_real(hetddft_ctx* c, void* dst, const double* src, size_t n) {
-  if (!c->mixed) {
+int upload_real(hetddft_ctx* c, void* dst, const double* src, size_t n, bool as_float) {
+  if (!as_float) {
     HIPCHK(c, hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice));
   } else {
     std::vector<float> tmp(n);
@@ -809,20 +965,58 @@ int upload_real(hetddft_ctx* c, void* dst, const double* src, size_t n) {
 
 // k-space kernel from the Fortran (mx,ny,nz) layout into the internal
 // spectrum order [kz][kx][ky]
-int upload_kernel(hetddft_ctx* c, void* dst, const double* src) {
+int upload_kernel(hetddft_ctx* c, void* dst, const double* src, bool as_float) {
   const Grid& g = c->g;
   std::vector<double> t((size_t)g.h);
   for (int iz = 0; iz < g.nz; ++iz)
     for (int iy = 0; iy < g.ny; ++iy)
       for (int ix = 0; ix < g.mx; ++ix)
         t[((size_t)iz * g.mx + ix) * g.ny + iy] = src[((size_t)iz * g.ny + iy) * g.mx + ix];
-  return upload_real(c, dst, t.data(), t.size());
+  return upload_real(c, dst, t.data(), t.size(), as_float);
+}
+
+template <class T>
+__global__ void k_to_double(const T* __restrict__ a, double* __restrict__ b, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (double)a[i];
 }
 
 template <class T>
 int energy_t(hetddft_ctx* c, double out[12]) {
   const Grid& g = c->g;
   auto F = [&](int id) { return (const T*)c->fld[id]; };
+  // The density-side terms (kinetic aside) are large and cancel; they are
+  // evaluated on FP64 fields: in mixed mode den is rebuilt from psi and the
+  // convolutions behind uimp, dencg and delj4 are redone in FP64.
+  const double *den64, *dencg64, *delj464, *uimp64;
+  if (c->mixed) {
+    double* d64 = (double*)c->e64[0];
+    double* dx64 = (double*)c->e64[1];
+    hipLaunchKernelGGL((k_density<double>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, (const c64*)c->psi, d64,
+                       g.n);
+    LAUNCHCHK(c);
+    hipLaunchKernelGGL((k_to_double<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, (const T*)c->denx, dx64, g.n);
+    LAUNCHCHK(c);
+    XfArgs<double> xa{};
+    xa.op[0] = XF_COPY;
+    xa.in[0][0] = dx64;
+    xa.op[1] = XF_COPY;
+    xa.in[1][0] = d64;
+    std::vector<ConvJob<double>> jobs;
+    jobs.push_back({0, c->fs64e.kvq, 1.0, c->e64[4]});
+    jobs.push_back({1, c->fs64e.kwcgk, 1.0, c->e64[2]});
+    jobs.push_back({1, c->fs64e.kfvlj4, 1.0, c->e64[3]});
+    RET(conv_batch<double>(c, c->fs64e, xa, 2, jobs));
+    den64 = d64;
+    dencg64 = (const double*)c->e64[2];
+    delj464 = (const double*)c->e64[3];
+    uimp64 = (const double*)c->e64[4];
+  } else {
+    den64 = (const double*)c->fld[hetddft_ctx::F_DEN];
+    dencg64 = (const double*)c->fld[hetddft_ctx::F_DENCG];
+    delj464 = (const double*)c->fld[hetddft_ctx::F_DELJ4];
+    uimp64 = (const double*)c->fld[hetddft_ctx::F_UIMP];
+  }
   EnergyArgs ea{};
   ea.g = g;
   ea.d1x = c->d1x;
@@ -835,11 +1029,10 @@ int energy_t(hetddft_ctx* c, double out[12]) {
   ea.den_m = c->p.den_m;
   ea.otc = c->p.core_otc;
   const unsigned nb = blocks_for(g.n);
-  hipLaunchKernelGGL((k_energy<T>), dim3(nb), dim3(kBlock), 0, c->st, ea, (const c64*)c->psi, F(hetddft_ctx::F_DEN),
-                     F(hetddft_ctx::F_DENCG), F(hetddft_ctx::F_DELJ4), F(hetddft_ctx::F_FALFS),
-                     F(hetddft_ctx::F_DX), F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ), F(hetddft_ctx::F_IX),
-                     F(hetddft_ctx::F_IY), F(hetddft_ctx::F_IZ), F(hetddft_ctx::F_PSIX),
-                     F(hetddft_ctx::F_HPSIX), F(hetddft_ctx::F_UIMP), c->part);
+  hipLaunchKernelGGL((k_energy<T>), dim3(nb), dim3(kBlock), 0, c->st, ea, (const c64*)c->psi, den64, dencg64,
+                     delj464, F(hetddft_ctx::F_FALFS), F(hetddft_ctx::F_DX), F(hetddft_ctx::F_DY),
+                     F(hetddft_ctx::F_DZ), F(hetddft_ctx::F_IX), F(hetddft_ctx::F_IY), F(hetddft_ctx::F_IZ),
+                     F(hetddft_ctx::F_PSIX), F(hetddft_ctx::F_HPSIX), uimp64, c->part);
   LAUNCHCHK(c);
   hipLaunchKernelGGL(k_finish_sums, dim3(8), dim3(kBlock), 0, c->st, (const double*)c->part, (int)nb, 8, c->res + 4);
   LAUNCHCHK(c);
@@ -916,6 +1109,11 @@ long long hetddft_device_bytes(const hetddft_ctx* ctx) { return ctx ? ctx->bytes
 int hetddft_destroy(hetddft_ctx* c) {
   if (!c) return HETDDFT_OK;
   if (c->st) hipStreamSynchronize(c->st);
+  for (auto& r : c->prof_pending) {
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
+  for (hipEvent_t e : c->prof_pool) hipEventDestroy(e);
   for (void* p : c->allocs) hipFree(p);
   if (c->hres) hipHostFree(c->hres);
   if (c->st) hipStreamDestroy(c->st);
@@ -987,7 +1185,9 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
     fail(c, HETDDFT_E_HIP, "hipStreamCreate failed");
     return bad(HETDDFT_E_HIP);
   }
-  int rc = c->mixed ? fft_alloc_tables<float>(c) : fft_alloc_tables<double>(c);
+  int rc = c->mixed ? fft_alloc_tables<float>(c, c->fs) : fft_alloc_tables<double>(c, c->fs);
+  if (rc) return bad(rc);
+  if (c->mixed) rc = fft_alloc_tables<double>(c, c->fs64e);
   if (rc) return bad(rc);
   const size_t n = (size_t)g.n, h = (size_t)g.h;
   auto A = [&](void** ptr, size_t bytes) { return dalloc(c, ptr, bytes); };
@@ -1002,11 +1202,19 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
   want.push_back({(void**)&c->counts, n * 4});
   want.push_back({(void**)&c->reg, kRegLen * sizeof(int)});
   want.push_back({(void**)&c->fcorner, (size_t)24 * std::max(p->n_par, 1) * 8});
-  for (int k = 0; k < 12; ++k) want.push_back({&c->spec[k], h * 2 * c->tsz});
-  want.push_back({&c->kfvlj4, h * c->tsz});
-  want.push_back({&c->kwcgk, h * c->tsz});
-  want.push_back({&c->kkalfs, h * c->tsz});
-  want.push_back({&c->kvq, h * c->tsz});
+  for (int k = 0; k < kSpectra; ++k) want.push_back({&c->fs.spec[k], h * 2 * c->tsz});
+  want.push_back({&c->fs.kfvlj4, h * c->tsz});
+  want.push_back({&c->fs.kwcgk, h * c->tsz});
+  want.push_back({&c->fs.kkalfs, h * c->tsz});
+  want.push_back({&c->fs.kvq, h * c->tsz});
+  if (c->mixed) {
+    // FP64 re-evaluation of the energy functional: 2 inputs + 3 outputs
+    for (int k = 0; k < 5; ++k) want.push_back({&c->fs64e.spec[k], h * 16});
+    want.push_back({&c->fs64e.kfvlj4, h * 8});
+    want.push_back({&c->fs64e.kwcgk, h * 8});
+    want.push_back({&c->fs64e.kvq, h * 8});
+    for (int k = 0; k < 5; ++k) want.push_back({&c->e64[k], n * 8});
+  }
   const size_t np3 = (size_t)3 * std::max(p->n_par, 1);
   for (double** pp : {&c->rimp, &c->vimp, &c->aimp, &c->qr, &c->qv, &c->pcr, &c->pcv})
     want.push_back({(void**)pp, np3 * 8});
@@ -1129,10 +1337,15 @@ int hetddft_set_kernels(hetddft_ctx* c, const double* fvlj4, const double* wcgk,
   if (!c || !fvlj4 || !wcgk || !kalfs || !vq) return fail(c, HETDDFT_E_ARG, "null kernel array");
   const size_t h = (size_t)c->g.h;
   (void)h;
-  RET(upload_kernel(c, c->kfvlj4, fvlj4));
-  RET(upload_kernel(c, c->kwcgk, wcgk));
-  RET(upload_kernel(c, c->kkalfs, kalfs));
-  RET(upload_kernel(c, c->kvq, vq));
+  RET(upload_kernel(c, c->fs.kfvlj4, fvlj4, c->mixed));
+  RET(upload_kernel(c, c->fs.kwcgk, wcgk, c->mixed));
+  RET(upload_kernel(c, c->fs.kkalfs, kalfs, c->mixed));
+  RET(upload_kernel(c, c->fs.kvq, vq, c->mixed));
+  if (c->mixed) {
+    RET(upload_kernel(c, c->fs64e.kfvlj4, fvlj4, false));
+    RET(upload_kernel(c, c->fs64e.kwcgk, wcgk, false));
+    RET(upload_kernel(c, c->fs64e.kvq, vq, false));
+  }
   c->kernels_set = true;
   return HETDDFT_OK;
 }
@@ -1282,8 +1495,8 @@ int hetddft_convolve(hetddft_ctx* c, int nf, const double* const* in, const doub
                         hetddft_ctx::F_UACONV};
   const int dst[5] = {hetddft_ctx::F_IX, hetddft_ctx::F_IY, hetddft_ctx::F_IZ, hetddft_ctx::F_DENCG,
                       hetddft_ctx::F_DELJ4};
-  for (int i = 0; i < nf; ++i) RET(upload_real(c, c->fld[stage[i]], in[i], n));
-  RET(upload_kernel(c, c->kkalfs, kern));
+  for (int i = 0; i < nf; ++i) RET(upload_real(c, c->fld[stage[i]], in[i], n, c->mixed));
+  RET(upload_kernel(c, c->fs.kkalfs, kern, c->mixed));
   c->kernels_set = false;
   auto run = [&](auto tag) -> int {
     using T = decltype(tag);
@@ -1292,13 +1505,49 @@ int hetddft_convolve(hetddft_ctx* c, int nf, const double* const* in, const doub
     for (int i = 0; i < nf; ++i) {
       xa.op[i] = XF_COPY;
       xa.in[i][0] = (const T*)c->fld[stage[i]];
-      jobs.push_back({i, c->kkalfs, 1.0, c->fld[dst[i]]});
+      jobs.push_back({i, c->fs.kkalfs, 1.0, c->fld[dst[i]]});
     }
-    RET(conv_batch<T>(c, xa, nf, jobs));
+    RET(conv_batch<T>(c, c->fs, xa, nf, jobs));
     for (int i = 0; i < nf; ++i) RET(field_to_host<T>(c, c->fld[dst[i]], out[i]));
     return HETDDFT_OK;
   };
   return c->mixed ? run(float{}) : run(double{});
+}
+
+int hetddft_profile(hetddft_ctx* c, int enable) {
+  if (!c) return HETDDFT_E_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (auto& r : c->prof_pending) {
+    c->prof_pool.push_back(r.a);
+    c->prof_pool.push_back(r.b);
+  }
+  c->prof_pending.clear();
+  for (int k = 0; k < HETDDFT_PROF_N; ++k) {
+    c->prof_ms[k] = 0.0;
+    c->prof_n[k] = 0;
+    c->prof_bytes[k] = 0.0;
+  }
+  c->prof_on = enable != 0;
+  return HETDDFT_OK;
+}
+
+int hetddft_profile_read(hetddft_ctx* c, int cls, double* ms_total, long long* launches, double* bytes_total) {
+  if (!c || cls < 0 || cls >= HETDDFT_PROF_N) return HETDDFT_E_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (auto& r : c->prof_pending) {
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, r.a, r.b));
+    c->prof_ms[r.cls] += ms;
+    c->prof_n[r.cls] += 1;
+    c->prof_bytes[r.cls] += r.bytes;
+    c->prof_pool.push_back(r.a);
+    c->prof_pool.push_back(r.b);
+  }
+  c->prof_pending.clear();
+  if (ms_total) *ms_total = c->prof_ms[cls];
+  if (launches) *launches = c->prof_n[cls];
+  if (bytes_total) *bytes_total = c->prof_bytes[cls];
+  return HETDDFT_OK;
 }
 
 int hetddft_sync(hetddft_ctx* c) {

@@ -28,6 +28,12 @@ FIELDS = {
     "correl": 18, "ualphas_conv": 19,
 }
 
+PROF_CLASSES = {
+    "fft_xy_forward": 0, "fft_x_forward": 1, "fft_y_forward": 2, "fft_z_kernel": 3,
+    "fft_y_inverse": 4, "fft_x_inverse": 5, "fft_xy_inverse": 6, "pot4_assembly": 7,
+    "pc_predict": 8, "pc_correct": 9, "grad_den": 10,
+}
+
 ENERGY_KEYS = ("etot", "etot4", "ekin4", "elj4", "ealphas", "esolid", "ecor4", "eimpu",
                "ekinx", "ekinqx", "ehex", "norm")
 
@@ -110,6 +116,8 @@ def load_library():
         "hetddft_get_field": ([C, ctypes.c_int, _VP], ctypes.c_int),
         "hetddft_convolve": ([C, ctypes.c_int, _VP, _VP, _VP], ctypes.c_int),
         "hetddft_sync": ([C], ctypes.c_int),
+        "hetddft_profile": ([C, ctypes.c_int], ctypes.c_int),
+        "hetddft_profile_read": ([C, ctypes.c_int, _VP, _VP, _VP], ctypes.c_int),
         "hetddft_device_bytes": ([C], ctypes.c_longlong),
     }
     for name, (args, res) in sig.items():
@@ -264,6 +272,23 @@ class Instance:
 
     def sync(self):
         self._check(self.lib.hetddft_sync(self._c), "sync")
+
+    def profile(self, enable=True):
+        """Reset and enable (or disable) per-kernel-class event timing."""
+        self._check(self.lib.hetddft_profile(self._c, int(bool(enable))), "profile")
+
+    def profile_read(self):
+        """{class: (total_ms, launches, algorithmic_bytes)} for classes that ran."""
+        out = {}
+        for name, cls in PROF_CLASSES.items():
+            ms = ctypes.c_double()
+            n = ctypes.c_longlong()
+            b = ctypes.c_double()
+            self._check(self.lib.hetddft_profile_read(self._c, cls, ctypes.byref(ms), ctypes.byref(n),
+                                                      ctypes.byref(b)), "profile_read")
+            if n.value:
+                out[name] = (ms.value, n.value, b.value)
+        return out
 
 
 def header_functions(header=None):

@@ -153,6 +153,28 @@ int hetddft_get_field(hetddft_ctx* ctx, int which, double* out);
 int hetddft_convolve(hetddft_ctx* ctx, int nf, const double* const* in, const double* kern,
                      double* const* out);
 
+/* Per-kernel-class timing.  hetddft_profile(ctx, 1) resets the counters and
+ * brackets every later launch of the classes below with HIP events on the
+ * context's stream; hetddft_profile_read returns the summed event time (ms),
+ * the number of launches and the algorithmic bytes those launches move
+ * (inputs read once + outputs written once).  Profiling adds events between
+ * kernels; leave it off for throughput runs. */
+#define HETDDFT_PROF_XYF 0   /* r2c along x and y of whole planes          */
+#define HETDDFT_PROF_XF 1    /* r2c along x (planes too large for LDS)      */
+#define HETDDFT_PROF_YF 2    /* forward FFT along y                         */
+#define HETDDFT_PROF_Z 3     /* z FFT, k-space kernel product, inverse z    */
+#define HETDDFT_PROF_YI 4    /* inverse FFT along y                         */
+#define HETDDFT_PROF_XI 5    /* c2r along x                                 */
+#define HETDDFT_PROF_XYI 6   /* c2r along y and x of whole planes           */
+#define HETDDFT_PROF_POT 7   /* alpha_s divergence + pot4 assembly          */
+#define HETDDFT_PROF_PCP 8   /* Hamming predictor/modifier of psi           */
+#define HETDDFT_PROF_PCC 9   /* Hamming corrector of psi + errHe            */
+#define HETDDFT_PROF_GRAD 10 /* grad(den)                                    */
+#define HETDDFT_PROF_N 11
+int hetddft_profile(hetddft_ctx* ctx, int enable);
+int hetddft_profile_read(hetddft_ctx* ctx, int cls, double* ms_total, long long* launches,
+                         double* bytes_total);
+
 /* Waits for all work queued on the context's stream. */
 int hetddft_sync(hetddft_ctx* ctx);
 
