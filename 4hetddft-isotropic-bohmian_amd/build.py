@@ -31,6 +31,43 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+FDIR = HERE / "fortran"
+FBIN = FDIR / "he4_tddft_gpu"
+
+
+def fortran_compiler():
+    for cand in (os.environ.get("FC_HETDDFT"), "/opt/rocm/bin/amdflang",
+                 "/opt/rocm/lib/llvm/bin/flang", shutil.which("amdflang"), shutil.which("flang")):
+        if cand and Path(cand).exists():
+            return cand
+    return None
+
+
+def build_fortran(verbose=False):
+    """Compile the ISO_C_BINDING module and the Fortran host loop against libhetddft.so.
+
+    Returns the driver path, or None when no Fortran compiler is installed."""
+    fc = fortran_compiler()
+    if fc is None:
+        return None
+    bdir = FDIR / "_build"
+    bdir.mkdir(exist_ok=True)
+    steps = [
+        [fc, "-O2", "-J", str(bdir), "-c", str(FDIR / "hetddft_mod.f90"), "-o", str(bdir / "hetddft_mod.o")],
+        [fc, "-O2", "-J", str(bdir), "-c", str(FDIR / "he4_tddft_gpu.f90"), "-o", str(bdir / "he4_tddft_gpu.o")],
+        [fc, "-O2", str(bdir / "hetddft_mod.o"), str(bdir / "he4_tddft_gpu.o"), "-L" + str(HERE),
+         "-lhetddft", "-Wl,-rpath,$ORIGIN/..", "-o", str(FBIN)],
+    ]
+    for cmd in steps:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"Fortran build failed ({res.returncode}): {' '.join(cmd)}")
+    return FBIN
+
+
 def build(verbose=False):
     deps = sorted((HERE / "csrc").glob("*")) + [HERE.parent / "include" / "hetddft.h"]
     tmp = OUT.with_suffix(".so.tmp")
@@ -48,3 +85,4 @@ def build(verbose=False):
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))
