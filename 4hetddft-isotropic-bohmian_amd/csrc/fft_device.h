@@ -1,4 +1,4 @@
-// Mixed-radix (2,3,4,5,8) Stockham FFT stages on LDS-resident lines.
+// Mixed-radix (2,3,4,5,8,16) Stockham FFT stages on LDS-resident lines.
 //
 // The 3D r2c/c2r transforms of the reference (FFTW plans, fft.f90:43-65) are
 // built from these 1D stages: each workgroup stages a tile of lines in LDS,
@@ -93,6 +93,44 @@ __device__ inline void bfly8(cplx<T>* v) {
   v[6] = e[2] - o2;
   v[3] = e[3] + o3;
   v[7] = e[3] - o3;
+}
+
+// Radix 16 as 4 x 4: inner length-4 DFTs over n1 (stride 4), twiddles
+// W16^(n2 k1), outer length-4 DFTs over n2; X[k1 + 4 k2] in natural order.
+template <class T, bool INV>
+__device__ inline void bfly16(cplx<T>* v) {
+  const T c1 = T(0.92387953251128675612818318939679);  // cos(pi/8)
+  const T s1 = T(0.38268343236508977172845998403040);  // sin(pi/8)
+  const T r = RadixConst<T>::r2;
+  const T sg = INV ? T(1) : T(-1);
+  cplx<T> y[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) y[n2][n1] = v[4 * n1 + n2];
+    bfly4<T, INV>(y[n2]);
+  }
+  const cplx<T> w1 = mk(c1, sg * s1);    // W16^1
+  const cplx<T> w2 = mk(r, sg * r);      // W16^2
+  const cplx<T> w3 = mk(s1, sg * c1);    // W16^3
+  const cplx<T> w6 = mk(-r, sg * r);     // W16^6
+  const cplx<T> w9 = mk(-c1, -sg * s1);  // W16^9
+  y[1][1] = cmul(y[1][1], w1);
+  y[1][2] = cmul(y[1][2], w2);
+  y[1][3] = cmul(y[1][3], w3);
+  y[2][1] = cmul(y[2][1], w2);
+  y[2][2] = rot90<T, INV>(y[2][2]);
+  y[2][3] = cmul(y[2][3], w6);
+  y[3][1] = cmul(y[3][1], w3);
+  y[3][2] = cmul(y[3][2], w6);
+  y[3][3] = cmul(y[3][3], w9);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    cplx<T> z[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+    bfly4<T, INV>(z);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = z[k2];
+  }
 }
 
 // One Stockham stage of radix R over L lines of length n (line stride ls in

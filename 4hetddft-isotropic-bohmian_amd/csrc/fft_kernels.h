@@ -45,6 +45,13 @@ __host__ __device__ constexpr int pick_radix(int m) {
   return (m % 8 == 0) ? 8 : (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 5 == 0) ? 5 : (m % 3 == 0) ? 3 : m;
 }
 
+// Radix for the next stage of a length-n transform over L lines on THR
+// threads: 16 first for n >= 128 (one LDS pass fewer than 8 x 8 x 2 for
+// 128) when the stage still gives at least half the threads a butterfly.
+__host__ __device__ constexpr int pick_radix_for(int m, int n, int L, int thr) {
+  return (m % 16 == 0 && n >= 128 && 2 * L * (n / 16) >= thr) ? 16 : pick_radix(m);
+}
+
 // One in-place Stockham stage over L lines: every thread loads its
 // butterflies into registers, the block synchronises, then stores.
 template <class T, bool INV, int R, int N, int NS, int L>
@@ -78,6 +85,7 @@ __device__ inline void stage_ip(cplx<T>* buf, int ls, const cplx<T>* __restrict_
       if constexpr (R == 4) bfly4<T, INV>(v[q]);
       if constexpr (R == 5) bfly5<T, INV>(v[q]);
       if constexpr (R == 8) bfly8<T, INV>(v[q]);
+      if constexpr (R == 16) bfly16<T, INV>(v[q]);
       dst[q] = l * ls + (j / NS) * NS * R + k;
     }
   }
@@ -95,7 +103,7 @@ __device__ inline void stage_ip(cplx<T>* buf, int ls, const cplx<T>* __restrict_
 template <class T, bool INV, int N, int NS, int REM, int L>
 __device__ inline void stages_ip(cplx<T>* buf, int ls, const cplx<T>* __restrict__ tw) {
   if constexpr (REM > 1) {
-    constexpr int R = pick_radix(REM);
+    constexpr int R = pick_radix_for(REM, N, L, kThreads);
     stage_ip<T, INV, R, N, NS, L>(buf, ls, tw);
     stages_ip<T, INV, N, NS * R, REM / R, L>(buf, ls, tw);
   }
@@ -376,6 +384,95 @@ __global__ void __launch_bounds__(kThreads) k_fft_zconv(ZArgs<T> a, Grid g, FftP
   }
 }
 
+// Z pass, persistent form for compile-time lengths.  gridDim.x workgroups walk
+// the work items (ky tile, kx, job), ky tile fastest, with stride gridDim.x.
+// While one item is transformed in LDS, the next item's column tile and
+// kernel values are already on their way into registers, so HBM traffic
+// overlaps the LDS work instead of every workgroup loading, transforming and
+// storing in lockstep.  Twiddles are staged in LDS once: no other global load
+// is outstanding behind the prefetch during the transform.  Arithmetic is
+// that of k_fft_zconv (same stages, twiddles and kernel product).
+template <class T, int N, int TY>
+__host__ __device__ constexpr size_t zpipe_bytes() {
+  return (size_t)(TY * (N + 1) + N) * sizeof(cplx<T>);
+}
+
+template <class T, int N, int TY>
+__global__ void __launch_bounds__(kThreads) k_fft_zpipe(ZArgs<T> a, int njobs, Grid g,
+                                                        const cplx<T>* __restrict__ twg) {
+  static_assert(N > 0 && kThreads % TY == 0, "tile width must divide the workgroup");
+  extern __shared__ unsigned char smem_raw[];
+  constexpr int ls = N + 1;
+  constexpr int RSTEP = kThreads / TY;            // rows between a thread's values
+  constexpr int PER = (N + RSTEP - 1) / RSTEP;    // values per thread
+  constexpr bool FULL = N % RSTEP == 0;           // every thread's last row exists
+  cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* tw = tile + TY * ls;
+  const int l = threadIdx.x % TY;         // ky within the tile
+  const int j0 = threadIdx.x / TY;        // first kz row of this thread
+  const int ntile = (g.ny + TY - 1) / TY;
+  const int per_job = ntile * g.mx;
+  const int total = per_job * njobs;
+  const long long pstride = (long long)g.mx * g.ny;
+  for (int i = threadIdx.x; i < N; i += kThreads) tw[i] = twg[i];
+
+  cplx<T> v[PER];
+  T kv[PER];
+  auto fetch = [&](int item) {
+    const int jb = item / per_job;
+    const int r = item - jb * per_job;
+    const int kx = r / ntile;
+    const int ky = (r - kx * ntile) * TY + l;
+    const bool ok = ky < g.ny;
+    const long long off = (long long)kx * g.ny + (ok ? ky : 0);
+    const cplx<T>* src = reinterpret_cast<const cplx<T>*>(a.in[jb]) + off;
+    const T* kp = a.kern[jb] + off;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const bool in = ok && (FULL || j0 + q * RSTEP < N);
+      const long long o = in ? (long long)(j0 + q * RSTEP) * pstride : 0;
+      v[q] = in ? src[o] : mk(T(0), T(0));
+      kv[q] = in ? kp[o] : T(0);
+    }
+  };
+
+  int w = blockIdx.x;
+  if (w < total) fetch(w);
+  for (; w < total; w += gridDim.x) {
+    const int jb = w / per_job;
+    const int r = w - jb * per_job;
+    const int kx = r / ntile;
+    const int ky = (r - kx * ntile) * TY + l;
+    T kc[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      if (FULL || j0 + q * RSTEP < N) tile[l * ls + j0 + q * RSTEP] = v[q];
+      kc[q] = kv[q];
+    }
+    __syncthreads();
+    if (w + (int)gridDim.x < total) fetch(w + gridDim.x);
+    stages_ip<T, false, N, 1, N, TY>(tile, ls, tw);
+    const T sg = a.ksign[jb];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      if (FULL || j0 + q * RSTEP < N) {
+        cplx<T>& x = tile[l * ls + j0 + q * RSTEP];
+        const cplx<T> p = mk(x.x * kc[q], x.y * kc[q]);
+        x = sg < T(0) ? mk(-p.x, -p.y) : p;
+      }
+    }
+    __syncthreads();
+    stages_ip<T, true, N, 1, N, TY>(tile, ls, tw);
+    if (ky < g.ny) {
+      cplx<T>* dst = reinterpret_cast<cplx<T>*>(a.out[jb]) + (long long)kx * g.ny + ky;
+#pragma unroll
+      for (int q = 0; q < PER; ++q)
+        if (FULL || j0 + q * RSTEP < N) dst[(long long)(j0 + q * RSTEP) * pstride] = tile[l * ls + j0 + q * RSTEP];
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace hetd
 
 namespace hetd {
@@ -423,6 +520,7 @@ __device__ inline void stage_g(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
       if constexpr (R == 4) bfly4<T, INV>(v[q]);
       if constexpr (R == 5) bfly5<T, INV>(v[q]);
       if constexpr (R == 8) bfly8<T, INV>(v[q]);
+      if constexpr (R == 16) bfly16<T, INV>(v[q]);
       dst[q] = l * LS + ((j / NS) * NS * R + k) * ES;
     }
   }
@@ -440,7 +538,7 @@ __device__ inline void stage_g(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
 template <class T, bool INV, int N, int NS, int REM, int L, int THR, int LS, int ES>
 __device__ inline void stages_g(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
   if constexpr (REM > 1) {
-    constexpr int R = pick_radix(REM);
+    constexpr int R = pick_radix_for(REM, N, L, THR);
     stage_g<T, INV, R, N, NS, L, THR, LS, ES>(buf, tw);
     stages_g<T, INV, N, NS * R, REM / R, L, THR, LS, ES>(buf, tw);
   }
@@ -449,6 +547,83 @@ __device__ inline void stages_g(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
 template <class T>
 __host__ __device__ constexpr size_t plane_bytes(int mh, int ny) {
   return (size_t)(mh + 1) * ny * sizeof(cplx<T>);
+}
+
+// Producer values of a real z plane (NY rows of NX) into LDS rows of stride
+// 2M reals.  Each thread's loads are all issued before the first value is
+// used (in chunks for the six-input producer), so a workgroup keeps its whole
+// plane in flight; the arithmetic is that of xf_value.
+template <class T, class R, int NX, int NY, int M, int THR>
+__device__ inline void load_plane_real(const XfArgs<R>& a, int f, long long src0, T* Pr) {
+  constexpr int NE = NX * NY;
+  constexpr int CNT = (NE + THR - 1) / THR;
+  constexpr bool FULL = NE % THR == 0;
+  const int tid = threadIdx.x;
+  // element q of this thread exists (the last round may be partial)
+  auto has = [&](int q) { return FULL || tid + q * THR < NE; };
+  auto put = [&](int q, double v) {
+    if (!has(q)) return;
+    const int e = tid + q * THR;
+    const int y = e / NX, c = e - y * NX;
+    Pr[2 * M * y + c] = (T)v;
+  };
+  // index of element q, clamped to a valid element for the partial round
+  auto at = [&](int q) { return src0 + (has(q) ? tid + q * THR : 0); };
+  switch (a.op[f]) {
+    case XF_CORREL: {
+      R u[CNT], w[CNT];
+#pragma unroll
+      for (int q = 0; q < CNT; ++q) {
+        u[q] = a.in[f][0][at(q)];
+        w[q] = a.in[f][1][at(q)];
+      }
+      const double c0 = a.c[f][0], c1 = a.c[f][1];
+#pragma unroll
+      for (int q = 0; q < CNT; ++q) {
+        const double den = (double)u[q], dcg = (double)w[q];
+        put(q, den * dcg * (c0 + c1 * dcg));
+      }
+      break;
+    }
+    case XF_MUL: {
+      R u[CNT], w[CNT];
+#pragma unroll
+      for (int q = 0; q < CNT; ++q) {
+        u[q] = a.in[f][0][at(q)];
+        w[q] = a.in[f][1][at(q)];
+      }
+#pragma unroll
+      for (int q = 0; q < CNT; ++q) put(q, (double)u[q] * (double)w[q]);
+      break;
+    }
+    case XF_GRAD_DOT: {
+      constexpr int CH = CNT >= 4 ? 4 : CNT;
+#pragma unroll
+      for (int q0 = 0; q0 < CNT; q0 += CH) {
+        R v[6][CH];
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+#pragma unroll
+          for (int q = 0; q < CH; ++q)
+            if (q0 + q < CNT) v[k][q] = a.in[f][k][at(q0 + q)];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+          if (q0 + q >= CNT) continue;
+          const double s = (double)v[0][q] * (double)v[3][q] + (double)v[1][q] * (double)v[4][q];
+          put(q0 + q, s + (double)v[2][q] * (double)v[5][q]);
+        }
+      }
+      break;
+    }
+    default: {
+      R u[CNT];
+#pragma unroll
+      for (int q = 0; q < CNT; ++q) u[q] = a.in[f][0][at(q)];
+#pragma unroll
+      for (int q = 0; q < CNT; ++q) put(q, (double)u[q]);
+      break;
+    }
+  }
 }
 
 // r2c in x and y of one z plane.  grid = (nz, nfields), kPlaneThreads.
@@ -466,20 +641,20 @@ __global__ void __launch_bounds__(kPlaneThreads) k_fft_xyf(XfArgs<R> a, Grid g, 
   const int iz = blockIdx.x;
   const int f = blockIdx.y;
   const long long src0 = (long long)iz * NY * NX;
-  for (int e = threadIdx.x; e < NY * NX; e += THR) {
-    const int y = e / NX, c = e - y * NX;
-    Pr[2 * M * y + c] = (T)xf_value(a, f, src0 + e);
-  }
+  load_plane_real<T, R, NX, NY, M, THR>(a, f, src0, Pr);
   __syncthreads();
   stages_g<T, false, MH, 1, MH, NY, THR, M, 1>(P, twx);
-  // Hermitian split per row (pairs k, MH-k)
-  constexpr int HALF = MH / 2 + 1;
-  for (int e = threadIdx.x; e < NY * HALF; e += THR) {
-    const int y = e / HALF, k = e - y * HALF;
+  // Hermitian split per row, one item per pair (k, MH-k), k = 1..MH/2.  The
+  // k = MH/2 item also forms the DC and Nyquist values (both real) and
+  // stores them packed as X0 + i X_MH in column 0, so that the y transform
+  // runs over MH columns: one full round of butterflies per stage.
+  constexpr int QH = MH / 2;
+  for (int e = threadIdx.x; e < NY * QH; e += THR) {
+    const int y = e / QH, k = 1 + (e - y * QH);
     cplx<T>* z = P + y * M;
     const int k2 = MH - k;
     const cplx<T> za = z[k];
-    const cplx<T> zb = z[k == 0 ? 0 : k2];
+    const cplx<T> zb = z[k2];
     {
       const cplx<T> zc = conj(zb);
       const cplx<T> E = scale(za + zc, T(0.5));
@@ -494,9 +669,28 @@ __global__ void __launch_bounds__(kPlaneThreads) k_fft_xyf(XfArgs<R> a, Grid g, 
       const cplx<T> O = mk(T(0.5) * D.y, T(-0.5) * D.x);
       z[k2] = E + cmul(twh[k2], O);
     }
+    if (k == QH) {
+      const cplx<T> z0 = z[0];
+      z[0] = mk(z0.x + z0.y, z0.x - z0.y);
+    }
   }
   __syncthreads();
-  stages_g<T, false, NY, 1, NY, M, THR, 1, M>(P, twy);
+  stages_g<T, false, NY, 1, NY, MH, THR, 1, M>(P, twy);
+  // unpack column 0: W = FFT(X0) + i FFT(X_MH) with both inputs real, so
+  // FFT(X0)[k] = (W[k] + conj W[-k])/2 and FFT(X_MH)[k] = -i (W[k] - conj W[-k])/2
+  for (int ky = threadIdx.x; ky <= NY / 2; ky += THR) {
+    const int km = (NY - ky) % NY;
+    const cplx<T> wa = P[ky * M], wb = P[km * M];
+    const cplx<T> sa = wa + conj(wb), da = wa - conj(wb);
+    const cplx<T> sb = wb + conj(wa), db = wb - conj(wa);
+    P[ky * M] = scale(sa, T(0.5));
+    P[ky * M + MH] = mk(T(0.5) * da.y, T(-0.5) * da.x);
+    if (km != ky) {
+      P[km * M] = scale(sb, T(0.5));
+      P[km * M + MH] = mk(T(0.5) * db.y, T(-0.5) * db.x);
+    }
+  }
+  __syncthreads();
   cplx<T>* out = reinterpret_cast<cplx<T>*>(a.out[f]) + (long long)iz * M * NY;
   for (int e = threadIdx.x; e < M * NY; e += THR) {
     const int kx = e / NY, ky = e - kx * NY;
@@ -518,15 +712,39 @@ __global__ void __launch_bounds__(kPlaneThreads) k_fft_xyi(XiArgs<T, R> a, Grid 
   const int iz = blockIdx.x;
   const int f = blockIdx.y;
   const cplx<T>* in = reinterpret_cast<const cplx<T>*>(a.in[f]) + (long long)iz * M * NY;
-  for (int e = threadIdx.x; e < M * NY; e += THR) {
-    const int kx = e / NY, ky = e - kx * NY;
-    P[ky * M + kx] = in[e];
+  // Columns kx = 0 and MH have Hermitian symmetry in ky (their inverse y
+  // transforms are real): they travel packed as col0 + i col_MH in column 0,
+  // so the y transform runs over MH columns.
+  {
+    constexpr int NE = MH * NY;
+    constexpr int CNT = (NE + THR - 1) / THR;
+    constexpr bool FULL = NE % THR == 0;
+    cplx<T> v[CNT];
+#pragma unroll
+    for (int q = 0; q < CNT; ++q) {
+      const int e = threadIdx.x + q * THR;
+      v[q] = (FULL || e < NE) ? in[e] : mk(T(0), T(0));
+    }
+    // threads holding kx = 0 (the first NY elements) also read column MH
+    cplx<T> u = mk(T(0), T(0));
+    if (threadIdx.x < NY) u = in[MH * NY + threadIdx.x];
+#pragma unroll
+    for (int q = 0; q < CNT; ++q) {
+      const int e = threadIdx.x + q * THR;
+      if (!(FULL || e < NE)) continue;
+      const int kx = e / NY, ky = e - kx * NY;
+      cplx<T> w = v[q];
+      if (kx == 0) w = mk(w.x - u.y, w.y + u.x);
+      P[ky * M + kx] = w;
+    }
   }
   __syncthreads();
-  stages_g<T, true, NY, 1, NY, M, THR, 1, M>(P, twy);
-  constexpr int HALF = MH / 2 + 1;
-  for (int e = threadIdx.x; e < NY * HALF; e += THR) {
-    const int y = e / HALF, k = e - y * HALF;
+  stages_g<T, true, NY, 1, NY, MH, THR, 1, M>(P, twy);
+  // Hermitian merge per row, one item per pair (k, MH-k), k = 1..MH/2; the
+  // k = MH/2 item also merges the real DC and Nyquist values held in column 0.
+  constexpr int QH = MH / 2;
+  for (int e = threadIdx.x; e < NY * QH; e += THR) {
+    const int y = e / QH, k = 1 + (e - y * QH);
     cplx<T>* x = P + y * M;
     const int k2 = MH - k;
     const cplx<T> xa = x[k], xb = x[k2];
@@ -536,11 +754,18 @@ __global__ void __launch_bounds__(kPlaneThreads) k_fft_xyi(XiArgs<T, R> a, Grid 
       const cplx<T> d = cmulc(xa - xc, twh[k]);
       x[k] = mk(s.x - d.y, s.y + d.x);
     }
-    if (k != 0 && k2 != k) {
+    if (k2 != k) {
       const cplx<T> xc = conj(xa);
       const cplx<T> s = xb + xc;
       const cplx<T> d = cmulc(xb - xc, twh[k2]);
       x[k2] = mk(s.x - d.y, s.y + d.x);
+    }
+    if (k == QH) {
+      const cplx<T> w = x[0];
+      const cplx<T> x0 = mk(w.x, T(0)), xm = mk(w.y, T(0));
+      const cplx<T> s = x0 + xm;
+      const cplx<T> d = cmulc(x0 - xm, twh[0]);
+      x[0] = mk(s.x - d.y, s.y + d.x);
     }
   }
   __syncthreads();

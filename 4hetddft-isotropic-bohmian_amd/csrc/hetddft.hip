@@ -4,6 +4,8 @@
 // work of the reference (steprk.f90, steppc.f90, potenimp.f90, poten.f90,
 // term_alfa.f90, energy.f90) is issued as a fixed sequence of hand-written
 // kernels on one HIP stream; history slots rotate by pointer, never by copy.
+#include <hip/hip_ext.h>
+
 #include "../../include/hetddft.h"
 #include "common.h"
 #include "fft_kernels.h"
@@ -92,6 +94,10 @@ struct FftKernelsT {
   void (*zc)(ZArgs<T>, Grid, FftPlan, const cplx<T>*) = nullptr;
   int tyz = 16;        // z-pass tile width (consecutive ky)
   size_t lds_x = 0, lds_y = 0, lds_z = 0;
+  // persistent z pass (compile-time lengths)
+  void (*zp)(ZArgs<T>, int, Grid, const cplx<T>*) = nullptr;
+  size_t lds_zp = 0;
+  int typ = 0;
   // whole-plane x+y kernels (when a plane fits in LDS) and the wide z pass
   void (*xyf)(XfArgs<T>, Grid, const cplx<T>*, const cplx<T>*, const cplx<T>*) = nullptr;
   void (*xyi)(XiArgs<T, T>, Grid, const cplx<T>*, const cplx<T>*, const cplx<T>*) = nullptr;
@@ -124,6 +130,12 @@ struct hetddft_ctx {
   Grid g{};
   double hx = 0, hy = 0, hz = 0, dxyz = 0, renor = 0, m_imp = 0, h2o2mx = 0;
   hipStream_t st = nullptr;
+  // Side stream for the particle work (histogram, ironing, force gather,
+  // particle predictor/corrector), which is latency-bound and runs beside the
+  // field kernels on st.  Every entry point forks st2 from st on entry and
+  // joins it back before returning, so callers see one ordered stream.
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_chain = nullptr, ev_b3 = nullptr, ev_join = nullptr;
   std::string err;
   long long bytes = 0;
   bool kernels_set = false, started = false;
@@ -134,6 +146,8 @@ struct hetddft_ctx {
   Stencil13 d1x{}, d1y{}, d1z{}, d2x{}, d2y{}, d2z{};
   FftPlan pl_xh{}, pl_y{}, pl_z{};
   int zc = 16;  // z planes per workgroup of the stencil engine
+  int cus = 256;      // compute units of the device
+  int zp_per_cu = 4;  // resident workgroups per CU of the persistent z pass
   bool st25 = true;  // z-marching stencil engine (false: direct neighbour loads)
 
   // device memory
@@ -160,6 +174,25 @@ struct hetddft_ctx {
   unsigned int* counts = nullptr;
   int* reg = nullptr;           // occupied-region bookkeeping (field_kernels.h)
   double* fcorner = nullptr;    // force at the 8 cell corners of each particle
+  // The particle density chain (histogram -> ironing -> psix -> hpsix) has
+  // two buffer sets, each with its own counts and live/work regions.  A chain
+  // queued on st2 writes the set that is not current; the potential
+  // evaluation that consumes it makes that set current (fld[] entries,
+  // counts, reg, denx).  Fields read between steps always belong to the
+  // positions of the last potential evaluation, as in the reference.
+  struct ChainSet {
+    void* denxa = nullptr;
+    void* denxb = nullptr;
+    void* psix = nullptr;
+    void* hpsix = nullptr;
+    unsigned int* counts = nullptr;
+    int* reg = nullptr;
+    void* denx = nullptr;
+  };
+  ChainSet cset[2];
+  int cs = 0;
+  bool chain_pending = false;  // a chain for the next potential evaluation is queued on st2
+  bool side_pending = false;   // st2 holds work that st has not waited for
   // particles
   double* rimp = nullptr;
   double* vimp = nullptr;
@@ -209,23 +242,32 @@ hipEvent_t prof_event(hetddft_ctx* c) {
   return e;
 }
 
-// Brackets one launch of kernel class `cls` with events when profiling.
+// Times one launch of kernel class `cls` when profiling: the launch goes
+// through PLAUNCH, which hands events a and b to hipExtLaunchKernel, so they
+// carry the dispatch's own start and end timestamps (the kernel's execution
+// time, as a profiler's kernel trace reports it, without queueing gaps).
+// With profiling off both events are null and PLAUNCH is a plain launch.
 struct ProfScope {
   hetddft_ctx* c;
   int cls;
   double bytes;
-  hipEvent_t a = nullptr;
-  ProfScope(hetddft_ctx* ctx, int k, double b) : c(ctx), cls(k), bytes(b) {
-    if (c->prof_on && (a = prof_event(c))) hipEventRecord(a, c->st);
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(hetddft_ctx* ctx, int k, double by) : c(ctx), cls(k), bytes(by) {
+    if (!c->prof_on) return;
+    a = prof_event(c);
+    b = a ? prof_event(c) : nullptr;
+    if (a && !b) {
+      c->prof_pool.push_back(a);
+      a = nullptr;
+    }
   }
   ~ProfScope() {
-    if (!a) return;
-    hipEvent_t b = prof_event(c);
-    if (!b) return;
-    hipEventRecord(b, c->st);
-    c->prof_pending.push_back({a, b, cls, bytes});
+    if (a && b) c->prof_pending.push_back({a, b, cls, bytes});
   }
 };
+
+#define PLAUNCH(ps, kern, grid, block, lds, ...) \
+  hipExtLaunchKernelGGL(kern, grid, block, lds, c->st, (ps).a, (ps).b, 0u, __VA_ARGS__)
 
 int fail(hetddft_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -255,6 +297,20 @@ int fail(hetddft_ctx* c, int code, const char* fmt, ...) {
     int r_ = (expr);          \
     if (r_ != HETDDFT_OK) return r_; \
   } while (0)
+
+// Work queued on st so far precedes work queued on st2 from now on.
+int fork_side(hetddft_ctx* c) {
+  HIPCHK(c, hipEventRecord(c->ev_fork, c->st));
+  HIPCHK(c, hipStreamWaitEvent(c->st2, c->ev_fork, 0));
+  return HETDDFT_OK;
+}
+
+// Work queued on st2 so far precedes work queued on st from now on.
+int join_side(hetddft_ctx* c, hipEvent_t e) {
+  HIPCHK(c, hipEventRecord(e, c->st2));
+  HIPCHK(c, hipStreamWaitEvent(c->st, e, 0));
+  return HETDDFT_OK;
+}
 
 int dalloc(hetddft_ctx* c, void** p, size_t bytes) {
   *p = nullptr;
@@ -321,6 +377,24 @@ void pick_z(FftKernels<T>& k, int n) {
     }
   }
   k.lds_z = (N > 0 ? 1 : 2) * tile_bytes<T>(k.tyz, n);
+  if constexpr (N > 0) {
+    const char* e = std::getenv("HETDDFT_ZPIPE");
+    if (!(e && std::atoi(e) == 0)) {
+      const char* t = std::getenv("HETDDFT_ZPTY");
+      const int want = t ? std::atoi(t) : 0;
+      if constexpr (N <= 128) {
+        if (want == 2 * TY) {
+          k.zp = k_fft_zpipe<T, N, 2 * TY>;
+          k.lds_zp = zpipe_bytes<T, N, 2 * TY>();
+          k.typ = 2 * TY;
+          return;
+        }
+      }
+      k.zp = k_fft_zpipe<T, N, TY>;
+      k.lds_zp = zpipe_bytes<T, N, TY>();
+      k.typ = TY;
+    }
+  }
 }
 
 template <class T, int MH, int NY>
@@ -431,6 +505,7 @@ int fft_alloc_tables(hetddft_ctx* c, FftSet& fs) {
     HIPCHK(c, hipFuncSetAttribute((const void*)k.xyi, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   }
   if (k.zw) HIPCHK(c, hipFuncSetAttribute((const void*)k.zw, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  if (k.zp) HIPCHK(c, hipFuncSetAttribute((const void*)k.zp, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   return HETDDFT_OK;
 }
 
@@ -446,13 +521,24 @@ struct ConvJob {
 };
 
 // Five passes over HBM (see fft_kernels.h): inputs i < nin are formed by
-// the x-pass producers in xf_in; every job reads one input spectrum.
+// the x-pass producers in xf_in; every job reads one input spectrum.  When
+// each input feeds exactly one job, the z pass works in place (a work item
+// reads its whole column tile before writing it back, and distinct items
+// cover disjoint columns), so a batch needs only nin spectrum buffers.
 template <class T>
 int conv_batch(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin,
                const std::vector<ConvJob<T>>& jobs) {
   const Grid& g = c->g;
   const int nout = (int)jobs.size();
-  if (nin < 1 || nout < 1 || nin > kMaxBatch || nout > kMaxBatch || nin + nout > kSpectra) return fail(c, HETDDFT_E_ARG, "conv_batch: %d in / %d out", nin, nout);
+  int uses[kMaxBatch] = {};
+  for (const ConvJob<T>& j : jobs)
+    if (j.input >= 0 && j.input < kMaxBatch) uses[j.input]++;
+  bool inplace = nout == nin;
+  for (int i = 0; i < nin && inplace; ++i) inplace = uses[i] == 1;
+  if (nin < 1 || nout < 1 || nin > kMaxBatch || nout > kMaxBatch || (!inplace && nin + nout > kSpectra))
+    return fail(c, HETDDFT_E_ARG, "conv_batch: %d in / %d out", nin, nout);
+  for (const ConvJob<T>& j : jobs)
+    if (j.input < 0 || j.input >= nin) return fail(c, HETDDFT_E_ARG, "conv_batch: job input %d", j.input);
   XfArgs<T> xa = xf_in;
   for (int i = 0; i < nin; ++i) xa.out[i] = fs.spec[i];
   FftKernels<T>& K = fft_of<T>(c);
@@ -468,20 +554,20 @@ int conv_batch(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin,
   }
   if (K.xyf) {
     ProfScope ps(c, HETDDFT_PROF_XYF, in_bytes + nin * Sb);
-    K.xyf<<<dim3(g.nz, nin), dim3(kPlaneThreads), K.lds_xy, c->st>>>(
+    PLAUNCH(ps, K.xyf, dim3(g.nz, nin), dim3(kPlaneThreads), K.lds_xy,
         xa, g, (const cplx<T>*)fs.tw_xh, (const cplx<T>*)fs.twh, (const cplx<T>*)fs.tw_y);
     LAUNCHCHK(c);
   } else {
     {
       ProfScope ps(c, HETDDFT_PROF_XF, in_bytes + nin * Sb);
-      K.xf<<<gx, dim3(kThreads), fs.lds_x, c->st>>>(xa, g, c->pl_xh, (const cplx<T>*)fs.tw_xh,
+      PLAUNCH(ps, K.xf, gx, dim3(kThreads), fs.lds_x, xa, g, c->pl_xh, (const cplx<T>*)fs.tw_xh,
                                                     (const cplx<T>*)fs.twh);
     }
     LAUNCHCHK(c);
     YArgs<T> ya{};
     for (int i = 0; i < nin; ++i) ya.buf[i] = fs.spec[i];
     ProfScope ps(c, HETDDFT_PROF_YF, nin * 2.0 * Sb);
-    K.yf<<<dim3(ylines, 1, nin), dim3(kThreads), fs.lds_y, c->st>>>(ya, g, c->pl_y, (const cplx<T>*)fs.tw_y);
+    PLAUNCH(ps, K.yf, dim3(ylines, 1, nin), dim3(kThreads), fs.lds_y, ya, g, c->pl_y, (const cplx<T>*)fs.tw_y);
     LAUNCHCHK(c);
   }
   ZArgs<T> za{};
@@ -493,37 +579,44 @@ int conv_batch(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin,
     za.in[k] = fs.spec[j.input];
     za.kern[k] = (const T*)j.kern;
     za.ksign[k] = (T)j.sign;
-    za.out[k] = fs.spec[nin + k];
-    yb.buf[k] = fs.spec[nin + k];
-    xi.in[k] = fs.spec[nin + k];
+    void* os = inplace ? fs.spec[j.input] : fs.spec[nin + k];
+    za.out[k] = os;
+    yb.buf[k] = os;
+    xi.in[k] = os;
     xi.out[k] = (T*)j.out;
     xi.epi[k] = j.epi;
     xi.ec[k] = j.ec;
   }
-  if (K.zw) {
+  if (K.zp) {
+    // persistent grid: a few workgroups per CU, each walking the work items
+    const long long items = (long long)((g.ny + K.typ - 1) / K.typ) * g.mx * nout;
+    const unsigned grid = (unsigned)std::min<long long>(items, (long long)c->cus * c->zp_per_cu);
     ProfScope ps(c, HETDDFT_PROF_Z, nout * (2.0 * Sb + Kb));
-    K.zw<<<dim3((g.ny + K.tyw - 1) / K.tyw, g.mx, nout), dim3(K.thrw), K.lds_zw, c->st>>>(za, g,
+    PLAUNCH(ps, K.zp, dim3(grid), dim3(kThreads), K.lds_zp, za, nout, g, (const cplx<T>*)fs.tw_z);
+  } else if (K.zw) {
+    ProfScope ps(c, HETDDFT_PROF_Z, nout * (2.0 * Sb + Kb));
+    PLAUNCH(ps, K.zw, dim3((g.ny + K.tyw - 1) / K.tyw, g.mx, nout), dim3(K.thrw), K.lds_zw, za, g,
                                                                                          (const cplx<T>*)fs.tw_z);
   } else {
     ProfScope ps(c, HETDDFT_PROF_Z, nout * (2.0 * Sb + Kb));
-    K.zc<<<dim3((g.ny + K.tyz - 1) / K.tyz, g.mx, nout), dim3(kThreads), fs.lds_z, c->st>>>(
+    PLAUNCH(ps, K.zc, dim3((g.ny + K.tyz - 1) / K.tyz, g.mx, nout), dim3(kThreads), fs.lds_z,
         za, g, c->pl_z, (const cplx<T>*)fs.tw_z);
   }
   LAUNCHCHK(c);
   if (K.xyi) {
     ProfScope ps(c, HETDDFT_PROF_XYI, nout * (Sb + Rb));
-    K.xyi<<<dim3(g.nz, nout), dim3(kPlaneThreads), K.lds_xy, c->st>>>(
+    PLAUNCH(ps, K.xyi, dim3(g.nz, nout), dim3(kPlaneThreads), K.lds_xy,
         xi, g, (const cplx<T>*)fs.tw_xh, (const cplx<T>*)fs.twh, (const cplx<T>*)fs.tw_y);
     LAUNCHCHK(c);
   } else {
     {
       ProfScope ps(c, HETDDFT_PROF_YI, nout * 2.0 * Sb);
-      K.yi<<<dim3(ylines, 1, nout), dim3(kThreads), fs.lds_y, c->st>>>(yb, g, c->pl_y, (const cplx<T>*)fs.tw_y);
+      PLAUNCH(ps, K.yi, dim3(ylines, 1, nout), dim3(kThreads), fs.lds_y, yb, g, c->pl_y, (const cplx<T>*)fs.tw_y);
     }
     LAUNCHCHK(c);
     ProfScope ps(c, HETDDFT_PROF_XI, nout * (Sb + Rb));
     const dim3 gxi((unsigned)((g.ny + kRowsX - 1) / kRowsX), (unsigned)g.nz, (unsigned)nout);
-    K.xi<<<gxi, dim3(kThreads), fs.lds_x, c->st>>>(xi, g, c->pl_xh, (const cplx<T>*)fs.tw_xh,
+    PLAUNCH(ps, K.xi, gxi, dim3(kThreads), fs.lds_x, xi, g, c->pl_xh, (const cplx<T>*)fs.tw_xh,
                                                    (const cplx<T>*)fs.twh);
     LAUNCHCHK(c);
   }
@@ -560,59 +653,113 @@ BoxArgs box_args(const hetddft_ctx* c) {
   return b;
 }
 
-// Grid-stride launches over a device-side region: enough blocks to cover a
-// 40^3 box in one sweep; larger regions loop.
-constexpr unsigned kRegionBlocks = 256;
+// Grid-stride launches over a device-side region.  The chain runs beside
+// the field kernels, so it takes few CUs (HETDDFT_REGION_BLOCKS overrides).
+unsigned region_blocks() {
+  static const unsigned n = [] {
+    const char* e = std::getenv("HETDDFT_REGION_BLOCKS");
+    const int v = e ? std::atoi(e) : 0;
+    return v > 0 ? (unsigned)v : 256u;
+  }();
+  return n;
+}
 
-// potenimp(); poten(); forceimp()
+// Particle density chain of potenimp (denpart.f90, ironing.f90).
 template <class T>
-int veval(hetddft_ctx* c) {
+int issue_chain(hetddft_ctx* c) {
   const Grid& g = c->g;
-  auto F = [&](int id) { return (T*)c->fld[id]; };
-  const bool otc = c->p.core_otc != 0;
-  // ---- potenimp: denpart (denpart.f90:5-49) on the occupied region only
+  hetddft_ctx::ChainSet& S = c->cset[c->cs ^ 1];
+  // ---- potenimp: denpart (denpart.f90:5-49) on the occupied region only,
+  // queued on the side stream into the set that is not current
   BoxArgs b = box_args(c);
-  hipLaunchKernelGGL(k_clear_counts, dim3(kRegionBlocks), dim3(kBlock), 0, c->st, g, c->reg, c->counts);
+  hipLaunchKernelGGL(k_clear_counts, dim3(region_blocks()), dim3(kBlock), 0, c->st2, g, S.reg, S.counts);
   LAUNCHCHK(c);
   if (c->p.n_par > 0) {
-    hipLaunchKernelGGL(k_hist, dim3(blocks_for(c->p.n_par)), dim3(kBlock), 0, c->st, b, c->rimp, c->counts, c->reg);
+    hipLaunchKernelGGL(k_hist, dim3(blocks_for(c->p.n_par)), dim3(kBlock), 0, c->st2, b, c->rimp, S.counts, S.reg);
     LAUNCHCHK(c);
   }
-  hipLaunchKernelGGL(k_reg_union, dim3(1), dim3(64), 0, c->st, g, c->reg, c->p.ironx + 6);
+  hipLaunchKernelGGL(k_reg_union, dim3(1), dim3(64), 0, c->st2, g, S.reg, c->p.ironx + 6);
   LAUNCHCHK(c);
   const double hxyz = c->hx * c->hy * c->hz;
   const double npar = (double)c->p.n_par;
-  T* cur = F(hetddft_ctx::F_DENXA);
-  T* oth = F(hetddft_ctx::F_DENXB);
-  T* psix = F(hetddft_ctx::F_PSIX);
+  T* cur = (T*)S.denxa;
+  T* oth = (T*)S.denxb;
+  T* psix = (T*)S.psix;
   const int npass = c->p.ironx > 0 ? c->p.ironx : 1;
-  hipLaunchKernelGGL((k_iron<T>), dim3(kRegionBlocks), dim3(kBlock), 0, c->st, g, (const int*)c->reg, c->counts,
+  hipLaunchKernelGGL((k_iron<T>), dim3(region_blocks()), dim3(kBlock), 0, c->st2, g, (const int*)S.reg, S.counts,
                      (const T*)nullptr, cur, npass == 1 ? psix : (T*)nullptr, npar, hxyz, c->p.ironx > 0 ? 1 : 0);
   LAUNCHCHK(c);
   for (int it = 1; it < npass; ++it) {
-    hipLaunchKernelGGL((k_iron<T>), dim3(kRegionBlocks), dim3(kBlock), 0, c->st, g, (const int*)c->reg,
+    hipLaunchKernelGGL((k_iron<T>), dim3(region_blocks()), dim3(kBlock), 0, c->st2, g, (const int*)S.reg,
                        (const unsigned int*)nullptr, (const T*)cur, oth, it == npass - 1 ? psix : (T*)nullptr, npar,
                        hxyz, 1);
     LAUNCHCHK(c);
     std::swap(cur, oth);
   }
-  c->denx = cur;
-  hipLaunchKernelGGL((k_hpsix<T>), dim3(kRegionBlocks), dim3(kBlock), 0, c->st, g, (const int*)c->reg,
-                     (const T*)psix, c->d2x, c->d2y, c->d2z, c->h2o2mx, F(hetddft_ctx::F_HPSIX));
+  S.denx = cur;
+  hipLaunchKernelGGL((k_hpsix<T>), dim3(region_blocks()), dim3(kBlock), 0, c->st2, g, (const int*)S.reg,
+                     (const T*)psix, c->d2x, c->d2y, c->d2z, c->h2o2mx, (T*)S.hpsix);
   LAUNCHCHK(c);
+  HIPCHK(c, hipEventRecord(c->ev_chain, c->st2));
+  c->chain_pending = true;
+  c->side_pending = true;
+  return HETDDFT_OK;
+}
+
+// Makes chain set cs current.
+void use_chain_set(hetddft_ctx* c) {
+  const hetddft_ctx::ChainSet& S = c->cset[c->cs];
+  c->fld[hetddft_ctx::F_DENXA] = S.denxa;
+  c->fld[hetddft_ctx::F_DENXB] = S.denxb;
+  c->fld[hetddft_ctx::F_PSIX] = S.psix;
+  c->fld[hetddft_ctx::F_HPSIX] = S.hpsix;
+  c->counts = S.counts;
+  c->reg = S.reg;
+  c->denx = S.denx;
+}
+
+// st waits for everything queued on st2.
+int settle(hetddft_ctx* c) {
+  if (!c->side_pending) return HETDDFT_OK;
+  RET(join_side(c, c->ev_join));
+  c->side_pending = false;
+  return HETDDFT_OK;
+}
+
+// potenimp(); poten(); forceimp().  The particle density chain runs on the
+// side stream (issued here, or by the caller when chain_issued) beside the
+// density-side work on st, which waits for it only where denx enters.
+template <class T>
+int veval(hetddft_ctx* c) {
+  const Grid& g = c->g;
+  auto F = [&](int id) { return (T*)c->fld[id]; };
+  const bool otc = c->p.core_otc != 0;
+  if (!c->chain_pending) {
+    RET(fork_side(c));
+    RET(issue_chain<T>(c));
+  }
+  BoxArgs b = box_args(c);
   // ---- poten: derden (derden.f90:26-28)
   {
   ProfScope psg(c, HETDDFT_PROF_GRAD, 4.0 * (double)g.n * sizeof(T));
   if (c->st25)
-    hipLaunchKernelGGL((k_grad3_25<T>), grid25(c), dim3(S_TX, S_TY), 0, c->st, g, c->zc,
+    PLAUNCH(psg, (k_grad3_25<T>), grid25(c), dim3(S_TX, S_TY), 0, g, c->zc,
                        (const T*)F(hetddft_ctx::F_DEN), c->d1x, c->d1y, c->d1z, F(hetddft_ctx::F_DX),
                        F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ));
   else
-    hipLaunchKernelGGL((k_grad3<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, g,
+    PLAUNCH(psg, (k_grad3<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, g,
                        (const T*)F(hetddft_ctx::F_DEN), c->d1x, c->d1y, c->d1z, F(hetddft_ctx::F_DX),
                        F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ));
   }
   LAUNCHCHK(c);
+  // the particle density chain enters here: its set becomes current
+  HIPCHK(c, hipStreamWaitEvent(c->st, c->ev_chain, 0));
+  c->cs ^= 1;
+  use_chain_set(c);
+  c->chain_pending = false;
+  c->side_pending = false;
+  T* cur = (T*)c->denx;
+  T* psix = F(hetddft_ctx::F_PSIX);
   // batch 1: denx*Vq -> uimp (potenimp.f90:15-17); den*{wcgk, fvlj4, kalfs}
   //          -> dencg, delj4, falfs = 1 - denalf/den0s (poten.f90:27-50,94-99;
   //          term_alfa.f90:28-30)
@@ -681,6 +828,23 @@ int veval(hetddft_ctx* c) {
     }
     RET(conv_batch<T>(c, c->fs, xa, nin, jobs));
   }
+  // ---- forceimp gather (potenimp.f90:52-71)
+  if (c->p.n_par > 0) {
+    GatherArgs ga{};
+    ga.b = b;
+    ga.d1[0] = c->d1x;
+    ga.d1[1] = c->d1y;
+    ga.d1[2] = c->d1z;
+    ga.m_imp = c->m_imp;
+    hipLaunchKernelGGL((k_force_corners<T>), dim3(blocks_for(24LL * c->p.n_par)), dim3(kBlock), 0, c->st, ga,
+                       c->rimp, (const T*)F(hetddft_ctx::F_FCX), (const T*)F(hetddft_ctx::F_FCY),
+                       (const T*)F(hetddft_ctx::F_FCZ), (const T*)cur, (const T*)psix,
+                       (const T*)F(hetddft_ctx::F_HPSIX), c->fcorner);
+    LAUNCHCHK(c);
+    hipLaunchKernelGGL(k_blend, dim3(blocks_for(3LL * c->p.n_par)), dim3(kBlock), 0, c->st, b, c->rimp,
+                       (const double*)c->fcorner, c->aimp);
+    LAUNCHCHK(c);
+  }
   // pot4 assembly with the alpha_s divergence (term_alfa.f90:85-113, poten.f90:80-124)
   PotArgs pa{};
   pa.g = g;
@@ -713,7 +877,7 @@ int veval(hetddft_ctx* c) {
     p5.otc = pa.otc;
     p5.zc = c->zc;
     ProfScope ps(c, HETDDFT_PROF_POT, (otc ? 11.0 : 6.0) * (double)g.n * sizeof(T));
-    hipLaunchKernelGGL((k_pot_final25<T>), grid25(c), dim3(S_TX, S_TY), 0, c->st, p5,
+    PLAUNCH(ps, (k_pot_final25<T>), grid25(c), dim3(S_TX, S_TY), 0, p5,
                        (const T*)F(hetddft_ctx::F_DEN), (const T*)F(hetddft_ctx::F_DENCG),
                        (const T*)F(hetddft_ctx::F_DELJ4), (const T*)F(hetddft_ctx::F_STO1),
                        (const T*)F(hetddft_ctx::F_UIMP), (const T*)F(hetddft_ctx::F_FALFS),
@@ -728,23 +892,6 @@ int veval(hetddft_ctx* c) {
                        (const T*)F(hetddft_ctx::F_IZ), (const T*)F(hetddft_ctx::F_UACONV), F(hetddft_ctx::F_POT4));
   }
   LAUNCHCHK(c);
-  // ---- forceimp gather (potenimp.f90:52-71)
-  if (c->p.n_par > 0) {
-    GatherArgs ga{};
-    ga.b = b;
-    ga.d1[0] = c->d1x;
-    ga.d1[1] = c->d1y;
-    ga.d1[2] = c->d1z;
-    ga.m_imp = c->m_imp;
-    hipLaunchKernelGGL((k_force_corners<T>), dim3(blocks_for(24LL * c->p.n_par)), dim3(kBlock), 0, c->st, ga,
-                       c->rimp, (const T*)F(hetddft_ctx::F_FCX), (const T*)F(hetddft_ctx::F_FCY),
-                       (const T*)F(hetddft_ctx::F_FCZ), (const T*)cur, (const T*)psix,
-                       (const T*)F(hetddft_ctx::F_HPSIX), c->fcorner);
-    LAUNCHCHK(c);
-    hipLaunchKernelGGL(k_blend, dim3(blocks_for(3LL * c->p.n_par)), dim3(kBlock), 0, c->st, b, c->rimp,
-                       (const double*)c->fcorner, c->aimp);
-    LAUNCHCHK(c);
-  }
   return HETDDFT_OK;
 }
 
@@ -838,17 +985,36 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   const Ham25 ha = ham_args(c);
   int* P = c->ioldp;
   int* H = c->ioldh;
-  // predictor + modifier (steppc.f90:36-62)
+  int* R = c->ioldr;
+  int* V = c->ioldv;
+  int* A = c->iolda;
+  const int n3 = 3 * c->p.n_par;
+  const dim3 g25 = grid25(c);
+  const unsigned nb = c->st25 ? g25.x * g25.y * g25.z : blocks_for(g.n);
+  RET(fork_side(c));
+  // Particle predictor (steppc.f90:64-87) and the density chain of the
+  // predicted positions on the side stream.  They are queued before the psi
+  // predictor so that they are dispatched first and run beside it.
+  if (n3 > 0) {
+    hipLaunchKernelGGL(k_part_predict, dim3(blocks_for(n3)), dim3(kBlock), 0, c->st2, n3, c->rimp,
+                       (const double*)c->vimp, (const double*)c->aimp, c->rold[R[2]], (const double*)c->vold[V[0]],
+                       (const double*)c->vold[V[1]], c->vold[V[2]], (const double*)c->aold[A[0]], c->aold[A[1]],
+                       c->pcr, c->pcv, c4o3 * dt, c112);
+    LAUNCHCHK(c);
+  }
+  std::swap(A[0], A[1]);
+  RET(issue_chain<T>(c));
+  // psi predictor + modifier (steppc.f90:36-62)
   c64* x = c->psi;
   c64* y = c->old[P[2]];
   {
   ProfScope pspred(c, HETDDFT_PROF_PCP, (double)g.n * (16.0 * 8 + 2.0 * sizeof(T)));
   if (c->st25)
-    hipLaunchKernelGGL((k_pc_predict25<T>), grid25(c), dim3(S_TX, S_TY), 0, c->st, ha, (const c64*)x,
+    PLAUNCH(pspred, (k_pc_predict25<T>), g25, dim3(S_TX, S_TY), 0, ha, (const c64*)x,
                        (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]], c->hold[H[1]], c->pc,
                        (T*)c->fld[hetddft_ctx::F_DEN], c4o3 * dt, c112);
   else
-    hipLaunchKernelGGL((k_pc_predict<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, ham_direct(ha),
+    PLAUNCH(pspred, (k_pc_predict<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, ham_direct(ha),
                        (const c64*)x, (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]],
                        c->hold[H[1]], c->pc, (T*)c->fld[hetddft_ctx::F_DEN], c4o3 * dt, c112);
   }
@@ -856,31 +1022,47 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   c->old[P[2]] = x;
   c->psi = y;
   std::swap(H[0], H[1]);
-  int* R = c->ioldr;
-  int* V = c->ioldv;
-  int* A = c->iolda;
-  const int n3 = 3 * c->p.n_par;
-  if (n3 > 0) {
-    hipLaunchKernelGGL(k_part_predict, dim3(blocks_for(n3)), dim3(kBlock), 0, c->st, n3, c->rimp,
-                       (const double*)c->vimp, (const double*)c->aimp, c->rold[R[2]], (const double*)c->vold[V[0]],
-                       (const double*)c->vold[V[1]], c->vold[V[2]], (const double*)c->aold[A[0]], c->aold[A[1]],
-                       c->pcr, c->pcv, c4o3 * dt, c112);
-    LAUNCHCHK(c);
-  }
-  std::swap(A[0], A[1]);
   RET(veval<T>(c));
-  // corrector (steppc.f90:97-151)
-  const dim3 g25 = grid25(c);
-  const unsigned nb = c->st25 ? g25.x * g25.y * g25.z : blocks_for(g.n);
+  // The particle corrector (steppc.f90:122-151) needs the accelerations just
+  // gathered; it and the density chain of the corrected positions go to the
+  // side stream ahead of the psi corrector.  The driver's potential
+  // evaluation (DFT4He3d.f90:861) consumes that chain.
+  RET(fork_side(c));
+  if (n3 > 0) {
+    const unsigned pb = blocks_for(n3);
+    double* pp = c->part + nb;  // after the psi partials
+    hipLaunchKernelGGL(k_part_correct, dim3(pb), dim3(kBlock), 0, c->st2, n3, c->rimp, c->vimp,
+                       (const double*)c->aimp, (const double*)c->rold[R[2]], (const double*)c->rold[R[1]],
+                       (const double*)c->vold[V[0]], (const double*)c->vold[V[1]], c->vold[V[2]],
+                       (const double*)c->aold[A[0]], (const double*)c->aold[A[1]], c->pcr, c->pcv, pp, 3.0 * dt, c9);
+    LAUNCHCHK(c);
+    hipLaunchKernelGGL(k_finish_sums, dim3(2), dim3(kBlock), 0, c->st2, (const double*)pp, (int)pb, 2, c->res + 1);
+    LAUNCHCHK(c);
+  } else {
+    HIPCHK(c, hipMemsetAsync(c->res + 1, 0, 2 * sizeof(double), c->st2));
+  }
+  {
+    int t = R[2];
+    R[2] = R[1];
+    R[1] = R[0];
+    R[0] = t;
+    t = V[2];
+    V[2] = V[1];
+    V[1] = V[0];
+    V[0] = t;
+  }
+  RET(issue_chain<T>(c));
+  // psi corrector (steppc.f90:97-120); the error sums stay on the device
+  // until fetch_errors asks for them
   {
   ProfScope pscorr(c, HETDDFT_PROF_PCC, (double)g.n * (16.0 * 10 + 2.0 * sizeof(T)));
   if (c->st25)
-    hipLaunchKernelGGL((k_pc_correct25<T>), g25, dim3(S_TX, S_TY), 0, c->st, ha, (const c64*)c->psi,
+    PLAUNCH(pscorr, (k_pc_correct25<T>), g25, dim3(S_TX, S_TY), 0, ha, (const c64*)c->psi,
                        (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
                        (const c64*)c->hold[H[0]], (const c64*)c->hold[H[1]], c->pc, c->spare,
                        (T*)c->fld[hetddft_ctx::F_DEN], c->part, 3.0 * dt, c9);
   else
-    hipLaunchKernelGGL((k_pc_correct<T>), dim3(nb), dim3(kBlock), 0, c->st, ham_direct(ha), (const c64*)c->psi,
+    PLAUNCH(pscorr, (k_pc_correct<T>), dim3(nb), dim3(kBlock), 0, ham_direct(ha), (const c64*)c->psi,
                        (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
                        (const c64*)c->hold[H[0]], (const c64*)c->hold[H[1]], c->pc, c->spare,
                        (T*)c->fld[hetddft_ctx::F_DEN], c->part, 3.0 * dt, c9);
@@ -895,34 +1077,12 @@ int step_pc_t(hetddft_ctx* c, double dt) {
     P[1] = P[0];
     P[0] = t;
   }
-  if (n3 > 0) {
-    const unsigned pb = blocks_for(n3);
-    double* pp = c->part + nb;  // after the psi partials
-    hipLaunchKernelGGL(k_part_correct, dim3(pb), dim3(kBlock), 0, c->st, n3, c->rimp, c->vimp,
-                       (const double*)c->aimp, (const double*)c->rold[R[2]], (const double*)c->rold[R[1]],
-                       (const double*)c->vold[V[0]], (const double*)c->vold[V[1]], c->vold[V[2]],
-                       (const double*)c->aold[A[0]], (const double*)c->aold[A[1]], c->pcr, c->pcv, pp, 3.0 * dt, c9);
-    LAUNCHCHK(c);
-    hipLaunchKernelGGL(k_finish_sums, dim3(2), dim3(kBlock), 0, c->st, (const double*)pp, (int)pb, 2, c->res + 1);
-    LAUNCHCHK(c);
-  } else {
-    HIPCHK(c, hipMemsetAsync(c->res + 1, 0, 2 * sizeof(double), c->st));
-  }
-  {
-    int t = R[2];
-    R[2] = R[1];
-    R[1] = R[0];
-    R[0] = t;
-    t = V[2];
-    V[2] = V[1];
-    V[1] = V[0];
-    V[0] = t;
-  }
-  HIPCHK(c, hipMemcpyAsync(c->hres, c->res, 3 * sizeof(double), hipMemcpyDeviceToHost, c->st));
   return HETDDFT_OK;
 }
 
 int fetch_errors(hetddft_ctx* c, double err[3]) {
+  RET(settle(c));
+  HIPCHK(c, hipMemcpyAsync(c->hres, c->res, 3 * sizeof(double), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   const double c9 = 9.0 / 121.0;
   const double e0 = c9 * c->hres[0] / (double)c->g.n;
@@ -1109,6 +1269,10 @@ long long hetddft_device_bytes(const hetddft_ctx* ctx) { return ctx ? ctx->bytes
 int hetddft_destroy(hetddft_ctx* c) {
   if (!c) return HETDDFT_OK;
   if (c->st) hipStreamSynchronize(c->st);
+  if (c->st2) hipStreamSynchronize(c->st2);
+  for (hipEvent_t e : {c->ev_fork, c->ev_chain, c->ev_b3, c->ev_join})
+    if (e) hipEventDestroy(e);
+  if (c->st2) hipStreamDestroy(c->st2);
   for (auto& r : c->prof_pending) {
     hipEventDestroy(r.a);
     hipEventDestroy(r.b);
@@ -1185,6 +1349,23 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
     fail(c, HETDDFT_E_HIP, "hipStreamCreate failed");
     return bad(HETDDFT_E_HIP);
   }
+  if (hipStreamCreate(&c->st2) != hipSuccess) {
+    fail(c, HETDDFT_E_HIP, "hipStreamCreate failed");
+    return bad(HETDDFT_E_HIP);
+  }
+  for (hipEvent_t* e : {&c->ev_fork, &c->ev_chain, &c->ev_b3, &c->ev_join})
+    if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
+      fail(c, HETDDFT_E_HIP, "hipEventCreate failed");
+      return bad(HETDDFT_E_HIP);
+    }
+  {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+      c->cus = cus;
+    const char* e = std::getenv("HETDDFT_ZOCC");
+    if (e && std::atoi(e) > 0) c->zp_per_cu = std::atoi(e);
+  }
   int rc = c->mixed ? fft_alloc_tables<float>(c, c->fs) : fft_alloc_tables<double>(c, c->fs);
   if (rc) return bad(rc);
   if (c->mixed) rc = fft_alloc_tables<double>(c, c->fs64e);
@@ -1234,15 +1415,36 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
     return bad(HETDDFT_E_NOMEM);
   }
   std::memset(c->hres, 0, 64 * sizeof(double));
-  c->denx = c->fld[hetddft_ctx::F_DENXA];
   {
+    // two particle-chain buffer sets; set 0 uses the fields allocated above
+    hetddft_ctx::ChainSet& s0 = c->cset[0];
+    s0.denxa = c->fld[hetddft_ctx::F_DENXA];
+    s0.denxb = c->fld[hetddft_ctx::F_DENXB];
+    s0.psix = c->fld[hetddft_ctx::F_PSIX];
+    s0.hpsix = c->fld[hetddft_ctx::F_HPSIX];
+    s0.counts = c->counts;
+    s0.reg = c->reg;
+    s0.denx = s0.denxa;
+    hetddft_ctx::ChainSet& s1 = c->cset[1];
+    for (void** f : {&s1.denxa, &s1.denxb, &s1.psix, &s1.hpsix}) {
+      rc = A(f, n * c->tsz);
+      if (rc) return bad(rc);
+    }
+    rc = A((void**)&s1.counts, n * 4);
+    if (rc) return bad(rc);
+    rc = A((void**)&s1.reg, kRegLen * sizeof(int));
+    if (rc) return bad(rc);
+    s1.denx = s1.denxa;
     // occupied region: nothing live yet
     const int r0[kRegLen] = {0x7fffffff, 0x7fffffff, 0x7fffffff, -1, -1, -1, 0x7fffffff, 0x7fffffff, 0x7fffffff,
                              -1, -1, -1, 0x7fffffff, 0x7fffffff, 0x7fffffff, -1, -1, -1};
-    if (hipMemcpy(c->reg, r0, sizeof r0, hipMemcpyHostToDevice) != hipSuccess) {
-      fail(c, HETDDFT_E_HIP, "region init failed");
-      return bad(HETDDFT_E_HIP);
-    }
+    for (int* rg : {s0.reg, s1.reg})
+      if (hipMemcpy(rg, r0, sizeof r0, hipMemcpyHostToDevice) != hipSuccess) {
+        fail(c, HETDDFT_E_HIP, "region init failed");
+        return bad(HETDDFT_E_HIP);
+      }
+    c->cs = 0;
+    use_chain_set(c);
   }
   // timec = cmplx(Lambdah*(1+tanh((z-tzmean)/tzsurf)), 1) with the real part
   // rounded to REAL(4) by the KIND-less cmplx (DFT4He3d.f90:773-782)
@@ -1352,6 +1554,9 @@ int hetddft_set_kernels(hetddft_ctx* c, const double* fvlj4, const double* wcgk,
 
 int hetddft_set_state(hetddft_ctx* c, const double* psi, const double* rimp, const double* vimp) {
   if (!c) return HETDDFT_E_ARG;
+  RET(settle(c));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->chain_pending = false;
   if (psi) HIPCHK(c, hipMemcpy(c->psi, psi, (size_t)c->g.n * 16, hipMemcpyHostToDevice));
   const size_t np3 = (size_t)3 * c->p.n_par;
   if (rimp && np3) HIPCHK(c, hipMemcpy(c->rimp, rimp, np3 * 8, hipMemcpyHostToDevice));
@@ -1363,6 +1568,8 @@ int hetddft_set_state(hetddft_ctx* c, const double* psi, const double* rimp, con
 int hetddft_start(hetddft_ctx* c) {
   if (!c) return HETDDFT_E_ARG;
   if (!c->kernels_set) return fail(c, HETDDFT_E_STATE, "hetddft_set_kernels not called");
+  RET(settle(c));
+  c->chain_pending = false;
   const Grid& g = c->g;
   if (c->mixed)
     hipLaunchKernelGGL((k_density<float>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, (const c64*)c->psi,
@@ -1393,6 +1600,10 @@ int hetddft_potential(hetddft_ctx* c) {
 int hetddft_step_rk(hetddft_ctx* c, double dt) {
   if (!c) return HETDDFT_E_ARG;
   if (!c->started) return fail(c, HETDDFT_E_STATE, "hetddft_start not called");
+  // the particles move on st in the RK stages: a chain queued for the old
+  // positions is not used
+  RET(settle(c));
+  c->chain_pending = false;
   RET(c->mixed ? step_rk_t<float>(c, dt) : step_rk_t<double>(c, dt));
   HIPCHK(c, hipStreamSynchronize(c->st));
   return HETDDFT_OK;
@@ -1415,6 +1626,8 @@ int hetddft_iterate(hetddft_ctx* c, double dt, double err[3]) {
   if (!c) return HETDDFT_E_ARG;
   if (!c->started) return fail(c, HETDDFT_E_STATE, "hetddft_start not called");
   if (c->iter_in_run < 3) {
+    RET(settle(c));
+    c->chain_pending = false;
     RET(c->mixed ? step_rk_t<float>(c, dt) : step_rk_t<double>(c, dt));
     RET(run_veval(c));
     if (err) {
@@ -1432,6 +1645,7 @@ int hetddft_iterate(hetddft_ctx* c, double dt, double err[3]) {
 
 int hetddft_particle_means(hetddft_ctx* c, double out[9]) {
   if (!c || !out) return HETDDFT_E_ARG;
+  RET(settle(c));
   const int np = c->p.n_par;
   if (np == 0) {
     std::memset(out, 0, 9 * sizeof(double));
@@ -1458,12 +1672,14 @@ int hetddft_particle_means(hetddft_ctx* c, double out[9]) {
 
 int hetddft_energy(hetddft_ctx* c, double out[12]) {
   if (!c || !out) return HETDDFT_E_ARG;
+  RET(settle(c));
   if (!c->started) return fail(c, HETDDFT_E_STATE, "hetddft_start not called");
   return c->mixed ? energy_t<float>(c, out) : energy_t<double>(c, out);
 }
 
 int hetddft_get_state(hetddft_ctx* c, double* psi, double* rimp, double* vimp, double* aimp) {
   if (!c) return HETDDFT_E_ARG;
+  RET(settle(c));
   HIPCHK(c, hipStreamSynchronize(c->st));
   const size_t np3 = (size_t)3 * c->p.n_par;
   if (psi) HIPCHK(c, hipMemcpy(psi, c->psi, (size_t)c->g.n * 16, hipMemcpyDeviceToHost));
@@ -1475,6 +1691,7 @@ int hetddft_get_state(hetddft_ctx* c, double* psi, double* rimp, double* vimp, d
 
 int hetddft_get_field(hetddft_ctx* c, int which, double* out) {
   if (!c || !out) return HETDDFT_E_ARG;
+  RET(settle(c));
   static const int map[20] = {hetddft_ctx::F_DEN,    hetddft_ctx::F_POT4,  hetddft_ctx::F_UIMP,
                               hetddft_ctx::F_DENCG,  hetddft_ctx::F_DELJ4, hetddft_ctx::F_FALFS,
                               hetddft_ctx::F_DX,     hetddft_ctx::F_DY,    hetddft_ctx::F_DZ,
@@ -1489,6 +1706,7 @@ int hetddft_get_field(hetddft_ctx* c, int which, double* out) {
 
 int hetddft_convolve(hetddft_ctx* c, int nf, const double* const* in, const double* kern, double* const* out) {
   if (!c || nf < 1 || nf > 5 || !in || !kern || !out) return fail(c, HETDDFT_E_ARG, "convolve: bad arguments");
+  RET(settle(c));
   const size_t n = (size_t)c->g.n;
   // stage inputs in the force-convolution fields, kernel in the alpha_s slot
   const int stage[5] = {hetddft_ctx::F_FCX, hetddft_ctx::F_FCY, hetddft_ctx::F_FCZ, hetddft_ctx::F_STO1,
@@ -1516,6 +1734,7 @@ int hetddft_convolve(hetddft_ctx* c, int nf, const double* const* in, const doub
 
 int hetddft_profile(hetddft_ctx* c, int enable) {
   if (!c) return HETDDFT_E_ARG;
+  RET(settle(c));
   HIPCHK(c, hipStreamSynchronize(c->st));
   for (auto& r : c->prof_pending) {
     c->prof_pool.push_back(r.a);
@@ -1533,6 +1752,7 @@ int hetddft_profile(hetddft_ctx* c, int enable) {
 
 int hetddft_profile_read(hetddft_ctx* c, int cls, double* ms_total, long long* launches, double* bytes_total) {
   if (!c || cls < 0 || cls >= HETDDFT_PROF_N) return HETDDFT_E_ARG;
+  RET(settle(c));
   HIPCHK(c, hipStreamSynchronize(c->st));
   for (auto& r : c->prof_pending) {
     float ms = 0.0f;
@@ -1552,6 +1772,7 @@ int hetddft_profile_read(hetddft_ctx* c, int cls, double* ms_total, long long* l
 
 int hetddft_sync(hetddft_ctx* c) {
   if (!c) return HETDDFT_E_ARG;
+  RET(settle(c));
   HIPCHK(c, hipStreamSynchronize(c->st));
   return HETDDFT_OK;
 }

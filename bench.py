@@ -84,14 +84,44 @@ def run_gpu(wl, precision, steps, warmup, prof_steps=0):
     return out
 
 
-def roofline(prof):
+TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic.json"
+
+
+def measured_traffic(workload, precision, cls):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration), or None."""
+    try:
+        rec = json.loads(TRAFFIC_FILE.read_text())
+        ent = rec["entries"][f"{workload}/{precision}/{cls}"]
+        return float(ent["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def roofline(prof, workload=None, precision=None):
     """Dominant kernel class: algorithmic bytes per launch / mean event time."""
     name, (ms, n, b) = max(prof.items(), key=lambda kv: kv[1][0])
     sec = ms / 1e3 / n
     achieved = b / n / sec / 1e9
     return {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_us": sec * 1e6,
-            "algorithmic_bytes_per_launch": b / n, "launches": n}
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload, precision, name),
+            "avg_us": sec * 1e6, "algorithmic_bytes_per_launch": b / n, "launches": n}
+
+
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """Plain device-to-device copy of nbytes, reps times: the reference kernel for
+    scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return {"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps,
+            "GB_per_s": 2 * nbytes * reps / (t1 - t0) / 1e9}
 
 
 def cpu_baseline(wl):
@@ -130,8 +160,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
-                    help="time one precision only (profiling runs)")
+    ap.add_argument("--only", choices=["fp64", "mixed", "copy"], default=None,
+                    help="time one precision only, or run the copy calibration (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the path has no multi-GPU mode "
@@ -139,10 +169,16 @@ def main():
         sys.exit(2)
     import inputs
     wl = inputs.WORKLOADS[args.workload]
+    if args.only == "copy":
+        print(json.dumps(copy_calibration()))
+        return
     if args.only:
-        res = run_gpu(wl, 0 if args.only == "fp64" else 1, args.steps, args.warmup)
+        res = run_gpu(wl, 0 if args.only == "fp64" else 1, args.steps, args.warmup, prof_steps=5)
+        kernels = {k: {"avg_us": round(ms * 1e3 / n, 2), "launches": n,
+                       "GB_per_s": round(b / (ms * 1e-3) / 1e9, 1)}
+                   for k, (ms, n, b) in res["prof"].items() if n}
         print(json.dumps({"only": args.only, "steps_per_s": res["steps_per_s"],
-                          "ms_per_step": 1000.0 / res["steps_per_s"]}))
+                          "ms_per_step": 1000.0 / res["steps_per_s"], "kernels": kernels}))
         return
     res64 = run_gpu(wl, 0, args.steps, args.warmup, prof_steps=5)
     res32 = run_gpu(wl, 1, args.steps, args.warmup, prof_steps=5)
@@ -171,7 +207,7 @@ def main():
         "config": {"workload": f"{wl.name}: {wl.n}^3 grid dx={wl.h} A, N={wl.n_he} He, "
                                f"N_par={wl.n_par}, dt={wl.dt_ps * 1000:.1f} fs, OTC, ironx={wl.ironx}",
                    "precision": "mixed" if use_mixed else "fp64"},
-        "roofline": roofline(best["prof"]),
+        "roofline": roofline(best["prof"], wl.name, "mixed" if use_mixed else "fp64"),
         "fp64_steps_per_s": res64["steps_per_s"],
         "mixed_steps_per_s": res32["steps_per_s"],
         "mixed_vs_fp64_speedup": res32["steps_per_s"] / res64["steps_per_s"],
@@ -180,7 +216,7 @@ def main():
         "model_gb_per_step": gb_step,
         "model_hbm_gbs": gb_step * best["steps_per_s"],
         "model_hbm_frac": gb_step * best["steps_per_s"] / HBM_PEAK_GBS,
-        "fp64_roofline": roofline(res64["prof"]),
+        "fp64_roofline": roofline(res64["prof"], wl.name, "fp64"),
         "device_bytes": best["device_bytes"],
     }
     if not args.no_cpu:
