@@ -146,6 +146,7 @@ struct hetddft_ctx {
   Stencil13 d1x{}, d1y{}, d1z{}, d2x{}, d2y{}, d2z{};
   FftPlan pl_xh{}, pl_y{}, pl_z{};
   int zc = 16;  // z planes per workgroup of the stencil engine
+  int xcd = 1;  // XCD-grouped tile order in the stencil engine
   int cus = 256;      // compute units of the device
   int zp_per_cu = 4;  // resident workgroups per CU of the persistent z pass
   bool st25 = true;  // z-marching stencil engine (false: direct neighbour loads)
@@ -743,7 +744,7 @@ int veval(hetddft_ctx* c) {
   {
   ProfScope psg(c, HETDDFT_PROF_GRAD, 4.0 * (double)g.n * sizeof(T));
   if (c->st25)
-    PLAUNCH(psg, (k_grad3_25<T>), grid25(c), dim3(S_TX, S_TY), 0, g, c->zc,
+    PLAUNCH(psg, (k_grad3_25<T>), grid25(c), dim3(S_TX, S_TY), 0, g, c->zc, c->xcd,
                        (const T*)F(hetddft_ctx::F_DEN), c->d1x, c->d1y, c->d1z, F(hetddft_ctx::F_DX),
                        F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ));
   else
@@ -876,6 +877,7 @@ int veval(hetddft_ctx* c) {
     p5.den_m = pa.den_m;
     p5.otc = pa.otc;
     p5.zc = c->zc;
+    p5.xcd = c->xcd;
     ProfScope ps(c, HETDDFT_PROF_POT, (otc ? 11.0 : 6.0) * (double)g.n * sizeof(T));
     PLAUNCH(ps, (k_pot_final25<T>), grid25(c), dim3(S_TX, S_TY), 0, p5,
                        (const T*)F(hetddft_ctx::F_DEN), (const T*)F(hetddft_ctx::F_DENCG),
@@ -906,6 +908,7 @@ Ham25 ham_args(const hetddft_ctx* c) {
   a.h2o2m4 = c->p.h2o2m4;
   a.timec_z = c->timec;
   a.zc = c->zc;
+  a.xcd = c->xcd;
   return a;
 }
 
@@ -991,18 +994,17 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   const int n3 = 3 * c->p.n_par;
   const dim3 g25 = grid25(c);
   const unsigned nb = c->st25 ? g25.x * g25.y * g25.z : blocks_for(g.n);
-  RET(fork_side(c));
-  // Particle predictor (steppc.f90:64-87) and the density chain of the
-  // predicted positions on the side stream.  They are queued before the psi
-  // predictor so that they are dispatched first and run beside it.
+  // Particle predictor (steppc.f90:64-87), then the density chain of the
+  // predicted positions on the side stream, beside the psi predictor.
   if (n3 > 0) {
-    hipLaunchKernelGGL(k_part_predict, dim3(blocks_for(n3)), dim3(kBlock), 0, c->st2, n3, c->rimp,
+    hipLaunchKernelGGL(k_part_predict, dim3(blocks_for(n3)), dim3(kBlock), 0, c->st, n3, c->rimp,
                        (const double*)c->vimp, (const double*)c->aimp, c->rold[R[2]], (const double*)c->vold[V[0]],
                        (const double*)c->vold[V[1]], c->vold[V[2]], (const double*)c->aold[A[0]], c->aold[A[1]],
                        c->pcr, c->pcv, c4o3 * dt, c112);
     LAUNCHCHK(c);
   }
   std::swap(A[0], A[1]);
+  RET(fork_side(c));
   RET(issue_chain<T>(c));
   // psi predictor + modifier (steppc.f90:36-62)
   c64* x = c->psi;
@@ -1023,23 +1025,21 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   c->psi = y;
   std::swap(H[0], H[1]);
   RET(veval<T>(c));
-  // The particle corrector (steppc.f90:122-151) needs the accelerations just
-  // gathered; it and the density chain of the corrected positions go to the
-  // side stream ahead of the psi corrector.  The driver's potential
-  // evaluation (DFT4He3d.f90:861) consumes that chain.
-  RET(fork_side(c));
+  // Particle corrector (steppc.f90:122-151), then the density chain of the
+  // corrected positions on the side stream beside the psi corrector; the
+  // driver's potential evaluation (DFT4He3d.f90:861) consumes that chain.
   if (n3 > 0) {
     const unsigned pb = blocks_for(n3);
     double* pp = c->part + nb;  // after the psi partials
-    hipLaunchKernelGGL(k_part_correct, dim3(pb), dim3(kBlock), 0, c->st2, n3, c->rimp, c->vimp,
+    hipLaunchKernelGGL(k_part_correct, dim3(pb), dim3(kBlock), 0, c->st, n3, c->rimp, c->vimp,
                        (const double*)c->aimp, (const double*)c->rold[R[2]], (const double*)c->rold[R[1]],
                        (const double*)c->vold[V[0]], (const double*)c->vold[V[1]], c->vold[V[2]],
                        (const double*)c->aold[A[0]], (const double*)c->aold[A[1]], c->pcr, c->pcv, pp, 3.0 * dt, c9);
     LAUNCHCHK(c);
-    hipLaunchKernelGGL(k_finish_sums, dim3(2), dim3(kBlock), 0, c->st2, (const double*)pp, (int)pb, 2, c->res + 1);
+    hipLaunchKernelGGL(k_finish_sums, dim3(2), dim3(kBlock), 0, c->st, (const double*)pp, (int)pb, 2, c->res + 1);
     LAUNCHCHK(c);
   } else {
-    HIPCHK(c, hipMemsetAsync(c->res + 1, 0, 2 * sizeof(double), c->st2));
+    HIPCHK(c, hipMemsetAsync(c->res + 1, 0, 2 * sizeof(double), c->st));
   }
   {
     int t = R[2];
@@ -1051,6 +1051,7 @@ int step_pc_t(hetddft_ctx* c, double dt) {
     V[1] = V[0];
     V[0] = t;
   }
+  RET(fork_side(c));
   RET(issue_chain<T>(c));
   // psi corrector (steppc.f90:97-120); the error sums stay on the device
   // until fetch_errors asks for them
@@ -1337,6 +1338,8 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
     if (e && std::strcmp(e, "direct") == 0) c->st25 = false;
     const char* z = std::getenv("HETDDFT_ZC");
     if (z && std::atoi(z) > 0) c->zc = std::atoi(z);
+    const char* x = std::getenv("HETDDFT_XCD");
+    if (x) c->xcd = std::atoi(x) != 0;
   }
   c->pl_xh = make_plan(p->nx / 2);
   c->pl_y = make_plan(p->ny);

@@ -36,16 +36,34 @@ __device__ inline c64 vload<c64, c64>(const c64* f, long long i) {
 
 struct Col {
   int x, y, tx, ty, tid;
+  int bx, by, bz;  // tile of this workgroup (x tile, y tile, z chunk)
   bool in;
 };
 
-__device__ inline Col col_of(const Grid& g) {
+// Tile of this workgroup.  Workgroups are dispatched round-robin over the 8
+// XCDs, each with its own L2.  With xcd set, XCD k takes a contiguous range
+// of tiles (x tile fastest, then y tile, then z chunk), so the tiles that
+// share x/y halos and z-queue planes read them through the same L2 at about
+// the same time.  The mapping is a bijection for any block count.
+__device__ inline Col col_of(const Grid& g, int xcd) {
   Col c;
   c.tx = threadIdx.x;
   c.ty = threadIdx.y;
   c.tid = c.tx + S_TX * c.ty;
-  c.x = blockIdx.x * S_TX + c.tx;
-  c.y = blockIdx.y * S_TY + c.ty;
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int p = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  int l = p;
+  if (xcd) {
+    constexpr int NXCD = 8;
+    const int nb = gx * gy * gridDim.z;
+    const int q = nb / NXCD, r = nb % NXCD, k = p % NXCD;
+    l = k * q + min(k, r) + p / NXCD;
+  }
+  c.bx = l % gx;
+  c.by = (l / gx) % gy;
+  c.bz = l / (gx * gy);
+  c.x = c.bx * S_TX + c.tx;
+  c.y = c.by * S_TY + c.ty;
   c.in = c.x < g.nx && c.y < g.ny;
   return c;
 }
@@ -91,7 +109,7 @@ struct ZQueue {
 template <class V, class R>
 __device__ inline void fill_plane(V* tile, const R* f, const Grid& g, const Col& c, int z, V centre) {
   tile[(c.ty + 6) * S_LX + c.tx + 6] = centre;
-  const int x0 = blockIdx.x * S_TX - 6, y0 = blockIdx.y * S_TY - 6;
+  const int x0 = c.bx * S_TX - 6, y0 = c.by * S_TY - 6;
   constexpr int NE = (S_LX * S_LY + S_THREADS - 1) / S_THREADS;
   V v[NE];
 #pragma unroll
@@ -149,7 +167,8 @@ struct Ham25 {
   Stencil13 d2x, d2y, d2z;
   double h2o2m4;
   const c64* timec_z;
-  int zc;  // z planes per workgroup
+  int zc;   // z planes per workgroup
+  int xcd;  // XCD-grouped tile order (col_of)
 };
 
 // Hpsi at the column's current plane (steppc.f90:36-46).
@@ -173,8 +192,8 @@ __global__ void __launch_bounds__(S_THREADS) k_pc_predict25(Ham25 a, const c64* 
                                                              double c4o3dt, double c112) {
   __shared__ c64 tile[S_LX * S_LY];
   const Grid& g = a.g;
-  const Col c = col_of(g);
-  const int z0 = blockIdx.z * a.zc;
+  const Col c = col_of(g, a.xcd);
+  const int z0 = c.bz * a.zc;
   const int z1 = min(z0 + a.zc, g.nz);
   ZQueue<c64, c64> q;
   q.init(psi, g, c, z0);
@@ -213,8 +232,8 @@ __global__ void __launch_bounds__(S_THREADS) k_pc_correct25(Ham25 a, const c64* 
   __shared__ c64 tile[S_LX * S_LY];
   __shared__ double sh[S_THREADS / 64];
   const Grid& g = a.g;
-  const Col c = col_of(g);
-  const int z0 = blockIdx.z * a.zc;
+  const Col c = col_of(g, a.xcd);
+  const int z0 = c.bz * a.zc;
   const int z1 = min(z0 + a.zc, g.nz);
   ZQueue<c64, c64> q;
   q.init(psi, g, c, z0);
@@ -253,8 +272,8 @@ __global__ void __launch_bounds__(S_THREADS) k_rk_stage25(Ham25 a, const c64* __
                                                            double cc, double dt) {
   __shared__ c64 tile[S_LX * S_LY];
   const Grid& g = a.g;
-  const Col c = col_of(g);
-  const int z0 = blockIdx.z * a.zc;
+  const Col c = col_of(g, a.xcd);
+  const int z0 = c.bz * a.zc;
   const int z1 = min(z0 + a.zc, g.nz);
   ZQueue<c64, c64> q;
   q.init(psi, g, c, z0);
@@ -281,12 +300,12 @@ __global__ void __launch_bounds__(S_THREADS) k_rk_stage25(Ham25 a, const c64* __
 
 // grad(den) with the first-derivative stencil (derden.f90:26-28).
 template <class R>
-__global__ void __launch_bounds__(S_THREADS) k_grad3_25(Grid g, int zc, const R* __restrict__ f, Stencil13 wx,
+__global__ void __launch_bounds__(S_THREADS) k_grad3_25(Grid g, int zc, int xcd, const R* __restrict__ f, Stencil13 wx,
                                                          Stencil13 wy, Stencil13 wz, R* __restrict__ fx,
                                                          R* __restrict__ fy, R* __restrict__ fz) {
   __shared__ double tile[S_LX * S_LY];
-  const Col c = col_of(g);
-  const int z0 = blockIdx.z * zc;
+  const Col c = col_of(g, xcd);
+  const int z0 = c.bz * zc;
   const int z1 = min(z0 + zc, g.nz);
   ZQueue<double, R> q;
   q.init(f, g, c, z0);
@@ -321,6 +340,7 @@ struct Pot25 {
   double C, beta, den_m;
   int otc;
   int zc;
+  int xcd;
 };
 
 template <class R>
@@ -344,8 +364,8 @@ __global__ void __launch_bounds__(S_THREADS) k_pot_final25(Pot25 a, const R* __r
   __shared__ double tx_tile[S_TY * S_LX];    // g_x, rows of this tile with x halo
   __shared__ double ty_tile[S_LY * S_TX];    // g_y, columns of this tile with y halo
   const Grid& g = a.g;
-  const Col c = col_of(g);
-  const int z0 = blockIdx.z * a.zc;
+  const Col c = col_of(g, a.xcd);
+  const int z0 = c.bz * a.zc;
   const int z1 = min(z0 + a.zc, g.nz);
   const int xc = min(c.x, g.nx - 1), yc = min(c.y, g.ny - 1);
   double q[13];
@@ -357,7 +377,7 @@ __global__ void __launch_bounds__(S_THREADS) k_pot_final25(Pot25 a, const R* __r
       q[k] = (c.in && z >= 0 && z < g.nz) ? t : 0.0;
     }
   }
-  const int x0 = blockIdx.x * S_TX - 6, y0 = blockIdx.y * S_TY - 6;
+  const int x0 = c.bx * S_TX - 6, y0 = c.by * S_TY - 6;
   for (int z = z0; z < z1; ++z) {
     if (a.otc) {
       const int zz = z + 6;
@@ -370,7 +390,7 @@ __global__ void __launch_bounds__(S_THREADS) k_pot_final25(Pot25 a, const R* __r
       for (int k = 0; k < NX; ++k) {
         const int e = c.tid + k * S_THREADS;
         const int ly = e / S_LX, lx = e - ly * S_LX;
-        const int x = x0 + lx, y = blockIdx.y * S_TY + ly;
+        const int x = x0 + lx, y = c.by * S_TY + ly;
         const bool inb = x >= 0 && x < g.nx && y < g.ny;
         const double v = prod_at(falfs, intx, g, min(max(x, 0), g.nx - 1), min(y, g.ny - 1), z);
         vx[k] = inb ? v : 0.0;
@@ -381,7 +401,7 @@ __global__ void __launch_bounds__(S_THREADS) k_pot_final25(Pot25 a, const R* __r
       for (int k = 0; k < NY; ++k) {
         const int e = c.tid + k * S_THREADS;
         const int ly = e / S_TX, lx = e - ly * S_TX;
-        const int x = blockIdx.x * S_TX + lx, y = y0 + ly;
+        const int x = c.bx * S_TX + lx, y = y0 + ly;
         const bool inb = x < g.nx && y >= 0 && y < g.ny;
         const double v = prod_at(falfs, inty, g, min(x, g.nx - 1), min(max(y, 0), g.ny - 1), z);
         vy[k] = inb ? v : 0.0;
