@@ -147,6 +147,7 @@ struct hetddft_ctx {
   FftPlan pl_xh{}, pl_y{}, pl_z{};
   int zc = 16;  // z planes per workgroup of the stencil engine
   int xcd = 1;  // XCD-grouped tile order in the stencil engine
+  int tile = 0; // tile shape of the stencil engine (with_tile)
   int cus = 256;      // compute units of the device
   int zp_per_cu = 4;  // resident workgroups per CU of the persistent z pass
   bool st25 = true;  // z-marching stencil engine (false: direct neighbour loads)
@@ -635,9 +636,26 @@ HamArgs ham_direct(const Ham25& h) {
   return a;
 }
 
-// Workgroups of the z-marching stencil engine: S_TX x S_TY columns, c->zc planes.
+// Tile shapes of the z-marching stencil engine (c->tile selects one).
+using TileA = Tile25<64, 4>;
+using TileB = Tile25<32, 8>;
+using TileC = Tile25<32, 16>;
+using TileD = Tile25<64, 8>;
+
+template <class Fn>
+auto with_tile(int tile, Fn&& fn) -> decltype(fn(TileA{})) {
+  switch (tile) {
+    case 1: return fn(TileB{});
+    case 2: return fn(TileC{});
+    case 3: return fn(TileD{});
+    default: return fn(TileA{});
+  }
+}
+
+// Workgroups of the engine: TX x TY columns, c->zc planes each.
+template <class SH>
 dim3 grid25(const hetddft_ctx* c) {
-  return dim3((unsigned)((c->g.nx + S_TX - 1) / S_TX), (unsigned)((c->g.ny + S_TY - 1) / S_TY),
+  return dim3((unsigned)((c->g.nx + SH::TX - 1) / SH::TX), (unsigned)((c->g.ny + SH::TY - 1) / SH::TY),
               (unsigned)((c->g.nz + c->zc - 1) / c->zc));
 }
 
@@ -744,9 +762,13 @@ int veval(hetddft_ctx* c) {
   {
   ProfScope psg(c, HETDDFT_PROF_GRAD, 4.0 * (double)g.n * sizeof(T));
   if (c->st25)
-    PLAUNCH(psg, (k_grad3_25<T>), grid25(c), dim3(S_TX, S_TY), 0, g, c->zc, c->xcd,
-                       (const T*)F(hetddft_ctx::F_DEN), c->d1x, c->d1y, c->d1z, F(hetddft_ctx::F_DX),
-                       F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ));
+    with_tile(c->tile, [&](auto sh) {
+      using SH = decltype(sh);
+      PLAUNCH(psg, (k_grad3_25<T, SH>), grid25<SH>(c), dim3(SH::TX, SH::TY), 0, g, c->zc, c->xcd,
+              (const T*)F(hetddft_ctx::F_DEN), c->d1x, c->d1y, c->d1z, F(hetddft_ctx::F_DX),
+              F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ));
+      return 0;
+    });
   else
     PLAUNCH(psg, (k_grad3<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, g,
                        (const T*)F(hetddft_ctx::F_DEN), c->d1x, c->d1y, c->d1z, F(hetddft_ctx::F_DX),
@@ -879,12 +901,16 @@ int veval(hetddft_ctx* c) {
     p5.zc = c->zc;
     p5.xcd = c->xcd;
     ProfScope ps(c, HETDDFT_PROF_POT, (otc ? 11.0 : 6.0) * (double)g.n * sizeof(T));
-    PLAUNCH(ps, (k_pot_final25<T>), grid25(c), dim3(S_TX, S_TY), 0, p5,
-                       (const T*)F(hetddft_ctx::F_DEN), (const T*)F(hetddft_ctx::F_DENCG),
-                       (const T*)F(hetddft_ctx::F_DELJ4), (const T*)F(hetddft_ctx::F_STO1),
-                       (const T*)F(hetddft_ctx::F_UIMP), (const T*)F(hetddft_ctx::F_FALFS),
-                       (const T*)F(hetddft_ctx::F_IX), (const T*)F(hetddft_ctx::F_IY),
-                       (const T*)F(hetddft_ctx::F_IZ), (const T*)F(hetddft_ctx::F_UACONV), F(hetddft_ctx::F_POT4));
+    with_tile(c->tile, [&](auto sh) {
+      using SH = decltype(sh);
+      PLAUNCH(ps, (k_pot_final25<T, SH>), grid25<SH>(c), dim3(SH::TX, SH::TY), 0, p5,
+              (const T*)F(hetddft_ctx::F_DEN), (const T*)F(hetddft_ctx::F_DENCG),
+              (const T*)F(hetddft_ctx::F_DELJ4), (const T*)F(hetddft_ctx::F_STO1),
+              (const T*)F(hetddft_ctx::F_UIMP), (const T*)F(hetddft_ctx::F_FALFS),
+              (const T*)F(hetddft_ctx::F_IX), (const T*)F(hetddft_ctx::F_IY),
+              (const T*)F(hetddft_ctx::F_IZ), (const T*)F(hetddft_ctx::F_UACONV), F(hetddft_ctx::F_POT4));
+      return 0;
+    });
   } else {
     hipLaunchKernelGGL((k_pot_final<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, pa,
                        (const T*)F(hetddft_ctx::F_DEN), (const T*)F(hetddft_ctx::F_DENCG),
@@ -954,9 +980,13 @@ int step_rk_t(hetddft_ctx* c, double dt) {
       out = c->spare;
     }
     if (c->st25)
-      hipLaunchKernelGGL((k_rk_stage25<T>), grid25(c), dim3(S_TX, S_TY), 0, c->st, ha, (const c64*)c->psi,
-                         (const T*)c->fld[hetddft_ctx::F_POT4], c->q, out, (T*)c->fld[hetddft_ctx::F_DEN], h1o,
-                         arun[j], brun[j], crun[j], dt);
+      with_tile(c->tile, [&](auto sh) {
+        using SH = decltype(sh);
+        hipLaunchKernelGGL((k_rk_stage25<T, SH>), grid25<SH>(c), dim3(SH::TX, SH::TY), 0, c->st, ha,
+                           (const c64*)c->psi, (const T*)c->fld[hetddft_ctx::F_POT4], c->q, out,
+                           (T*)c->fld[hetddft_ctx::F_DEN], h1o, arun[j], brun[j], crun[j], dt);
+        return 0;
+      });
     else
       hipLaunchKernelGGL((k_rk_stage<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, ham_direct(ha),
                          (const c64*)c->psi, (const T*)c->fld[hetddft_ctx::F_POT4], c->q, out,
@@ -992,7 +1022,7 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   int* V = c->ioldv;
   int* A = c->iolda;
   const int n3 = 3 * c->p.n_par;
-  const dim3 g25 = grid25(c);
+  const dim3 g25 = with_tile(c->tile, [&](auto sh) { return grid25<decltype(sh)>(c); });
   const unsigned nb = c->st25 ? g25.x * g25.y * g25.z : blocks_for(g.n);
   // Particle predictor (steppc.f90:64-87), then the density chain of the
   // predicted positions on the side stream, beside the psi predictor.
@@ -1012,9 +1042,13 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   {
   ProfScope pspred(c, HETDDFT_PROF_PCP, (double)g.n * (16.0 * 8 + 2.0 * sizeof(T)));
   if (c->st25)
-    PLAUNCH(pspred, (k_pc_predict25<T>), g25, dim3(S_TX, S_TY), 0, ha, (const c64*)x,
-                       (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]], c->hold[H[1]], c->pc,
-                       (T*)c->fld[hetddft_ctx::F_DEN], c4o3 * dt, c112);
+    with_tile(c->tile, [&](auto sh) {
+      using SH = decltype(sh);
+      PLAUNCH(pspred, (k_pc_predict25<T, SH>), g25, dim3(SH::TX, SH::TY), 0, ha, (const c64*)x,
+              (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]], c->hold[H[1]], c->pc,
+              (T*)c->fld[hetddft_ctx::F_DEN], c4o3 * dt, c112);
+      return 0;
+    });
   else
     PLAUNCH(pspred, (k_pc_predict<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, ham_direct(ha),
                        (const c64*)x, (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]],
@@ -1058,10 +1092,14 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   {
   ProfScope pscorr(c, HETDDFT_PROF_PCC, (double)g.n * (16.0 * 10 + 2.0 * sizeof(T)));
   if (c->st25)
-    PLAUNCH(pscorr, (k_pc_correct25<T>), g25, dim3(S_TX, S_TY), 0, ha, (const c64*)c->psi,
-                       (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
-                       (const c64*)c->hold[H[0]], (const c64*)c->hold[H[1]], c->pc, c->spare,
-                       (T*)c->fld[hetddft_ctx::F_DEN], c->part, 3.0 * dt, c9);
+    with_tile(c->tile, [&](auto sh) {
+      using SH = decltype(sh);
+      PLAUNCH(pscorr, (k_pc_correct25<T, SH>), g25, dim3(SH::TX, SH::TY), 0, ha, (const c64*)c->psi,
+              (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
+              (const c64*)c->hold[H[0]], (const c64*)c->hold[H[1]], c->pc, c->spare,
+              (T*)c->fld[hetddft_ctx::F_DEN], c->part, 3.0 * dt, c9);
+      return 0;
+    });
   else
     PLAUNCH(pscorr, (k_pc_correct<T>), dim3(nb), dim3(kBlock), 0, ham_direct(ha), (const c64*)c->psi,
                        (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
@@ -1330,9 +1368,17 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
   c->d2y = make_stencil(2, c->hy);
   c->d2z = make_stencil(2, c->hz);
   {
-    // z chunk of the stencil engine: about 512 workgroups, 4..32 planes each
-    const long long cols = (long long)((p->nx + S_TX - 1) / S_TX) * ((p->ny + S_TY - 1) / S_TY);
-    long long zc = cols * p->nz / 1024;
+    // tile shape, and a z chunk giving about HETDDFT_ZBLOCKS workgroups
+    // (4..32 planes each)
+    const char* t = std::getenv("HETDDFT_TILE");
+    if (t) c->tile = std::max(0, std::min(3, std::atoi(t)));
+    const char* zb = std::getenv("HETDDFT_ZBLOCKS");
+    const long long target = zb && std::atoll(zb) > 0 ? std::atoll(zb) : 1024;
+    const long long cols = with_tile(c->tile, [&](auto sh) {
+      using SH = decltype(sh);
+      return (int)(((p->nx + SH::TX - 1) / SH::TX) * ((p->ny + SH::TY - 1) / SH::TY));
+    });
+    long long zc = cols * p->nz / target;
     c->zc = (int)std::max(4LL, std::min(32LL, zc));
     const char* e = std::getenv("HETDDFT_STENCIL");
     if (e && std::strcmp(e, "direct") == 0) c->st25 = false;

@@ -1,7 +1,7 @@
 // z-marching ("2.5D") engine for the 13-point star stencils of derivnD
 // (icon=13, zero outside the box: init_deriv_parallel.f90:428-535).
 //
-// A workgroup of S_TX x S_TY threads owns one (x, y) column tile and walks
+// A workgroup of TX x TY threads owns one (x, y) column tile and walks
 // a chunk of z planes.  Each thread keeps the 13 z-neighbours of its column
 // in a register queue; the x and y neighbours come from an LDS copy of the
 // current plane with a 6-point halo.  Per output point this costs one fresh
@@ -14,11 +14,14 @@
 
 namespace hetd {
 
-constexpr int S_TX = 64;
-constexpr int S_TY = 4;
-constexpr int S_LX = S_TX + 12;
-constexpr int S_LY = S_TY + 12;
-constexpr int S_THREADS = S_TX * S_TY;
+// Tile shape: TX x TY columns per workgroup (one thread each), LDS plane
+// tile of LX x LY with the 6-point halo.  Per output point a tile reads
+// about 1 + 12/TX + 12/TY + 12/zc values of the stencilled field, so wider
+// tiles and longer z chunks cut re-reads at the cost of fewer workgroups.
+template <int TXv, int TYv>
+struct Tile25 {
+  static constexpr int TX = TXv, TY = TYv, LX = TXv + 12, LY = TYv + 12, N = TXv * TYv;
+};
 
 __device__ inline double vzero(double) { return 0.0; }
 __device__ inline c64 vzero(c64) { return mk(0.0, 0.0); }
@@ -45,11 +48,12 @@ struct Col {
 // of tiles (x tile fastest, then y tile, then z chunk), so the tiles that
 // share x/y halos and z-queue planes read them through the same L2 at about
 // the same time.  The mapping is a bijection for any block count.
+template <class SH>
 __device__ inline Col col_of(const Grid& g, int xcd) {
   Col c;
   c.tx = threadIdx.x;
   c.ty = threadIdx.y;
-  c.tid = c.tx + S_TX * c.ty;
+  c.tid = c.tx + SH::TX * c.ty;
   const int gx = gridDim.x, gy = gridDim.y;
   const int p = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
   int l = p;
@@ -62,8 +66,8 @@ __device__ inline Col col_of(const Grid& g, int xcd) {
   c.bx = l % gx;
   c.by = (l / gx) % gy;
   c.bz = l / (gx * gy);
-  c.x = c.bx * S_TX + c.tx;
-  c.y = c.by * S_TY + c.ty;
+  c.x = c.bx * SH::TX + c.tx;
+  c.y = c.by * SH::TY + c.ty;
   c.in = c.x < g.nx && c.y < g.ny;
   return c;
 }
@@ -85,12 +89,14 @@ struct ZQueue {
       q[k] = (c.in && z >= 0 && z < g.nz) ? t : vzero(V());
     }
   }
-  __device__ inline void fetch(const R* f, const Grid& g, const Col& c, int z) {
+  // Value entering the queue at plane z: f(z + 6), zero outside the box.
+  __device__ inline V next(const R* f, const Grid& g, const Col& c, int z) const {
     const int zz = z + 6;
     const int xc = min(c.x, g.nx - 1), yc = min(c.y, g.ny - 1);
     const V t = vload<V, R>(f, gidx(g, xc, yc, min(zz, g.nz - 1)));
-    q[12] = (c.in && zz < g.nz) ? t : vzero(V());
+    return (c.in && zz < g.nz) ? t : vzero(V());
   }
+  __device__ inline void fetch(const R* f, const Grid& g, const Col& c, int z) { q[12] = next(f, g, c, z); }
   __device__ inline void shift() {
 #pragma unroll
     for (int k = 0; k < 12; ++k) q[k] = q[k + 1];
@@ -104,52 +110,62 @@ struct ZQueue {
   }
 };
 
-// Fill the LDS plane tile (S_LY x S_LX) of plane z: centre from the queue,
-// halo from memory (zero outside the box).
-template <class V, class R>
-__device__ inline void fill_plane(V* tile, const R* f, const Grid& g, const Col& c, int z, V centre) {
-  tile[(c.ty + 6) * S_LX + c.tx + 6] = centre;
-  const int x0 = c.bx * S_TX - 6, y0 = c.by * S_TY - 6;
-  constexpr int NE = (S_LX * S_LY + S_THREADS - 1) / S_THREADS;
+// The same tile in two steps, so that the loads of the next plane can be in
+// flight while the current plane is computed: plane_load issues the halo
+// loads into registers, plane_store writes them (and the centre) to LDS.
+template <class SH, class V>
+struct PlaneRegs {
+  static constexpr int NE = (SH::LX * SH::LY + SH::N - 1) / SH::N;
   V v[NE];
+};
+
+template <class SH, class V, class R>
+__device__ inline void plane_load(PlaneRegs<SH, V>& pr, const R* f, const Grid& g, const Col& c, int z) {
+  const int x0 = c.bx * SH::TX - 6, y0 = c.by * SH::TY - 6;
 #pragma unroll
-  for (int k = 0; k < NE; ++k) {
-    const int e = c.tid + k * S_THREADS;
-    const int ly = e / S_LX, lx = e - ly * S_LX;
+  for (int k = 0; k < PlaneRegs<SH, V>::NE; ++k) {
+    const int e = c.tid + k * SH::N;
+    const int ly = e / SH::LX, lx = e - ly * SH::LX;
     const int x = x0 + lx, y = y0 + ly;
     const bool inb = x >= 0 && x < g.nx && y >= 0 && y < g.ny;
     const int xc = min(max(x, 0), g.nx - 1), yc = min(max(y, 0), g.ny - 1);
     const V t = vload<V, R>(f, gidx(g, xc, yc, z));
-    v[k] = inb ? t : vzero(V());
-  }
-#pragma unroll
-  for (int k = 0; k < NE; ++k) {
-    const int e = c.tid + k * S_THREADS;
-    const int ly = e / S_LX, lx = e - ly * S_LX;
-    const bool centre_region = ly >= 6 && ly < 6 + S_TY && lx >= 6 && lx < 6 + S_TX;
-    if (e < S_LX * S_LY && !centre_region) tile[e] = v[k];
+    pr.v[k] = inb ? t : vzero(V());
   }
 }
 
-template <class V>
+template <class SH, class V>
+__device__ inline void plane_store(V* tile, const PlaneRegs<SH, V>& pr, const Col& c, V centre) {
+  tile[(c.ty + 6) * SH::LX + c.tx + 6] = centre;
+#pragma unroll
+  for (int k = 0; k < PlaneRegs<SH, V>::NE; ++k) {
+    const int e = c.tid + k * SH::N;
+    const int ly = e / SH::LX, lx = e - ly * SH::LX;
+    const bool centre_region = ly >= 6 && ly < 6 + SH::TY && lx >= 6 && lx < 6 + SH::TX;
+    if (e < SH::LX * SH::LY && !centre_region) tile[e] = pr.v[k];
+  }
+}
+
+template <class SH, class V>
 __device__ inline V tile_dx(const V* tile, const Col& c, const Stencil13& w) {
-  const V* row = tile + (c.ty + 6) * S_LX + c.tx;
+  const V* row = tile + (c.ty + 6) * SH::LX + c.tx;
   V acc = vzero(V());
 #pragma unroll
   for (int m = 0; m < 13; ++m) acc = vmadd(acc, w.w[m], row[m]);
   return acc;
 }
 
-template <class V>
+template <class SH, class V>
 __device__ inline V tile_dy(const V* tile, const Col& c, const Stencil13& w) {
-  const V* colp = tile + c.ty * S_LX + c.tx + 6;
+  const V* colp = tile + c.ty * SH::LX + c.tx + 6;
   V acc = vzero(V());
 #pragma unroll
-  for (int m = 0; m < 13; ++m) acc = vmadd(acc, w.w[m], colp[m * S_LX]);
+  for (int m = 0; m < 13; ++m) acc = vmadd(acc, w.w[m], colp[m * SH::LX]);
   return acc;
 }
 
-// Block sum for the 2D engine blocks (S_THREADS threads).
+// Block sum for the 2D engine blocks (SH::N threads).
+template <class SH>
 __device__ inline double block_sum25(double v, double* sh, int tid) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   __syncthreads();
@@ -157,7 +173,7 @@ __device__ inline double block_sum25(double v, double* sh, int tid) {
   __syncthreads();
   double s = 0.0;
   if (tid == 0)
-    for (int k = 0; k < S_THREADS / 64; ++k) s += sh[k];
+    for (int k = 0; k < SH::N / 64; ++k) s += sh[k];
   return s;
 }
 
@@ -172,10 +188,11 @@ struct Ham25 {
 };
 
 // Hpsi at the column's current plane (steppc.f90:36-46).
+template <class SH>
 __device__ inline c64 ham25(const Ham25& a, const c64* tile, const ZQueue<c64, c64>& q, const Col& c,
                             double pv, int z) {
-  const c64 sx = tile_dx(tile, c, a.d2x);
-  const c64 sy = tile_dy(tile, c, a.d2y);
+  const c64 sx = tile_dx<SH>(tile, c, a.d2x);
+  const c64 sy = tile_dy<SH>(tile, c, a.d2y);
   const c64 sz = q.dz(a.d2z);
   const c64 lap = (sx + sy) + sz;
   const c64 ps = q.centre();
@@ -183,28 +200,47 @@ __device__ inline c64 ham25(const Ham25& a, const c64* tile, const ZQueue<c64, c
   return cmul(a.timec_z[z], t);
 }
 
-// Hamming predictor + modifier (steppc.f90:46-56).
-template <class R>
-__global__ void __launch_bounds__(S_THREADS) k_pc_predict25(Ham25 a, const c64* __restrict__ psi,
+// Hamming predictor + modifier (steppc.f90:46-56).  Software-pipelined: the
+// tile, queue entry and centre values of plane z+1 are loaded while plane z
+// is computed.
+template <class R, class SH>
+__global__ void __launch_bounds__(SH::N) k_pc_predict25(Ham25 a, const c64* __restrict__ psi,
                                                              const R* __restrict__ pot4, c64* old3,
                                                              const c64* __restrict__ h1, c64* h2,
                                                              c64* __restrict__ pc, R* __restrict__ den,
                                                              double c4o3dt, double c112) {
-  __shared__ c64 tile[S_LX * S_LY];
+  __shared__ c64 tile[SH::LX * SH::LY];
   const Grid& g = a.g;
-  const Col c = col_of(g, a.xcd);
+  const Col c = col_of<SH>(g, a.xcd);
   const int z0 = c.bz * a.zc;
   const int z1 = min(z0 + a.zc, g.nz);
+  const int xc = min(c.x, g.nx - 1), yc = min(c.y, g.ny - 1);
   ZQueue<c64, c64> q;
   q.init(psi, g, c, z0);
+  PlaneRegs<SH, c64> pr;
+  c64 qn, o3n, a1n, a2n, pcn;
+  double pvn;
+  auto prefetch = [&](int z) {
+    qn = q.next(psi, g, c, z);
+    plane_load<SH>(pr, psi, g, c, z);
+    const long long j = gidx(g, xc, yc, z);
+    pvn = (double)pot4[j];
+    o3n = old3[j];
+    a1n = h1[j];
+    a2n = h2[j];
+    pcn = pc[j];
+  };
+  if (z0 < z1) prefetch(z0);
   for (int z = z0; z < z1; ++z) {
-    q.fetch(psi, g, c, z);
-    fill_plane(tile, psi, g, c, z, q.q[6]);
+    q.q[12] = qn;
+    plane_store<SH>(tile, pr, c, q.q[6]);
+    const double pv = pvn;
+    const c64 o3 = o3n, a1 = a1n, a2 = a2n, pci = pcn;
     __syncthreads();
+    if (z + 1 < z1) prefetch(z + 1);
     if (c.in) {
       const long long i = gidx(g, c.x, c.y, z);
-      const c64 s4 = ham25(a, tile, q, c, (double)pot4[i], z);
-      const c64 o3 = old3[i], a1 = h1[i], a2 = h2[i], pci = pc[i];
+      const c64 s4 = ham25<SH>(a, tile, q, c, pv, z);
       const c64 s1 = mk(o3.x + c4o3dt * ((2.0 * s4.x - a1.x) + 2.0 * a2.x),
                         o3.y + c4o3dt * ((2.0 * s4.y - a1.y) + 2.0 * a2.y));
       const c64 np = mk(s1.x - c112 * pci.x, s1.y - c112 * pci.y);
@@ -218,9 +254,10 @@ __global__ void __launch_bounds__(S_THREADS) k_pc_predict25(Ham25 a, const c64* 
   }
 }
 
-// Hamming corrector + final value + errHe partial sums (steppc.f90:97-112).
-template <class R>
-__global__ void __launch_bounds__(S_THREADS) k_pc_correct25(Ham25 a, const c64* __restrict__ psi,
+// Hamming corrector + final value + errHe partial sums (steppc.f90:97-112),
+// software-pipelined like the predictor.
+template <class R, class SH>
+__global__ void __launch_bounds__(SH::N) k_pc_correct25(Ham25 a, const c64* __restrict__ psi,
                                                              const R* __restrict__ pot4,
                                                              const c64* __restrict__ o3p,
                                                              const c64* __restrict__ o2p,
@@ -229,26 +266,43 @@ __global__ void __launch_bounds__(S_THREADS) k_pc_correct25(Ham25 a, const c64* 
                                                              c64* __restrict__ pc, c64* __restrict__ psi_out,
                                                              R* __restrict__ den, double* __restrict__ part,
                                                              double dt3, double c9) {
-  __shared__ c64 tile[S_LX * S_LY];
-  __shared__ double sh[S_THREADS / 64];
+  __shared__ c64 tile[SH::LX * SH::LY];
+  __shared__ double sh[SH::N / 64];
   const Grid& g = a.g;
-  const Col c = col_of(g, a.xcd);
+  const Col c = col_of<SH>(g, a.xcd);
   const int z0 = c.bz * a.zc;
   const int z1 = min(z0 + a.zc, g.nz);
+  const int xc = min(c.x, g.nx - 1), yc = min(c.y, g.ny - 1);
   ZQueue<c64, c64> q;
   q.init(psi, g, c, z0);
+  PlaneRegs<SH, c64> pr;
+  c64 qn, v3n, v2n, a1n, a2n, pcn;
+  double pvn;
+  auto prefetch = [&](int z) {
+    qn = q.next(psi, g, c, z);
+    plane_load<SH>(pr, psi, g, c, z);
+    const long long j = gidx(g, xc, yc, z);
+    pvn = (double)pot4[j];
+    v3n = o3p[j];
+    v2n = o2p[j];
+    a1n = h1p[j];
+    a2n = h2p[j];
+    pcn = pc[j];
+  };
+  if (z0 < z1) prefetch(z0);
   double e = 0.0;
   for (int z = z0; z < z1; ++z) {
-    q.fetch(psi, g, c, z);
-    fill_plane(tile, psi, g, c, z, q.q[6]);
+    q.q[12] = qn;
+    plane_store<SH>(tile, pr, c, q.q[6]);
+    const double pv = pvn;
+    const c64 v3 = v3n, v2 = v2n, a1 = a1n, a2 = a2n, pci = pcn;
     __syncthreads();
+    if (z + 1 < z1) prefetch(z + 1);
     if (c.in) {
       const long long i = gidx(g, c.x, c.y, z);
-      const c64 s4 = ham25(a, tile, q, c, (double)pot4[i], z);
-      const c64 v3 = o3p[i], v2 = o2p[i], a1 = h1p[i], a2 = h2p[i];
+      const c64 s4 = ham25<SH>(a, tile, q, c, pv, z);
       const c64 s5 = mk(0.125 * ((9.0 * v3.x - v2.x) + dt3 * ((s4.x + 2.0 * a1.x) - a2.x)),
                         0.125 * ((9.0 * v3.y - v2.y) + dt3 * ((s4.y + 2.0 * a1.y) - a2.y)));
-      const c64 pci = pc[i];
       const c64 npc = mk(pci.x - s5.x, pci.y - s5.y);
       pc[i] = npc;
       const c64 np = mk(s5.x + c9 * npc.x, s5.y + c9 * npc.y);
@@ -259,32 +313,46 @@ __global__ void __launch_bounds__(S_THREADS) k_pc_correct25(Ham25 a, const c64* 
     __syncthreads();
     q.shift();
   }
-  const double s = block_sum25(e, sh, c.tid);
+  const double s = block_sum25<SH>(e, sh, c.tid);
   if (c.tid == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
 }
 
-// RK-Gill stage (steprk.f90:56-74).
-template <class R>
-__global__ void __launch_bounds__(S_THREADS) k_rk_stage25(Ham25 a, const c64* __restrict__ psi,
+// RK-Gill stage (steprk.f90:56-74), software-pipelined like the predictor.
+template <class R, class SH>
+__global__ void __launch_bounds__(SH::N) k_rk_stage25(Ham25 a, const c64* __restrict__ psi,
                                                            const R* __restrict__ pot4, c64* __restrict__ qacc,
                                                            c64* __restrict__ psi_out, R* __restrict__ den,
                                                            c64* __restrict__ h1_out, double ca, double cb,
                                                            double cc, double dt) {
-  __shared__ c64 tile[S_LX * S_LY];
+  __shared__ c64 tile[SH::LX * SH::LY];
   const Grid& g = a.g;
-  const Col c = col_of(g, a.xcd);
+  const Col c = col_of<SH>(g, a.xcd);
   const int z0 = c.bz * a.zc;
   const int z1 = min(z0 + a.zc, g.nz);
+  const int xc = min(c.x, g.nx - 1), yc = min(c.y, g.ny - 1);
   ZQueue<c64, c64> q;
   q.init(psi, g, c, z0);
+  PlaneRegs<SH, c64> pr;
+  c64 qn, qin;
+  double pvn;
+  auto prefetch = [&](int z) {
+    qn = q.next(psi, g, c, z);
+    plane_load<SH>(pr, psi, g, c, z);
+    const long long j = gidx(g, xc, yc, z);
+    pvn = (double)pot4[j];
+    qin = qacc[j];
+  };
+  if (z0 < z1) prefetch(z0);
   for (int z = z0; z < z1; ++z) {
-    q.fetch(psi, g, c, z);
-    fill_plane(tile, psi, g, c, z, q.q[6]);
+    q.q[12] = qn;
+    plane_store<SH>(tile, pr, c, q.q[6]);
+    const double pv = pvn;
+    const c64 qi = qin;
     __syncthreads();
+    if (z + 1 < z1) prefetch(z + 1);
     if (c.in) {
       const long long i = gidx(g, c.x, c.y, z);
-      const c64 s4 = ham25(a, tile, q, c, (double)pot4[i], z);
-      const c64 qi = qacc[i];
+      const c64 s4 = ham25<SH>(a, tile, q, c, pv, z);
       const c64 s1 = mk(ca * (s4.x - cb * qi.x), ca * (s4.y - cb * qi.y));
       qacc[i] = mk((qi.x + 3.0 * s1.x) - cc * s4.x, (qi.y + 3.0 * s1.y) - cc * s4.y);
       if (h1_out) h1_out[i] = s4;
@@ -298,25 +366,34 @@ __global__ void __launch_bounds__(S_THREADS) k_rk_stage25(Ham25 a, const c64* __
   }
 }
 
-// grad(den) with the first-derivative stencil (derden.f90:26-28).
-template <class R>
-__global__ void __launch_bounds__(S_THREADS) k_grad3_25(Grid g, int zc, int xcd, const R* __restrict__ f, Stencil13 wx,
+// grad(den) with the first-derivative stencil (derden.f90:26-28),
+// software-pipelined like the predictor.
+template <class R, class SH>
+__global__ void __launch_bounds__(SH::N) k_grad3_25(Grid g, int zc, int xcd, const R* __restrict__ f, Stencil13 wx,
                                                          Stencil13 wy, Stencil13 wz, R* __restrict__ fx,
                                                          R* __restrict__ fy, R* __restrict__ fz) {
-  __shared__ double tile[S_LX * S_LY];
-  const Col c = col_of(g, xcd);
+  __shared__ double tile[SH::LX * SH::LY];
+  const Col c = col_of<SH>(g, xcd);
   const int z0 = c.bz * zc;
   const int z1 = min(z0 + zc, g.nz);
   ZQueue<double, R> q;
   q.init(f, g, c, z0);
+  PlaneRegs<SH, double> pr;
+  double qn = 0.0;
+  auto prefetch = [&](int z) {
+    qn = q.next(f, g, c, z);
+    plane_load<SH>(pr, f, g, c, z);
+  };
+  if (z0 < z1) prefetch(z0);
   for (int z = z0; z < z1; ++z) {
-    q.fetch(f, g, c, z);
-    fill_plane(tile, f, g, c, z, q.q[6]);
+    q.q[12] = qn;
+    plane_store<SH>(tile, pr, c, q.q[6]);
     __syncthreads();
+    if (z + 1 < z1) prefetch(z + 1);
     if (c.in) {
       const long long i = gidx(g, c.x, c.y, z);
-      fx[i] = (R)tile_dx(tile, c, wx);
-      fy[i] = (R)tile_dy(tile, c, wy);
+      fx[i] = (R)tile_dx<SH>(tile, c, wx);
+      fy[i] = (R)tile_dy<SH>(tile, c, wy);
       fz[i] = (R)q.dz(wz);
     }
     __syncthreads();
@@ -350,8 +427,8 @@ __device__ inline double prod_at(const R* __restrict__ a, const R* __restrict__ 
   return (double)a[j] * (double)b[j];
 }
 
-template <class R>
-__global__ void __launch_bounds__(S_THREADS) k_pot_final25(Pot25 a, const R* __restrict__ den,
+template <class R, class SH>
+__global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25 a, const R* __restrict__ den,
                                                             const R* __restrict__ dencg,
                                                             const R* __restrict__ delj4,
                                                             const R* __restrict__ sto1c,
@@ -361,10 +438,12 @@ __global__ void __launch_bounds__(S_THREADS) k_pot_final25(Pot25 a, const R* __r
                                                             const R* __restrict__ inty,
                                                             const R* __restrict__ intz,
                                                             const R* __restrict__ uaconv, R* __restrict__ pot4) {
-  __shared__ double tx_tile[S_TY * S_LX];    // g_x, rows of this tile with x halo
-  __shared__ double ty_tile[S_LY * S_TX];    // g_y, columns of this tile with y halo
+  __shared__ double tx_tile[SH::TY * SH::LX];    // g_x, rows of this tile with x halo
+  __shared__ double ty_tile[SH::LY * SH::TX];    // g_y, columns of this tile with y halo
+  constexpr int NX = (SH::TY * SH::LX + SH::N - 1) / SH::N;
+  constexpr int NY = (SH::LY * SH::TX + SH::N - 1) / SH::N;
   const Grid& g = a.g;
-  const Col c = col_of(g, a.xcd);
+  const Col c = col_of<SH>(g, a.xcd);
   const int z0 = c.bz * a.zc;
   const int z1 = min(z0 + a.zc, g.nz);
   const int xc = min(c.x, g.nx - 1), yc = min(c.y, g.ny - 1);
@@ -377,72 +456,91 @@ __global__ void __launch_bounds__(S_THREADS) k_pot_final25(Pot25 a, const R* __r
       q[k] = (c.in && z >= 0 && z < g.nz) ? t : 0.0;
     }
   }
-  const int x0 = c.bx * S_TX - 6, y0 = c.by * S_TY - 6;
-  for (int z = z0; z < z1; ++z) {
+  const int x0 = c.bx * SH::TX - 6, y0 = c.by * SH::TY - 6;
+  // Software pipeline: the raw operands of plane z+1 (tile products, queue
+  // entry, centre values) are loaded while plane z is computed; products
+  // are formed when the plane is used, as (double)a * (double)b.
+  R fxn[NX], ixn[NX], fyn[NY], iyn[NY], fzn = R(0), izn = R(0);
+  bool okx[NX], oky[NY], okz = false;
+  R dnn = R(0), dcgn = R(0), dljn = R(0), s1n = R(0), uin = R(0), uan = R(0);
+  auto prefetch = [&](int z) {
     if (a.otc) {
       const int zz = z + 6;
-      const double t = prod_at(falfs, intz, g, xc, yc, min(zz, g.nz - 1));
-      q[12] = (c.in && zz < g.nz) ? t : 0.0;
-      // g_x rows: S_TY x S_LX
-      constexpr int NX = (S_TY * S_LX + S_THREADS - 1) / S_THREADS;
-      double vx[NX];
+      const long long jz = gidx(g, xc, yc, min(zz, g.nz - 1));
+      fzn = falfs[jz];
+      izn = intz[jz];
+      okz = c.in && zz < g.nz;
 #pragma unroll
       for (int k = 0; k < NX; ++k) {
-        const int e = c.tid + k * S_THREADS;
-        const int ly = e / S_LX, lx = e - ly * S_LX;
-        const int x = x0 + lx, y = c.by * S_TY + ly;
-        const bool inb = x >= 0 && x < g.nx && y < g.ny;
-        const double v = prod_at(falfs, intx, g, min(max(x, 0), g.nx - 1), min(y, g.ny - 1), z);
-        vx[k] = inb ? v : 0.0;
-      }
-      constexpr int NY = (S_LY * S_TX + S_THREADS - 1) / S_THREADS;
-      double vy[NY];
-#pragma unroll
-      for (int k = 0; k < NY; ++k) {
-        const int e = c.tid + k * S_THREADS;
-        const int ly = e / S_TX, lx = e - ly * S_TX;
-        const int x = c.bx * S_TX + lx, y = y0 + ly;
-        const bool inb = x < g.nx && y >= 0 && y < g.ny;
-        const double v = prod_at(falfs, inty, g, min(x, g.nx - 1), min(max(y, 0), g.ny - 1), z);
-        vy[k] = inb ? v : 0.0;
-      }
-#pragma unroll
-      for (int k = 0; k < NX; ++k) {
-        const int e = c.tid + k * S_THREADS;
-        if (e < S_TY * S_LX) tx_tile[e] = vx[k];
+        const int e = c.tid + k * SH::N;
+        const int ly = e / SH::LX, lx = e - ly * SH::LX;
+        const int x = x0 + lx, y = c.by * SH::TY + ly;
+        okx[k] = x >= 0 && x < g.nx && y < g.ny;
+        const long long j = gidx(g, min(max(x, 0), g.nx - 1), min(y, g.ny - 1), z);
+        fxn[k] = falfs[j];
+        ixn[k] = intx[j];
       }
 #pragma unroll
       for (int k = 0; k < NY; ++k) {
-        const int e = c.tid + k * S_THREADS;
-        if (e < S_LY * S_TX) ty_tile[e] = vy[k];
+        const int e = c.tid + k * SH::N;
+        const int ly = e / SH::TX, lx = e - ly * SH::TX;
+        const int x = c.bx * SH::TX + lx, y = y0 + ly;
+        oky[k] = x < g.nx && y >= 0 && y < g.ny;
+        const long long j = gidx(g, min(x, g.nx - 1), min(max(y, 0), g.ny - 1), z);
+        fyn[k] = falfs[j];
+        iyn[k] = inty[j];
       }
-      __syncthreads();
+      uan = uaconv[gidx(g, xc, yc, z)];
     }
+    const long long i = gidx(g, xc, yc, z);
+    dnn = den[i];
+    dcgn = dencg[i];
+    dljn = delj4[i];
+    s1n = sto1c[i];
+    uin = uimp[i];
+  };
+  if (z0 < z1) prefetch(z0);
+  for (int z = z0; z < z1; ++z) {
+    if (a.otc) {
+      q[12] = okz ? (double)fzn * (double)izn : 0.0;
+#pragma unroll
+      for (int k = 0; k < NX; ++k) {
+        const int e = c.tid + k * SH::N;
+        if (e < SH::TY * SH::LX) tx_tile[e] = okx[k] ? (double)fxn[k] * (double)ixn[k] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < NY; ++k) {
+        const int e = c.tid + k * SH::N;
+        if (e < SH::LY * SH::TX) ty_tile[e] = oky[k] ? (double)fyn[k] * (double)iyn[k] : 0.0;
+      }
+    }
+    const double dn = (double)dnn, dcg = (double)dcgn, dlj = (double)dljn, s1 = (double)s1n;
+    const double ui = (double)uin, uac = (double)uan;
+    __syncthreads();
+    if (z + 1 < z1) prefetch(z + 1);
     if (c.in) {
       const long long i = gidx(g, c.x, c.y, z);
-      const double dn = (double)den[i];
-      const double dcg = (double)dencg[i];
-      double p4 = ((double)delj4[i] + dcg * dcg * (a.a0 + a.a1 * dcg)) + (double)sto1c[i];
+      double p4 = (dlj + dcg * dcg * (a.a0 + a.a1 * dcg)) + s1;
       if (a.otc) {
         double s4 = 0.0, s5 = 0.0, s6 = 0.0;
-        const double* rx = tx_tile + c.ty * S_LX + c.tx;
-        const double* cy = ty_tile + c.tx + c.ty * S_TX;
+        const double* rx = tx_tile + c.ty * SH::LX + c.tx;
+        const double* cy = ty_tile + c.tx + c.ty * SH::TX;
 #pragma unroll
         for (int m = 0; m < 13; ++m) s4 = s4 + a.d1x.w[m] * rx[m];
 #pragma unroll
-        for (int m = 0; m < 13; ++m) s5 = s5 + a.d1y.w[m] * cy[m * S_TX];
+        for (int m = 0; m < 13; ++m) s5 = s5 + a.d1y.w[m] * cy[m * SH::TX];
 #pragma unroll
         for (int m = 0; m < 13; ++m) s6 = s6 + a.d1z.w[m] * q[m];
-        const double ua = (double)uaconv[i] / a.den0s;
+        const double ua = uac / a.den0s;
         const double ual = (((ua + s4) + s5) + s6) * a.h2o2m4 * a.alphas;
         p4 = p4 + ual;
       }
       const double t = tanh(a.beta * (dn - a.den_m));
       const double pen = a.C * (1.0 + t + a.beta * dn * (1.0 - t * t));
-      pot4[i] = (R)((p4 + (double)uimp[i]) + pen);
+      pot4[i] = (R)((p4 + ui) + pen);
     }
+    __syncthreads();
     if (a.otc) {
-      __syncthreads();
 #pragma unroll
       for (int k = 0; k < 12; ++k) q[k] = q[k + 1];
     }

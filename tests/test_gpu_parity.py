@@ -110,3 +110,26 @@ def test_timec_lambda(hip):
     psi, r, v, a = g.get_state()
     assert rel_max(psi, o.psi) < TOL
     g.close()
+
+
+@pytest.mark.parametrize("tile,zc,xcd", [("1", "5", "1"), ("2", "4", "1"), ("3", "7", "0"), ("0", "3", "0")])
+@pytest.mark.parametrize("name", ["t24", "t32"])
+def test_trajectory_tile_shapes(hip, name, tile, zc, xcd, monkeypatch):
+    """Every tile shape of the z-marching stencil engine, z chunks that do not
+    divide nz, partial x tiles (24 < 32) and both block orders give the same
+    trajectory as the oracle."""
+    monkeypatch.setenv("HETDDFT_TILE", tile)
+    monkeypatch.setenv("HETDDFT_ZC", zc)
+    monkeypatch.setenv("HETDDFT_XCD", xcd)
+    wl, o, g = _setup(hip, name)
+    print(f"trajectory {name} tile {tile} zc {zc} xcd {xcd}", flush=True)
+    o.start()
+    g.start()
+    for it in range(3 + 2):
+        o.iterate(it, wl.deltat)
+        eg = g.iterate(wl.deltat)
+        psi, r, v, a = g.get_state()
+        assert rel_max(psi, o.psi) < TOL, (it, rel_max(psi, o.psi))
+        assert rel_max(r, o.rimp) < TOL, it
+        assert rel_max(g.get_field("pot4"), o.pot4) < TOL, it
+    g.close()
