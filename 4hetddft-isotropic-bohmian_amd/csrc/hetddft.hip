@@ -239,16 +239,23 @@ hipEvent_t prof_event(hetddft_ctx* c) {
     c->prof_pool.pop_back();
     return e;
   }
+  // timing only: no system-scope fence (no cache writeback inside the
+  // measured interval)
   hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
   return e;
 }
 
-// Times one launch of kernel class `cls` when profiling: the launch goes
-// through PLAUNCH, which hands events a and b to hipExtLaunchKernel, so they
-// carry the dispatch's own start and end timestamps (the kernel's execution
-// time, as a profiler's kernel trace reports it, without queueing gaps).
-// With profiling off both events are null and PLAUNCH is a plain launch.
+// Times launches on st when profiling.  Every kernel launch on st in the
+// step path goes through PLAUNCH (profiled class cls >= 0) or SLAUNCH
+// (cls = -1), which hand an end event to hipExtLaunchKernel: the event
+// carries the dispatch's completion timestamp.  Kernels on st run back to
+// back, so a profiled launch lasts from the end of the previous launch on
+// st to its own end -- the convention of a profiler's kernel trace, where
+// each kernel of a stream starts when the previous one ends.  No start event
+// is attached: a separate start packet would add its own gap to the
+// measurement.  With profiling off the event is null and PLAUNCH is a plain
+// launch.
 struct ProfScope {
   hetddft_ctx* c;
   int cls;
@@ -256,20 +263,22 @@ struct ProfScope {
   hipEvent_t a = nullptr, b = nullptr;
   ProfScope(hetddft_ctx* ctx, int k, double by) : c(ctx), cls(k), bytes(by) {
     if (!c->prof_on) return;
-    a = prof_event(c);
-    b = a ? prof_event(c) : nullptr;
-    if (a && !b) {
-      c->prof_pool.push_back(a);
-      a = nullptr;
-    }
+    b = prof_event(c);
   }
   ~ProfScope() {
-    if (a && b) c->prof_pending.push_back({a, b, cls, bytes});
+    if (b) c->prof_pending.push_back({a, b, cls, bytes});
   }
 };
 
 #define PLAUNCH(ps, kern, grid, block, lds, ...) \
   hipExtLaunchKernelGGL(kern, grid, block, lds, c->st, (ps).a, (ps).b, 0u, __VA_ARGS__)
+
+// A launch on st outside the profiled classes (keeps the end-event chain).
+#define SLAUNCH(kern, grid, block, lds, ...)                  \
+  do {                                                        \
+    ProfScope ps_(c, -1, 0.0);                                \
+    PLAUNCH(ps_, kern, grid, block, lds, __VA_ARGS__);        \
+  } while (0)
 
 int fail(hetddft_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -859,12 +868,12 @@ int veval(hetddft_ctx* c) {
     ga.d1[1] = c->d1y;
     ga.d1[2] = c->d1z;
     ga.m_imp = c->m_imp;
-    hipLaunchKernelGGL((k_force_corners<T>), dim3(blocks_for(24LL * c->p.n_par)), dim3(kBlock), 0, c->st, ga,
+    SLAUNCH((k_force_corners<T>), dim3(blocks_for(24LL * c->p.n_par)), dim3(kBlock), 0, ga,
                        c->rimp, (const T*)F(hetddft_ctx::F_FCX), (const T*)F(hetddft_ctx::F_FCY),
                        (const T*)F(hetddft_ctx::F_FCZ), (const T*)cur, (const T*)psix,
                        (const T*)F(hetddft_ctx::F_HPSIX), c->fcorner);
     LAUNCHCHK(c);
-    hipLaunchKernelGGL(k_blend, dim3(blocks_for(3LL * c->p.n_par)), dim3(kBlock), 0, c->st, b, c->rimp,
+    SLAUNCH(k_blend, dim3(blocks_for(3LL * c->p.n_par)), dim3(kBlock), 0, b, c->rimp,
                        (const double*)c->fcorner, c->aimp);
     LAUNCHCHK(c);
   }
@@ -912,7 +921,7 @@ int veval(hetddft_ctx* c) {
       return 0;
     });
   } else {
-    hipLaunchKernelGGL((k_pot_final<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, pa,
+    SLAUNCH((k_pot_final<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, pa,
                        (const T*)F(hetddft_ctx::F_DEN), (const T*)F(hetddft_ctx::F_DENCG),
                        (const T*)F(hetddft_ctx::F_DELJ4), (const T*)F(hetddft_ctx::F_STO1),
                        (const T*)F(hetddft_ctx::F_UIMP), (const T*)F(hetddft_ctx::F_FALFS),
@@ -982,13 +991,13 @@ int step_rk_t(hetddft_ctx* c, double dt) {
     if (c->st25)
       with_tile(c->tile, [&](auto sh) {
         using SH = decltype(sh);
-        hipLaunchKernelGGL((k_rk_stage25<T, SH>), grid25<SH>(c), dim3(SH::TX, SH::TY), 0, c->st, ha,
+        SLAUNCH((k_rk_stage25<T, SH>), grid25<SH>(c), dim3(SH::TX, SH::TY), 0, ha,
                            (const c64*)c->psi, (const T*)c->fld[hetddft_ctx::F_POT4], c->q, out,
                            (T*)c->fld[hetddft_ctx::F_DEN], h1o, arun[j], brun[j], crun[j], dt);
         return 0;
       });
     else
-      hipLaunchKernelGGL((k_rk_stage<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, ham_direct(ha),
+      SLAUNCH((k_rk_stage<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, ham_direct(ha),
                          (const c64*)c->psi, (const T*)c->fld[hetddft_ctx::F_POT4], c->q, out,
                          (T*)c->fld[hetddft_ctx::F_DEN], h1o, arun[j], brun[j], crun[j], dt);
     LAUNCHCHK(c);
@@ -999,7 +1008,7 @@ int step_rk_t(hetddft_ctx* c, double dt) {
       c->psi = out;
     }
     if (n3 > 0) {
-      hipLaunchKernelGGL(k_part_rk, dim3(blocks_for(n3)), dim3(kBlock), 0, c->st, n3, c->rimp, c->vimp,
+      SLAUNCH(k_part_rk, dim3(blocks_for(n3)), dim3(kBlock), 0, n3, c->rimp, c->vimp,
                          (const double*)c->aimp, c->qr, c->qv, r1o, v1o, a1o, arun[j], brun[j], crun[j], dt);
       LAUNCHCHK(c);
     }
@@ -1027,7 +1036,7 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   // Particle predictor (steppc.f90:64-87), then the density chain of the
   // predicted positions on the side stream, beside the psi predictor.
   if (n3 > 0) {
-    hipLaunchKernelGGL(k_part_predict, dim3(blocks_for(n3)), dim3(kBlock), 0, c->st, n3, c->rimp,
+    SLAUNCH(k_part_predict, dim3(blocks_for(n3)), dim3(kBlock), 0, n3, c->rimp,
                        (const double*)c->vimp, (const double*)c->aimp, c->rold[R[2]], (const double*)c->vold[V[0]],
                        (const double*)c->vold[V[1]], c->vold[V[2]], (const double*)c->aold[A[0]], c->aold[A[1]],
                        c->pcr, c->pcv, c4o3 * dt, c112);
@@ -1065,12 +1074,12 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   if (n3 > 0) {
     const unsigned pb = blocks_for(n3);
     double* pp = c->part + nb;  // after the psi partials
-    hipLaunchKernelGGL(k_part_correct, dim3(pb), dim3(kBlock), 0, c->st, n3, c->rimp, c->vimp,
+    SLAUNCH(k_part_correct, dim3(pb), dim3(kBlock), 0, n3, c->rimp, c->vimp,
                        (const double*)c->aimp, (const double*)c->rold[R[2]], (const double*)c->rold[R[1]],
                        (const double*)c->vold[V[0]], (const double*)c->vold[V[1]], c->vold[V[2]],
                        (const double*)c->aold[A[0]], (const double*)c->aold[A[1]], c->pcr, c->pcv, pp, 3.0 * dt, c9);
     LAUNCHCHK(c);
-    hipLaunchKernelGGL(k_finish_sums, dim3(2), dim3(kBlock), 0, c->st, (const double*)pp, (int)pb, 2, c->res + 1);
+    SLAUNCH(k_finish_sums, dim3(2), dim3(kBlock), 0, (const double*)pp, (int)pb, 2, c->res + 1);
     LAUNCHCHK(c);
   } else {
     HIPCHK(c, hipMemsetAsync(c->res + 1, 0, 2 * sizeof(double), c->st));
@@ -1108,7 +1117,7 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   }
   LAUNCHCHK(c);
   std::swap(c->psi, c->spare);
-  hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(kBlock), 0, c->st, (const double*)c->part, (int)nb, 1, c->res);
+  SLAUNCH(k_finish_sums, dim3(1), dim3(kBlock), 0, (const double*)c->part, (int)nb, 1, c->res);
   LAUNCHCHK(c);
   {
     const int t = P[2];
@@ -1313,7 +1322,7 @@ int hetddft_destroy(hetddft_ctx* c) {
     if (e) hipEventDestroy(e);
   if (c->st2) hipStreamDestroy(c->st2);
   for (auto& r : c->prof_pending) {
-    hipEventDestroy(r.a);
+    if (r.a) hipEventDestroy(r.a);
     hipEventDestroy(r.b);
   }
   for (hipEvent_t e : c->prof_pool) hipEventDestroy(e);
@@ -1786,7 +1795,7 @@ int hetddft_profile(hetddft_ctx* c, int enable) {
   RET(settle(c));
   HIPCHK(c, hipStreamSynchronize(c->st));
   for (auto& r : c->prof_pending) {
-    c->prof_pool.push_back(r.a);
+    if (r.a) c->prof_pool.push_back(r.a);
     c->prof_pool.push_back(r.b);
   }
   c->prof_pending.clear();
@@ -1803,13 +1812,19 @@ int hetddft_profile_read(hetddft_ctx* c, int cls, double* ms_total, long long* l
   if (!c || cls < 0 || cls >= HETDDFT_PROF_N) return HETDDFT_E_ARG;
   RET(settle(c));
   HIPCHK(c, hipStreamSynchronize(c->st));
+  hipEvent_t prev = nullptr;  // end event of the previous launch on st
   for (auto& r : c->prof_pending) {
-    float ms = 0.0f;
-    HIPCHK(c, hipEventElapsedTime(&ms, r.a, r.b));
-    c->prof_ms[r.cls] += ms;
-    c->prof_n[r.cls] += 1;
-    c->prof_bytes[r.cls] += r.bytes;
-    c->prof_pool.push_back(r.a);
+    if (r.cls >= 0 && prev) {
+      float ms = 0.0f;
+      HIPCHK(c, hipEventElapsedTime(&ms, prev, r.b));
+      c->prof_ms[r.cls] += ms;
+      c->prof_n[r.cls] += 1;
+      c->prof_bytes[r.cls] += r.bytes;
+    }
+    prev = r.b;
+  }
+  for (auto& r : c->prof_pending) {
+    if (r.a) c->prof_pool.push_back(r.a);
     c->prof_pool.push_back(r.b);
   }
   c->prof_pending.clear();

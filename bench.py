@@ -180,8 +180,8 @@ def main():
         print(json.dumps({"only": args.only, "steps_per_s": res["steps_per_s"],
                           "ms_per_step": 1000.0 / res["steps_per_s"], "kernels": kernels}))
         return
-    res64 = run_gpu(wl, 0, args.steps, args.warmup, prof_steps=5)
-    res32 = run_gpu(wl, 1, args.steps, args.warmup, prof_steps=5)
+    res64 = run_gpu(wl, 0, args.steps, args.warmup, prof_steps=20)
+    res32 = run_gpu(wl, 1, args.steps, args.warmup, prof_steps=20)
     e64 = res64["energy"]["etot"]
     e32 = res32["energy"]["etot"]
     rel_e = abs(e32 - e64) / abs(e64)
