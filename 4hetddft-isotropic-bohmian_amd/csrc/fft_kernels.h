@@ -473,6 +473,165 @@ __global__ void __launch_bounds__(kThreads) k_fft_zpipe(ZArgs<T> a, int njobs, G
   }
 }
 
+// Radix-R butterfly on v[0..R) (R = 8 or 16).
+template <class T, bool INV, int R>
+__device__ inline void bfly(cplx<T>* v) {
+  if constexpr (R == 8) bfly8<T, INV>(v);
+  if constexpr (R == 16) bfly16<T, INV>(v);
+}
+
+// Z pass with the transforms held in registers between two LDS exchanges,
+// for lengths N = R1 * R2 (R1, R2 in {8, 16}).  Per work item (TY consecutive
+// ky of one kx, one job) the four Stockham stages run as:
+//   A  forward radix-R1 stage straight from HBM (rows kz = j + R2 t), written
+//      to LDS in Stockham order (j R1 + t);
+//   B  forward radix-R2 stage from LDS (rows j + R1 t) with twiddles W_N^(t j):
+//      the thread then holds X[j + R1 t], multiplies it by the kernel values
+//      read from HBM,
+//   C  and runs the first inverse stage (radix R2, no twiddles) on the same
+//      registers, written to LDS (j R2 + t);
+//   D  inverse radix-R1 stage from LDS (rows j + R2 t) with twiddles
+//      conj W_N^(t j), stored straight to HBM (rows kz = j + R2 t).
+// Four LDS passes per item instead of twelve.  Threads take consecutive ky
+// first, so every HBM access moves rows of TY consecutive complex values and
+// LDS accesses of a wavefront walk columns (stride N+1: no bank conflicts).
+// Persistent over items like k_fft_zpipe; the next item's stage-A rows are
+// loaded while the current item runs B, C and D.
+template <class T, int N, int TY, int R1, int R2>
+__host__ __device__ constexpr size_t zreg_bytes() {
+  return (size_t)(TY * (N + 1) + N) * sizeof(cplx<T>);
+}
+
+template <class T, int N, int TY, int R1, int R2>
+__global__ void __launch_bounds__(kThreads) k_fft_zreg(ZArgs<T> a, int njobs, Grid g,
+                                                       const cplx<T>* __restrict__ twg) {
+  static_assert(R1 * R2 == N && (R1 == 8 || R1 == 16) && (R2 == 8 || R2 == 16), "N = R1 * R2");
+  extern __shared__ unsigned char smem_raw[];
+  constexpr int ls = N + 1;
+  constexpr int NA = TY * R2;  // stage A / D butterflies
+  constexpr int NB = TY * R1;  // stage B / C butterflies
+  constexpr int PA = (NA + kThreads - 1) / kThreads;
+  constexpr int PB = (NB + kThreads - 1) / kThreads;
+  cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* tw = tile + TY * ls;
+  const int tid = threadIdx.x;
+  const int ntile = (g.ny + TY - 1) / TY;
+  const int per_job = ntile * g.mx;
+  const int total = per_job * njobs;
+  const long long pstride = (long long)g.mx * g.ny;
+  for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
+
+  struct Item {
+    int jb, kx, ky0;
+  };
+  auto decode = [&](int w) {
+    Item it;
+    it.jb = w / per_job;
+    const int r = w - it.jb * per_job;
+    it.kx = r / ntile;
+    it.ky0 = (r - it.kx * ntile) * TY;
+    return it;
+  };
+  // stage-A rows of an item: butterfly b = l + TY j, elements kz = j + R2 t
+  cplx<T> va[PA][R1];
+  auto load_a = [&](const Item& it) {
+    const cplx<T>* src = reinterpret_cast<const cplx<T>*>(a.in[it.jb]) + (long long)it.kx * g.ny + it.ky0;
+#pragma unroll
+    for (int q = 0; q < PA; ++q) {
+      const int b = tid + q * kThreads;
+      const int l = b % TY, j = b / TY;
+      const bool ok = (NA % kThreads == 0 || b < NA) && it.ky0 + l < g.ny;
+#pragma unroll
+      for (int t = 0; t < R1; ++t) {
+        const long long o = ok ? (long long)(j + R2 * t) * pstride + l : 0;
+        va[q][t] = ok ? src[o] : mk(T(0), T(0));
+      }
+    }
+  };
+
+  int w = blockIdx.x;
+  if (w < total) load_a(decode(w));
+  __syncthreads();  // twiddles
+  for (; w < total; w += gridDim.x) {
+    const Item it = decode(w);
+    // A: forward radix R1 (first stage, no twiddles) -> LDS
+#pragma unroll
+    for (int q = 0; q < PA; ++q) {
+      const int b = tid + q * kThreads;
+      if (NA % kThreads == 0 || b < NA) {
+        const int l = b % TY, j = b / TY;
+        bfly<T, false, R1>(va[q]);
+#pragma unroll
+        for (int t = 0; t < R1; ++t) tile[l * ls + j * R1 + t] = va[q][t];
+      }
+    }
+    __syncthreads();
+    if (w + (int)gridDim.x < total) load_a(decode(w + gridDim.x));
+    // B reads (kernel values from HBM first, so their latency overlaps)
+    const T* kern = a.kern[it.jb] + (long long)it.kx * g.ny + it.ky0;
+    const T sg = a.ksign[it.jb];
+    cplx<T> u[PB][R2];
+    T kv[PB][R2];
+#pragma unroll
+    for (int q = 0; q < PB; ++q) {
+      const int b = tid + q * kThreads;
+      const int l = b % TY, j = b / TY;
+      const bool ok = (NB % kThreads == 0 || b < NB) && it.ky0 + l < g.ny;
+#pragma unroll
+      for (int t = 0; t < R2; ++t) {
+        const long long o = ok ? (long long)(j + R1 * t) * pstride + l : 0;
+        kv[q][t] = ok ? kern[o] : T(0);
+      }
+      if (NB % kThreads == 0 || b < NB) {
+#pragma unroll
+        for (int t = 0; t < R2; ++t) u[q][t] = tile[l * ls + j + R1 * t];
+      }
+    }
+    // every stage-B read of a column precedes the stage-C writes into it
+    __syncthreads();
+    // B: twiddles + forward radix R2; kernel product; C: inverse radix R2
+#pragma unroll
+    for (int q = 0; q < PB; ++q) {
+      const int b = tid + q * kThreads;
+      if (NB % kThreads == 0 || b < NB) {
+        const int l = b % TY, j = b / TY;
+#pragma unroll
+        for (int t = 1; t < R2; ++t) u[q][t] = cmul(u[q][t], tw[t * j]);
+        bfly<T, false, R2>(u[q]);
+#pragma unroll
+        for (int t = 0; t < R2; ++t) {
+          const cplx<T> pr = mk(u[q][t].x * kv[q][t], u[q][t].y * kv[q][t]);
+          u[q][t] = sg < T(0) ? mk(-pr.x, -pr.y) : pr;
+        }
+        bfly<T, true, R2>(u[q]);
+#pragma unroll
+        for (int t = 0; t < R2; ++t) tile[l * ls + j * R2 + t] = u[q][t];
+      }
+    }
+    __syncthreads();
+    // D: twiddles + inverse radix R1 -> HBM
+    cplx<T>* dst = reinterpret_cast<cplx<T>*>(a.out[it.jb]) + (long long)it.kx * g.ny + it.ky0;
+#pragma unroll
+    for (int q = 0; q < PA; ++q) {
+      const int b = tid + q * kThreads;
+      if (NA % kThreads == 0 || b < NA) {
+        const int l = b % TY, j = b / TY;
+        cplx<T> v[R1];
+#pragma unroll
+        for (int t = 0; t < R1; ++t) v[t] = tile[l * ls + j + R2 * t];
+#pragma unroll
+        for (int t = 1; t < R1; ++t) v[t] = cmulc(v[t], tw[t * j]);
+        bfly<T, true, R1>(v);
+        if (it.ky0 + l < g.ny) {
+#pragma unroll
+          for (int t = 0; t < R1; ++t) dst[(long long)(j + R2 * t) * pstride + l] = v[t];
+        }
+      }
+    }
+    __syncthreads();  // stage-D reads done before the next item's stage-A writes
+  }
+}
+
 }  // namespace hetd
 
 namespace hetd {

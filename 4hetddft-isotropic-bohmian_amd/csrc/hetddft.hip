@@ -388,6 +388,30 @@ void pick_z(FftKernels<T>& k, int n) {
     }
   }
   k.lds_z = (N > 0 ? 1 : 2) * tile_bytes<T>(k.tyz, n);
+  if constexpr (N == 64 || N == 128 || N == 256) {
+    // register-resident z pass (two LDS exchanges per transform pair)
+    const char* r = std::getenv("HETDDFT_ZREG");
+    if (!(r && std::atoi(r) == 0)) {
+      constexpr int R1 = N == 256 ? 16 : 8;
+      constexpr int R2 = N / R1;
+      const char* t = std::getenv("HETDDFT_ZRTY");
+      const int want = t ? std::atoi(t) : 32;
+      if (want == 8) {
+        k.zp = k_fft_zreg<T, N, 8, R1, R2>;
+        k.lds_zp = zreg_bytes<T, N, 8, R1, R2>();
+        k.typ = 8;
+      } else if (want == 32) {
+        k.zp = k_fft_zreg<T, N, 32, R1, R2>;
+        k.lds_zp = zreg_bytes<T, N, 32, R1, R2>();
+        k.typ = 32;
+      } else {
+        k.zp = k_fft_zreg<T, N, 16, R1, R2>;
+        k.lds_zp = zreg_bytes<T, N, 16, R1, R2>();
+        k.typ = 16;
+      }
+      return;
+    }
+  }
   if constexpr (N > 0) {
     const char* e = std::getenv("HETDDFT_ZPIPE");
     if (!(e && std::atoi(e) == 0)) {

@@ -82,7 +82,9 @@ _VP = ctypes.c_void_p
 
 
 def lib_path():
-    return _HERE / LIB_NAME
+    """libhetddft.so next to this file, or the file named by HETDDFT_LIBRARY."""
+    env = os.environ.get("HETDDFT_LIBRARY")
+    return Path(env) if env else _HERE / LIB_NAME
 
 
 def load_library():
