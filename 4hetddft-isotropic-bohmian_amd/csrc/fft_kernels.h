@@ -980,4 +980,339 @@ __global__ void __launch_bounds__(THR) k_fft_zwide(ZArgs<T> a, Grid g, const cpl
   }
 }
 
+
+// ---- register-resident kernels for 128 x 128 planes ------------------------
+// The same r2c / c2r of one z plane as k_fft_xyf / k_fft_xyi, with every
+// radix stage in registers.  Per plane (1024 threads, 16 wavefronts):
+//   x   each wavefront owns 8 rows; a row's 64-point half-length transform is
+//       8 x 8: radix 8 straight from HBM, one exchange inside the wavefront,
+//       radix 8.  The Hermitian split needs X[k] and X[64-k]: the stage-B
+//       lanes of a row hold k1 in the order 1 7 2 6 3 5 0 4, so partner
+//       columns sit in neighbouring lanes and one DPP lane swap brings them
+//       together (k1 = 0 and 4 pair with themselves).
+//   xy  one block-wide exchange through the plane (rows of 64 complex values,
+//       column index xor-swizzled per row, no bank conflicts for the row
+//       writes and the column reads);
+//   y   each wavefront owns 4 columns; the 128-point transform is 8 x 16:
+//       radix 8, one exchange inside the wavefront, radix 8, and a radix-2
+//       step across neighbouring lanes (DPP swap), stored straight to HBM.
+// XYI runs the same steps in reverse with conjugate twiddles.  Column 0
+// carries DC + i Nyquist as in k_fft_xyf; its unpack is a fourth, small
+// exchange inside wavefront 0.  tw[m] = exp(-2 pi i m/128), m < 128.
+constexpr int kPlane128Lds = (128 * 64 + 128);  // plane + staged twiddles, in complex values
+// FP32 planes take 66.5 KB of LDS: two workgroups per CU when each thread
+// keeps to 64 VGPRs (8 wavefronts per SIMD)
+template <class T>
+constexpr int kPlane128Waves = sizeof(T) == 4 ? 8 : 4;
+
+// Wavefront-local LDS ordering: lanes of one wavefront run in lockstep and
+// LDS executes one wavefront's instructions in order, so an exchange inside a
+// wavefront needs only that the compiler keep its stores before its loads.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// value held by lane ^ 1 (DPP quad_perm [1,0,3,2])
+__device__ inline float lane_swap1(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ inline double lane_swap1(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, 0xB1, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+template <class T>
+__device__ inline cplx<T> lane_swap1(cplx<T> v) {
+  return mk(lane_swap1(v.x), lane_swap1(v.y));
+}
+
+// X[64 - k] for k = k1 + 8 k2: own value for k1 = 0 and 4, the partner
+// lane's otherwise.  Selects values (a conditional over array elements would
+// select addresses and keep the arrays out of registers).
+template <class T>
+__device__ inline cplx<T> pick3(int k1, cplx<T> a0, cplx<T> a4, cplx<T> ap) {
+  const cplx<T> r = k1 == 4 ? a4 : ap;
+  return k1 == 0 ? a0 : r;
+}
+
+// plane element (y, x), x < 64
+__device__ inline int p128_at(int y, int x) {
+  return y * 64 + (x ^ ((y & 15) ^ ((y & 1) << 3) ^ ((y >> 6) * 12)));
+}
+// x exchange of a wavefront: row yl < 8, first-stage butterfly j, output k1
+__device__ inline int p128_xe(int yl, int j, int k1) { return yl * 64 + 8 * (j ^ yl) + (k1 ^ j); }
+// y exchange of a wavefront: column c < 4, first-stage butterfly j < 16, output m1
+__device__ inline int p128_ye(int c, int j, int m1) {
+  const int rot = ((j & 1) << 2) | ((j >> 1) & 3);
+  return c * 128 + 8 * (j ^ (2 * c)) + (m1 ^ rot ^ ((j >> 3) << 2));
+}
+// stage-B lane s of a row holds k1 = 1 7 2 6 3 5 0 4 (partner k1 of 8-k1 at s ^ 1)
+__device__ inline int p128_k1(int s) { return (0x40536271 >> (4 * s)) & 7; }
+
+// Producer values z[t] = (f[2n], f[2n+1]), n = s + 8t, of one row of 128.
+template <class T, class R>
+__device__ inline void p128_load_row(const XfArgs<R>& a, int f, long long row0, int s, cplx<T>* z) {
+  auto ld = [&](int k, int t) { return reinterpret_cast<const cplx<R>*>(a.in[f][k] + row0)[s + 8 * t]; };
+  switch (a.op[f]) {
+    case XF_CORREL: {
+      cplx<R> u[8], w[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) u[t] = ld(0, t), w[t] = ld(1, t);
+      const double c0 = a.c[f][0], c1 = a.c[f][1];
+      auto fn = [&](R den, R dcg) {
+        const double d = (double)den, g = (double)dcg;
+        return (T)(d * g * (c0 + c1 * g));
+      };
+#pragma unroll
+      for (int t = 0; t < 8; ++t) z[t] = mk(fn(u[t].x, w[t].x), fn(u[t].y, w[t].y));
+      break;
+    }
+    case XF_MUL: {
+      cplx<R> u[8], w[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) u[t] = ld(0, t), w[t] = ld(1, t);
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        z[t] = mk((T)((double)u[t].x * (double)w[t].x), (T)((double)u[t].y * (double)w[t].y));
+      break;
+    }
+    case XF_GRAD_DOT: {
+      constexpr int CH = 2;
+#pragma unroll
+      for (int t0 = 0; t0 < 8; t0 += CH) {
+        cplx<R> v[6][CH];
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+#pragma unroll
+          for (int q = 0; q < CH; ++q) v[k][q] = ld(k, t0 + q);
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+          const double sx = (double)v[0][q].x * (double)v[3][q].x + (double)v[1][q].x * (double)v[4][q].x;
+          const double sy = (double)v[0][q].y * (double)v[3][q].y + (double)v[1][q].y * (double)v[4][q].y;
+          z[t0 + q] = mk((T)(sx + (double)v[2][q].x * (double)v[5][q].x),
+                         (T)(sy + (double)v[2][q].y * (double)v[5][q].y));
+        }
+        // keep the next chunk's loads below this one (bounds live registers)
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      break;
+    }
+    default: {
+      cplx<R> u[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) u[t] = ld(0, t);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) z[t] = mk((T)u[t].x, (T)u[t].y);
+      break;
+    }
+  }
+}
+
+// r2c in x and y of one 128 x 128 z plane.  grid = (nz, nfields), 1024 threads.
+template <class T, class R>
+__global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyf128(XfArgs<R> a, Grid g, const cplx<T>* __restrict__ twx,
+                                                     const cplx<T>* __restrict__ twh,
+                                                     const cplx<T>* __restrict__ tw) {
+  extern __shared__ unsigned char smem_raw[];
+  constexpr int NX = 128, NY = 128, MH = 64, M = MH + 1;
+  cplx<T>* P = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* tws = P + NY * 64;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int iz = blockIdx.x, f = blockIdx.y;
+  // ---- x: thread (row y, lane s of the row)
+  const int yl = lane >> 3, s = lane & 7, y = 8 * w + yl;
+  const int k1 = p128_k1(s);
+  cplx<T> v[8];
+  p128_load_row<T, R>(a, f, ((long long)iz * NY + y) * NX, s, v);
+  if (tid < 128) tws[tid] = tw[tid];
+  bfly<T, false, 8>(v);  // v[o] = Y[j = s][o]
+  cplx<T>* xe = P + w * 512;
+#pragma unroll
+  for (int o = 0; o < 8; ++o) xe[p128_xe(yl, s, o)] = v[o];
+  wave_lds_sync();
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] = xe[p128_xe(yl, j, k1)];
+#pragma unroll
+  for (int j = 1; j < 8; ++j) v[j] = cmul(v[j], tw[2 * j * k1]);
+  bfly<T, false, 8>(v);  // v[k2] = X[k1 + 8 k2]
+  // Hermitian split Z[k] = E + W_128^k O with X[64 - k] from the partner lane
+  {
+    cplx<T> pv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) pv[q] = lane_swap1(v[q]);
+    cplx<T> zk[8];
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) {
+      const cplx<T> za = v[k2];
+      const cplx<T> zb = pick3(k1, v[(8 - k2) & 7], v[7 - k2], pv[7 - k2]);
+      const cplx<T> zc = conj(zb);
+      const cplx<T> E = scale(za + zc, T(0.5));
+      const cplx<T> D = za - zc;
+      const cplx<T> O = mk(T(0.5) * D.y, T(-0.5) * D.x);
+      zk[k2] = E + cmul(tw[k1 + 8 * k2], O);
+      // k = 0: DC + i Nyquist, both real
+      if (k2 == 0 && k1 == 0) zk[0] = mk(za.x + za.y, za.x - za.y);
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) P[p128_at(y, k1 + 8 * k2)] = zk[k2];
+  }
+  __syncthreads();
+  // ---- y: thread (column kx = 4w + c, lane q of the column)
+  const int c = lane >> 4, q = lane & 15, kx = 4 * w + c;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) v[t] = P[p128_at(q + 16 * t, kx)];
+  __syncthreads();  // every plane read precedes the column exchanges below
+  bfly<T, false, 8>(v);  // v[m1]
+#pragma unroll
+  for (int m = 1; m < 8; ++m) v[m] = cmul(v[m], tws[q * m]);
+  cplx<T>* ye = P + w * 512;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) ye[p128_ye(c, q, m)] = v[m];
+  wave_lds_sync();
+  const int m1 = q >> 1, i0 = q & 1;
+#pragma unroll
+  for (int i1 = 0; i1 < 8; ++i1) v[i1] = ye[p128_ye(c, i0 + 2 * i1, m1)];
+  bfly<T, false, 8>(v);  // B_i0[r]
+#pragma unroll
+  for (int r = 0; r < 8; ++r) v[r] = cmul(v[r], tws[8 * r * i0]);
+  {
+    cplx<T> o[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) o[r] = lane_swap1(v[r]);
+    // ky = m1 + 8 r + 64 i0
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = i0 ? o[r] - v[r] : v[r] + o[r];
+  }
+  cplx<T>* out = reinterpret_cast<cplx<T>*>(a.out[f]) + (long long)iz * M * NY;
+  if (w == 0) {
+    // column 0 holds FFT(X0) + i FFT(X_MH) of two real columns:
+    // FFT(X0)[k] = (W[k] + conj W[-k])/2, FFT(X_MH)[k] = -i (W[k] - conj W[-k])/2
+    if (c == 0) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) ye[m1 + 8 * r + 64 * i0] = v[r];
+    }
+    wave_lds_sync();
+    if (c == 0) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int ky = m1 + 8 * r + 64 * i0;
+        const cplx<T> wa = v[r], wb = ye[(NY - ky) & (NY - 1)];
+        const cplx<T> sa = wa + conj(wb), da = wa - conj(wb);
+        out[ky] = scale(sa, T(0.5));
+        out[MH * NY + ky] = mk(T(0.5) * da.y, T(-0.5) * da.x);
+      }
+    }
+  }
+  if (kx != 0) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) out[kx * NY + m1 + 8 * r + 64 * i0] = v[r];
+  }
+}
+
+// c2r in y and x of one 128 x 128 z plane, times renor, with the XI epilogues.
+template <class T, class R>
+__global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyi128(XiArgs<T, R> a, Grid g, const cplx<T>* __restrict__ twx,
+                                                     const cplx<T>* __restrict__ twh,
+                                                     const cplx<T>* __restrict__ tw) {
+  extern __shared__ unsigned char smem_raw[];
+  constexpr int NX = 128, NY = 128, MH = 64, M = MH + 1;
+  cplx<T>* P = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* tws = P + NY * 64;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int iz = blockIdx.x, f = blockIdx.y;
+  const cplx<T>* in = reinterpret_cast<const cplx<T>*>(a.in[f]) + (long long)iz * M * NY;
+  // ---- y: thread (column kx = 4w + c, lane q), rows ky = q + 16 t
+  const int c = lane >> 4, q = lane & 15, kx = 4 * w + c;
+  cplx<T> v[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) v[t] = in[kx * NY + q + 16 * t];
+  if (kx == 0) {
+    // columns 0 and MH (real inverse transforms) travel packed as col0 + i col_MH
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const cplx<T> u = in[MH * NY + q + 16 * t];
+      v[t] = mk(v[t].x - u.y, v[t].y + u.x);
+    }
+  }
+  if (tid < 128) tws[tid] = tw[tid];
+  bfly<T, true, 8>(v);
+#pragma unroll
+  for (int m = 1; m < 8; ++m) v[m] = cmulc(v[m], tw[q * m]);
+  cplx<T>* ex = P + w * 512;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) ex[p128_ye(c, q, m)] = v[m];
+  wave_lds_sync();
+  {
+    const int m1 = q >> 1, i0 = q & 1;
+#pragma unroll
+    for (int i1 = 0; i1 < 8; ++i1) v[i1] = ex[p128_ye(c, i0 + 2 * i1, m1)];
+    bfly<T, true, 8>(v);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = cmulc(v[r], tw[8 * r * i0]);
+    cplx<T> o[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) o[r] = lane_swap1(v[r]);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = i0 ? o[r] - v[r] : v[r] + o[r];
+    __syncthreads();  // every wavefront's column exchange is read before the plane is written
+#pragma unroll
+    for (int r = 0; r < 8; ++r) P[p128_at(m1 + 8 * r + 64 * i0, kx)] = v[r];
+  }
+  __syncthreads();
+  // ---- x: thread (row y, lane s), stage-A lanes hold k1 = p128_k1(s)
+  const int yl = lane >> 3, s = lane & 7, y = 8 * w + yl;
+  const int k1 = p128_k1(s);
+#pragma unroll
+  for (int k2 = 0; k2 < 8; ++k2) v[k2] = P[p128_at(y, k1 + 8 * k2)];
+  {
+    // Hermitian merge X[k] = s + i d, s = Z[k] + conj Z[64-k],
+    // d = (Z[k] - conj Z[64-k]) conj(W_128^k); k = 0 merges DC and Nyquist
+    cplx<T> pv[8];
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) pv[k2] = lane_swap1(v[k2]);
+    cplx<T> xk[8];
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) {
+      const cplx<T> xa = v[k2];
+      cplx<T> xc = conj(pick3(k1, v[(8 - k2) & 7], v[7 - k2], pv[7 - k2]));
+      cplx<T> xd = xa;
+      if (k2 == 0 && k1 == 0) {
+        xd = mk(xa.x, T(0));
+        xc = mk(xa.y, T(0));
+      }
+      const cplx<T> sm = xd + xc;
+      const cplx<T> d = cmulc(xd - xc, tws[k1 + 8 * k2]);
+      xk[k2] = mk(sm.x - d.y, sm.y + d.x);
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) v[k2] = xk[k2];
+  }
+  bfly<T, true, 8>(v);  // v[j] = G[k1][j]
+#pragma unroll
+  for (int j = 1; j < 8; ++j) v[j] = cmulc(v[j], tws[2 * j * k1]);
+  cplx<T>* xe = P + w * 512;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) xe[p128_xe(yl, j, k1)] = v[j];
+  wave_lds_sync();
+#pragma unroll
+  for (int o = 0; o < 8; ++o) v[o] = xe[p128_xe(yl, s, o)];
+  bfly<T, true, 8>(v);  // v[t] = z[s + 8t] = (x[2n], x[2n+1])
+  cplx<R>* dst = reinterpret_cast<cplx<R>*>(a.out[f] + ((long long)iz * NY + y) * NX);
+  const T renor = (T)a.renor;
+  const bool omd = a.epi[f] == XI_ONE_MINUS_DIV;
+  const double ec = a.ec[f];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const T vx = v[t].x * renor, vy = v[t].y * renor;
+    cplx<R> o;
+    o.x = omd ? (R)(1.0 - (double)(R)vx / ec) : (R)vx;
+    o.y = omd ? (R)(1.0 - (double)(R)vy / ec) : (R)vy;
+    dst[s + 8 * t] = o;
+  }
+}
+
 }  // namespace hetd

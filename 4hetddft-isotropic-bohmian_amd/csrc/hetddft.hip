@@ -435,6 +435,15 @@ void pick_z(FftKernels<T>& k, int n) {
 template <class T, int MH, int NY>
 void pick_xy(FftKernels<T>& k) {
   constexpr size_t b = plane_bytes<T>(MH, NY);
+  if constexpr (MH == 64 && NY == 128) {
+    const char* e = std::getenv("HETDDFT_XYREG");
+    if (!(e && std::atoi(e) == 0)) {
+      k.xyf = k_fft_xyf128<T, T>;
+      k.xyi = k_fft_xyi128<T, T>;
+      k.lds_xy = kPlane128Lds * sizeof(cplx<T>);
+      return;
+    }
+  }
   if constexpr (b <= 160 * 1024) {
     k.xyf = k_fft_xyf<T, T, MH, NY>;
     k.xyi = k_fft_xyi<T, T, MH, NY>;

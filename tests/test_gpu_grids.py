@@ -1,6 +1,7 @@
 """Non-cubic grids: every compile-time transform length outside the whole-plane
-kernels (x rows of 256 points, y and z lines of 256) and odd mixes of lengths,
-against the oracle in FP64 and against the FFT definition in both precisions.
+kernels (x rows of 256 points, y and z lines of 256), odd mixes of lengths and
+the 128 x 128 plane kernels, against the oracle in FP64 and against the FFT
+definition in both precisions.
 
 These grids reach the separate x / y / z passes that a 256^3 grid uses (a
 256^3 plane does not fit in LDS), at sizes the oracle finishes in seconds.
@@ -14,8 +15,10 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
 
-# (nx, ny, nz): x rows of 256 (half length 128), y lines of 256, z lines of 256
-GRIDS = [(256, 24, 32), (32, 256, 24), (24, 32, 256)]
+# (nx, ny, nz): x rows of 256 (half length 128), y lines of 256, z lines of 256,
+# and 128 x 128 planes (the register-resident whole-plane kernels of the
+# 128^3 workloads) with every producer and epilogue of the potential
+GRIDS = [(256, 24, 32), (32, 256, 24), (24, 32, 256), (128, 128, 24)]
 
 
 def _state(nx, ny, nz, h, n_par, seed):
