@@ -270,6 +270,19 @@ struct ProfScope {
   }
 };
 
+// Marks the current point of st in the end-event chain, so that a wait for
+// the other stream is not counted into the next profiled launch.
+int prof_mark(hetddft_ctx* c) {
+  if (!c->prof_on) return HETDDFT_OK;
+  hipEvent_t e = prof_event(c);
+  if (!e || hipEventRecord(e, c->st) != hipSuccess) {
+    c->err = "prof_mark: hipEventRecord failed";
+    return HETDDFT_E_HIP;
+  }
+  c->prof_pending.push_back({nullptr, e, -1, 0.0});
+  return HETDDFT_OK;
+}
+
 #define PLAUNCH(ps, kern, grid, block, lds, ...) \
   hipExtLaunchKernelGGL(kern, grid, block, lds, c->st, (ps).a, (ps).b, 0u, __VA_ARGS__)
 
@@ -819,6 +832,7 @@ int veval(hetddft_ctx* c) {
   LAUNCHCHK(c);
   // the particle density chain enters here: its set becomes current
   HIPCHK(c, hipStreamWaitEvent(c->st, c->ev_chain, 0));
+  RET(prof_mark(c));
   c->cs ^= 1;
   use_chain_set(c);
   c->chain_pending = false;
@@ -1444,8 +1458,16 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
     fail(c, HETDDFT_E_HIP, "hipStreamCreate failed");
     return bad(HETDDFT_E_HIP);
   }
+  // Cross-stream ordering between the two streams of this device: a
+  // device-scope release is enough (no system-scope cache writeback per
+  // record).  HETDDFT_SYSFENCE=1 restores the system-scope events.
+  unsigned evflags = hipEventDisableTiming | hipEventDisableSystemFence;
+  {
+    const char* sf = std::getenv("HETDDFT_SYSFENCE");
+    if (sf && std::atoi(sf) == 1) evflags = hipEventDisableTiming;
+  }
   for (hipEvent_t* e : {&c->ev_fork, &c->ev_chain, &c->ev_b3, &c->ev_join})
-    if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
+    if (hipEventCreateWithFlags(e, evflags) != hipSuccess) {
       fail(c, HETDDFT_E_HIP, "hipEventCreate failed");
       return bad(HETDDFT_E_HIP);
     }
