@@ -105,6 +105,13 @@ __device__ inline double block_sum(double v, double* sh) {
   return s;  // valid in thread 0
 }
 
+// p[0..n) = 0 (a kernel rather than a memset, so that it can sit in a
+// captured graph).
+__global__ void k_fill_zero(double* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
 // Sum of nblocks partial sums per slot; one block per slot, fixed order.
 __global__ void k_finish_sums(const double* __restrict__ part, int nblocks, int nslots,
                               double* __restrict__ out) {

@@ -59,7 +59,7 @@ module hetddft_mod
             hetddft_potential, hetddft_step_rk, hetddft_step_pc, hetddft_last_errors, &
             hetddft_iterate, hetddft_particle_means, hetddft_energy, hetddft_get_state, &
             hetddft_get_field, hetddft_convolve, hetddft_profile, hetddft_profile_read, &
-            hetddft_sync, hetddft_device_bytes
+            hetddft_sync, hetddft_device_bytes, hetddft_graph_info
   public :: hetddft_message, hetddft_check, gpu_steprk, gpu_steppc, gpu_veval, gpu_energy
 
   interface
@@ -197,6 +197,12 @@ module hetddft_mod
     integer(c_int) function hetddft_sync(ctx) bind(C, name="hetddft_sync")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function hetddft_graph_info(ctx, count, enabled) bind(C, name="hetddft_graph_info")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: count, enabled
     end function
 
     integer(c_long_long) function hetddft_device_bytes(ctx) bind(C, name="hetddft_device_bytes")

@@ -118,6 +118,7 @@ def load_library():
         "hetddft_get_field": ([C, ctypes.c_int, _VP], ctypes.c_int),
         "hetddft_convolve": ([C, ctypes.c_int, _VP, _VP, _VP], ctypes.c_int),
         "hetddft_sync": ([C], ctypes.c_int),
+        "hetddft_graph_info": ([C, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "hetddft_profile": ([C, ctypes.c_int], ctypes.c_int),
         "hetddft_profile_read": ([C, ctypes.c_int, _VP, _VP, _VP], ctypes.c_int),
         "hetddft_device_bytes": ([C], ctypes.c_longlong),
@@ -274,6 +275,12 @@ class Instance:
 
     def sync(self):
         self._check(self.lib.hetddft_sync(self._c), "sync")
+
+    def graph_info(self):
+        """(captured PC-step graphs, graphs enabled) of this instance."""
+        n, on = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self.lib.hetddft_graph_info(self._c, ctypes.byref(n), ctypes.byref(on)), "graph_info")
+        return n.value, bool(on.value)
 
     def profile(self, enable=True):
         """Reset and enable (or disable) per-kernel-class event timing."""

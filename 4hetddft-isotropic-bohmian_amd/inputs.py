@@ -68,6 +68,30 @@ def _rng(seed):
     return np.random.Generator(np.random.Philox(seed))
 
 
+def ensemble_c4(count=32, seed=4, n_pc=500):
+    """Ensemble of uneven instances: grids 48^3, 64^3, 80^3, 96^3 in turn at
+    dx = 0.45 A; N uniform in [50, 300], clipped so that the droplet radius
+    R = 2.22 N^(1/3) stays 5 A inside the box (a 48^3 box holds N <= 17 and
+    takes that N); 2000 test particles (sigma 0.5 A) with a speed uniform in
+    0-300 m/s along a random direction."""
+    g = _rng(seed)
+    out = []
+    for i in range(count):
+        n = (48, 64, 80, 96)[i % 4]
+        h = 0.45
+        half = n * h / 2.0
+        nmax = int(((half - 5.0) / 2.22) ** 3)
+        hi = min(300, nmax)
+        lo = min(50, hi)
+        n_he = int(g.integers(lo, hi + 1))
+        speed = float(g.uniform(0.0, 3.0))          # A/ps; 100 m/s = 1 A/ps
+        d = g.standard_normal(3)
+        d /= np.linalg.norm(d)
+        out.append(Workload(f"c4-{i}", n, h, n_he, 2000, 0.5, v0=speed, vdir=tuple(float(x) for x in d),
+                            seed=400 + i, n_pc=n_pc))
+    return out
+
+
 def _ylm_real(l, m, ct, phi):
     """Real spherical harmonics (unnormalised) for the ripple, l <= 4."""
     from scipy.special import lpmv

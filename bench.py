@@ -84,6 +84,92 @@ def run_gpu(wl, precision, steps, warmup, prof_steps=0):
     return out
 
 
+def run_ensemble(wls, precision, steps, warmup, threads=8):
+    """Every instance of the ensemble advances one PC step per round; the
+    rounds are enqueued without host synchronisation (each instance has its
+    own two streams), and the timed region ends when every instance is done.
+    Launching is host-bound for instances this small, so `threads` host
+    threads each drive every threads-th instance (instances are independent;
+    the library releases the GIL for each call)."""
+    import threading
+    import hetddft
+    import inputs
+    kcache = {}
+    insts = []
+    for wl in wls:
+        p = inputs.params_for(wl, precision=precision)
+        if wl.n not in kcache:
+            kcache[wl.n] = hetddft.make_kernels(p)
+        g = hetddft.Instance(p)
+        g.set_kernels(kcache[wl.n])
+        psi, r, v = inputs.initial_state(wl)
+        g.set_state(psi, r, v)
+        g.start()
+        insts.append(g)
+    dts = [wl.deltat for wl in wls]
+    for _ in range(3 + warmup):
+        for g, dt in zip(insts, dts):
+            g.iterate(dt, want_errors=False)
+    for g in insts:
+        g.sync()
+    groups = [list(range(t, len(insts), threads)) for t in range(min(threads, len(insts)))]
+    errors = []
+
+    def drive(idx):
+        try:
+            for _ in range(steps):
+                for i in idx:
+                    insts[i].iterate(dts[i], want_errors=False)
+            for i in idx:
+                insts[i].sync()
+        except Exception as e:  # reported after the join
+            errors.append(e)
+
+    workers = [threading.Thread(target=drive, args=(idx,)) for idx in groups]
+    t0 = time.perf_counter()
+    for w in workers:
+        w.start()
+    for w in workers:
+        w.join()
+    t1 = time.perf_counter()
+    if errors:
+        raise errors[0]
+    energies = [g.energy()["etot"] for g in insts]
+    dev = sum(g.device_bytes for g in insts)
+    for g in insts:
+        g.close()
+    return dict(seconds=t1 - t0, instance_steps_per_s=len(insts) * steps / (t1 - t0), energies=energies,
+                device_bytes=dev)
+
+
+def ensemble_main(args):
+    import inputs
+    wls = inputs.ensemble_c4(n_pc=args.steps)
+    r64 = run_ensemble(wls, 0, args.steps, args.warmup, args.threads)
+    r32 = run_ensemble(wls, 1, args.steps, args.warmup, args.threads)
+    rel = max(abs(a - b) / abs(a) for a, b in zip(r64["energies"], r32["energies"]))
+    sizes = {}
+    for wl in wls:
+        sizes[f"{wl.n}^3"] = sizes.get(f"{wl.n}^3", 0) + 1
+    mb = sum(model_bytes(wl.n, wl.n_par, 1) for wl in wls) / len(wls)
+    print(json.dumps({
+        "metric": "instance-steps/s, ensemble of 32 uneven instances (mixed vs FP64)",
+        "value": r32["instance_steps_per_s"], "unit": "instance-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+        "data": "synthetic (seeded Woods-Saxon droplets, Gaussian particle clouds)",
+        "config": {"workload": f"c4: 32 instances {sizes}, dx=0.45 A, N in [17, 300], N_par=2000, "
+                               f"dt=0.5 fs, OTC, ironx=3", "precision": "mixed"},
+        "fp64_instance_steps_per_s": r64["instance_steps_per_s"],
+        "mixed_instance_steps_per_s": r32["instance_steps_per_s"],
+        "mixed_vs_fp64_speedup": r32["instance_steps_per_s"] / r64["instance_steps_per_s"],
+        "mixed_max_rel_energy_err": rel,
+        "model_gb_per_instance_step_mixed": mb / 1e9,
+        "model_hbm_gbs": mb * r32["instance_steps_per_s"] / 1e9,
+        "device_bytes_fp64": r64["device_bytes"], "device_bytes_mixed": r32["device_bytes"],
+        "host_threads": args.threads,
+    }))
+
+
 TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic.json"
 
 
@@ -160,6 +246,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--threads", type=int, default=8, help="host threads driving the ensemble (c4)")
     ap.add_argument("--only", choices=["fp64", "mixed", "copy"], default=None,
                     help="time one precision only, or run the copy calibration (profiling runs)")
     args = ap.parse_args()
@@ -168,6 +255,9 @@ def main():
               "(an instance is one GPU's share of the work)", file=sys.stderr)
         sys.exit(2)
     import inputs
+    if args.workload == "c4":
+        ensemble_main(args)
+        return
     wl = inputs.WORKLOADS[args.workload]
     if args.only == "copy":
         print(json.dumps(copy_calibration()))

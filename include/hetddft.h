@@ -178,6 +178,16 @@ int hetddft_profile_read(hetddft_ctx* ctx, int cls, double* ms_total, long long*
 /* Waits for all work queued on the context's stream. */
 int hetddft_sync(hetddft_ctx* ctx);
 
+/* With HETDDFT_GRAPH=1 (both streams) or 2 (particle chain on the main
+ * stream) in the environment at create time, hetddft_iterate captures its PC
+ * steps as HIP graphs, one per state of the rotating history slots (two in
+ * steady state), and replays them; by default the steps are launched
+ * directly.  *count receives the number of captured graphs of the context;
+ * *enabled is 0 when graphs are off (by default, or after a failed capture).
+ * No counterpart in the reference (a diagnostic of this library's launch
+ * path). */
+int hetddft_graph_info(hetddft_ctx* ctx, int* count, int* enabled);
+
 /* Device bytes allocated by the context. */
 long long hetddft_device_bytes(const hetddft_ctx* ctx);
 

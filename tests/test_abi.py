@@ -58,3 +58,20 @@ def test_inputs_are_normalised_and_seeded():
     psi2, r2, v2 = inputs.initial_state(wl)
     assert np.array_equal(psi, psi2) and np.array_equal(r, r2)
     assert np.allclose(v[2], wl.v0 * inputs.KTOPS) and np.all(v[:2] == 0)
+
+
+def test_ensemble_workloads():
+    """Ensemble of uneven instances: sizes in turn, N in range and clipped so
+    the droplet stays 5 A inside its box, speeds 0-300 m/s, deterministic."""
+    import inputs
+    a = inputs.ensemble_c4()
+    b = inputs.ensemble_c4()
+    assert len(a) == 32
+    assert [w.n for w in a[:4]] == [48, 64, 80, 96]
+    for w in a:
+        assert w.n_par == 2000 and w.h == 0.45
+        assert 2.22 * w.n_he ** (1.0 / 3.0) + 5.0 < w.n * w.h / 2.0
+        assert w.n_he <= 300 and (w.n_he >= 50 or w.n == 48)
+        assert 0.0 <= w.v0 <= 3.0
+        assert abs(sum(x * x for x in w.vdir) - 1.0) < 1e-12
+    assert [(w.n_he, w.v0, w.vdir) for w in a] == [(w.n_he, w.v0, w.vdir) for w in b]
