@@ -29,11 +29,11 @@ __device__ inline double vmadd(double acc, double w, double v) { return acc + w 
 __device__ inline c64 vmadd(c64 acc, double w, c64 v) { return mk(acc.x + w * v.x, acc.y + w * v.y); }
 
 template <class V, class R>
-__device__ inline V vload(const R* f, long long i) {
+__device__ inline V vload(const R* f, int i) {
   return (V)f[i];
 }
 template <>
-__device__ inline c64 vload<c64, c64>(const c64* f, long long i) {
+__device__ inline c64 vload<c64, c64>(const c64* f, int i) {
   return f[i];
 }
 
@@ -72,8 +72,9 @@ __device__ inline Col col_of(const Grid& g, int xcd) {
   return c;
 }
 
-__device__ inline long long gidx(const Grid& g, int x, int y, int z) {
-  return x + (long long)g.nx * (y + (long long)g.ny * z);
+// Element offset of (x, y, z); grids are at most 256^3, so 32 bits hold it.
+__device__ inline int gidx(const Grid& g, int x, int y, int z) {
+  return x + g.nx * (y + g.ny * z);
 }
 
 // Queue of the 13 z-neighbours of column (x, y): q[k] = f(z - 6 + k).
@@ -223,7 +224,7 @@ __global__ void __launch_bounds__(SH::N) k_pc_predict25(Ham25 a, const c64* __re
   auto prefetch = [&](int z) {
     qn = q.next(psi, g, c, z);
     plane_load<SH>(pr, psi, g, c, z);
-    const long long j = gidx(g, xc, yc, z);
+    const int j = gidx(g, xc, yc, z);
     pvn = (double)pot4[j];
     o3n = old3[j];
     a1n = h1[j];
@@ -239,7 +240,7 @@ __global__ void __launch_bounds__(SH::N) k_pc_predict25(Ham25 a, const c64* __re
     __syncthreads();
     if (z + 1 < z1) prefetch(z + 1);
     if (c.in) {
-      const long long i = gidx(g, c.x, c.y, z);
+      const int i = gidx(g, c.x, c.y, z);
       const c64 s4 = ham25<SH>(a, tile, q, c, pv, z);
       const c64 s1 = mk(o3.x + c4o3dt * ((2.0 * s4.x - a1.x) + 2.0 * a2.x),
                         o3.y + c4o3dt * ((2.0 * s4.y - a1.y) + 2.0 * a2.y));
@@ -281,7 +282,7 @@ __global__ void __launch_bounds__(SH::N) k_pc_correct25(Ham25 a, const c64* __re
   auto prefetch = [&](int z) {
     qn = q.next(psi, g, c, z);
     plane_load<SH>(pr, psi, g, c, z);
-    const long long j = gidx(g, xc, yc, z);
+    const int j = gidx(g, xc, yc, z);
     pvn = (double)pot4[j];
     v3n = o3p[j];
     v2n = o2p[j];
@@ -299,7 +300,7 @@ __global__ void __launch_bounds__(SH::N) k_pc_correct25(Ham25 a, const c64* __re
     __syncthreads();
     if (z + 1 < z1) prefetch(z + 1);
     if (c.in) {
-      const long long i = gidx(g, c.x, c.y, z);
+      const int i = gidx(g, c.x, c.y, z);
       const c64 s4 = ham25<SH>(a, tile, q, c, pv, z);
       const c64 s5 = mk(0.125 * ((9.0 * v3.x - v2.x) + dt3 * ((s4.x + 2.0 * a1.x) - a2.x)),
                         0.125 * ((9.0 * v3.y - v2.y) + dt3 * ((s4.y + 2.0 * a1.y) - a2.y)));
@@ -338,7 +339,7 @@ __global__ void __launch_bounds__(SH::N) k_rk_stage25(Ham25 a, const c64* __rest
   auto prefetch = [&](int z) {
     qn = q.next(psi, g, c, z);
     plane_load<SH>(pr, psi, g, c, z);
-    const long long j = gidx(g, xc, yc, z);
+    const int j = gidx(g, xc, yc, z);
     pvn = (double)pot4[j];
     qin = qacc[j];
   };
@@ -351,7 +352,7 @@ __global__ void __launch_bounds__(SH::N) k_rk_stage25(Ham25 a, const c64* __rest
     __syncthreads();
     if (z + 1 < z1) prefetch(z + 1);
     if (c.in) {
-      const long long i = gidx(g, c.x, c.y, z);
+      const int i = gidx(g, c.x, c.y, z);
       const c64 s4 = ham25<SH>(a, tile, q, c, pv, z);
       const c64 s1 = mk(ca * (s4.x - cb * qi.x), ca * (s4.y - cb * qi.y));
       qacc[i] = mk((qi.x + 3.0 * s1.x) - cc * s4.x, (qi.y + 3.0 * s1.y) - cc * s4.y);
@@ -391,7 +392,7 @@ __global__ void __launch_bounds__(SH::N) k_grad3_25(Grid g, int zc, int xcd, con
     __syncthreads();
     if (z + 1 < z1) prefetch(z + 1);
     if (c.in) {
-      const long long i = gidx(g, c.x, c.y, z);
+      const int i = gidx(g, c.x, c.y, z);
       fx[i] = (R)tile_dx<SH>(tile, c, wx);
       fy[i] = (R)tile_dy<SH>(tile, c, wy);
       fz[i] = (R)q.dz(wz);
@@ -423,7 +424,7 @@ struct Pot25 {
 template <class R>
 __device__ inline double prod_at(const R* __restrict__ a, const R* __restrict__ b, const Grid& g, int x, int y,
                                  int z) {
-  const long long j = gidx(g, x, y, z);
+  const int j = gidx(g, x, y, z);
   return (double)a[j] * (double)b[j];
 }
 
@@ -466,7 +467,7 @@ __global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25 a, const R* __restr
   auto prefetch = [&](int z) {
     if (a.otc) {
       const int zz = z + 6;
-      const long long jz = gidx(g, xc, yc, min(zz, g.nz - 1));
+      const int jz = gidx(g, xc, yc, min(zz, g.nz - 1));
       fzn = falfs[jz];
       izn = intz[jz];
       okz = c.in && zz < g.nz;
@@ -476,7 +477,7 @@ __global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25 a, const R* __restr
         const int ly = e / SH::LX, lx = e - ly * SH::LX;
         const int x = x0 + lx, y = c.by * SH::TY + ly;
         okx[k] = x >= 0 && x < g.nx && y < g.ny;
-        const long long j = gidx(g, min(max(x, 0), g.nx - 1), min(y, g.ny - 1), z);
+        const int j = gidx(g, min(max(x, 0), g.nx - 1), min(y, g.ny - 1), z);
         fxn[k] = falfs[j];
         ixn[k] = intx[j];
       }
@@ -486,13 +487,13 @@ __global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25 a, const R* __restr
         const int ly = e / SH::TX, lx = e - ly * SH::TX;
         const int x = c.bx * SH::TX + lx, y = y0 + ly;
         oky[k] = x < g.nx && y >= 0 && y < g.ny;
-        const long long j = gidx(g, min(x, g.nx - 1), min(max(y, 0), g.ny - 1), z);
+        const int j = gidx(g, min(x, g.nx - 1), min(max(y, 0), g.ny - 1), z);
         fyn[k] = falfs[j];
         iyn[k] = inty[j];
       }
       uan = uaconv[gidx(g, xc, yc, z)];
     }
-    const long long i = gidx(g, xc, yc, z);
+    const int i = gidx(g, xc, yc, z);
     dnn = den[i];
     dcgn = dencg[i];
     dljn = delj4[i];
@@ -519,7 +520,7 @@ __global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25 a, const R* __restr
     __syncthreads();
     if (z + 1 < z1) prefetch(z + 1);
     if (c.in) {
-      const long long i = gidx(g, c.x, c.y, z);
+      const int i = gidx(g, c.x, c.y, z);
       double p4 = (dlj + dcg * dcg * (a.a0 + a.a1 * dcg)) + s1;
       if (a.otc) {
         double s4 = 0.0, s5 = 0.0, s6 = 0.0;
