@@ -678,4 +678,110 @@ __global__ void k_sq_sum(int n, const double* __restrict__ v, double* __restrict
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// ---- per-step diagnostics ---------------------------------------------------
+// The driver's per-step output (DFT4He3d.f90:879-930): the He count
+// sum(den) dxyz, the centre-of-mass moments of r_cm (r_cm.f90:23-58), the
+// impurity mean-square radii sqrt(sum(denx x^2) dxyz) (:918-928); and the
+// chemical potential <psi|H|psi>/<psi|psi> with H = -h2o2m4 lap + pot4 and the
+// current j = 2 h2o2m4 Im(conj(psi) grad psi), with the 13-point stencils of
+// the step.  x(ix) = -xmax + hx (ix-1) (DFT4He3d.f90:302-310).
+struct DiagArgs {
+  Grid g;
+  Stencil13 d1x, d1y, d1z, d2x, d2y, d2z;
+  double xmin, ymin, zmin, hx, hy, hz;
+  double h2o2m4;
+};
+
+constexpr int kDiagSlots = 11;
+
+// Slots: 0 |psi|^2, 1-3 |psi|^2 (x, y, z), 4-6 denx (x^2, y^2, z^2),
+// 7 Re conj(psi) (-h2o2m4 lap psi + pot4 psi), 8-10 Im conj(psi) d_c psi.
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_diag(DiagArgs a, const c64* __restrict__ psi,
+                                                  const R* __restrict__ pot4, const R* __restrict__ denx,
+                                                  double* __restrict__ part) {
+  __shared__ double sh[16];
+  const Grid& g = a.g;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[kDiagSlots] = {};
+  if (i < g.n) {
+    const Idx3 p = idx3(i, g);
+    const double x = a.xmin + a.hx * p.ix, y = a.ymin + a.hy * p.iy, z = a.zmin + a.hz * p.iz;
+    const c64 ps = psi[i];
+    const double rho = ps.x * ps.x + ps.y * ps.y;
+    v[0] = rho;
+    v[1] = rho * x;
+    v[2] = rho * y;
+    v[3] = rho * z;
+    const double dx = (double)denx[i];
+    v[4] = dx * (x * x);
+    v[5] = dx * (y * y);
+    v[6] = dx * (z * z);
+    const long long sxy = (long long)g.nx * g.ny;
+    const c64 lap = (d13c(psi, i, p.ix, g.nx, 1, a.d2x) + d13c(psi, i, p.iy, g.ny, g.nx, a.d2y)) +
+                    d13c(psi, i, p.iz, g.nz, sxy, a.d2z);
+    const double pv = (double)pot4[i];
+    const c64 hp = mk(pv * ps.x - a.h2o2m4 * lap.x, pv * ps.y - a.h2o2m4 * lap.y);
+    v[7] = ps.x * hp.x + ps.y * hp.y;
+    const c64 gx = d13c(psi, i, p.ix, g.nx, 1, a.d1x);
+    const c64 gy = d13c(psi, i, p.iy, g.ny, g.nx, a.d1y);
+    const c64 gz = d13c(psi, i, p.iz, g.nz, sxy, a.d1z);
+    v[8] = ps.x * gx.y - ps.y * gx.x;
+    v[9] = ps.x * gy.y - ps.y * gy.x;
+    v[10] = ps.x * gz.y - ps.y * gz.x;
+  }
+  for (int s = 0; s < kDiagSlots; ++s) {
+    const double t = block_sum(v[s], sh);
+    if (threadIdx.x == 0) part[(long long)s * gridDim.x + blockIdx.x] = t;
+    __syncthreads();
+  }
+}
+
+// j = 2 h2o2m4 Im(conj(psi) grad psi) at every point.
+__global__ void __launch_bounds__(kBlock) k_current(DiagArgs a, const c64* __restrict__ psi,
+                                                     double* __restrict__ jx, double* __restrict__ jy,
+                                                     double* __restrict__ jz) {
+  const Grid& g = a.g;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n) return;
+  const Idx3 p = idx3(i, g);
+  const c64 ps = psi[i];
+  const long long sxy = (long long)g.nx * g.ny;
+  const c64 gx = d13c(psi, i, p.ix, g.nx, 1, a.d1x);
+  const c64 gy = d13c(psi, i, p.iy, g.ny, g.nx, a.d1y);
+  const c64 gz = d13c(psi, i, p.iz, g.nz, sxy, a.d1z);
+  const double c = 2.0 * a.h2o2m4;
+  jx[i] = c * (ps.x * gx.y - ps.y * gx.x);
+  jy[i] = c * (ps.x * gy.y - ps.y * gy.x);
+  jz[i] = c * (ps.x * gz.y - ps.y * gz.x);
+}
+
+// min and max of column c of an (N_par, 3) particle array: one block per
+// (array, component); out[2k] = min, out[2k+1] = max, k = 3*array + c.
+__global__ void __launch_bounds__(kBlock) k_part_minmax(int np, const double* __restrict__ v,
+                                                         const double* __restrict__ acc, double* __restrict__ out) {
+  __shared__ double smin[kBlock], smax[kBlock];
+  const int k = blockIdx.x;  // 0..5
+  const double* col = (k < 3 ? v : acc) + (long long)(k % 3) * np;
+  double lo = INFINITY, hi = -INFINITY;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    lo = fmin(lo, col[i]);
+    hi = fmax(hi, col[i]);
+  }
+  smin[threadIdx.x] = lo;
+  smax[threadIdx.x] = hi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * k] = smin[0];
+    out[2 * k + 1] = smax[0];
+  }
+}
+
 }  // namespace hetd

@@ -212,6 +212,7 @@ struct hetddft_ctx {
   double* part = nullptr;
   long long part_len = 0;
   double* res = nullptr;       // device results
+  double* jcur = nullptr;      // current density (3 fields, FP64), allocated by the first hetddft_current
   double* hres = nullptr;      // pinned host copy
   double last_err[3] = {0, 0, 0};
   // PC steps replayed from hipGraphs (hetddft_iterate): one graph per host
@@ -1483,6 +1484,25 @@ int iterate_pc(hetddft_ctx* c, double dt) {
   return HETDDFT_OK;
 }
 
+DiagArgs diag_args(const hetddft_ctx* c) {
+  DiagArgs a{};
+  a.g = c->g;
+  a.d1x = c->d1x;
+  a.d1y = c->d1y;
+  a.d1z = c->d1z;
+  a.d2x = c->d2x;
+  a.d2y = c->d2y;
+  a.d2z = c->d2z;
+  a.xmin = -c->p.xmax;
+  a.ymin = -c->p.ymax;
+  a.zmin = -c->p.zmax;
+  a.hx = c->hx;
+  a.hy = c->hy;
+  a.hz = c->hz;
+  a.h2o2m4 = c->p.h2o2m4;
+  return a;
+}
+
 extern "C" {
 
 int hetddft_default_params(hetddft_params* p) {
@@ -1675,7 +1695,7 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
   for (int k = 0; k < 3; ++k) want.push_back({(void**)&c->rold[k], np3 * 8});
   for (int k = 0; k < 3; ++k) want.push_back({(void**)&c->vold[k], np3 * 8});
   for (int k = 0; k < 2; ++k) want.push_back({(void**)&c->aold[k], np3 * 8});
-  c->part_len = 8LL * blocks_for(g.n) + 2LL * blocks_for((long long)np3) + 2048;
+  c->part_len = (long long)kDiagSlots * blocks_for(g.n) + 2LL * blocks_for((long long)np3) + 2048;
   want.push_back({(void**)&c->part, (size_t)c->part_len * 8});
   want.push_back({(void**)&c->res, 64 * 8});
   want.push_back({(void**)&c->timec, (size_t)g.nz * 16});
@@ -1939,6 +1959,62 @@ int hetddft_particle_means(hetddft_ctx* c, double out[9]) {
     if (k >= 6) v = v * pstok * pstok;
     out[k] = v;
   }
+  return HETDDFT_OK;
+}
+
+int hetddft_diagnostics(hetddft_ctx* c, double out[HETDDFT_DIAG_N]) {
+  if (!c || !out) return HETDDFT_E_ARG;
+  if (!c->started) return fail(c, HETDDFT_E_STATE, "hetddft_start not called");
+  RET(settle(c));
+  const Grid& g = c->g;
+  const DiagArgs da = diag_args(c);
+  const unsigned nb = blocks_for(g.n);
+  if (c->mixed)
+    hipLaunchKernelGGL((k_diag<float>), dim3(nb), dim3(kBlock), 0, c->st, da, (const c64*)c->psi,
+                       (const float*)c->fld[hetddft_ctx::F_POT4], (const float*)c->denx, c->part);
+  else
+    hipLaunchKernelGGL((k_diag<double>), dim3(nb), dim3(kBlock), 0, c->st, da, (const c64*)c->psi,
+                       (const double*)c->fld[hetddft_ctx::F_POT4], (const double*)c->denx, c->part);
+  LAUNCHCHK(c);
+  hipLaunchKernelGGL(k_finish_sums, dim3(kDiagSlots), dim3(kBlock), 0, c->st, (const double*)c->part, (int)nb,
+                     kDiagSlots, c->res + 32);
+  LAUNCHCHK(c);
+  const int np = c->p.n_par;
+  if (np > 0) {
+    hipLaunchKernelGGL(k_part_minmax, dim3(6), dim3(kBlock), 0, c->st, np, (const double*)c->vimp,
+                       (const double*)c->aimp, c->res + 32 + kDiagSlots);
+    LAUNCHCHK(c);
+  }
+  HIPCHK(c, hipMemcpyAsync(c->hres + 32, c->res + 32, (kDiagSlots + 12) * sizeof(double), hipMemcpyDeviceToHost,
+                           c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  const double* r = c->hres + 32;
+  const double dv = c->dxyz;
+  out[HETDDFT_DIAG_NORM] = r[0] * dv;
+  for (int k = 0; k < 3; ++k) {
+    out[HETDDFT_DIAG_CM + k] = r[1 + k] * dv;
+    out[HETDDFT_DIAG_JTOT + k] = 2.0 * c->p.h2o2m4 * r[8 + k] * dv;
+    out[HETDDFT_DIAG_MSX + k] = std::sqrt(r[4 + k] * dv);
+  }
+  out[HETDDFT_DIAG_MU] = r[7] / r[0];
+  for (int k = 0; k < 12; ++k) out[HETDDFT_DIAG_PMINMAX + k] = np > 0 ? r[kDiagSlots + k] : 0.0;
+  return HETDDFT_OK;
+}
+
+int hetddft_current(hetddft_ctx* c, double* jx, double* jy, double* jz) {
+  if (!c) return HETDDFT_E_ARG;
+  if (!c->started) return fail(c, HETDDFT_E_STATE, "hetddft_start not called");
+  RET(settle(c));
+  const Grid& g = c->g;
+  if (!c->jcur) RET(dalloc(c, (void**)&c->jcur, (size_t)3 * g.n * sizeof(double)));
+  double* d[3] = {c->jcur, c->jcur + g.n, c->jcur + 2 * g.n};
+  hipLaunchKernelGGL(k_current, dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, diag_args(c), (const c64*)c->psi, d[0],
+                     d[1], d[2]);
+  LAUNCHCHK(c);
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  double* h[3] = {jx, jy, jz};
+  for (int k = 0; k < 3; ++k)
+    if (h[k]) HIPCHK(c, hipMemcpy(h[k], d[k], (size_t)g.n * sizeof(double), hipMemcpyDeviceToHost));
   return HETDDFT_OK;
 }
 

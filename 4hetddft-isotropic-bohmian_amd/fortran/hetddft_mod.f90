@@ -59,7 +59,8 @@ module hetddft_mod
             hetddft_potential, hetddft_step_rk, hetddft_step_pc, hetddft_last_errors, &
             hetddft_iterate, hetddft_particle_means, hetddft_energy, hetddft_get_state, &
             hetddft_get_field, hetddft_convolve, hetddft_profile, hetddft_profile_read, &
-            hetddft_sync, hetddft_device_bytes, hetddft_graph_info
+            hetddft_sync, hetddft_device_bytes, hetddft_graph_info, hetddft_diagnostics, &
+            hetddft_current
   public :: hetddft_message, hetddft_check, gpu_steprk, gpu_steppc, gpu_veval, gpu_energy
 
   interface
@@ -197,6 +198,18 @@ module hetddft_mod
     integer(c_int) function hetddft_sync(ctx) bind(C, name="hetddft_sync")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function hetddft_diagnostics(ctx, out) bind(C, name="hetddft_diagnostics")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: out(23)
+    end function
+
+    integer(c_int) function hetddft_current(ctx, jx, jy, jz) bind(C, name="hetddft_current")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: jx(*), jy(*), jz(*)
     end function
 
     integer(c_int) function hetddft_graph_info(ctx, count, enabled) bind(C, name="hetddft_graph_info")

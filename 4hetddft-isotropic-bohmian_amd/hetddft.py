@@ -119,6 +119,8 @@ def load_library():
         "hetddft_convolve": ([C, ctypes.c_int, _VP, _VP, _VP], ctypes.c_int),
         "hetddft_sync": ([C], ctypes.c_int),
         "hetddft_graph_info": ([C, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "hetddft_diagnostics": ([C, _VP], ctypes.c_int),
+        "hetddft_current": ([C, _VP, _VP, _VP], ctypes.c_int),
         "hetddft_profile": ([C, ctypes.c_int], ctypes.c_int),
         "hetddft_profile_read": ([C, ctypes.c_int, _VP, _VP, _VP], ctypes.c_int),
         "hetddft_device_bytes": ([C], ctypes.c_longlong),
@@ -275,6 +277,23 @@ class Instance:
 
     def sync(self):
         self._check(self.lib.hetddft_sync(self._c), "sync")
+
+    def diagnostics(self):
+        """Device-side per-step diagnostics (hetddft_diagnostics): He count, the
+        centre-of-mass moments sum(den x_c) dxyz, chemical potential, integral of
+        the current, impurity mean-square radii, min/max of vimp and aimp."""
+        out = np.zeros(23)
+        self._check(self.lib.hetddft_diagnostics(self._c, _ptr(out)), "diagnostics")
+        return {"norm": out[0], "cm_moments": out[1:4].copy(), "mu": out[4], "jtot": out[5:8].copy(),
+                "msx": out[8:11].copy(), "vmin": out[11:17:2].copy(), "vmax": out[12:17:2].copy(),
+                "amin": out[17:23:2].copy(), "amax": out[18:23:2].copy()}
+
+    def current(self):
+        """Current density j = 2 h2o2m4 Im(conj(psi) grad psi), three (nz, ny, nx) arrays."""
+        shape = (self.p.nz, self.p.ny, self.p.nx)
+        j = [np.zeros(shape) for _ in range(3)]
+        self._check(self.lib.hetddft_current(self._c, _ptr(j[0]), _ptr(j[1]), _ptr(j[2])), "current")
+        return tuple(j)
 
     def graph_info(self):
         """(captured PC-step graphs, graphs enabled) of this instance."""

@@ -175,6 +175,33 @@ int hetddft_profile(hetddft_ctx* ctx, int enable);
 int hetddft_profile_read(hetddft_ctx* ctx, int cls, double* ms_total, long long* launches,
                          double* bytes_total);
 
+/* Per-step diagnostics on the device, in one pass over the grid (the
+ * driver's per-step output, DFT4He3d.f90:879-930, and the chemical potential
+ * and current of the dynamics):
+ *   out[HETDDFT_DIAG_NORM]         sum(den) dxyz                (DFT4He3d.f90:884)
+ *   out[HETDDFT_DIAG_CM + c]       sum(den x_c) dxyz; the reference's centre of
+ *                                  mass is this / n4        (r_cm.f90:23-58)
+ *   out[HETDDFT_DIAG_MU]           <psi|H|psi>/<psi|psi>, H = -h2o2m4 lap + pot4
+ *   out[HETDDFT_DIAG_JTOT + c]     integral of j_c, j = 2 h2o2m4 Im(conj(psi) grad psi)
+ *   out[HETDDFT_DIAG_MSX + c]      sqrt(sum(denx x_c^2) dxyz)  (DFT4He3d.f90:918-928)
+ *   out[HETDDFT_DIAG_PMINMAX + 2k] min, + 2k + 1 max of vimp(:,k) (k = 0..2) and
+ *                                  aimp(:,k-3) (k = 3..5)   (DFT4He3d.f90:900-903)
+ * Derivatives are the 13-point stencils of the step (zero outside the box). */
+#define HETDDFT_DIAG_NORM 0
+#define HETDDFT_DIAG_CM 1
+#define HETDDFT_DIAG_MU 4
+#define HETDDFT_DIAG_JTOT 5
+#define HETDDFT_DIAG_MSX 8
+#define HETDDFT_DIAG_PMINMAX 11
+#define HETDDFT_DIAG_N 23
+int hetddft_diagnostics(hetddft_ctx* ctx, double out[HETDDFT_DIAG_N]);
+
+/* Current density j = 2 h2o2m4 Im(conj(psi) grad psi) (units of psi^2 A/K^-1)
+ * into host arrays of nx*ny*nz doubles (x fastest); any pointer may be NULL.
+ * The reference writes no current (pcurr is a dump period only); this is the
+ * field behind HETDDFT_DIAG_JTOT. */
+int hetddft_current(hetddft_ctx* ctx, double* jx, double* jy, double* jz);
+
 /* Waits for all work queued on the context's stream. */
 int hetddft_sync(hetddft_ctx* ctx);
 

@@ -636,6 +636,43 @@ def interpola3(prm: Params, rimp, fx, fy, fz):
     return out
 
 
+def diagnostics(prm: Params, psi, pot4, denx, vimp, aimp):
+    """Per-step diagnostics: the He count sum(den) dxyz (DFT4He3d.f90:884),
+    the centre-of-mass moments sum(den x_c) dxyz of r_cm (r_cm.f90:23-58; the
+    reference divides them by n4), the impurity mean-square radii
+    sqrt(sum(denx x_c^2) dxyz) (DFT4He3d.f90:918-928), min/max of vimp and
+    aimp per component (:900-903); and the chemical potential
+    <psi|H|psi>/<psi|psi>, H = -h2o2m4 lap + pot4, and the integral of the
+    current j = 2 h2o2m4 Im(conj(psi) grad psi).  Layout of hetddft_diagnostics."""
+    den = np.abs(psi) ** 2
+    dv = prm.dxyz
+    x, y, z = prm.axis_grid("x"), prm.axis_grid("y"), prm.axis_grid("z")
+    xs = [x[None, None, :], y[None, :, None], z[:, None, None]]
+    out = np.zeros(23)
+    out[0] = den.sum() * dv
+    for k in range(3):
+        out[1 + k] = (den * xs[k]).sum() * dv
+    lap = (deriv(psi, "x", 2, prm.hx) + deriv(psi, "y", 2, prm.hy)) + deriv(psi, "z", 2, prm.hz)
+    hp = pot4 * psi - prm.h2o2m4 * lap
+    out[4] = np.sum(np.real(np.conj(psi) * hp)) / den.sum()
+    for k, j in enumerate(current(prm, psi)):
+        out[5 + k] = j.sum() * dv
+    for k in range(3):
+        out[8 + k] = math.sqrt((denx * xs[k] ** 2).sum() * dv)
+    if vimp.shape[1] > 0:
+        for k in range(6):
+            col = vimp[k] if k < 3 else aimp[k - 3]
+            out[11 + 2 * k] = col.min()
+            out[12 + 2 * k] = col.max()
+    return out
+
+
+def current(prm: Params, psi):
+    """j = 2 h2o2m4 Im(conj(psi) d_c psi), c = x, y, z, with the 13-point stencil."""
+    hs = {"x": prm.hx, "y": prm.hy, "z": prm.hz}
+    return tuple(2.0 * prm.h2o2m4 * np.imag(np.conj(psi) * deriv(psi, a, 1, hs[a])) for a in ("x", "y", "z"))
+
+
 class Oracle:
     """State and per-step operators of one droplet+impurity instance."""
 
