@@ -1484,6 +1484,48 @@ int iterate_pc(hetddft_ctx* c, double dt) {
   return HETDDFT_OK;
 }
 
+// ---- full-history checkpoint ------------------------------------------------
+// Header, then the device arrays in a fixed order: psi; its PC slots (psi
+// history x3, H psi history x2, pc) and the RK accumulator q, by buffer
+// index; the particles r, v, a; their slots r x3, v x3, a x2; qr, qv, pcr,
+// pcv.  The slot order (ioldp, ioldh, ioldr, ioldv, iolda) and the step
+// counter are in the header.
+struct CkptHeader {
+  char magic[8];  // "HETDCKP1"
+  int nx, ny, nz, n_par, precision, iter_in_run;
+  int ioldp[3], ioldh[2], ioldr[3], ioldv[3], iolda[2];
+  int reserved[8];
+};
+
+struct CkptPart {
+  void* dev;
+  size_t bytes;
+};
+
+std::vector<CkptPart> ckpt_parts(hetddft_ctx* c) {
+  const size_t n = (size_t)c->g.n * 16, np3 = (size_t)3 * c->p.n_par * 8;
+  std::vector<CkptPart> v;
+  v.push_back({c->psi, n});
+  for (int k = 0; k < 3; ++k) v.push_back({c->old[k], n});
+  for (int k = 0; k < 2; ++k) v.push_back({c->hold[k], n});
+  v.push_back({c->pc, n});
+  v.push_back({c->q, n});
+  if (np3) {
+    for (double* a : {c->rimp, c->vimp, c->aimp}) v.push_back({a, np3});
+    for (int k = 0; k < 3; ++k) v.push_back({c->rold[k], np3});
+    for (int k = 0; k < 3; ++k) v.push_back({c->vold[k], np3});
+    for (int k = 0; k < 2; ++k) v.push_back({c->aold[k], np3});
+    for (double* a : {c->qr, c->qv, c->pcr, c->pcv}) v.push_back({a, np3});
+  }
+  return v;
+}
+
+long long ckpt_bytes(hetddft_ctx* c) {
+  long long b = sizeof(CkptHeader);
+  for (const CkptPart& q : ckpt_parts(c)) b += (long long)q.bytes;
+  return b;
+}
+
 DiagArgs diag_args(const hetddft_ctx* c) {
   DiagArgs a{};
   a.g = c->g;
@@ -1998,6 +2040,93 @@ int hetddft_diagnostics(hetddft_ctx* c, double out[HETDDFT_DIAG_N]) {
   }
   out[HETDDFT_DIAG_MU] = r[7] / r[0];
   for (int k = 0; k < 12; ++k) out[HETDDFT_DIAG_PMINMAX + k] = np > 0 ? r[kDiagSlots + k] : 0.0;
+  return HETDDFT_OK;
+}
+
+int hetddft_checkpoint_size(hetddft_ctx* c, long long* bytes) {
+  if (!c || !bytes) return HETDDFT_E_ARG;
+  *bytes = ckpt_bytes(c);
+  return HETDDFT_OK;
+}
+
+int hetddft_checkpoint_save(hetddft_ctx* c, void* buf) {
+  if (!c || !buf) return HETDDFT_E_ARG;
+  if (!c->started) return fail(c, HETDDFT_E_STATE, "hetddft_start not called");
+  RET(settle(c));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  CkptHeader h;
+  std::memset(&h, 0, sizeof h);
+  std::memcpy(h.magic, "HETDCKP1", 8);
+  h.nx = c->p.nx;
+  h.ny = c->p.ny;
+  h.nz = c->p.nz;
+  h.n_par = c->p.n_par;
+  h.precision = c->p.precision;
+  h.iter_in_run = c->iter_in_run;
+  std::copy(c->ioldp, c->ioldp + 3, h.ioldp);
+  std::copy(c->ioldh, c->ioldh + 2, h.ioldh);
+  std::copy(c->ioldr, c->ioldr + 3, h.ioldr);
+  std::copy(c->ioldv, c->ioldv + 3, h.ioldv);
+  std::copy(c->iolda, c->iolda + 2, h.iolda);
+  unsigned char* o = static_cast<unsigned char*>(buf);
+  std::memcpy(o, &h, sizeof h);
+  o += sizeof h;
+  for (const CkptPart& q : ckpt_parts(c)) {
+    HIPCHK(c, hipMemcpy(o, q.dev, q.bytes, hipMemcpyDeviceToHost));
+    o += q.bytes;
+  }
+  return HETDDFT_OK;
+}
+
+int hetddft_checkpoint_load(hetddft_ctx* c, const void* buf) {
+  if (!c || !buf) return HETDDFT_E_ARG;
+  if (!c->kernels_set) return fail(c, HETDDFT_E_STATE, "hetddft_set_kernels not called");
+  CkptHeader h;
+  std::memcpy(&h, buf, sizeof h);
+  if (std::memcmp(h.magic, "HETDCKP1", 8) != 0) return fail(c, HETDDFT_E_ARG, "checkpoint: bad magic");
+  if (h.nx != c->p.nx || h.ny != c->p.ny || h.nz != c->p.nz || h.n_par != c->p.n_par ||
+      h.precision != c->p.precision)
+    return fail(c, HETDDFT_E_ARG, "checkpoint of a %dx%dx%d grid, N_par %d, precision %d does not fit this context",
+                h.nx, h.ny, h.nz, h.n_par, h.precision);
+  auto perm_ok = [](const int* a, int n) {
+    int seen = 0;
+    for (int k = 0; k < n; ++k) {
+      if (a[k] < 0 || a[k] >= n || (seen >> a[k]) & 1) return false;
+      seen |= 1 << a[k];
+    }
+    return true;
+  };
+  if (!perm_ok(h.ioldp, 3) || !perm_ok(h.ioldh, 2) || !perm_ok(h.ioldr, 3) || !perm_ok(h.ioldv, 3) ||
+      !perm_ok(h.iolda, 2) || h.iter_in_run < 0)
+    return fail(c, HETDDFT_E_ARG, "checkpoint: corrupt slot order");
+  RET(settle(c));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  const unsigned char* o = static_cast<const unsigned char*>(buf) + sizeof h;
+  for (const CkptPart& q : ckpt_parts(c)) {
+    HIPCHK(c, hipMemcpy(q.dev, o, q.bytes, hipMemcpyHostToDevice));
+    o += q.bytes;
+  }
+  std::copy(h.ioldp, h.ioldp + 3, c->ioldp);
+  std::copy(h.ioldh, h.ioldh + 2, c->ioldh);
+  std::copy(h.ioldr, h.ioldr + 3, c->ioldr);
+  std::copy(h.ioldv, h.ioldv + 3, c->ioldv);
+  std::copy(h.iolda, h.iolda + 2, c->iolda);
+  c->iter_in_run = h.iter_in_run;
+  c->chain_pending = false;
+  clear_graphs(c);
+  // den and the potential fields of the restored state (the fields the saved
+  // context held after its last potential evaluation)
+  if (c->mixed)
+    hipLaunchKernelGGL((k_density<float>), dim3(blocks_for(c->g.n)), dim3(kBlock), 0, c->st, (const c64*)c->psi,
+                       (float*)c->fld[hetddft_ctx::F_DEN], c->g.n);
+  else
+    hipLaunchKernelGGL((k_density<double>), dim3(blocks_for(c->g.n)), dim3(kBlock), 0, c->st, (const c64*)c->psi,
+                       (double*)c->fld[hetddft_ctx::F_DEN], c->g.n);
+  LAUNCHCHK(c);
+  RET(run_veval(c));
+  RET(settle(c));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->started = true;
   return HETDDFT_OK;
 }
 

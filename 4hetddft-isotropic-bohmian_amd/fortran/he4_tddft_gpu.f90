@@ -18,7 +18,11 @@
 !   fileout  stream file: psi, rimp, vimp, aimp after the last iteration
 !            (with dry_run: fvlj4, wcgk, kalfs, vq from hetddft_make_kernels)
 !   filelog  text: "means iter time errHe errimp errvimp <r> <v> <a>" per
-!            iteration and "energy iter <12 terms>" at start and every pener
+!            iteration, "energy iter <12 terms>" at start and every pener,
+!            and "diag iter <23 values of hetddft_diagnostics>" every pener
+!            (He count and centre-of-mass moments, chemical potential, total
+!            current, impurity mean-square radii, particle min/max: the
+!            driver's output of DFT4He3d.f90:879-930)
 program he4_tddft_gpu
   use, intrinsic :: iso_c_binding
   use hetddft_mod
@@ -44,7 +48,7 @@ program he4_tddft_gpu
   complex(c_double_complex), allocatable :: psi(:,:,:)
   real(c_double), allocatable :: rimp(:,:), vimp(:,:), aimp(:,:)
   real(c_double), allocatable :: fvlj4(:,:,:), wcgk(:,:,:), kalfs(:,:,:), vq(:,:,:)
-  real(c_double) :: errHe, errimp, errvimp, means(9), e(12), time
+  real(c_double) :: errHe, errimp, errvimp, means(9), e(12), diag(23), time
   integer :: iter, narg, u, ulog, ios
   character(len=512) :: nmlfile
 
@@ -141,6 +145,8 @@ program he4_tddft_gpu
     if (mod(iter, pener) == 0) then
       call gpu_energy(ctx, e)
       write(ulog, '("energy ",i8,12es26.17e3)') iter, e
+      call hetddft_check(hetddft_diagnostics(ctx, diag), ctx, 'hetddft_diagnostics')
+      write(ulog, '("diag ",i8,23es26.17e3)') iter, diag
     end if
   end do
   close(ulog)

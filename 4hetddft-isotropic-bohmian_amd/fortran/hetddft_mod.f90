@@ -60,7 +60,7 @@ module hetddft_mod
             hetddft_iterate, hetddft_particle_means, hetddft_energy, hetddft_get_state, &
             hetddft_get_field, hetddft_convolve, hetddft_profile, hetddft_profile_read, &
             hetddft_sync, hetddft_device_bytes, hetddft_graph_info, hetddft_diagnostics, &
-            hetddft_current
+            hetddft_current, hetddft_checkpoint_size, hetddft_checkpoint_save, hetddft_checkpoint_load
   public :: hetddft_message, hetddft_check, gpu_steprk, gpu_steppc, gpu_veval, gpu_energy
 
   interface
@@ -204,6 +204,22 @@ module hetddft_mod
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: out(23)
+    end function
+
+    integer(c_int) function hetddft_checkpoint_size(ctx, bytes) bind(C, name="hetddft_checkpoint_size")
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: ctx
+      integer(c_long_long), intent(out) :: bytes
+    end function
+
+    integer(c_int) function hetddft_checkpoint_save(ctx, buf) bind(C, name="hetddft_checkpoint_save")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, buf
+    end function
+
+    integer(c_int) function hetddft_checkpoint_load(ctx, buf) bind(C, name="hetddft_checkpoint_load")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, buf
     end function
 
     integer(c_int) function hetddft_current(ctx, jx, jy, jz) bind(C, name="hetddft_current")

@@ -120,6 +120,9 @@ def load_library():
         "hetddft_sync": ([C], ctypes.c_int),
         "hetddft_graph_info": ([C, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "hetddft_diagnostics": ([C, _VP], ctypes.c_int),
+        "hetddft_checkpoint_size": ([C, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+        "hetddft_checkpoint_save": ([C, _VP], ctypes.c_int),
+        "hetddft_checkpoint_load": ([C, _VP], ctypes.c_int),
         "hetddft_current": ([C, _VP, _VP, _VP], ctypes.c_int),
         "hetddft_profile": ([C, ctypes.c_int], ctypes.c_int),
         "hetddft_profile_read": ([C, ctypes.c_int, _VP, _VP, _VP], ctypes.c_int),
@@ -287,6 +290,19 @@ class Instance:
         return {"norm": out[0], "cm_moments": out[1:4].copy(), "mu": out[4], "jtot": out[5:8].copy(),
                 "msx": out[8:11].copy(), "vmin": out[11:17:2].copy(), "vmax": out[12:17:2].copy(),
                 "amin": out[17:23:2].copy(), "amax": out[18:23:2].copy()}
+
+    def checkpoint(self):
+        """Full-history checkpoint image (bytes) for bit-exact resume."""
+        n = ctypes.c_longlong(0)
+        self._check(self.lib.hetddft_checkpoint_size(self._c, ctypes.byref(n)), "checkpoint_size")
+        buf = np.zeros(n.value, dtype=np.uint8)
+        self._check(self.lib.hetddft_checkpoint_save(self._c, _ptr(buf)), "checkpoint_save")
+        return buf.tobytes()
+
+    def restore(self, image):
+        """Restore a checkpoint image (after set_kernels) and re-run the potential evaluation."""
+        buf = np.frombuffer(image, dtype=np.uint8).copy()
+        self._check(self.lib.hetddft_checkpoint_load(self._c, _ptr(buf)), "checkpoint_load")
 
     def current(self):
         """Current density j = 2 h2o2m4 Im(conj(psi) grad psi), three (nz, ny, nx) arrays."""

@@ -196,6 +196,20 @@ int hetddft_profile_read(hetddft_ctx* ctx, int cls, double* ms_total, long long*
 #define HETDDFT_DIAG_N 23
 int hetddft_diagnostics(hetddft_ctx* ctx, double out[HETDDFT_DIAG_N]);
 
+/* Full-history checkpoint for bit-exact resume.  No counterpart in the
+ * reference, whose restarts read psi and the particles back (readenc.f90:96-114,
+ * mode 2) and begin again with three RK steps; this one keeps everything the
+ * next steps read: psi with its PC slots (psi history x3, H psi history x2,
+ * pc) and the RK accumulator, the particles with their slots and qr, qv, pcr,
+ * pcv, the slot order and the step counter.  _size gives the bytes of the
+ * image; _save writes it to a host buffer of that size; _load restores it into
+ * a context of the same grid, N_par and precision (after
+ * hetddft_set_kernels) and re-runs the potential evaluation, after which
+ * hetddft_iterate continues exactly as the saved context would have. */
+int hetddft_checkpoint_size(hetddft_ctx* ctx, long long* bytes);
+int hetddft_checkpoint_save(hetddft_ctx* ctx, void* buf);
+int hetddft_checkpoint_load(hetddft_ctx* ctx, const void* buf);
+
 /* Current density j = 2 h2o2m4 Im(conj(psi) grad psi) (units of psi^2 A/K^-1)
  * into host arrays of nx*ny*nz doubles (x fastest); any pointer may be NULL.
  * The reference writes no current (pcurr is a dump period only); this is the

@@ -11,6 +11,10 @@
 !   ref_harness deriv      <dir>   derivnD with npd=13, icon=13
 !                                  (init_deriv_parallel.f90, redef.f, newder.f)
 !   ref_harness ironing    <dir>   ironing (ironing.f90:4-33)
+!   ref_harness printoutc  <dir>   printoutc (printoutc.f90:13-124) writing den.out/imp.out
+!   ref_harness readenc    <dir>   readenc mode 2 (readenc.f90:96-114) reading den.in/imp.in
+!                                  (run with <dir> as the working directory: the
+!                                  file names are character(len=60))
 !
 ! <dir>/params.txt holds the scalar inputs (list-directed); arrays travel as
 ! raw little-endian stream files in Fortran order.
@@ -42,6 +46,10 @@ program ref_harness
     call run_deriv()
   case ('ironing')
     call run_ironing()
+  case ('printoutc')
+    call run_printoutc()
+  case ('readenc')
+    call run_readenc()
   case default
     write(0, *) 'unknown command ', trim(cmd)
     error stop 2
@@ -149,5 +157,50 @@ contains
     end do
     call write_array('out.bin', f, n1 * n2 * n3)
   end subroutine run_ironing
+
+  ! params: nx ny nz n_par xmax ymax zmax hx hy hz time iprint iter paflv ;
+  ! psi.bin rimp.bin vimp.bin -> den.out imp.out
+  subroutine run_printoutc()
+    integer :: n1, n2, n3, np, iprint, iter, m
+    real(kind=8) :: xm, ym, zm, h1, h2, h3, time, paflv
+    real(kind=8), allocatable :: raw(:), ri(:), vi(:)
+    complex(kind=8), allocatable :: ps(:,:,:)
+    character(len=60) :: f1, f2
+    read(u, *) n1, n2, n3, np, xm, ym, zm, h1, h2, h3, time, iprint, iter, paflv
+    m = n1 * n2 * n3
+    allocate(raw(2 * m), ps(n1, n2, n3), ri(3 * np), vi(3 * np))
+    call read_array('psi.bin', raw, 2 * m)
+    ps = reshape(cmplx(raw(1::2), raw(2::2), kind=8), [n1, n2, n3])
+    call read_array('rimp.bin', ri, 3 * np)
+    call read_array('vimp.bin', vi, 3 * np)
+    f1 = 'den.out'
+    f2 = 'imp.out'
+    call printoutc(time, iprint, f1, f2, ps, n1, n2, n3, h1, h2, h3, xm, ym, zm, &
+                   reshape(ri, [np, 3]), reshape(vi, [np, 3]), np, paflv, iter)
+  end subroutine run_printoutc
+
+  ! params: nx ny nz n_par ; den.in imp.in -> psi.bin rimp.bin vimp.bin
+  subroutine run_readenc()
+    use rho, only: psi, den
+    use impur, only: N_par, rimp, vimp
+    integer :: n1, n2, n3, m
+    real(kind=8) :: r_clust
+    real(kind=8), allocatable :: raw(:)
+    character(len=60) :: f1, f2
+    read(u, *) n1, n2, n3, N_par
+    nx = n1
+    ny = n2
+    nz = n3
+    m = n1 * n2 * n3
+    allocate(psi(n1, n2, n3), den(n1, n2, n3), rimp(N_par, 3), vimp(N_par, 3), raw(2 * m))
+    f1 = 'den.in'
+    f2 = 'imp.in'
+    call readenc(100, 0.0218d0, f1, f2, 2, 0.d0, r_clust)
+    raw(1::2) = reshape(real(psi, kind=8), [m])
+    raw(2::2) = reshape(aimag(psi), [m])
+    call write_array('psi.bin', raw, 2 * m)
+    call write_array('rimp.bin', reshape(rimp, [3 * N_par]), 3 * N_par)
+    call write_array('vimp.bin', reshape(vimp, [3 * N_par]), 3 * N_par)
+  end subroutine run_readenc
 
 end program ref_harness

@@ -84,19 +84,19 @@ def test_fortran_rejects_bad_grid(tmp_path):
 
 
 def _parse_log(path: Path):
-    means, energy = {}, {}
+    means, energy, diag = {}, {}, {}
     for line in path.read_text().splitlines():
         tok = line.split()
         vals = np.array([float(t) for t in tok[2:]])
-        (means if tok[0] == "means" else energy)[int(tok[1])] = vals
-    return means, energy
+        {"means": means, "energy": energy, "diag": diag}[tok[0]][int(tok[1])] = vals
+    return means, energy, diag
 
 
 def _run_python(wl, precision, niter, pener):
     p = inputs.params_for(wl, precision=precision)
     k = hetddft.make_kernels(p, hetddft.Morse(20.0, 5.0, 0.8, 2.0, 0.0, 1500))
     psi, r, v = inputs.initial_state(wl)
-    means, energy = {}, {}
+    means, energy, diag = {}, {}, {}
     with hetddft.Instance(p) as g:
         g.set_kernels(k)
         g.set_state(psi, r, v)
@@ -113,8 +113,12 @@ def _run_python(wl, precision, niter, pener):
             means[it] = np.concatenate([[it * wl.dt_ps], err, g.particle_means()])
             if it % pener == 0:
                 energy[it] = np.array(list(g.energy().values()))
+                d = g.diagnostics()
+                mm = np.empty(12)
+                mm[0:6:2], mm[1:6:2], mm[6::2], mm[7::2] = d["vmin"], d["vmax"], d["amin"], d["amax"]
+                diag[it] = np.concatenate([[d["norm"]], d["cm_moments"], [d["mu"]], d["jtot"], d["msx"], mm])
         state = g.get_state()
-    return state, means, energy
+    return state, means, energy, diag
 
 
 @pytest.mark.gpu
@@ -138,9 +142,9 @@ def test_fortran_host_loop_matches(tmp_path, precision):
     raw = np.fromfile(tmp_path / "state.out", dtype=np.float64)
     fpsi = raw[: 2 * n**3].view(np.complex128).reshape(n, n, n)
     rest = raw[2 * n**3:].reshape(3, 3, npar)
-    fmeans, fenergy = _parse_log(tmp_path / "steps.log")
+    fmeans, fenergy, fdiag = _parse_log(tmp_path / "steps.log")
 
-    (gpsi, gr, gv, ga), gmeans, genergy = _run_python(wl, precision, niter, pener)
+    (gpsi, gr, gv, ga), gmeans, genergy, gdiag = _run_python(wl, precision, niter, pener)
     # same library, same inputs, same call sequence: identical bits
     assert np.array_equal(fpsi, gpsi)
     assert np.array_equal(rest[0], gr) and np.array_equal(rest[1], gv) and np.array_equal(rest[2], ga)
@@ -149,6 +153,9 @@ def test_fortran_host_loop_matches(tmp_path, precision):
         np.testing.assert_allclose(fmeans[it], gmeans[it], rtol=1e-16, atol=0)
     for it in genergy:
         np.testing.assert_allclose(fenergy[it], genergy[it], rtol=1e-16, atol=0)
+    assert sorted(fdiag) == sorted(gdiag) and len(gdiag) > 0
+    for it in gdiag:
+        np.testing.assert_allclose(fdiag[it], gdiag[it], rtol=1e-16, atol=0)
 
     if precision == 0:
         from oracle import hetddft_oracle as orc
