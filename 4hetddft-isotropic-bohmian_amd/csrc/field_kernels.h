@@ -784,4 +784,89 @@ __global__ void __launch_bounds__(kBlock) k_part_minmax(int np, const double* __
   }
 }
 
+// ---- particle sampling (sorteo.f90:1-37) --------------------------------------
+// Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC11): a counter-based generator,
+// so particle i's attempt a draws the block of counter (i, a, 0, 0) under the
+// key (seed, 0) with no state shared between threads.
+struct U64x4 {
+  unsigned long long v[4];
+};
+
+__device__ inline U64x4 philox4x64(U64x4 c, unsigned long long k0, unsigned long long k1) {
+  constexpr unsigned long long M0 = 0xD2E7470EE14C6C93ull, M1 = 0xCA5A826395121157ull;
+  constexpr unsigned long long W0 = 0x9E3779B97F4A7C15ull, W1 = 0xBB67AE8584CAA73Bull;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += W0;
+      k1 += W1;
+    }
+    const unsigned long long hi0 = __umul64hi(M0, c.v[0]), lo0 = M0 * c.v[0];
+    const unsigned long long hi1 = __umul64hi(M1, c.v[2]), lo1 = M1 * c.v[2];
+    U64x4 o;
+    o.v[0] = hi1 ^ c.v[1] ^ k0;
+    o.v[1] = lo1;
+    o.v[2] = hi0 ^ c.v[3] ^ k1;
+    o.v[3] = lo0;
+    c = o;
+  }
+  return c;
+}
+
+__device__ inline double u53(unsigned long long v) { return (double)(v >> 11) * (1.0 / 9007199254740992.0); }
+
+struct SortArgs {
+  Grid g;
+  double x0, y0, z0, dx, dy, dz;  // x(1) and x(2) - x(1) of the grid arrays
+  double lo[3], hi[3];
+  double denmax;
+  unsigned long long seed;
+  long long max_attempts;
+};
+
+// One thread per particle: uniform point in [lo, hi], uniform dum in
+// [0, denmax); accepted when dum < trilinear denx at the point (interpola3denx,
+// sorteo.f90:40-74).  rimp is (N_par, 3); a particle not accepted within
+// max_attempts sets *failed.
+__global__ void __launch_bounds__(kBlock) k_sorteo(SortArgs a, int np, const double* __restrict__ denx,
+                                                    double* __restrict__ rimp, int* __restrict__ failed) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const Grid& g = a.g;
+  for (long long t = 0; t < a.max_attempts; ++t) {
+    U64x4 c;
+    c.v[0] = (unsigned long long)i;
+    c.v[1] = (unsigned long long)t;
+    c.v[2] = 0;
+    c.v[3] = 0;
+    const U64x4 w = philox4x64(c, a.seed, 0ull);
+    double xyz[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) xyz[k] = u53(w.v[k]) * (a.hi[k] - a.lo[k]) + a.lo[k];
+    const double dum = u53(w.v[3]) * a.denmax;
+    const double ax = (xyz[0] - a.x0) / a.dx + 1.0;
+    const double ay = (xyz[1] - a.y0) / a.dy + 1.0;
+    const double az = (xyz[2] - a.z0) / a.dz + 1.0;
+    const int ix = (int)ax, iy = (int)ay, iz = (int)az;
+    double f = 0.0;
+    if (!(ix >= g.nx || iy >= g.ny || iz >= g.nz || ix <= 0 || iy <= 0 || iz <= 0)) {
+      const double r = ax - ix, s = ay - iy, u = az - iz;
+      auto d = [&](int p, int q, int k) { return denx[(p - 1) + (long long)g.nx * ((q - 1) + (long long)g.ny * (k - 1))]; };
+      const double x000 = d(ix, iy, iz), x001 = d(ix, iy, iz + 1), x010 = d(ix, iy + 1, iz), x011 = d(ix, iy + 1, iz + 1);
+      const double x100 = d(ix + 1, iy, iz), x101 = d(ix + 1, iy, iz + 1), x110 = d(ix + 1, iy + 1, iz),
+                   x111 = d(ix + 1, iy + 1, iz + 1);
+      f = 1.0 * (+x000) + r * (-x000 + x100) + s * (-x000 + x010) + u * (-x000 + x001) +
+          r * s * (+x000 - x100 - x010 + x110) + r * u * (+x000 - x100 - x001 + x101) +
+          s * u * (+x000 - x010 - x001 + x011) + r * s * u * (-x000 + x100 + x010 + x001 - x011 - x101 - x110 + x111);
+    }
+    if (dum < f) {
+      rimp[i] = xyz[0];
+      rimp[np + i] = xyz[1];
+      rimp[2 * np + i] = xyz[2];
+      return;
+    }
+  }
+  atomicOr(failed, 1);
+}
+
 }  // namespace hetd

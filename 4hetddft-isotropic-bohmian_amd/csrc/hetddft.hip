@@ -11,6 +11,7 @@
 #include "fft_kernels.h"
 #include "field_kernels.h"
 #include "kinit.h"
+#include "kinit_device.h"
 #include "stencil25.h"
 
 #include <algorithm>
@@ -1869,6 +1870,137 @@ int hetddft_make_kernels(const hetddft_params* p, const hetddft_morse* mo, doubl
   return HETDDFT_OK;
 }
 
+int hetddft_init_kernels(hetddft_ctx* c, const hetddft_morse* mo) {
+  if (!c) return HETDDFT_E_ARG;
+  using namespace kinit;
+  const hetddft_params* p = &c->p;
+  const double hx = 2.0 * std::fabs(p->xmax) / p->nx, hy = 2.0 * std::fabs(p->ymax) / p->ny,
+               hz = 2.0 * std::fabs(p->zmax) / p->nz;
+  MorsePot pot{};
+  pot.D = mo ? mo->D : 20.0;
+  pot.re = mo ? mo->re : 5.0;
+  pot.alpha = mo ? mo->alpha : 0.8;
+  pot.rc = mo ? mo->rc : 2.0;
+  {
+    const double e = std::exp(-pot.alpha * (pot.rc - pot.re));
+    pot.umax = (mo && mo->umax != 0.0) ? mo->umax : pot.D * (e * e - 2.0 * e);
+  }
+  const int nq = (mo && mo->nq > 1) ? mo->nq : 1500;
+  const double pmaxx = 1.0 / (2.0 * hx), pmaxy = 1.0 / (2.0 * hy), pmaxz = 1.0 / (2.0 * hz);
+  const double qmax = std::sqrt(pmaxx * pmaxx + pmaxy * pmaxy + pmaxz * pmaxz);
+  const double hq = qmax / (nq - 1);
+  const FtvSetup su(pot, qmax, hq);
+  const int N = FtvSetup::N, np = (int)su.P.size();
+  RET(settle(c));
+  std::vector<void*> tmp;
+  auto talloc = [&](size_t bytes) -> double* {
+    void* q = nullptr;
+    if (hipMalloc(&q, bytes) != hipSuccess) return nullptr;
+    tmp.push_back(q);
+    return (double*)q;
+  };
+  auto cleanup = [&]() {
+    for (void* q : tmp) hipFree(q);
+  };
+  double *dx = talloc(N * 8), *da = talloc(N * 8), *db = talloc(N * 8), *dc = talloc(N * 8), *dd = talloc(N * 8);
+  double *dP = talloc(np * 8), *dv = talloc(np * 8), *sA = talloc(np * 8), *sB = talloc(np * 8), *sC = talloc(np * 8),
+         *sD = talloc(np * 8);
+  for (double* q : {dx, da, db, dc, dd, dP, dv, sA, sB, sC, sD})
+    if (!q) {
+      cleanup();
+      return fail(c, HETDDFT_E_NOMEM, "init_kernels: hipMalloc failed");
+    }
+  hipError_t e = hipSuccess;
+  auto up = [&](double* dst, const std::vector<double>& v) {
+    if (e == hipSuccess) e = hipMemcpy(dst, v.data(), v.size() * 8, hipMemcpyHostToDevice);
+  };
+  up(dx, su.x);
+  up(da, su.a);
+  up(db, su.b);
+  up(dc, su.c);
+  up(dd, su.d);
+  up(dP, su.P);
+  std::vector<double> vpp(np);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_ftv_sums, dim3(np), dim3(256), 0, c->st, (const double*)dx, (const double*)da,
+                       (const double*)db, (const double*)dc, (const double*)dd, N, (const double*)dP, su.rc, su.fc, dv);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (e == hipSuccess) e = hipMemcpy(vpp.data(), dv, np * 8, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(c, HETDDFT_E_HIP, "init_kernels: %s", hipGetErrorString(e));
+  }
+  const FtvSpline ft(su.P, vpp);
+  up(dP, ft.P);
+  up(sA, ft.A);
+  up(sB, ft.B);
+  up(sC, ft.C);
+  up(sD, ft.D);
+  KinitArgs ka{};
+  ka.nx = p->nx;
+  ka.ny = p->ny;
+  ka.nz = p->nz;
+  ka.mx = p->nx / 2 + 1;
+  ka.hx = hx;
+  ka.hy = hy;
+  ka.hz = hz;
+  ka.h4 = p->h4;
+  ka.l = p->l;
+  ka.eps = p->eps4;
+  ka.sigma = p->sigma4;
+  ka.P = dP;
+  ka.A = sA;
+  ka.B = sB;
+  ka.C = sC;
+  ka.D = sD;
+  ka.np = np;
+  const unsigned nb = blocks_for(c->g.h, 256);
+  if (e == hipSuccess) {
+    if (c->mixed)
+      hipLaunchKernelGGL((k_make_kernels<float>), dim3(nb), dim3(256), 0, c->st, ka, (double*)c->fs64e.kfvlj4,
+                         (double*)c->fs64e.kwcgk, (double*)nullptr, (double*)c->fs64e.kvq, (float*)c->fs.kfvlj4,
+                         (float*)c->fs.kwcgk, (float*)c->fs.kkalfs, (float*)c->fs.kvq);
+    else
+      hipLaunchKernelGGL((k_make_kernels<double>), dim3(nb), dim3(256), 0, c->st, ka, (double*)c->fs.kfvlj4,
+                         (double*)c->fs.kwcgk, (double*)c->fs.kkalfs, (double*)c->fs.kvq, (double*)nullptr,
+                         (double*)nullptr, (double*)nullptr, (double*)nullptr);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  cleanup();
+  if (e != hipSuccess) return fail(c, HETDDFT_E_HIP, "init_kernels: %s", hipGetErrorString(e));
+  c->kernels_set = true;
+  return HETDDFT_OK;
+}
+
+int hetddft_get_kernels(hetddft_ctx* c, double* fvlj4, double* wcgk, double* kalfs, double* vq) {
+  if (!c) return HETDDFT_E_ARG;
+  if (!c->kernels_set) return fail(c, HETDDFT_E_STATE, "kernels not set");
+  RET(settle(c));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  const Grid& g = c->g;
+  std::vector<double> t((size_t)g.h);
+  std::vector<float> tf(c->mixed ? (size_t)g.h : 0);
+  const void* src[4] = {c->fs.kfvlj4, c->fs.kwcgk, c->fs.kkalfs, c->fs.kvq};
+  double* dst[4] = {fvlj4, wcgk, kalfs, vq};
+  for (int k = 0; k < 4; ++k) {
+    if (!dst[k]) continue;
+    if (c->mixed) {
+      HIPCHK(c, hipMemcpy(tf.data(), src[k], (size_t)g.h * 4, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < (size_t)g.h; ++i) t[i] = (double)tf[i];
+    } else {
+      HIPCHK(c, hipMemcpy(t.data(), src[k], (size_t)g.h * 8, hipMemcpyDeviceToHost));
+    }
+    for (int iz = 0; iz < g.nz; ++iz)
+      for (int iy = 0; iy < g.ny; ++iy)
+        for (int ix = 0; ix < g.mx; ++ix)
+          dst[k][((size_t)iz * g.ny + iy) * g.mx + ix] = t[((size_t)iz * g.mx + ix) * g.ny + iy];
+  }
+  return HETDDFT_OK;
+}
+
 int hetddft_set_kernels(hetddft_ctx* c, const double* fvlj4, const double* wcgk, const double* kalfs,
                         const double* vq) {
   if (!c || !fvlj4 || !wcgk || !kalfs || !vq) return fail(c, HETDDFT_E_ARG, "null kernel array");
@@ -2127,6 +2259,62 @@ int hetddft_checkpoint_load(hetddft_ctx* c, const void* buf) {
   RET(settle(c));
   HIPCHK(c, hipStreamSynchronize(c->st));
   c->started = true;
+  return HETDDFT_OK;
+}
+
+int hetddft_sample_particles(hetddft_ctx* c, const double* denx, const double lo[3], const double hi[3],
+                             unsigned long long seed, long long max_attempts, double* rimp) {
+  if (!c || !denx || !lo || !hi || !rimp || max_attempts <= 0) return HETDDFT_E_ARG;
+  const int np = c->p.n_par;
+  if (np == 0) return HETDDFT_OK;
+  const Grid& g = c->g;
+  SortArgs a{};
+  a.g = g;
+  // x(1) and x(2) - x(1) from the grid arrays x(ix) = -xmax + hx (ix-1)
+  a.x0 = -c->p.xmax;
+  a.y0 = -c->p.ymax;
+  a.z0 = -c->p.zmax;
+  a.dx = (-c->p.xmax + c->hx) - a.x0;
+  a.dy = (-c->p.ymax + c->hy) - a.y0;
+  a.dz = (-c->p.zmax + c->hz) - a.z0;
+  double mx = denx[0];
+  for (long long i = 1; i < g.n; ++i) mx = std::max(mx, denx[i]);
+  if (!(mx > 0.0)) return fail(c, HETDDFT_E_ARG, "sample_particles: denx has no positive value");
+  for (int k = 0; k < 3; ++k) {
+    a.lo[k] = lo[k];
+    a.hi[k] = hi[k];
+  }
+  a.denmax = mx;
+  a.seed = seed;
+  a.max_attempts = max_attempts;
+  RET(settle(c));
+  double* dd = nullptr;
+  double* dr = nullptr;
+  int* flag = nullptr;
+  auto cleanup = [&]() {
+    if (dd) hipFree(dd);
+    if (dr) hipFree(dr);
+    if (flag) hipFree(flag);
+  };
+  if (hipMalloc((void**)&dd, (size_t)g.n * 8) != hipSuccess || hipMalloc((void**)&dr, (size_t)3 * np * 8) != hipSuccess ||
+      hipMalloc((void**)&flag, sizeof(int)) != hipSuccess) {
+    cleanup();
+    return fail(c, HETDDFT_E_NOMEM, "sample_particles: hipMalloc failed");
+  }
+  hipError_t e = hipMemcpy(dd, denx, (size_t)g.n * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(flag, 0, sizeof(int));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_sorteo, dim3(blocks_for(np)), dim3(kBlock), 0, c->st, a, np, (const double*)dd, dr, flag);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  int failed = 0;
+  if (e == hipSuccess) e = hipMemcpy(&failed, flag, sizeof(int), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && !failed) e = hipMemcpy(rimp, dr, (size_t)3 * np * 8, hipMemcpyDeviceToHost);
+  cleanup();
+  if (e != hipSuccess) return fail(c, HETDDFT_E_HIP, "sample_particles: %s", hipGetErrorString(e));
+  if (failed) return fail(c, HETDDFT_E_STATE, "sample_particles: a particle was not accepted in %lld attempts",
+                          max_attempts);
   return HETDDFT_OK;
 }
 

@@ -146,77 +146,111 @@ struct MorsePot {
   }
 };
 
-// Spline Fourier transform of a radial potential (FT_V_spline.f90:14-276).
-struct FtvSpline {
-  std::vector<double> P, A, B, C, D;
+// Spline Fourier transform of a radial potential (FT_V_spline.f90:14-276), in
+// three phases so that the momentum sums can also run on the device:
+//   FtvSetup   the cubic spline of V on 100001 points in [rc + 1e-8, 2000 A]
+//              (FT_V_spline.f90:84-123) and the momentum grid P;
+//   ftv_sums   V(p) = 4 pi/p [vp(p) + sum over spline pieces of the piece's
+//              analytic Fourier integral] for every p of the grid (:151-225);
+//   FtvSpline  the small-q redefinition (:230-264) and the spline over p used
+//              for lookups (:47-53, 268).
+struct FtvSetup {
+  static constexpr int N = 100001;
+  std::vector<double> x, a, b, c, d, P;
+  double rc = 0.0, fc = 0.0;
 
-  FtvSpline(const MorsePot& pot, double pmax, double hp) {
-    const int N = 100001;
-    const double rmax = 2000.0, eps = 1e-8, small_q = 1e-2, dr = 0.5;
-    const double rc = pot.rc, fc = pot.umax;
+  FtvSetup(const MorsePot& pot, double pmax, double hp) {
+    const double rmax = 2000.0, eps = 1e-8, dr = 0.5;
+    rc = pot.rc;
+    fc = pot.umax;
     const double X1 = rc + eps, XN = rmax;
     const double S1 = (pot(X1) - 2.0 * pot(X1 + dr) + pot(X1 + dr + dr)) / (dr * dr);
     const double Sn = (pot(XN - dr) - 2.0 * pot(XN) + pot(XN + dr)) / (dr * dr);
     const double h = (XN - X1) / (N - 1);
-    std::vector<double> x(N), y(N);
+    x.resize(N);
+    std::vector<double> y(N);
     for (int i = 0; i < N; ++i) {
       x[i] = i * h + X1;
       y[i] = pot(x[i]);
     }
-    std::vector<double> a, b, c, d;
     spline(x, y, S1, Sn, a, b, c, d);
-    const double Qpi = 4.0 * kPi;
     const int np = (int)(pmax / hp + 1.5);
     P.resize(np);
-    std::vector<double> Vpp(np);
     for (int i = 0; i < np; ++i) P[i] = i * hp;
-    auto work = [&](int i0, int i1) {
-      for (int i = i0; i < i1; ++i) {
-        const double p = P[i];
-        const double q = 2.0 * kPi * p;
-        if (p == 0.0) {
-          const double vp = std::pow(rc, 3) * fc / 3.0;
-          double Fi = 0.0;
-          for (int j = 0; j < N - 1; ++j) {
-            const double u = x[j], w = x[j + 1];
-            Fi = Fi + (a[j] / 3.0 * (std::pow(w, 3) - std::pow(u, 3)) + b[j] / 4.0 * (std::pow(w, 4) - std::pow(u, 4)) +
-                       c[j] / 5.0 * (std::pow(w, 5) - std::pow(u, 5)) + d[j] / 6.0 * (std::pow(w, 6) - std::pow(u, 6)));
-          }
-          Vpp[i] = (vp + Fi) * Qpi;
-          continue;
-        }
-        const double c1 = 1.0 / q, c2 = c1 * c1, c3 = c2 * c1, c4 = c2 * c2, c5 = c4 * c1;
-        const double vp = fc * c1 * (std::sin(q * rc) * c1 - rc * std::cos(q * rc));
+  }
+};
+
+// The sums of phase two for momenta [i0, i1), on host threads.
+inline void ftv_sums(const FtvSetup& s, std::vector<double>& Vpp) {
+  const int N = FtvSetup::N;
+  const std::vector<double>&x = s.x, &a = s.a, &b = s.b, &c = s.c, &d = s.d, &P = s.P;
+  const double rc = s.rc, fc = s.fc, Qpi = 4.0 * kPi;
+  const int np = (int)P.size();
+  Vpp.assign(np, 0.0);
+  auto work = [&](int i0, int i1) {
+    for (int i = i0; i < i1; ++i) {
+      const double p = P[i];
+      const double q = 2.0 * kPi * p;
+      if (p == 0.0) {
+        const double vp = std::pow(rc, 3) * fc / 3.0;
         double Fi = 0.0;
-        double sj = std::sin(q * x[0]), cj = std::cos(q * x[0]);
         for (int j = 0; j < N - 1; ++j) {
-          const double rj = x[j], rj1 = x[j + 1];
-          const double sj1 = std::sin(q * rj1), cj1 = std::cos(q * rj1);
-          const double A = a[j], B = b[j], C = c[j], D = d[j];
-          Fi = Fi + ((A + (B + (C + D * rj) * rj) * rj) * rj * c1 * cj -
-                     (A + (B + (C + D * rj1) * rj1) * rj1) * rj1 * c1 * cj1 +
-                     (A + (2.0 * B + (3.0 * C + 4.0 * D * rj1) * rj1) * rj1) * c2 * sj1 -
-                     (A + (2.0 * B + (3.0 * C + 4.0 * D * rj) * rj) * rj) * c2 * sj +
-                     (2.0 * B + (6.0 * C + 12.0 * D * rj1) * rj1) * c3 * cj1 -
-                     (2.0 * B + (6.0 * C + 12.0 * D * rj) * rj) * c3 * cj +
-                     (6.0 * C + 24.0 * D * rj) * c4 * sj - (6.0 * C + 24.0 * D * rj1) * c4 * sj1 +
-                     24.0 * D * c5 * (cj - cj1));
-          sj = sj1;
-          cj = cj1;
+          const double u = x[j], w = x[j + 1];
+          Fi = Fi + (a[j] / 3.0 * (std::pow(w, 3) - std::pow(u, 3)) + b[j] / 4.0 * (std::pow(w, 4) - std::pow(u, 4)) +
+                     c[j] / 5.0 * (std::pow(w, 5) - std::pow(u, 5)) + d[j] / 6.0 * (std::pow(w, 6) - std::pow(u, 6)));
         }
-        Vpp[i] = (vp + Fi) * c1 * Qpi;
+        Vpp[i] = (vp + Fi) * Qpi;
+        continue;
       }
-    };
-    unsigned nt = std::thread::hardware_concurrency();
-    if (nt == 0) nt = 1;
-    if (nt > 16) nt = 16;
-    std::vector<std::thread> th;
-    const int chunk = (np + (int)nt - 1) / (int)nt;
-    for (unsigned t = 0; t < nt; ++t) {
-      const int i0 = (int)t * chunk, i1 = std::min(np, i0 + chunk);
-      if (i0 < i1) th.emplace_back(work, i0, i1);
+      const double c1 = 1.0 / q, c2 = c1 * c1, c3 = c2 * c1, c4 = c2 * c2, c5 = c4 * c1;
+      const double vp = fc * c1 * (std::sin(q * rc) * c1 - rc * std::cos(q * rc));
+      double Fi = 0.0;
+      double sj = std::sin(q * x[0]), cj = std::cos(q * x[0]);
+      for (int j = 0; j < N - 1; ++j) {
+        const double rj = x[j], rj1 = x[j + 1];
+        const double sj1 = std::sin(q * rj1), cj1 = std::cos(q * rj1);
+        const double A = a[j], B = b[j], C = c[j], D = d[j];
+        Fi = Fi + ((A + (B + (C + D * rj) * rj) * rj) * rj * c1 * cj -
+                   (A + (B + (C + D * rj1) * rj1) * rj1) * rj1 * c1 * cj1 +
+                   (A + (2.0 * B + (3.0 * C + 4.0 * D * rj1) * rj1) * rj1) * c2 * sj1 -
+                   (A + (2.0 * B + (3.0 * C + 4.0 * D * rj) * rj) * rj) * c2 * sj +
+                   (2.0 * B + (6.0 * C + 12.0 * D * rj1) * rj1) * c3 * cj1 -
+                   (2.0 * B + (6.0 * C + 12.0 * D * rj) * rj) * c3 * cj +
+                   (6.0 * C + 24.0 * D * rj) * c4 * sj - (6.0 * C + 24.0 * D * rj1) * c4 * sj1 +
+                   24.0 * D * c5 * (cj - cj1));
+        sj = sj1;
+        cj = cj1;
+      }
+      Vpp[i] = (vp + Fi) * c1 * Qpi;
     }
-    for (auto& t : th) t.join();
+  };
+  unsigned nt = std::thread::hardware_concurrency();
+  if (nt == 0) nt = 1;
+  if (nt > 16) nt = 16;
+  std::vector<std::thread> th;
+  const int chunk = (np + (int)nt - 1) / (int)nt;
+  for (unsigned t = 0; t < nt; ++t) {
+    const int i0 = (int)t * chunk, i1 = std::min(np, i0 + chunk);
+    if (i0 < i1) th.emplace_back(work, i0, i1);
+  }
+  for (auto& t : th) t.join();
+}
+
+struct FtvSpline {
+  std::vector<double> P, A, B, C, D;
+
+  FtvSpline(const MorsePot& pot, double pmax, double hp) {
+    const FtvSetup s(pot, pmax, hp);
+    std::vector<double> Vpp;
+    ftv_sums(s, Vpp);
+    finish(s.P, Vpp);
+  }
+  FtvSpline(const std::vector<double>& p, std::vector<double> Vpp) { finish(p, std::move(Vpp)); }
+
+  void finish(const std::vector<double>& p, std::vector<double> Vpp) {
+    const double small_q = 1e-2;
+    P = p;
+    const int np = (int)P.size();
     // small-q redefinition (FT_V_spline.f90:230-264)
     std::vector<double> xs{0.0}, ys{Vpp[0]};
     for (int i = 1; i < np && xs.size() < 4; ++i) {
@@ -228,9 +262,9 @@ struct FtvSpline {
     std::vector<double> a4, b4, c4, d4;
     spline(xs, ys, 0.0, 0.0, a4, b4, c4, d4);
     for (int i = 1; i < np; ++i) {
-      const double p = P[i];
-      if (p <= small_q)
-        Vpp[i] = a4[0] + p * (b4[0] + p * (c4[0] + p * d4[0]));
+      const double q = P[i];
+      if (q <= small_q)
+        Vpp[i] = a4[0] + q * (b4[0] + q * (c4[0] + q * d4[0]));
       else
         break;
     }

@@ -60,7 +60,8 @@ module hetddft_mod
             hetddft_iterate, hetddft_particle_means, hetddft_energy, hetddft_get_state, &
             hetddft_get_field, hetddft_convolve, hetddft_profile, hetddft_profile_read, &
             hetddft_sync, hetddft_device_bytes, hetddft_graph_info, hetddft_diagnostics, &
-            hetddft_current, hetddft_checkpoint_size, hetddft_checkpoint_save, hetddft_checkpoint_load
+            hetddft_current, hetddft_checkpoint_size, hetddft_checkpoint_save, hetddft_checkpoint_load, &
+            hetddft_sample_particles, hetddft_init_kernels, hetddft_get_kernels
   public :: hetddft_message, hetddft_check, gpu_steprk, gpu_steppc, gpu_veval, gpu_energy
 
   interface
@@ -204,6 +205,27 @@ module hetddft_mod
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: out(23)
+    end function
+
+    integer(c_int) function hetddft_init_kernels(ctx, morse) bind(C, name="hetddft_init_kernels")
+      import :: c_int, c_ptr, hetddft_morse
+      type(c_ptr), value :: ctx
+      type(hetddft_morse), intent(in) :: morse
+    end function
+
+    integer(c_int) function hetddft_get_kernels(ctx, fvlj4, wcgk, kalfs, vq) bind(C, name="hetddft_get_kernels")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: fvlj4(*), wcgk(*), kalfs(*), vq(*)
+    end function
+
+    integer(c_int) function hetddft_sample_particles(ctx, denx, lo, hi, seed, max_attempts, rimp) &
+        bind(C, name="hetddft_sample_particles")
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: denx(*), lo(3), hi(3)
+      integer(c_long_long), value :: seed, max_attempts
+      real(c_double), intent(out) :: rimp(*)
     end function
 
     integer(c_int) function hetddft_checkpoint_size(ctx, bytes) bind(C, name="hetddft_checkpoint_size")

@@ -121,6 +121,9 @@ def load_library():
         "hetddft_graph_info": ([C, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "hetddft_diagnostics": ([C, _VP], ctypes.c_int),
         "hetddft_checkpoint_size": ([C, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+        "hetddft_sample_particles": ([C, _VP, _VP, _VP, ctypes.c_ulonglong, ctypes.c_longlong, _VP], ctypes.c_int),
+        "hetddft_init_kernels": ([C, ctypes.POINTER(Morse)], ctypes.c_int),
+        "hetddft_get_kernels": ([C, _VP, _VP, _VP, _VP], ctypes.c_int),
         "hetddft_checkpoint_save": ([C, _VP], ctypes.c_int),
         "hetddft_checkpoint_load": ([C, _VP], ctypes.c_int),
         "hetddft_current": ([C, _VP, _VP, _VP], ctypes.c_int),
@@ -290,6 +293,29 @@ class Instance:
         return {"norm": out[0], "cm_moments": out[1:4].copy(), "mu": out[4], "jtot": out[5:8].copy(),
                 "msx": out[8:11].copy(), "vmin": out[11:17:2].copy(), "vmax": out[12:17:2].copy(),
                 "amin": out[17:23:2].copy(), "amax": out[18:23:2].copy()}
+
+    def init_kernels(self, morse=None):
+        """Build the k-space kernels on the device (hetddft_init_kernels)."""
+        m = ctypes.byref(morse) if morse is not None else None
+        self._check(self.lib.hetddft_init_kernels(self._c, m), "init_kernels")
+
+    def get_kernels(self):
+        """The context's kernels (dict of (nz, ny, nx//2+1) arrays)."""
+        shape = (self.p.nz, self.p.ny, self.p.nx // 2 + 1)
+        k = {n: np.zeros(shape) for n in ("fvlj4", "wcgk", "kalfs", "vq")}
+        self._check(self.lib.hetddft_get_kernels(self._c, _ptr(k["fvlj4"]), _ptr(k["wcgk"]), _ptr(k["kalfs"]),
+                                                 _ptr(k["vq"])), "get_kernels")
+        return k
+
+    def sample_particles(self, denx, lo, hi, seed, max_attempts=1 << 24):
+        """N_par positions (3, N_par) drawn by rejection from denx (nz, ny, nx), as sorteo."""
+        d = np.ascontiguousarray(denx, dtype=np.float64)
+        lo = np.ascontiguousarray(lo, dtype=np.float64)
+        hi = np.ascontiguousarray(hi, dtype=np.float64)
+        out = np.zeros((3, self.p.n_par))
+        self._check(self.lib.hetddft_sample_particles(self._c, _ptr(d), _ptr(lo), _ptr(hi), int(seed),
+                                                      int(max_attempts), _ptr(out)), "sample_particles")
+        return out
 
     def checkpoint(self):
         """Full-history checkpoint image (bytes) for bit-exact resume."""

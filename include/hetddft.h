@@ -196,6 +196,35 @@ int hetddft_profile_read(hetddft_ctx* ctx, int cls, double* ms_total, long long*
 #define HETDDFT_DIAG_N 23
 int hetddft_diagnostics(hetddft_ctx* ctx, double out[HETDDFT_DIAG_N]);
 
+/* The four k-space kernels of hetddft_make_kernels computed on the device and
+ * stored in the context as hetddft_set_kernels stores them (replaces the pair
+ * make_kernels + set_kernels, with no host arrays).  The spline of the
+ * potential and the spline over momenta stay on the host (sequential and
+ * small); the momentum sums of FT_V_spline (FT_V_spline.f90:151-225, ~10^5
+ * terms per momentum) and every half-grid point of fvlj4 (fforma.f90:44-56),
+ * wcgk (initcg.f90:37-42), kalfs (DFT4He3d.f90:566-570) and Vq (rounded to
+ * FP32 as DFT4He3d.f90:533) run on the device.  The sums are added in another
+ * order than on the host, so values differ from hetddft_make_kernels in the
+ * last bits (and a Vq entry by one FP32 ulp where that moves a rounding). */
+int hetddft_init_kernels(hetddft_ctx* ctx, const hetddft_morse* morse);
+
+/* The context's kernels back in the layout of hetddft_make_kernels (FP32
+ * values widened in mixed precision); any pointer may be NULL. */
+int hetddft_get_kernels(hetddft_ctx* ctx, double* fvlj4, double* wcgk, double* kalfs, double* vq);
+
+/* Test-particle positions drawn by rejection from a density, as sorteo
+ * (sorteo.f90:1-37, called at DFT4He3d.f90:598 for starts from density files):
+ * a uniform point in the box [lo[c], hi[c]] and a uniform value in
+ * [0, max(denx)) are drawn until the value lies below the trilinear
+ * interpolation of denx at the point (interpola3denx, sorteo.f90:40-74).
+ * denx is nx*ny*nz doubles (x fastest); rimp receives (N_par, 3).  The random
+ * numbers are Philox4x64-10 keyed by (seed, 0), block (particle, attempt, 0, 0),
+ * 53-bit uniforms: the reference's random_number stream is compiler-specific,
+ * so the positions follow the reference's distribution, not its values.
+ * HETDDFT_E_STATE if a particle is not accepted within max_attempts. */
+int hetddft_sample_particles(hetddft_ctx* ctx, const double* denx, const double lo[3], const double hi[3],
+                             unsigned long long seed, long long max_attempts, double* rimp);
+
 /* Full-history checkpoint for bit-exact resume.  No counterpart in the
  * reference, whose restarts read psi and the particles back (readenc.f90:96-114,
  * mode 2) and begin again with three RK steps; this one keeps everything the

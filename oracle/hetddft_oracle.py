@@ -636,6 +636,73 @@ def interpola3(prm: Params, rimp, fx, fy, fz):
     return out
 
 
+# --------------------------------------------------------------------------
+# Particle sampling (sorteo.f90:1-37 with interpola3denx :40-74)
+# --------------------------------------------------------------------------
+_M64 = (1 << 64) - 1
+_PHILOX_M = (0xD2E7470EE14C6C93, 0xCA5A826395121157)
+_PHILOX_W = (0x9E3779B97F4A7C15, 0xBB67AE8584CAA73B)
+
+
+def philox4x64(ctr, key, rounds=10):
+    """Philox4x64-10 block (Salmon, Moraes, Dror, Shaw, SC11): four 64-bit
+    words from a 256-bit counter and a 128-bit key."""
+    c = [int(v) & _M64 for v in ctr]
+    k = [int(v) & _M64 for v in key]
+    for r in range(rounds):
+        if r > 0:
+            k = [(k[0] + _PHILOX_W[0]) & _M64, (k[1] + _PHILOX_W[1]) & _M64]
+        p0 = _PHILOX_M[0] * c[0]
+        p1 = _PHILOX_M[1] * c[2]
+        c = [(p1 >> 64) ^ c[1] ^ k[0], p1 & _M64, (p0 >> 64) ^ c[3] ^ k[1], p0 & _M64]
+    return c
+
+
+def _u53(v):
+    return (v >> 11) * (1.0 / 9007199254740992.0)
+
+
+def interpola3denx(prm: Params, denx, xyz):
+    """Trilinear interpolation of denx at a point (sorteo.f90:40-74): index
+    int((r - x1)/h + 1), zero outside [1, n-1], blend_103 weights."""
+    xg, yg, zg = prm.axis_grid("x"), prm.axis_grid("y"), prm.axis_grid("z")
+    # x0 = x(1), Dx = x(2) - x(1) from the grid arrays (sorteo.f90:10-12)
+    ax = (xyz[0] - xg[0]) / (xg[1] - xg[0]) + 1.0
+    ay = (xyz[1] - yg[0]) / (yg[1] - yg[0]) + 1.0
+    az = (xyz[2] - zg[0]) / (zg[1] - zg[0]) + 1.0
+    ix, iy, iz = int(ax), int(ay), int(az)
+    r, s, t = ax - ix, ay - iy, az - iz
+    if ix >= prm.nx or iy >= prm.ny or iz >= prm.nz or ix <= 0 or iy <= 0 or iz <= 0:
+        return 0.0
+    d = lambda i, j, k: float(denx[k - 1, j - 1, i - 1])
+    return blend_103(r, s, t, d(ix, iy, iz), d(ix, iy, iz + 1), d(ix, iy + 1, iz), d(ix, iy + 1, iz + 1),
+                     d(ix + 1, iy, iz), d(ix + 1, iy, iz + 1), d(ix + 1, iy + 1, iz), d(ix + 1, iy + 1, iz + 1))
+
+
+def sorteo(prm: Params, denx, n_par, lo, hi, seed, max_attempts=1 << 20):
+    """Rejection sampling of sorteo.f90:19-30: a uniform point in the box
+    [lo, hi] and a uniform dum in [0, max(denx)); accepted when dum < f(point).
+    The random numbers come from Philox4x64-10 keyed by (seed, 0): attempt a of
+    particle i uses the block of counter (i, a, 0, 0), words 0-2 for the point
+    and word 3 for dum, each as a 53-bit uniform.  Returns rimp (3, n_par) and
+    the attempts used per particle."""
+    denmax = float(np.max(denx))
+    out = np.zeros((3, n_par))
+    tries = np.zeros(n_par, dtype=np.int64)
+    for i in range(n_par):
+        for a in range(max_attempts):
+            w = philox4x64((i, a, 0, 0), (seed, 0))
+            xyz = [_u53(w[c]) * (hi[c] - lo[c]) + lo[c] for c in range(3)]
+            dum = _u53(w[3]) * denmax
+            if dum < interpola3denx(prm, denx, xyz):
+                out[:, i] = xyz
+                tries[i] = a + 1
+                break
+        else:
+            raise RuntimeError(f"particle {i} not accepted in {max_attempts} attempts")
+    return out, tries
+
+
 def diagnostics(prm: Params, psi, pot4, denx, vimp, aimp):
     """Per-step diagnostics: the He count sum(den) dxyz (DFT4He3d.f90:884),
     the centre-of-mass moments sum(den x_c) dxyz of r_cm (r_cm.f90:23-58; the
