@@ -58,6 +58,37 @@ struct FftPlan {
 struct Stencil13 {
   double w[13];
 };
+// The same weights rounded to FP32, for the stencils of the mixed-precision
+// potential side (FP32 fields, FP32 arithmetic).
+struct Stencil13f {
+  float w[13];
+};
+template <class A>
+struct StencilOf {
+  using type = Stencil13;
+};
+template <>
+struct StencilOf<float> {
+  using type = Stencil13f;
+};
+template <class A>
+using StencilA = typename StencilOf<A>::type;
+
+inline Stencil13f to_f32(const Stencil13& s) {
+  Stencil13f r;
+  for (int m = 0; m < 13; ++m) r.w[m] = (float)s.w[m];
+  return r;
+}
+template <class A>
+inline StencilA<A> stencil_as(const Stencil13& s);
+template <>
+inline Stencil13 stencil_as<double>(const Stencil13& s) {
+  return s;
+}
+template <>
+inline Stencil13f stencil_as<float>(const Stencil13& s) {
+  return to_f32(s);
+}
 
 struct Grid {
   int nx, ny, nz;

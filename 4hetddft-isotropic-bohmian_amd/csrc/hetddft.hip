@@ -846,7 +846,8 @@ int veval(hetddft_ctx* c) {
     with_tile(c->tile, [&](auto sh) {
       using SH = decltype(sh);
       PLAUNCH(psg, (k_grad3_25<T, SH>), grid25<SH>(c), dim3(SH::TX, SH::TY), 0, g, c->zc, c->xcd,
-              (const T*)F(hetddft_ctx::F_DEN), c->d1x, c->d1y, c->d1z, F(hetddft_ctx::F_DX),
+              (const T*)F(hetddft_ctx::F_DEN), stencil_as<T>(c->d1x), stencil_as<T>(c->d1y),
+              stencil_as<T>(c->d1z), F(hetddft_ctx::F_DX),
               F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ));
       return 0;
     });
@@ -966,19 +967,19 @@ int veval(hetddft_ctx* c) {
   pa.den_m = c->p.den_m;
   pa.otc = otc ? 1 : 0;
   if (c->st25) {
-    Pot25 p5{};
+    Pot25T<T> p5{};
     p5.g = g;
-    p5.d1x = c->d1x;
-    p5.d1y = c->d1y;
-    p5.d1z = c->d1z;
-    p5.a0 = pa.a0;
-    p5.a1 = pa.a1;
-    p5.den0s = pa.den0s;
-    p5.h2o2m4 = pa.h2o2m4;
-    p5.alphas = pa.alphas;
-    p5.C = pa.C;
-    p5.beta = pa.beta;
-    p5.den_m = pa.den_m;
+    p5.d1x = stencil_as<T>(c->d1x);
+    p5.d1y = stencil_as<T>(c->d1y);
+    p5.d1z = stencil_as<T>(c->d1z);
+    p5.a0 = (T)pa.a0;
+    p5.a1 = (T)pa.a1;
+    p5.den0s = (T)pa.den0s;
+    p5.h2o2m4 = (T)pa.h2o2m4;
+    p5.alphas = (T)pa.alphas;
+    p5.C = (T)pa.C;
+    p5.beta = (T)pa.beta;
+    p5.den_m = (T)pa.den_m;
     p5.otc = pa.otc;
     p5.zc = c->zc;
     p5.xcd = c->xcd;

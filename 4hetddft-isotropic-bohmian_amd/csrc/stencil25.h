@@ -24,8 +24,10 @@ struct Tile25 {
 };
 
 __device__ inline double vzero(double) { return 0.0; }
+__device__ inline float vzero(float) { return 0.0f; }
 __device__ inline c64 vzero(c64) { return mk(0.0, 0.0); }
 __device__ inline double vmadd(double acc, double w, double v) { return acc + w * v; }
+__device__ inline float vmadd(float acc, float w, float v) { return acc + w * v; }
 __device__ inline c64 vmadd(c64 acc, double w, c64 v) { return mk(acc.x + w * v.x, acc.y + w * v.y); }
 
 template <class V, class R>
@@ -103,7 +105,8 @@ struct ZQueue {
     for (int k = 0; k < 12; ++k) q[k] = q[k + 1];
   }
   __device__ inline V centre() const { return q[6]; }
-  __device__ inline V dz(const Stencil13& w) const {
+  template <class W>
+  __device__ inline V dz(const W& w) const {
     V acc = vzero(V());
 #pragma unroll
     for (int m = 0; m < 13; ++m) acc = vmadd(acc, w.w[m], q[m]);
@@ -147,8 +150,8 @@ __device__ inline void plane_store(V* tile, const PlaneRegs<SH, V>& pr, const Co
   }
 }
 
-template <class SH, class V>
-__device__ inline V tile_dx(const V* tile, const Col& c, const Stencil13& w) {
+template <class SH, class V, class W>
+__device__ inline V tile_dx(const V* tile, const Col& c, const W& w) {
   const V* row = tile + (c.ty + 6) * SH::LX + c.tx;
   V acc = vzero(V());
 #pragma unroll
@@ -156,8 +159,8 @@ __device__ inline V tile_dx(const V* tile, const Col& c, const Stencil13& w) {
   return acc;
 }
 
-template <class SH, class V>
-__device__ inline V tile_dy(const V* tile, const Col& c, const Stencil13& w) {
+template <class SH, class V, class W>
+__device__ inline V tile_dy(const V* tile, const Col& c, const W& w) {
   const V* colp = tile + c.ty * SH::LX + c.tx + 6;
   V acc = vzero(V());
 #pragma unroll
@@ -368,19 +371,22 @@ __global__ void __launch_bounds__(SH::N) k_rk_stage25(Ham25 a, const c64* __rest
 }
 
 // grad(den) with the first-derivative stencil (derden.f90:26-28),
-// software-pipelined like the predictor.
+// software-pipelined like the predictor.  The arithmetic type is the field
+// type: FP64 in the FP64 path, FP32 (weights rounded to FP32) on the FP32
+// potential side of the mixed path.
 template <class R, class SH>
-__global__ void __launch_bounds__(SH::N) k_grad3_25(Grid g, int zc, int xcd, const R* __restrict__ f, Stencil13 wx,
-                                                         Stencil13 wy, Stencil13 wz, R* __restrict__ fx,
-                                                         R* __restrict__ fy, R* __restrict__ fz) {
-  __shared__ double tile[SH::LX * SH::LY];
+__global__ void __launch_bounds__(SH::N) k_grad3_25(Grid g, int zc, int xcd, const R* __restrict__ f,
+                                                         StencilA<R> wx, StencilA<R> wy, StencilA<R> wz,
+                                                         R* __restrict__ fx, R* __restrict__ fy,
+                                                         R* __restrict__ fz) {
+  __shared__ R tile[SH::LX * SH::LY];
   const Col c = col_of<SH>(g, xcd);
   const int z0 = c.bz * zc;
   const int z1 = min(z0 + zc, g.nz);
-  ZQueue<double, R> q;
+  ZQueue<R, R> q;
   q.init(f, g, c, z0);
-  PlaneRegs<SH, double> pr;
-  double qn = 0.0;
+  PlaneRegs<SH, R> pr;
+  R qn = R(0);
   auto prefetch = [&](int z) {
     qn = q.next(f, g, c, z);
     plane_load<SH>(pr, f, g, c, z);
@@ -409,27 +415,38 @@ namespace hetd {
 // ---- alpha_s divergence + pot4 on the engine ---------------------------
 // term_alfa.f90:85-113 and poten.f90:80-124: each product g_c = falfs*int_c
 // is differentiated along its own axis only, so g_x needs an x-halo row
-// tile, g_y a y-halo column tile and g_z the z queue.
-struct Pot25 {
+// tile, g_y a y-halo column tile and g_z the z queue.  A is the arithmetic
+// type: FP64 in the FP64 path (the reference's operation order), FP32 on the
+// FP32 potential side of the mixed path.
+template <class A>
+struct Pot25T {
   Grid g;
-  Stencil13 d1x, d1y, d1z;
-  double a0, a1;                 // cp4/2, cpp4/3
-  double den0s, h2o2m4, alphas;
-  double C, beta, den_m;
+  StencilA<A> d1x, d1y, d1z;
+  A a0, a1;                 // cp4/2, cpp4/3
+  A den0s, h2o2m4, alphas;
+  A C, beta, den_m;
   int otc;
   int zc;
   int xcd;
 };
 
-template <class R>
-__device__ inline double prod_at(const R* __restrict__ a, const R* __restrict__ b, const Grid& g, int x, int y,
-                                 int z) {
-  const int j = gidx(g, x, y, z);
-  return (double)a[j] * (double)b[j];
+// Solid penalty C (1 + t + beta rho (1 - t^2)), t = tanh(beta (rho - rho_m))
+// (poten.f90:114-118).  FP64: the reference's form.  FP32: with
+// s = 1 + t = 2 / (1 + exp(-2x)) and 1 - t^2 = s (2 - s), which keeps the
+// relative accuracy of s where t is close to -1 (1 + t would lose it all to
+// the rounding of t, and C is large).
+__device__ inline double penalty(double dn, double C, double beta, double den_m) {
+  const double t = tanh(beta * (dn - den_m));
+  return C * (1.0 + t + beta * dn * (1.0 - t * t));
+}
+__device__ inline float penalty(float dn, float C, float beta, float den_m) {
+  const float x = beta * (dn - den_m);
+  const float sg = 2.0f / (1.0f + __expf(-2.0f * x));
+  return C * (sg + beta * dn * (sg * (2.0f - sg)));
 }
 
 template <class R, class SH>
-__global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25 a, const R* __restrict__ den,
+__global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25T<R> a, const R* __restrict__ den,
                                                             const R* __restrict__ dencg,
                                                             const R* __restrict__ delj4,
                                                             const R* __restrict__ sto1c,
@@ -439,8 +456,9 @@ __global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25 a, const R* __restr
                                                             const R* __restrict__ inty,
                                                             const R* __restrict__ intz,
                                                             const R* __restrict__ uaconv, R* __restrict__ pot4) {
-  __shared__ double tx_tile[SH::TY * SH::LX];    // g_x, rows of this tile with x halo
-  __shared__ double ty_tile[SH::LY * SH::TX];    // g_y, columns of this tile with y halo
+  using A = R;
+  __shared__ A tx_tile[SH::TY * SH::LX];    // g_x, rows of this tile with x halo
+  __shared__ A ty_tile[SH::LY * SH::TX];    // g_y, columns of this tile with y halo
   constexpr int NX = (SH::TY * SH::LX + SH::N - 1) / SH::N;
   constexpr int NY = (SH::LY * SH::TX + SH::N - 1) / SH::N;
   const Grid& g = a.g;
@@ -448,19 +466,20 @@ __global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25 a, const R* __restr
   const int z0 = c.bz * a.zc;
   const int z1 = min(z0 + a.zc, g.nz);
   const int xc = min(c.x, g.nx - 1), yc = min(c.y, g.ny - 1);
-  double q[13];
+  A q[13];
   if (a.otc) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
       const int z = z0 - 6 + k;
-      const double t = prod_at(falfs, intz, g, xc, yc, min(max(z, 0), g.nz - 1));
-      q[k] = (c.in && z >= 0 && z < g.nz) ? t : 0.0;
+      const int j = gidx(g, xc, yc, min(max(z, 0), g.nz - 1));
+      const A t = (A)falfs[j] * (A)intz[j];
+      q[k] = (c.in && z >= 0 && z < g.nz) ? t : A(0);
     }
   }
   const int x0 = c.bx * SH::TX - 6, y0 = c.by * SH::TY - 6;
   // Software pipeline: the raw operands of plane z+1 (tile products, queue
   // entry, centre values) are loaded while plane z is computed; products
-  // are formed when the plane is used, as (double)a * (double)b.
+  // are formed when the plane is used, as (A)a * (A)b.
   R fxn[NX], ixn[NX], fyn[NY], iyn[NY], fzn = R(0), izn = R(0);
   bool okx[NX], oky[NY], okz = false;
   R dnn = R(0), dcgn = R(0), dljn = R(0), s1n = R(0), uin = R(0), uan = R(0);
@@ -503,42 +522,40 @@ __global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25 a, const R* __restr
   if (z0 < z1) prefetch(z0);
   for (int z = z0; z < z1; ++z) {
     if (a.otc) {
-      q[12] = okz ? (double)fzn * (double)izn : 0.0;
+      q[12] = okz ? (A)fzn * (A)izn : A(0);
 #pragma unroll
       for (int k = 0; k < NX; ++k) {
         const int e = c.tid + k * SH::N;
-        if (e < SH::TY * SH::LX) tx_tile[e] = okx[k] ? (double)fxn[k] * (double)ixn[k] : 0.0;
+        if (e < SH::TY * SH::LX) tx_tile[e] = okx[k] ? (A)fxn[k] * (A)ixn[k] : A(0);
       }
 #pragma unroll
       for (int k = 0; k < NY; ++k) {
         const int e = c.tid + k * SH::N;
-        if (e < SH::LY * SH::TX) ty_tile[e] = oky[k] ? (double)fyn[k] * (double)iyn[k] : 0.0;
+        if (e < SH::LY * SH::TX) ty_tile[e] = oky[k] ? (A)fyn[k] * (A)iyn[k] : A(0);
       }
     }
-    const double dn = (double)dnn, dcg = (double)dcgn, dlj = (double)dljn, s1 = (double)s1n;
-    const double ui = (double)uin, uac = (double)uan;
+    const A dn = (A)dnn, dcg = (A)dcgn, dlj = (A)dljn, s1 = (A)s1n;
+    const A ui = (A)uin, uac = (A)uan;
     __syncthreads();
     if (z + 1 < z1) prefetch(z + 1);
     if (c.in) {
       const int i = gidx(g, c.x, c.y, z);
-      double p4 = (dlj + dcg * dcg * (a.a0 + a.a1 * dcg)) + s1;
+      A p4 = (dlj + dcg * dcg * (a.a0 + a.a1 * dcg)) + s1;
       if (a.otc) {
-        double s4 = 0.0, s5 = 0.0, s6 = 0.0;
-        const double* rx = tx_tile + c.ty * SH::LX + c.tx;
-        const double* cy = ty_tile + c.tx + c.ty * SH::TX;
+        A s4 = A(0), s5 = A(0), s6 = A(0);
+        const A* rx = tx_tile + c.ty * SH::LX + c.tx;
+        const A* cy = ty_tile + c.tx + c.ty * SH::TX;
 #pragma unroll
         for (int m = 0; m < 13; ++m) s4 = s4 + a.d1x.w[m] * rx[m];
 #pragma unroll
         for (int m = 0; m < 13; ++m) s5 = s5 + a.d1y.w[m] * cy[m * SH::TX];
 #pragma unroll
         for (int m = 0; m < 13; ++m) s6 = s6 + a.d1z.w[m] * q[m];
-        const double ua = uac / a.den0s;
-        const double ual = (((ua + s4) + s5) + s6) * a.h2o2m4 * a.alphas;
+        const A ua = uac / a.den0s;
+        const A ual = (((ua + s4) + s5) + s6) * a.h2o2m4 * a.alphas;
         p4 = p4 + ual;
       }
-      const double t = tanh(a.beta * (dn - a.den_m));
-      const double pen = a.C * (1.0 + t + a.beta * dn * (1.0 - t * t));
-      pot4[i] = (R)((p4 + ui) + pen);
+      pot4[i] = (R)((p4 + ui) + penalty(dn, a.C, a.beta, a.den_m));
     }
     __syncthreads();
     if (a.otc) {
