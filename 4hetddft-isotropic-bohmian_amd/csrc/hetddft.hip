@@ -15,6 +15,8 @@
 #include "stencil25.h"
 
 #include <algorithm>
+#include <chrono>
+#include <mutex>
 #include <cmath>
 #include <cstdarg>
 #include <cstdlib>
@@ -100,6 +102,12 @@ struct FftKernelsT {
   void (*zp)(ZArgs<T>, int, Grid, const cplx<T>*) = nullptr;
   size_t lds_zp = 0;
   int typ = 0;
+  // register-resident z pass at ky-tile widths 8, 16, 32 (same arithmetic
+  // per column, so the same results); zp/lds_zp/typ hold the selected one
+  int nzv = 0;
+  void (*zpv[3])(ZArgs<T>, int, Grid, const cplx<T>*) = {};
+  size_t lds_zpv[3] = {};
+  int typv[3] = {};
   // whole-plane x+y kernels (when a plane fits in LDS) and the wide z pass
   void (*xyf)(XfArgs<T>, Grid, const cplx<T>*, const cplx<T>*, const cplx<T>*) = nullptr;
   void (*xyi)(XiArgs<T, T>, Grid, const cplx<T>*, const cplx<T>*, const cplx<T>*) = nullptr;
@@ -137,7 +145,7 @@ struct hetddft_ctx {
   // field kernels on st.  Every entry point forks st2 from st on entry and
   // joins it back before returning, so callers see one ordered stream.
   hipStream_t st2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_chain = nullptr, ev_b3 = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_chain = nullptr, ev_b3 = nullptr, ev_join = nullptr, ev_force = nullptr;
   std::string err;
   long long bytes = 0;
   bool kernels_set = false, started = false;
@@ -147,9 +155,14 @@ struct hetddft_ctx {
 
   Stencil13 d1x{}, d1y{}, d1z{}, d2x{}, d2y{}, d2z{};
   FftPlan pl_xh{}, pl_y{}, pl_z{};
-  int zc = 16;  // z planes per workgroup of the stencil engine
+  int zc = 16;  // default z planes per workgroup of the stencil engine
   int xcd = 1;  // XCD-grouped tile order in the stencil engine
-  int tile = 0; // tile shape of the stencil engine (with_tile)
+  int tile = 0; // default tile shape of the stencil engine (with_tile)
+  // launch shape of each stencil kernel class (HETDDFT_TUNE_*): tile shape
+  // and z planes per workgroup; set from tile/zc at create, changed by
+  // hetddft_set_tuning / hetddft_autotune.  Results do not depend on them.
+  int tile_k[HETDDFT_TUNE_NSTENCIL] = {};
+  int zc_k[HETDDFT_TUNE_NSTENCIL] = {};
   int cus = 256;      // compute units of the device
   int zp_per_cu = 4;  // resident workgroups per CU of the persistent z pass
   bool st25 = true;  // z-marching stencil engine (false: direct neighbour loads)
@@ -434,21 +447,22 @@ void pick_z(FftKernels<T>& k, int n) {
     if (!(r && std::atoi(r) == 0)) {
       constexpr int R1 = N == 256 ? 16 : 8;
       constexpr int R2 = N / R1;
+      k.nzv = 3;
+      k.zpv[0] = k_fft_zreg<T, N, 8, R1, R2>;
+      k.lds_zpv[0] = zreg_bytes<T, N, 8, R1, R2>();
+      k.typv[0] = 8;
+      k.zpv[1] = k_fft_zreg<T, N, 16, R1, R2>;
+      k.lds_zpv[1] = zreg_bytes<T, N, 16, R1, R2>();
+      k.typv[1] = 16;
+      k.zpv[2] = k_fft_zreg<T, N, 32, R1, R2>;
+      k.lds_zpv[2] = zreg_bytes<T, N, 32, R1, R2>();
+      k.typv[2] = 32;
       const char* t = std::getenv("HETDDFT_ZRTY");
       const int want = t ? std::atoi(t) : 32;
-      if (want == 8) {
-        k.zp = k_fft_zreg<T, N, 8, R1, R2>;
-        k.lds_zp = zreg_bytes<T, N, 8, R1, R2>();
-        k.typ = 8;
-      } else if (want == 32) {
-        k.zp = k_fft_zreg<T, N, 32, R1, R2>;
-        k.lds_zp = zreg_bytes<T, N, 32, R1, R2>();
-        k.typ = 32;
-      } else {
-        k.zp = k_fft_zreg<T, N, 16, R1, R2>;
-        k.lds_zp = zreg_bytes<T, N, 16, R1, R2>();
-        k.typ = 16;
-      }
+      const int v = want == 8 ? 0 : want == 16 ? 1 : 2;
+      k.zp = k.zpv[v];
+      k.lds_zp = k.lds_zpv[v];
+      k.typ = k.typv[v];
       return;
     }
   }
@@ -590,6 +604,8 @@ int fft_alloc_tables(hetddft_ctx* c, FftSet& fs) {
   }
   if (k.zw) HIPCHK(c, hipFuncSetAttribute((const void*)k.zw, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   if (k.zp) HIPCHK(c, hipFuncSetAttribute((const void*)k.zp, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  for (int v = 0; v < k.nzv; ++v)
+    HIPCHK(c, hipFuncSetAttribute((const void*)k.zpv[v], hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   return HETDDFT_OK;
 }
 
@@ -734,11 +750,21 @@ auto with_tile(int tile, Fn&& fn) -> decltype(fn(TileA{})) {
   }
 }
 
-// Workgroups of the engine: TX x TY columns, c->zc planes each.
+// Workgroups of the engine: TX x TY columns, zc planes each.
 template <class SH>
-dim3 grid25(const hetddft_ctx* c) {
-  return dim3((unsigned)((c->g.nx + SH::TX - 1) / SH::TX), (unsigned)((c->g.ny + SH::TY - 1) / SH::TY),
-              (unsigned)((c->g.nz + c->zc - 1) / c->zc));
+dim3 grid25z(const Grid& g, int zc) {
+  return dim3((unsigned)((g.nx + SH::TX - 1) / SH::TX), (unsigned)((g.ny + SH::TY - 1) / SH::TY),
+              (unsigned)((g.nz + zc - 1) / zc));
+}
+// ... for stencil class k (HETDDFT_TUNE_*)
+template <class SH>
+dim3 grid25(const hetddft_ctx* c, int k) {
+  return grid25z<SH>(c->g, c->zc_k[k]);
+}
+// Workgroup count of class k's launch.
+long long blocks25(const hetddft_ctx* c, int k) {
+  const dim3 d = with_tile(c->tile_k[k], [&](auto sh) { return grid25<decltype(sh)>(c, k); });
+  return (long long)d.x * d.y * d.z;
 }
 
 BoxArgs box_args(const hetddft_ctx* c) {
@@ -843,9 +869,10 @@ int veval(hetddft_ctx* c) {
   {
   ProfScope psg(c, HETDDFT_PROF_GRAD, 4.0 * (double)g.n * sizeof(T));
   if (c->st25)
-    with_tile(c->tile, [&](auto sh) {
+    with_tile(c->tile_k[HETDDFT_TUNE_GRAD], [&](auto sh) {
       using SH = decltype(sh);
-      PLAUNCH(psg, (k_grad3_25<T, SH>), grid25<SH>(c), dim3(SH::TX, SH::TY), 0, g, c->zc, c->xcd,
+      PLAUNCH(psg, (k_grad3_25<T, SH>), grid25<SH>(c, HETDDFT_TUNE_GRAD), dim3(SH::TX, SH::TY), 0, g,
+              c->zc_k[HETDDFT_TUNE_GRAD], c->xcd,
               (const T*)F(hetddft_ctx::F_DEN), stencil_as<T>(c->d1x), stencil_as<T>(c->d1y),
               stencil_as<T>(c->d1z), F(hetddft_ctx::F_DX),
               F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ));
@@ -934,21 +961,24 @@ int veval(hetddft_ctx* c) {
     }
     RET(conv_batch<T>(c, c->fs, xa, nin, jobs));
   }
-  // ---- forceimp gather (potenimp.f90:52-71)
+  // ---- forceimp gather (potenimp.f90:52-71), on the side stream beside
+  // the pot4 assembly below: the two read disjoint fields and write aimp and
+  // pot4 respectively; st waits for the gather after the assembly.
   if (c->p.n_par > 0) {
+    RET(fork_side(c));
     GatherArgs ga{};
     ga.b = b;
     ga.d1[0] = c->d1x;
     ga.d1[1] = c->d1y;
     ga.d1[2] = c->d1z;
     ga.m_imp = c->m_imp;
-    SLAUNCH((k_force_corners<T>), dim3(blocks_for(24LL * c->p.n_par)), dim3(kBlock), 0, ga,
-                       c->rimp, (const T*)F(hetddft_ctx::F_FCX), (const T*)F(hetddft_ctx::F_FCY),
-                       (const T*)F(hetddft_ctx::F_FCZ), (const T*)cur, (const T*)psix,
-                       (const T*)F(hetddft_ctx::F_HPSIX), c->fcorner);
+    S2LAUNCH((k_force_corners<T>), dim3(blocks_for(24LL * c->p.n_par)), dim3(kBlock), 0, ga,
+             c->rimp, (const T*)F(hetddft_ctx::F_FCX), (const T*)F(hetddft_ctx::F_FCY),
+             (const T*)F(hetddft_ctx::F_FCZ), (const T*)cur, (const T*)psix,
+             (const T*)F(hetddft_ctx::F_HPSIX), c->fcorner);
     LAUNCHCHK(c);
-    SLAUNCH(k_blend, dim3(blocks_for(3LL * c->p.n_par)), dim3(kBlock), 0, b, c->rimp,
-                       (const double*)c->fcorner, c->aimp);
+    S2LAUNCH(k_blend, dim3(blocks_for(3LL * c->p.n_par)), dim3(kBlock), 0, b, c->rimp,
+             (const double*)c->fcorner, c->aimp);
     LAUNCHCHK(c);
   }
   // pot4 assembly with the alpha_s divergence (term_alfa.f90:85-113, poten.f90:80-124)
@@ -981,12 +1011,12 @@ int veval(hetddft_ctx* c) {
     p5.beta = (T)pa.beta;
     p5.den_m = (T)pa.den_m;
     p5.otc = pa.otc;
-    p5.zc = c->zc;
+    p5.zc = c->zc_k[HETDDFT_TUNE_POT];
     p5.xcd = c->xcd;
     ProfScope ps(c, HETDDFT_PROF_POT, (otc ? 11.0 : 6.0) * (double)g.n * sizeof(T));
-    with_tile(c->tile, [&](auto sh) {
+    with_tile(c->tile_k[HETDDFT_TUNE_POT], [&](auto sh) {
       using SH = decltype(sh);
-      PLAUNCH(ps, (k_pot_final25<T, SH>), grid25<SH>(c), dim3(SH::TX, SH::TY), 0, p5,
+      PLAUNCH(ps, (k_pot_final25<T, SH>), grid25<SH>(c, HETDDFT_TUNE_POT), dim3(SH::TX, SH::TY), 0, p5,
               (const T*)F(hetddft_ctx::F_DEN), (const T*)F(hetddft_ctx::F_DENCG),
               (const T*)F(hetddft_ctx::F_DELJ4), (const T*)F(hetddft_ctx::F_STO1),
               (const T*)F(hetddft_ctx::F_UIMP), (const T*)F(hetddft_ctx::F_FALFS),
@@ -1003,6 +1033,10 @@ int veval(hetddft_ctx* c) {
                        (const T*)F(hetddft_ctx::F_IZ), (const T*)F(hetddft_ctx::F_UACONV), F(hetddft_ctx::F_POT4));
   }
   LAUNCHCHK(c);
+  if (c->p.n_par > 0) {
+    RET(join_side(c, c->ev_force));
+    RET(prof_mark(c));
+  }
   return HETDDFT_OK;
 }
 
@@ -1020,6 +1054,12 @@ Ham25 ham_args(const hetddft_ctx* c) {
   a.xcd = c->xcd;
   return a;
 }
+// ... with the z chunk of stencil class k
+Ham25 ham_args(const hetddft_ctx* c, int k) {
+  Ham25 a = ham_args(c);
+  a.zc = c->zc_k[k];
+  return a;
+}
 
 template <class T>
 int step_rk_t(hetddft_ctx* c, double dt) {
@@ -1028,7 +1068,7 @@ int step_rk_t(hetddft_ctx* c, double dt) {
   const double brun[4] = {2.0, 1.0, 1.0, 2.0};
   const double crun[4] = {0.5, 1.0 - 1.0 / s2, 1.0 + 1.0 / s2, 0.5};
   const Grid& g = c->g;
-  const Ham25 ha = ham_args(c);
+  const Ham25 ha = ham_args(c, HETDDFT_TUNE_RK);
   const int n3 = 3 * c->p.n_par;
   for (int j = 0; j < 4; ++j) {
     c64* out;
@@ -1063,9 +1103,9 @@ int step_rk_t(hetddft_ctx* c, double dt) {
       out = c->spare;
     }
     if (c->st25)
-      with_tile(c->tile, [&](auto sh) {
+      with_tile(c->tile_k[HETDDFT_TUNE_RK], [&](auto sh) {
         using SH = decltype(sh);
-        SLAUNCH((k_rk_stage25<T, SH>), grid25<SH>(c), dim3(SH::TX, SH::TY), 0, ha,
+        SLAUNCH((k_rk_stage25<T, SH>), grid25<SH>(c, HETDDFT_TUNE_RK), dim3(SH::TX, SH::TY), 0, ha,
                            (const c64*)c->psi, (const T*)c->fld[hetddft_ctx::F_POT4], c->q, out,
                            (T*)c->fld[hetddft_ctx::F_DEN], h1o, arun[j], brun[j], crun[j], dt);
         return 0;
@@ -1098,15 +1138,16 @@ template <class T>
 int step_pc_t(hetddft_ctx* c, double dt) {
   const double c112 = 112.0 / 121.0, c9 = 9.0 / 121.0, c4o3 = 4.0 / 3.0;
   const Grid& g = c->g;
-  const Ham25 ha = ham_args(c);
+  const Ham25 hp = ham_args(c, HETDDFT_TUNE_PCP);
+  const Ham25 hc = ham_args(c, HETDDFT_TUNE_PCC);
   int* P = c->ioldp;
   int* H = c->ioldh;
   int* R = c->ioldr;
   int* V = c->ioldv;
   int* A = c->iolda;
   const int n3 = 3 * c->p.n_par;
-  const dim3 g25 = with_tile(c->tile, [&](auto sh) { return grid25<decltype(sh)>(c); });
-  const unsigned nb = c->st25 ? g25.x * g25.y * g25.z : blocks_for(g.n);
+  // errHe partial sums: one per workgroup of the corrector
+  const unsigned nb = c->st25 ? (unsigned)blocks25(c, HETDDFT_TUNE_PCC) : blocks_for(g.n);
   // Particle predictor (steppc.f90:64-87), then the density chain of the
   // predicted positions on the side stream, beside the psi predictor.
   if (n3 > 0) {
@@ -1125,15 +1166,16 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   {
   ProfScope pspred(c, HETDDFT_PROF_PCP, (double)g.n * (16.0 * 8 + 2.0 * sizeof(T)));
   if (c->st25)
-    with_tile(c->tile, [&](auto sh) {
+    with_tile(c->tile_k[HETDDFT_TUNE_PCP], [&](auto sh) {
       using SH = decltype(sh);
-      PLAUNCH(pspred, (k_pc_predict25<T, SH>), g25, dim3(SH::TX, SH::TY), 0, ha, (const c64*)x,
+      PLAUNCH(pspred, (k_pc_predict25<T, SH>), grid25<SH>(c, HETDDFT_TUNE_PCP), dim3(SH::TX, SH::TY), 0, hp,
+              (const c64*)x,
               (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]], c->hold[H[1]], c->pc,
               (T*)c->fld[hetddft_ctx::F_DEN], c4o3 * dt, c112);
       return 0;
     });
   else
-    PLAUNCH(pspred, (k_pc_predict<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, ham_direct(ha),
+    PLAUNCH(pspred, (k_pc_predict<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, ham_direct(hp),
                        (const c64*)x, (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]],
                        c->hold[H[1]], c->pc, (T*)c->fld[hetddft_ctx::F_DEN], c4o3 * dt, c112);
   }
@@ -1176,16 +1218,17 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   {
   ProfScope pscorr(c, HETDDFT_PROF_PCC, (double)g.n * (16.0 * 10 + 2.0 * sizeof(T)));
   if (c->st25)
-    with_tile(c->tile, [&](auto sh) {
+    with_tile(c->tile_k[HETDDFT_TUNE_PCC], [&](auto sh) {
       using SH = decltype(sh);
-      PLAUNCH(pscorr, (k_pc_correct25<T, SH>), g25, dim3(SH::TX, SH::TY), 0, ha, (const c64*)c->psi,
+      PLAUNCH(pscorr, (k_pc_correct25<T, SH>), grid25<SH>(c, HETDDFT_TUNE_PCC), dim3(SH::TX, SH::TY), 0, hc,
+              (const c64*)c->psi,
               (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
               (const c64*)c->hold[H[0]], (const c64*)c->hold[H[1]], c->pc, c->spare,
               (T*)c->fld[hetddft_ctx::F_DEN], c->part, 3.0 * dt, c9);
       return 0;
     });
   else
-    PLAUNCH(pscorr, (k_pc_correct<T>), dim3(nb), dim3(kBlock), 0, ham_direct(ha), (const c64*)c->psi,
+    PLAUNCH(pscorr, (k_pc_correct<T>), dim3(nb), dim3(kBlock), 0, ham_direct(hc), (const c64*)c->psi,
                        (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
                        (const c64*)c->hold[H[0]], (const c64*)c->hold[H[1]], c->pc, c->spare,
                        (T*)c->fld[hetddft_ctx::F_DEN], c->part, 3.0 * dt, c9);
@@ -1547,6 +1590,93 @@ DiagArgs diag_args(const hetddft_ctx* c) {
   return a;
 }
 
+// ---- launch-shape tuning ----------------------------------------------
+template <class T>
+int fft_z_tile_of(hetddft_ctx* c) {
+  const FftKernels<T>& k = fft_of<T>(c);
+  return k.nzv > 0 ? k.typ : 0;
+}
+int fft_z_tile(hetddft_ctx* c) { return c->mixed ? fft_z_tile_of<float>(c) : fft_z_tile_of<double>(c); }
+
+// Selects the register-resident z transform with ky tile width w; false if
+// this grid offers no such kernel.
+template <class T>
+bool select_fft_z_tile(hetddft_ctx* c, int w) {
+  FftKernels<T>& k = fft_of<T>(c);
+  for (int v = 0; v < k.nzv; ++v)
+    if (k.typv[v] == w) {
+      k.zp = k.zpv[v];
+      k.lds_zp = k.lds_zpv[v];
+      k.typ = k.typv[v];
+      return true;
+    }
+  return false;
+}
+
+void tuning_of(hetddft_ctx* c, hetddft_tuning* t) {
+  for (int k = 0; k < HETDDFT_TUNE_NSTENCIL; ++k) {
+    t->tile[k] = c->tile_k[k];
+    t->zc[k] = c->zc_k[k];
+  }
+  t->fft_z_tile = fft_z_tile(c);
+}
+
+int apply_tuning(hetddft_ctx* c, const hetddft_tuning* t) {
+  for (int k = 0; k < HETDDFT_TUNE_NSTENCIL; ++k)
+    if (t->tile[k] < 0 || t->tile[k] > 3 || t->zc[k] < 1)
+      return fail(c, HETDDFT_E_ARG, "tuning: class %d has tile %d, zc %d", k, t->tile[k], t->zc[k]);
+  {
+    // the corrector's errHe partials (one per workgroup) must fit the
+    // partial-sum buffer
+    const int k = HETDDFT_TUNE_PCC;
+    const dim3 d = with_tile(t->tile[k], [&](auto sh) {
+      return grid25z<decltype(sh)>(c->g, std::min(t->zc[k], c->g.nz));
+    });
+    if ((long long)d.x * d.y * d.z > (long long)kDiagSlots * blocks_for(c->g.n))
+      return fail(c, HETDDFT_E_ARG, "tuning: corrector grid of %u workgroups is too large", d.x * d.y * d.z);
+  }
+  if (t->fft_z_tile != 0) {
+    const bool ok = c->mixed ? select_fft_z_tile<float>(c, t->fft_z_tile) : select_fft_z_tile<double>(c, t->fft_z_tile);
+    if (!ok) return fail(c, HETDDFT_E_ARG, "tuning: no z transform with ky tile %d on this grid", t->fft_z_tile);
+  }
+  for (int k = 0; k < HETDDFT_TUNE_NSTENCIL; ++k) {
+    c->tile_k[k] = t->tile[k];
+    c->zc_k[k] = std::min(t->zc[k], c->g.nz);
+  }
+  clear_graphs(c);  // captured steps hold the old launch shapes
+  return HETDDFT_OK;
+}
+
+// z chunks giving about 512, 768, 1024 and 2048 workgroups for a tile shape
+// (2..32 planes each), without repeats.
+std::vector<int> zc_candidates(const hetddft_ctx* c, int tile) {
+  const long long cols = with_tile(tile, [&](auto sh) {
+    using SH = decltype(sh);
+    return (long long)((c->g.nx + SH::TX - 1) / SH::TX) * ((c->g.ny + SH::TY - 1) / SH::TY);
+  });
+  std::vector<int> out;
+  for (long long target : {512LL, 768LL, 1024LL, 2048LL}) {
+    long long zc = (cols * c->g.nz + target - 1) / target;
+    zc = std::max(2LL, std::min(32LL, std::min<long long>(zc, c->g.nz)));
+    if (std::find(out.begin(), out.end(), (int)zc) == out.end()) out.push_back((int)zc);
+  }
+  return out;
+}
+
+// Mean ms per launch of each profiled class over `steps` PC iterations
+// (one untimed iteration first).
+static int time_classes(hetddft_ctx* c, double dt, int steps, double ms[HETDDFT_PROF_N]);
+
+static std::mutex g_tune_mu;
+static std::map<std::string, hetddft_tuning> g_tune_cache;
+
+static std::string tune_key(const hetddft_ctx* c) {
+  char b[128];
+  std::snprintf(b, sizeof b, "%d/%d/%d/%d/%d/%d", c->p.nx, c->p.ny, c->p.nz, c->p.precision, c->p.n_par,
+                c->p.core_otc);
+  return b;
+}
+
 extern "C" {
 
 int hetddft_default_params(hetddft_params* p) {
@@ -1586,7 +1716,7 @@ int hetddft_destroy(hetddft_ctx* c) {
   if (c->st) hipStreamSynchronize(c->st);
   if (c->st2) hipStreamSynchronize(c->st2);
   clear_graphs(c);
-  for (hipEvent_t e : {c->ev_fork, c->ev_chain, c->ev_b3, c->ev_join})
+  for (hipEvent_t e : {c->ev_fork, c->ev_chain, c->ev_b3, c->ev_join, c->ev_force})
     if (e) hipEventDestroy(e);
   if (c->st2) hipStreamDestroy(c->st2);
   for (auto& r : c->prof_pending) {
@@ -1666,6 +1796,10 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
     const char* gph = std::getenv("HETDDFT_GRAPH");
     c->graph_on = gph && std::atoi(gph) != 0;
     c->graph_one_stream = gph && std::atoi(gph) == 2;
+    for (int k = 0; k < HETDDFT_TUNE_NSTENCIL; ++k) {
+      c->tile_k[k] = c->tile;
+      c->zc_k[k] = c->zc;
+    }
   }
   c->pl_xh = make_plan(p->nx / 2);
   c->pl_y = make_plan(p->ny);
@@ -1690,7 +1824,7 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
     const char* sf = std::getenv("HETDDFT_SYSFENCE");
     if (sf && std::atoi(sf) == 1) evflags = hipEventDisableTiming;
   }
-  for (hipEvent_t* e : {&c->ev_fork, &c->ev_chain, &c->ev_b3, &c->ev_join})
+  for (hipEvent_t* e : {&c->ev_fork, &c->ev_chain, &c->ev_b3, &c->ev_join, &c->ev_force})
     if (hipEventCreateWithFlags(e, evflags) != hipSuccess) {
       fail(c, HETDDFT_E_HIP, "hipEventCreate failed");
       return bad(HETDDFT_E_HIP);
@@ -2456,4 +2590,126 @@ int hetddft_sync(hetddft_ctx* c) {
   return HETDDFT_OK;
 }
 
+int hetddft_get_tuning(hetddft_ctx* c, hetddft_tuning* out) {
+  if (!c || !out) return HETDDFT_E_ARG;
+  tuning_of(c, out);
+  out->ms_per_step = 0.0;
+  return HETDDFT_OK;
+}
+
+int hetddft_set_tuning(hetddft_ctx* c, const hetddft_tuning* t) {
+  if (!c || !t) return HETDDFT_E_ARG;
+  RET(settle(c));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return apply_tuning(c, t);
+}
+
+int hetddft_autotune(hetddft_ctx* c, double dt, int steps, hetddft_tuning* out) {
+  if (!c) return HETDDFT_E_ARG;
+  if (!c->started) return fail(c, HETDDFT_E_STATE, "hetddft_start not called");
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(c, HETDDFT_E_ARG, "autotune: deltat %g", dt);
+  if (steps <= 0) steps = 4;
+  const std::string key = tune_key(c);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      RET(hetddft_set_tuning(c, &it->second));
+      if (out) *out = it->second;
+      return HETDDFT_OK;
+    }
+  }
+  long long sz = 0;
+  RET(hetddft_checkpoint_size(c, &sz));
+  std::vector<unsigned char> img((size_t)sz);
+  RET(hetddft_checkpoint_save(c, img.data()));
+  const bool graphs = c->graph_on;
+  c->graph_on = false;
+  hetddft_tuning best;
+  tuning_of(c, &best);
+  int rc = HETDDFT_OK;
+  auto run = [&]() -> int {
+    // the PC classes need the RK start-up behind them
+    while (c->iter_in_run < 3) RET(hetddft_iterate(c, dt, nullptr));
+    // stencil classes: every tile shape x z chunk, fastest per class
+    const int cls_prof[4] = {HETDDFT_PROF_GRAD, HETDDFT_PROF_POT, HETDDFT_PROF_PCP, HETDDFT_PROF_PCC};
+    const int cls_tune[4] = {HETDDFT_TUNE_GRAD, HETDDFT_TUNE_POT, HETDDFT_TUNE_PCP, HETDDFT_TUNE_PCC};
+    double best_ms[4] = {1e300, 1e300, 1e300, 1e300};
+    if (c->st25) {
+      for (int tile = 0; tile < 4; ++tile)
+        for (int zc : zc_candidates(c, tile)) {
+          hetddft_tuning t = best;
+          for (int k = 0; k < HETDDFT_TUNE_NSTENCIL; ++k) {
+            t.tile[k] = tile;
+            t.zc[k] = zc;
+          }
+          RET(apply_tuning(c, &t));
+          double ms[HETDDFT_PROF_N];
+          RET(time_classes(c, dt, steps, ms));
+          for (int j = 0; j < 4; ++j)
+            if (ms[cls_prof[j]] > 0.0 && ms[cls_prof[j]] < best_ms[j]) {
+              best_ms[j] = ms[cls_prof[j]];
+              best.tile[cls_tune[j]] = tile;
+              best.zc[cls_tune[j]] = zc;
+            }
+        }
+      best.tile[HETDDFT_TUNE_RK] = best.tile[HETDDFT_TUNE_PCP];
+      best.zc[HETDDFT_TUNE_RK] = best.zc[HETDDFT_TUNE_PCP];
+    }
+    RET(apply_tuning(c, &best));
+    // z transform tile width
+    if (best.fft_z_tile != 0) {
+      double zbest = 1e300;
+      for (int w : {8, 16, 32}) {
+        hetddft_tuning t = best;
+        t.fft_z_tile = w;
+        if (apply_tuning(c, &t) != HETDDFT_OK) continue;
+        double ms[HETDDFT_PROF_N];
+        RET(time_classes(c, dt, steps, ms));
+        if (ms[HETDDFT_PROF_Z] > 0.0 && ms[HETDDFT_PROF_Z] < zbest) {
+          zbest = ms[HETDDFT_PROF_Z];
+          best.fft_z_tile = w;
+        }
+      }
+      RET(apply_tuning(c, &best));
+    }
+    // wall time of the chosen shapes
+    RET(hetddft_iterate(c, dt, nullptr));
+    RET(settle(c));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < steps; ++i) RET(hetddft_iterate(c, dt, nullptr));
+    RET(settle(c));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const auto t1 = std::chrono::steady_clock::now();
+    best.ms_per_step = std::chrono::duration<double, std::milli>(t1 - t0).count() / steps;
+    return HETDDFT_OK;
+  };
+  rc = run();
+  c->prof_on = false;
+  c->graph_on = graphs;
+  const int rc2 = hetddft_checkpoint_load(c, img.data());
+  if (rc != HETDDFT_OK) return rc;
+  if (rc2 != HETDDFT_OK) return rc2;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
+  }
+  if (out) *out = best;
+  return HETDDFT_OK;
+}
+
 }  // extern "C"
+
+static int time_classes(hetddft_ctx* c, double dt, int steps, double ms[HETDDFT_PROF_N]) {
+  RET(hetddft_iterate(c, dt, nullptr));
+  RET(hetddft_profile(c, 1));
+  for (int i = 0; i < steps; ++i) RET(hetddft_iterate(c, dt, nullptr));
+  for (int k = 0; k < HETDDFT_PROF_N; ++k) {
+    double t = 0.0;
+    long long n = 0;
+    RET(hetddft_profile_read(c, k, &t, &n, nullptr));
+    ms[k] = n > 0 ? t / (double)n : 0.0;
+  }
+  return hetddft_profile(c, 0);
+}

@@ -10,6 +10,7 @@
 // Sums run over the offsets -6..6 in order with zeros outside the box, the
 // same sequence of operations as the direct form (d13/d13c).
 #pragma once
+#include <type_traits>
 #include "common.h"
 
 namespace hetd {
@@ -29,6 +30,26 @@ __device__ inline c64 vzero(c64) { return mk(0.0, 0.0); }
 __device__ inline double vmadd(double acc, double w, double v) { return acc + w * v; }
 __device__ inline float vmadd(float acc, float w, float v) { return acc + w * v; }
 __device__ inline c64 vmadd(c64 acc, double w, c64 v) { return mk(acc.x + w * v.x, acc.y + w * v.y); }
+
+// acc + w*v as written (FU = false: separate rounding of the product, the
+// reference's order) or fused (FU = true: one rounding, half the FP64
+// instructions; used by the mixed path, whose psi update has no bitwise
+// reference to follow).
+template <bool FU>
+__device__ inline double madd(double acc, double w, double v) {
+  if constexpr (FU) return fma(w, v, acc);
+  else return vmadd(acc, w, v);
+}
+template <bool FU>
+__device__ inline float madd(float acc, float w, float v) {
+  if constexpr (FU) return fmaf(w, v, acc);
+  else return vmadd(acc, w, v);
+}
+template <bool FU>
+__device__ inline c64 madd(c64 acc, double w, c64 v) {
+  if constexpr (FU) return mk(fma(w, v.x, acc.x), fma(w, v.y, acc.y));
+  else return vmadd(acc, w, v);
+}
 
 template <class V, class R>
 __device__ inline V vload(const R* f, int i) {
@@ -105,11 +126,11 @@ struct ZQueue {
     for (int k = 0; k < 12; ++k) q[k] = q[k + 1];
   }
   __device__ inline V centre() const { return q[6]; }
-  template <class W>
+  template <bool FU = false, class W>
   __device__ inline V dz(const W& w) const {
     V acc = vzero(V());
 #pragma unroll
-    for (int m = 0; m < 13; ++m) acc = vmadd(acc, w.w[m], q[m]);
+    for (int m = 0; m < 13; ++m) acc = madd<FU>(acc, w.w[m], q[m]);
     return acc;
   }
 };
@@ -150,21 +171,21 @@ __device__ inline void plane_store(V* tile, const PlaneRegs<SH, V>& pr, const Co
   }
 }
 
-template <class SH, class V, class W>
+template <class SH, bool FU = false, class V, class W>
 __device__ inline V tile_dx(const V* tile, const Col& c, const W& w) {
   const V* row = tile + (c.ty + 6) * SH::LX + c.tx;
   V acc = vzero(V());
 #pragma unroll
-  for (int m = 0; m < 13; ++m) acc = vmadd(acc, w.w[m], row[m]);
+  for (int m = 0; m < 13; ++m) acc = madd<FU>(acc, w.w[m], row[m]);
   return acc;
 }
 
-template <class SH, class V, class W>
+template <class SH, bool FU = false, class V, class W>
 __device__ inline V tile_dy(const V* tile, const Col& c, const W& w) {
   const V* colp = tile + c.ty * SH::LX + c.tx + 6;
   V acc = vzero(V());
 #pragma unroll
-  for (int m = 0; m < 13; ++m) acc = vmadd(acc, w.w[m], colp[m * SH::LX]);
+  for (int m = 0; m < 13; ++m) acc = madd<FU>(acc, w.w[m], colp[m * SH::LX]);
   return acc;
 }
 
@@ -191,13 +212,14 @@ struct Ham25 {
   int xcd;  // XCD-grouped tile order (col_of)
 };
 
-// Hpsi at the column's current plane (steppc.f90:36-46).
-template <class SH>
+// Hpsi at the column's current plane (steppc.f90:36-46).  FU: fused
+// stencil sums (the mixed path).
+template <class SH, bool FU = false>
 __device__ inline c64 ham25(const Ham25& a, const c64* tile, const ZQueue<c64, c64>& q, const Col& c,
                             double pv, int z) {
-  const c64 sx = tile_dx<SH>(tile, c, a.d2x);
-  const c64 sy = tile_dy<SH>(tile, c, a.d2y);
-  const c64 sz = q.dz(a.d2z);
+  const c64 sx = tile_dx<SH, FU>(tile, c, a.d2x);
+  const c64 sy = tile_dy<SH, FU>(tile, c, a.d2y);
+  const c64 sz = q.template dz<FU>(a.d2z);
   const c64 lap = (sx + sy) + sz;
   const c64 ps = q.centre();
   const c64 t = mk(lap.x * a.h2o2m4 - pv * ps.x, lap.y * a.h2o2m4 - pv * ps.y);
@@ -244,7 +266,7 @@ __global__ void __launch_bounds__(SH::N) k_pc_predict25(Ham25 a, const c64* __re
     if (z + 1 < z1) prefetch(z + 1);
     if (c.in) {
       const int i = gidx(g, c.x, c.y, z);
-      const c64 s4 = ham25<SH>(a, tile, q, c, pv, z);
+      const c64 s4 = ham25<SH, std::is_same<R, float>::value>(a, tile, q, c, pv, z);
       const c64 s1 = mk(o3.x + c4o3dt * ((2.0 * s4.x - a1.x) + 2.0 * a2.x),
                         o3.y + c4o3dt * ((2.0 * s4.y - a1.y) + 2.0 * a2.y));
       const c64 np = mk(s1.x - c112 * pci.x, s1.y - c112 * pci.y);
@@ -304,7 +326,7 @@ __global__ void __launch_bounds__(SH::N) k_pc_correct25(Ham25 a, const c64* __re
     if (z + 1 < z1) prefetch(z + 1);
     if (c.in) {
       const int i = gidx(g, c.x, c.y, z);
-      const c64 s4 = ham25<SH>(a, tile, q, c, pv, z);
+      const c64 s4 = ham25<SH, std::is_same<R, float>::value>(a, tile, q, c, pv, z);
       const c64 s5 = mk(0.125 * ((9.0 * v3.x - v2.x) + dt3 * ((s4.x + 2.0 * a1.x) - a2.x)),
                         0.125 * ((9.0 * v3.y - v2.y) + dt3 * ((s4.y + 2.0 * a1.y) - a2.y)));
       const c64 npc = mk(pci.x - s5.x, pci.y - s5.y);
@@ -356,7 +378,7 @@ __global__ void __launch_bounds__(SH::N) k_rk_stage25(Ham25 a, const c64* __rest
     if (z + 1 < z1) prefetch(z + 1);
     if (c.in) {
       const int i = gidx(g, c.x, c.y, z);
-      const c64 s4 = ham25<SH>(a, tile, q, c, pv, z);
+      const c64 s4 = ham25<SH, std::is_same<R, float>::value>(a, tile, q, c, pv, z);
       const c64 s1 = mk(ca * (s4.x - cb * qi.x), ca * (s4.y - cb * qi.y));
       qacc[i] = mk((qi.x + 3.0 * s1.x) - cc * s4.x, (qi.y + 3.0 * s1.y) - cc * s4.y);
       if (h1_out) h1_out[i] = s4;
@@ -399,9 +421,10 @@ __global__ void __launch_bounds__(SH::N) k_grad3_25(Grid g, int zc, int xcd, con
     if (z + 1 < z1) prefetch(z + 1);
     if (c.in) {
       const int i = gidx(g, c.x, c.y, z);
-      fx[i] = (R)tile_dx<SH>(tile, c, wx);
-      fy[i] = (R)tile_dy<SH>(tile, c, wy);
-      fz[i] = (R)q.dz(wz);
+      constexpr bool FU = std::is_same<R, float>::value;
+      fx[i] = (R)tile_dx<SH, FU>(tile, c, wx);
+      fy[i] = (R)tile_dy<SH, FU>(tile, c, wy);
+      fz[i] = (R)q.template dz<FU>(wz);
     }
     __syncthreads();
     q.shift();
@@ -545,12 +568,13 @@ __global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25T<R> a, const R* __r
         A s4 = A(0), s5 = A(0), s6 = A(0);
         const A* rx = tx_tile + c.ty * SH::LX + c.tx;
         const A* cy = ty_tile + c.tx + c.ty * SH::TX;
+        constexpr bool FU = std::is_same<R, float>::value;
 #pragma unroll
-        for (int m = 0; m < 13; ++m) s4 = s4 + a.d1x.w[m] * rx[m];
+        for (int m = 0; m < 13; ++m) s4 = madd<FU>(s4, a.d1x.w[m], rx[m]);
 #pragma unroll
-        for (int m = 0; m < 13; ++m) s5 = s5 + a.d1y.w[m] * cy[m * SH::TX];
+        for (int m = 0; m < 13; ++m) s5 = madd<FU>(s5, a.d1y.w[m], cy[m * SH::TX]);
 #pragma unroll
-        for (int m = 0; m < 13; ++m) s6 = s6 + a.d1z.w[m] * q[m];
+        for (int m = 0; m < 13; ++m) s6 = madd<FU>(s6, a.d1z.w[m], q[m]);
         const A ua = uac / a.den0s;
         const A ual = (((ua + s4) + s5) + s6) * a.h2o2m4 * a.alphas;
         p4 = p4 + ual;

@@ -75,6 +75,23 @@ class Morse(ctypes.Structure):
                 ("rc", ctypes.c_double), ("umax", ctypes.c_double), ("nq", ctypes.c_int)]
 
 
+TUNE_CLASSES = {"grad_den": 0, "pot4_assembly": 1, "pc_predict": 2, "pc_correct": 3, "rk_stage": 4}
+TILES = {0: (64, 4), 1: (32, 8), 2: (32, 16), 3: (64, 8)}
+
+
+class Tuning(ctypes.Structure):
+    """Mirror of ``hetddft_tuning``: launch shape per stencil class and the z-transform tile."""
+    _fields_ = [("tile", ctypes.c_int * 5), ("zc", ctypes.c_int * 5), ("fft_z_tile", ctypes.c_int),
+                ("ms_per_step", ctypes.c_double)]
+
+    def as_dict(self):
+        d = {name: {"tile": "x".join(map(str, TILES[self.tile[k]])), "zc": self.zc[k]}
+             for name, k in TUNE_CLASSES.items()}
+        d["fft_z_tile"] = self.fft_z_tile
+        d["ms_per_step"] = self.ms_per_step
+        return d
+
+
 _lib = None
 
 _DP = ctypes.POINTER(ctypes.c_double)
@@ -130,6 +147,9 @@ def load_library():
         "hetddft_profile": ([C, ctypes.c_int], ctypes.c_int),
         "hetddft_profile_read": ([C, ctypes.c_int, _VP, _VP, _VP], ctypes.c_int),
         "hetddft_device_bytes": ([C], ctypes.c_longlong),
+        "hetddft_autotune": ([C, ctypes.c_double, ctypes.c_int, ctypes.POINTER(Tuning)], ctypes.c_int),
+        "hetddft_get_tuning": ([C, ctypes.POINTER(Tuning)], ctypes.c_int),
+        "hetddft_set_tuning": ([C, ctypes.POINTER(Tuning)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -336,6 +356,20 @@ class Instance:
         j = [np.zeros(shape) for _ in range(3)]
         self._check(self.lib.hetddft_current(self._c, _ptr(j[0]), _ptr(j[1]), _ptr(j[2])), "current")
         return tuple(j)
+
+    def autotune(self, deltat, steps=4):
+        """Time the launch shapes on this instance's state and keep the fastest (a Tuning)."""
+        t = Tuning()
+        self._check(self.lib.hetddft_autotune(self._c, float(deltat), int(steps), ctypes.byref(t)), "autotune")
+        return t
+
+    def get_tuning(self):
+        t = Tuning()
+        self._check(self.lib.hetddft_get_tuning(self._c, ctypes.byref(t)), "get_tuning")
+        return t
+
+    def set_tuning(self, t):
+        self._check(self.lib.hetddft_set_tuning(self._c, ctypes.byref(t)), "set_tuning")
 
     def graph_info(self):
         """(captured PC-step graphs, graphs enabled) of this instance."""

@@ -48,7 +48,7 @@ def model_bytes(n, n_par, precision, ironx=3):
     return 2 * veval + psi_side + parts
 
 
-def run_gpu(wl, precision, steps, warmup, prof_steps=0):
+def run_gpu(wl, precision, steps, warmup, prof_steps=0, tune=True):
     import hetddft
     import inputs
     p = inputs.params_for(wl, precision=precision)
@@ -62,6 +62,11 @@ def run_gpu(wl, precision, steps, warmup, prof_steps=0):
         g.iterate(wl.deltat, want_errors=False)
     for _ in range(warmup):
         g.iterate(wl.deltat, want_errors=False)
+    tuning = None
+    if tune:                                # launch shapes picked by the autotuner (state restored)
+        ta = time.perf_counter()
+        tuning = g.autotune(wl.deltat, steps=4).as_dict()
+        tuning["seconds"] = time.perf_counter() - ta
     g.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -79,12 +84,12 @@ def run_gpu(wl, precision, steps, warmup, prof_steps=0):
         prof = g.profile_read()
         g.profile(False)
     out = dict(seconds=t1 - t0, steps_per_s=steps / (t1 - t0), energy=energy, errs=errs, den=den,
-               device_bytes=g.device_bytes, prof=prof)
+               device_bytes=g.device_bytes, prof=prof, tuning=tuning)
     g.close()
     return out
 
 
-def run_ensemble(wls, precision, steps, warmup, threads=8):
+def run_ensemble(wls, precision, steps, warmup, threads=8, tune=True):
     """Every instance of the ensemble advances one PC step per round; the
     rounds are enqueued without host synchronisation (each instance has its
     own two streams), and the timed region ends when every instance is done.
@@ -110,6 +115,11 @@ def run_ensemble(wls, precision, steps, warmup, threads=8):
     for _ in range(3 + warmup):
         for g, dt in zip(insts, dts):
             g.iterate(dt, want_errors=False)
+    ta = time.perf_counter()
+    if tune:                                # one autotuner run per grid size (cached by the library)
+        for g, dt in zip(insts, dts):
+            g.autotune(dt, steps=2)
+    tune_s = time.perf_counter() - ta
     for g in insts:
         g.sync()
     groups = [list(range(t, len(insts), threads)) for t in range(min(threads, len(insts)))]
@@ -139,14 +149,14 @@ def run_ensemble(wls, precision, steps, warmup, threads=8):
     for g in insts:
         g.close()
     return dict(seconds=t1 - t0, instance_steps_per_s=len(insts) * steps / (t1 - t0), energies=energies,
-                device_bytes=dev)
+                device_bytes=dev, tune_seconds=tune_s)
 
 
 def ensemble_main(args):
     import inputs
     wls = inputs.ensemble_c4(n_pc=args.steps)
-    r64 = run_ensemble(wls, 0, args.steps, args.warmup, args.threads)
-    r32 = run_ensemble(wls, 1, args.steps, args.warmup, args.threads)
+    r64 = run_ensemble(wls, 0, args.steps, args.warmup, args.threads, tune=not args.no_tune)
+    r32 = run_ensemble(wls, 1, args.steps, args.warmup, args.threads, tune=not args.no_tune)
     rel = max(abs(a - b) / abs(a) for a, b in zip(r64["energies"], r32["energies"]))
     sizes = {}
     for wl in wls:
@@ -167,6 +177,7 @@ def ensemble_main(args):
         "model_hbm_gbs": mb * r32["instance_steps_per_s"] / 1e9,
         "device_bytes_fp64": r64["device_bytes"], "device_bytes_mixed": r32["device_bytes"],
         "host_threads": args.threads,
+        "autotune_seconds": {"fp64": r64["tune_seconds"], "mixed": r32["tune_seconds"]},
     }))
 
 
@@ -246,6 +257,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes (no autotuner)")
     ap.add_argument("--threads", type=int, default=8, help="host threads driving the ensemble (c4)")
     ap.add_argument("--only", choices=["fp64", "mixed", "copy"], default=None,
                     help="time one precision only, or run the copy calibration (profiling runs)")
@@ -263,15 +275,16 @@ def main():
         print(json.dumps(copy_calibration()))
         return
     if args.only:
-        res = run_gpu(wl, 0 if args.only == "fp64" else 1, args.steps, args.warmup, prof_steps=5)
+        res = run_gpu(wl, 0 if args.only == "fp64" else 1, args.steps, args.warmup, prof_steps=5,
+                      tune=not args.no_tune)
         kernels = {k: {"avg_us": round(ms * 1e3 / n, 2), "launches": n,
                        "GB_per_s": round(b / (ms * 1e-3) / 1e9, 1)}
                    for k, (ms, n, b) in res["prof"].items() if n}
         print(json.dumps({"only": args.only, "steps_per_s": res["steps_per_s"],
                           "ms_per_step": 1000.0 / res["steps_per_s"], "kernels": kernels}))
         return
-    res64 = run_gpu(wl, 0, args.steps, args.warmup, prof_steps=20)
-    res32 = run_gpu(wl, 1, args.steps, args.warmup, prof_steps=20)
+    res64 = run_gpu(wl, 0, args.steps, args.warmup, prof_steps=20, tune=not args.no_tune)
+    res32 = run_gpu(wl, 1, args.steps, args.warmup, prof_steps=20, tune=not args.no_tune)
     e64 = res64["energy"]["etot"]
     e32 = res32["energy"]["etot"]
     rel_e = abs(e32 - e64) / abs(e64)
@@ -306,7 +319,9 @@ def main():
         "model_gb_per_step": gb_step,
         "model_hbm_gbs": gb_step * best["steps_per_s"],
         "model_hbm_frac": gb_step * best["steps_per_s"] / HBM_PEAK_GBS,
+        "fp64_model_hbm_frac": model_bytes(wl.n, wl.n_par, 0, wl.ironx) / 1e9 * res64["steps_per_s"] / HBM_PEAK_GBS,
         "fp64_roofline": roofline(res64["prof"], wl.name, "fp64"),
+        "tuning": {"mixed": res32["tuning"], "fp64": res64["tuning"]},
         "device_bytes": best["device_bytes"],
     }
     if not args.no_cpu:
