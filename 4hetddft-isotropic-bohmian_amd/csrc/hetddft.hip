@@ -15,6 +15,9 @@
 #include "stencil25.h"
 
 #include <algorithm>
+#include <atomic>
+#include <new>
+#include <thread>
 #include <chrono>
 #include <mutex>
 #include <cmath>
@@ -1677,6 +1680,13 @@ static std::string tune_key(const hetddft_ctx* c) {
   return b;
 }
 
+// ---- ensemble executor -------------------------------------------------
+struct hetddft_batch {
+  std::vector<hetddft_ctx*> ctx;
+  std::vector<std::vector<int>> part;  // instances of each host thread
+  std::string err;
+};
+
 extern "C" {
 
 int hetddft_default_params(hetddft_params* p) {
@@ -2696,6 +2706,98 @@ int hetddft_autotune(hetddft_ctx* c, double dt, int steps, hetddft_tuning* out) 
     g_tune_cache[key] = best;
   }
   if (out) *out = best;
+  return HETDDFT_OK;
+}
+
+int hetddft_batch_create(hetddft_ctx* const* ctxs, int n, int nthreads, hetddft_batch** out) {
+  if (!out) return HETDDFT_E_ARG;
+  *out = nullptr;
+  if (!ctxs || n <= 0) return HETDDFT_E_ARG;
+  for (int i = 0; i < n; ++i)
+    if (!ctxs[i]) return HETDDFT_E_ARG;
+  auto* b = new (std::nothrow) hetddft_batch();
+  if (!b) return HETDDFT_E_NOMEM;
+  b->ctx.assign(ctxs, ctxs + n);
+  const int nt = std::max(1, std::min(n, nthreads > 0 ? nthreads : 8));
+  // largest grids first, each to the thread with the fewest grid points so far
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return ctxs[a]->g.n > ctxs[c]->g.n; });
+  b->part.assign(nt, {});
+  std::vector<long long> load(nt, 0);
+  for (int i : order) {
+    const int t = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    b->part[t].push_back(i);
+    load[t] += ctxs[i]->g.n;
+  }
+  for (auto& v : b->part) std::sort(v.begin(), v.end());
+  *out = b;
+  return HETDDFT_OK;
+}
+
+int hetddft_batch_iterate(hetddft_batch* b, const double* dt, int steps, double* err, int* failed) {
+  if (failed) *failed = -1;
+  if (!b || !dt || steps < 0) return HETDDFT_E_ARG;
+  const int n = (int)b->ctx.size();
+  std::atomic<bool> stop{false};
+  std::mutex mu;
+  int first_rc = HETDDFT_OK, first_i = -1;
+  auto record = [&](int i, int rc) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (first_i < 0) {
+      first_i = i;
+      first_rc = rc;
+    }
+    stop = true;
+  };
+  auto drive = [&](const std::vector<int>& mine) {
+    for (int s = 0; s < steps && !stop; ++s)
+      for (int i : mine) {
+        const int rc = hetddft_iterate(b->ctx[i], dt[i], nullptr);
+        if (rc != HETDDFT_OK) {
+          record(i, rc);
+          return;
+        }
+      }
+    for (int i : mine) {
+      const int rc = hetddft_sync(b->ctx[i]);
+      if (rc != HETDDFT_OK) {
+        record(i, rc);
+        return;
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  th.reserve(b->part.size());
+  for (size_t t = 1; t < b->part.size(); ++t) th.emplace_back(drive, std::cref(b->part[t]));
+  drive(b->part[0]);
+  for (auto& x : th) x.join();
+  if (first_i >= 0) {
+    b->err = std::string("instance ") + std::to_string(first_i) + ": " + hetddft_last_error(b->ctx[first_i]);
+    if (failed) *failed = first_i;
+    return first_rc;
+  }
+  if (err)
+    for (int i = 0; i < n; ++i) {
+      if (b->ctx[i]->iter_in_run <= 3) {  // the last iteration was an RK step
+        err[3 * i] = err[3 * i + 1] = err[3 * i + 2] = 0.0;
+        continue;
+      }
+      const int rc = hetddft_last_errors(b->ctx[i], err + 3 * i);
+      if (rc != HETDDFT_OK) {
+        b->err = std::string("instance ") + std::to_string(i) + ": " + hetddft_last_error(b->ctx[i]);
+        if (failed) *failed = i;
+        return rc;
+      }
+    }
+  b->err.clear();
+  return HETDDFT_OK;
+}
+
+const char* hetddft_batch_last_error(const hetddft_batch* b) { return b ? b->err.c_str() : ""; }
+
+int hetddft_batch_destroy(hetddft_batch* b) {
+  delete b;
   return HETDDFT_OK;
 }
 

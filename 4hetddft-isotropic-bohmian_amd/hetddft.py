@@ -150,6 +150,10 @@ def load_library():
         "hetddft_autotune": ([C, ctypes.c_double, ctypes.c_int, ctypes.POINTER(Tuning)], ctypes.c_int),
         "hetddft_get_tuning": ([C, ctypes.POINTER(Tuning)], ctypes.c_int),
         "hetddft_set_tuning": ([C, ctypes.POINTER(Tuning)], ctypes.c_int),
+        "hetddft_batch_create": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.POINTER(C)], ctypes.c_int),
+        "hetddft_batch_iterate": ([C, _VP, ctypes.c_int, _VP, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "hetddft_batch_last_error": ([C], ctypes.c_char_p),
+        "hetddft_batch_destroy": ([C], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -393,6 +397,48 @@ class Instance:
             if n.value:
                 out[name] = (ms.value, n.value, b.value)
         return out
+
+
+class Batch:
+    """An ensemble of started instances advanced together by the library's
+    host threads (hetddft_batch_*): ``iterate(steps)`` runs ``steps``
+    iterations of the reference time loop on every instance."""
+
+    def __init__(self, instances, deltats, nthreads=0):
+        self.lib = load_library()
+        self.instances = list(instances)
+        self._ptrs = (ctypes.c_void_p * len(self.instances))(*[g._c.value for g in self.instances])
+        self.deltats = np.ascontiguousarray(deltats, dtype=np.float64)
+        if self.deltats.shape != (len(self.instances),):
+            raise ValueError("one deltat per instance")
+        self._b = ctypes.c_void_p()
+        rc = self.lib.hetddft_batch_create(ctypes.cast(self._ptrs, ctypes.c_void_p), len(self.instances),
+                                           int(nthreads), ctypes.byref(self._b))
+        if rc != 0:
+            raise HetddftError(f"batch_create failed ({rc})")
+
+    def iterate(self, steps=1, want_errors=False):
+        """Advance every instance `steps` iterations; (n, 3) PC errors of the last one if asked."""
+        err = np.zeros((len(self.instances), 3)) if want_errors else None
+        failed = ctypes.c_int(-1)
+        rc = self.lib.hetddft_batch_iterate(self._b, _ptr(self.deltats), int(steps),
+                                            _ptr(err) if err is not None else None, ctypes.byref(failed))
+        if rc != 0:
+            msg = self.lib.hetddft_batch_last_error(self._b)
+            raise HetddftError(f"batch_iterate failed ({rc}) at instance {failed.value}: "
+                               f"{msg.decode() if msg else ''}", rc, failed.value)
+        return err
+
+    def close(self):
+        if self._b:
+            self.lib.hetddft_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def header_functions(header=None):

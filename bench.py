@@ -89,13 +89,14 @@ def run_gpu(wl, precision, steps, warmup, prof_steps=0, tune=True):
     return out
 
 
-def run_ensemble(wls, precision, steps, warmup, threads=8, tune=True):
+def run_ensemble(wls, precision, steps, warmup, threads=8, tune=True, executor="library"):
     """Every instance of the ensemble advances one PC step per round; the
     rounds are enqueued without host synchronisation (each instance has its
     own two streams), and the timed region ends when every instance is done.
     Launching is host-bound for instances this small, so `threads` host
-    threads each drive every threads-th instance (instances are independent;
-    the library releases the GIL for each call)."""
+    threads each drive a share of the instances: by default the library's
+    batch executor (hetddft_batch_iterate, C++ threads, largest grids spread
+    first); executor="python" drives them from Python threads instead."""
     import threading
     import hetddft
     import inputs
@@ -122,6 +123,18 @@ def run_ensemble(wls, precision, steps, warmup, threads=8, tune=True):
     tune_s = time.perf_counter() - ta
     for g in insts:
         g.sync()
+    if executor == "library":
+        b = hetddft.Batch(insts, dts, nthreads=threads)
+        t0 = time.perf_counter()
+        b.iterate(steps)
+        t1 = time.perf_counter()
+        b.close()
+        energies = [g.energy()["etot"] for g in insts]
+        dev = sum(g.device_bytes for g in insts)
+        for g in insts:
+            g.close()
+        return dict(seconds=t1 - t0, instance_steps_per_s=len(insts) * steps / (t1 - t0), energies=energies,
+                    device_bytes=dev, tune_seconds=tune_s)
     groups = [list(range(t, len(insts), threads)) for t in range(min(threads, len(insts)))]
     errors = []
 
@@ -155,8 +168,8 @@ def run_ensemble(wls, precision, steps, warmup, threads=8, tune=True):
 def ensemble_main(args):
     import inputs
     wls = inputs.ensemble_c4(n_pc=args.steps)
-    r64 = run_ensemble(wls, 0, args.steps, args.warmup, args.threads, tune=not args.no_tune)
-    r32 = run_ensemble(wls, 1, args.steps, args.warmup, args.threads, tune=not args.no_tune)
+    r64 = run_ensemble(wls, 0, args.steps, args.warmup, args.threads, tune=not args.no_tune, executor=args.executor)
+    r32 = run_ensemble(wls, 1, args.steps, args.warmup, args.threads, tune=not args.no_tune, executor=args.executor)
     rel = max(abs(a - b) / abs(a) for a, b in zip(r64["energies"], r32["energies"]))
     sizes = {}
     for wl in wls:
@@ -177,6 +190,7 @@ def ensemble_main(args):
         "model_hbm_gbs": mb * r32["instance_steps_per_s"] / 1e9,
         "device_bytes_fp64": r64["device_bytes"], "device_bytes_mixed": r32["device_bytes"],
         "host_threads": args.threads,
+        "executor": "hetddft_batch_iterate" if args.executor == "library" else "python threads",
         "autotune_seconds": {"fp64": r64["tune_seconds"], "mixed": r32["tune_seconds"]},
     }))
 
@@ -258,6 +272,8 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes (no autotuner)")
+    ap.add_argument("--executor", choices=["library", "python"], default="library",
+                    help="ensemble (c4) driver: the library's batch executor or Python threads")
     ap.add_argument("--threads", type=int, default=8, help="host threads driving the ensemble (c4)")
     ap.add_argument("--only", choices=["fp64", "mixed", "copy"], default=None,
                     help="time one precision only, or run the copy calibration (profiling runs)")

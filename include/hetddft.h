@@ -302,6 +302,29 @@ int hetddft_get_tuning(hetddft_ctx* ctx, hetddft_tuning* out);
  * (other than 0, which keeps the current one) is HETDDFT_E_ARG. */
 int hetddft_set_tuning(hetddft_ctx* ctx, const hetddft_tuning* t);
 
+/* Batched variant: an ensemble of independent instances (contexts of any
+ * grids, N_par and precisions, each created, loaded and started as above)
+ * advanced together -- the reference's time loop (DFT4He3d.f90:848-863) run
+ * for every instance.  No counterpart in the reference, which runs one
+ * droplet per process.  hetddft_batch_create borrows the n context pointers
+ * (they must outlive the batch and must not be used by other threads while
+ * hetddft_batch_iterate runs) and spreads them over nthreads host threads
+ * (nthreads <= 0: min(n, 8)), largest grids first, so each thread holds about
+ * the same number of grid points.  hetddft_batch_iterate runs `steps`
+ * hetddft_iterate calls on every instance with deltat[i] for instance i: each
+ * thread submits one iteration of each of its instances in turn, with no
+ * synchronisation between iterations, and the call returns when every
+ * instance has finished.  err (may be NULL) receives 3 doubles per instance,
+ * the PC error estimates of its last iteration.  On a failure the other
+ * threads stop after their current iteration, and the return code and the
+ * message (hetddft_batch_last_error) are those of the first failing instance,
+ * whose index goes to *failed (may be NULL; -1 when all succeed). */
+typedef struct hetddft_batch hetddft_batch;
+int hetddft_batch_create(hetddft_ctx* const* ctxs, int n, int nthreads, hetddft_batch** out);
+int hetddft_batch_iterate(hetddft_batch* b, const double* deltat, int steps, double* err, int* failed);
+const char* hetddft_batch_last_error(const hetddft_batch* b);
+int hetddft_batch_destroy(hetddft_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

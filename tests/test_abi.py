@@ -75,3 +75,16 @@ def test_ensemble_workloads():
         assert 0.0 <= w.v0 <= 3.0
         assert abs(sum(x * x for x in w.vdir) - 1.0) < 1e-12
     assert [(w.n_he, w.v0, w.vdir) for w in a] == [(w.n_he, w.v0, w.vdir) for w in b]
+
+
+def test_batch_arguments(hip):
+    lib = hip.load_library()
+    b = ctypes.c_void_p()
+    assert lib.hetddft_batch_create(None, 0, 1, ctypes.byref(b)) == -1
+    assert b.value is None
+    assert lib.hetddft_batch_create(None, 1, 1, None) == -1
+    failed = ctypes.c_int(7)
+    assert lib.hetddft_batch_iterate(None, None, 1, None, ctypes.byref(failed)) == -1
+    assert failed.value == -1
+    assert lib.hetddft_batch_last_error(None) == b""
+    assert lib.hetddft_batch_destroy(None) == 0

@@ -41,3 +41,55 @@ def test_interleaved_instances_match_solo(hip, precision):
         for a, b in zip(got, ref):
             assert np.array_equal(a, b)
         g.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_executor_matches_solo(hip, precision):
+    """hetddft_batch_iterate (the library's host threads drive the instances)
+    ends in the solo states, with the solo PC error estimates."""
+    import inputs
+    wls = [inputs.WORKLOADS["t24"], inputs.WORKLOADS["t32"],
+           inputs.Workload("e48", 48, 0.45, 17, 500, 0.5, v0=1.2, vdir=(0.6, 0.0, 0.8), seed=41)]
+    print(f"batch executor vs solo, precision={precision}", flush=True)
+    solo, solo_err = [], []
+    for wl in wls:
+        g = _make(hip, wl, precision)
+        for _ in range(9):
+            g.iterate(wl.deltat, want_errors=False)
+        solo.append(g.get_state())
+        solo_err.append(g.last_errors())
+        g.close()
+    insts = [_make(hip, wl, precision) for wl in wls]
+    b = hip.Batch(insts, [wl.deltat for wl in wls], nthreads=2)
+    e = b.iterate(2, want_errors=True)
+    assert np.array_equal(e, np.zeros((3, 3)))          # still in the RK start-up
+    b.iterate(4)
+    err = b.iterate(3, want_errors=True)
+    for g, ref, ref_err, got_err in zip(insts, solo, solo_err, err):
+        for x, y in zip(g.get_state(), ref):
+            assert np.array_equal(x, y)
+        assert np.array_equal(np.asarray(ref_err), got_err)
+    b.close()
+    for g in insts:
+        g.close()
+
+
+def test_batch_executor_reports_the_failing_instance(hip):
+    """An instance that was never started fails its first iteration
+    (HETDDFT_E_STATE, raised on the host); the batch names it."""
+    import inputs
+    wls = [inputs.WORKLOADS["t24"], inputs.WORKLOADS["t32"], inputs.WORKLOADS["t24"]]
+    insts = [_make(hip, wls[0], 0)]
+    p = inputs.params_for(wls[1], precision=0)
+    idle = hip.Instance(p)
+    idle.set_kernels(hip.make_kernels(p))
+    insts.append(idle)
+    insts.append(_make(hip, wls[2], 0))
+    b = hip.Batch(insts, [wl.deltat for wl in wls], nthreads=3)
+    with pytest.raises(hip.HetddftError) as ei:
+        b.iterate(3)
+    assert ei.value.args[1] == -4 and ei.value.args[2] == 1
+    assert "instance 1" in ei.value.args[0]
+    b.close()
+    for g in insts:
+        g.close()
