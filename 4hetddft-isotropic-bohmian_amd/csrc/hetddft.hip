@@ -166,6 +166,7 @@ struct hetddft_ctx {
   // hetddft_set_tuning / hetddft_autotune.  Results do not depend on them.
   int tile_k[HETDDFT_TUNE_NSTENCIL] = {};
   int zc_k[HETDDFT_TUNE_NSTENCIL] = {};
+  int pf_k[HETDDFT_TUNE_NSTENCIL] = {1, 1, 1, 1, 1};
   int cus = 256;      // compute units of the device
   int zp_per_cu = 4;  // resident workgroups per CU of the persistent z pass
   bool st25 = true;  // z-marching stencil engine (false: direct neighbour loads)
@@ -752,6 +753,18 @@ auto with_tile(int tile, Fn&& fn) -> decltype(fn(TileA{})) {
     default: return fn(TileA{});
   }
 }
+// Tile shape and software pipelining (pf) of a stencil launch:
+// fn(tile type, std::integral_constant<bool, pf>).
+template <class Fn>
+void with_shape(int tile, int pf, Fn&& fn) {
+  with_tile(tile, [&](auto sh) {
+    if (pf)
+      fn(sh, std::integral_constant<bool, true>{});
+    else
+      fn(sh, std::integral_constant<bool, false>{});
+    return 0;
+  });
+}
 
 // Workgroups of the engine: TX x TY columns, zc planes each.
 template <class SH>
@@ -872,9 +885,10 @@ int veval(hetddft_ctx* c) {
   {
   ProfScope psg(c, HETDDFT_PROF_GRAD, 4.0 * (double)g.n * sizeof(T));
   if (c->st25)
-    with_tile(c->tile_k[HETDDFT_TUNE_GRAD], [&](auto sh) {
+    with_shape(c->tile_k[HETDDFT_TUNE_GRAD], c->pf_k[HETDDFT_TUNE_GRAD], [&](auto sh, auto pf) {
       using SH = decltype(sh);
-      PLAUNCH(psg, (k_grad3_25<T, SH>), grid25<SH>(c, HETDDFT_TUNE_GRAD), dim3(SH::TX, SH::TY), 0, g,
+      constexpr bool PF = decltype(pf)::value;
+      PLAUNCH(psg, (k_grad3_25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_GRAD), dim3(SH::TX, SH::TY), 0, g,
               c->zc_k[HETDDFT_TUNE_GRAD], c->xcd,
               (const T*)F(hetddft_ctx::F_DEN), stencil_as<T>(c->d1x), stencil_as<T>(c->d1y),
               stencil_as<T>(c->d1z), F(hetddft_ctx::F_DX),
@@ -1017,9 +1031,10 @@ int veval(hetddft_ctx* c) {
     p5.zc = c->zc_k[HETDDFT_TUNE_POT];
     p5.xcd = c->xcd;
     ProfScope ps(c, HETDDFT_PROF_POT, (otc ? 11.0 : 6.0) * (double)g.n * sizeof(T));
-    with_tile(c->tile_k[HETDDFT_TUNE_POT], [&](auto sh) {
+    with_shape(c->tile_k[HETDDFT_TUNE_POT], c->pf_k[HETDDFT_TUNE_POT], [&](auto sh, auto pf) {
       using SH = decltype(sh);
-      PLAUNCH(ps, (k_pot_final25<T, SH>), grid25<SH>(c, HETDDFT_TUNE_POT), dim3(SH::TX, SH::TY), 0, p5,
+      constexpr bool PF = decltype(pf)::value;
+      PLAUNCH(ps, (k_pot_final25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_POT), dim3(SH::TX, SH::TY), 0, p5,
               (const T*)F(hetddft_ctx::F_DEN), (const T*)F(hetddft_ctx::F_DENCG),
               (const T*)F(hetddft_ctx::F_DELJ4), (const T*)F(hetddft_ctx::F_STO1),
               (const T*)F(hetddft_ctx::F_UIMP), (const T*)F(hetddft_ctx::F_FALFS),
@@ -1106,9 +1121,10 @@ int step_rk_t(hetddft_ctx* c, double dt) {
       out = c->spare;
     }
     if (c->st25)
-      with_tile(c->tile_k[HETDDFT_TUNE_RK], [&](auto sh) {
+      with_shape(c->tile_k[HETDDFT_TUNE_RK], c->pf_k[HETDDFT_TUNE_RK], [&](auto sh, auto pf) {
         using SH = decltype(sh);
-        SLAUNCH((k_rk_stage25<T, SH>), grid25<SH>(c, HETDDFT_TUNE_RK), dim3(SH::TX, SH::TY), 0, ha,
+        constexpr bool PF = decltype(pf)::value;
+        SLAUNCH((k_rk_stage25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_RK), dim3(SH::TX, SH::TY), 0, ha,
                            (const c64*)c->psi, (const T*)c->fld[hetddft_ctx::F_POT4], c->q, out,
                            (T*)c->fld[hetddft_ctx::F_DEN], h1o, arun[j], brun[j], crun[j], dt);
         return 0;
@@ -1169,9 +1185,10 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   {
   ProfScope pspred(c, HETDDFT_PROF_PCP, (double)g.n * (16.0 * 8 + 2.0 * sizeof(T)));
   if (c->st25)
-    with_tile(c->tile_k[HETDDFT_TUNE_PCP], [&](auto sh) {
+    with_shape(c->tile_k[HETDDFT_TUNE_PCP], c->pf_k[HETDDFT_TUNE_PCP], [&](auto sh, auto pf) {
       using SH = decltype(sh);
-      PLAUNCH(pspred, (k_pc_predict25<T, SH>), grid25<SH>(c, HETDDFT_TUNE_PCP), dim3(SH::TX, SH::TY), 0, hp,
+      constexpr bool PF = decltype(pf)::value;
+      PLAUNCH(pspred, (k_pc_predict25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_PCP), dim3(SH::TX, SH::TY), 0, hp,
               (const c64*)x,
               (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]], c->hold[H[1]], c->pc,
               (T*)c->fld[hetddft_ctx::F_DEN], c4o3 * dt, c112);
@@ -1221,9 +1238,10 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   {
   ProfScope pscorr(c, HETDDFT_PROF_PCC, (double)g.n * (16.0 * 10 + 2.0 * sizeof(T)));
   if (c->st25)
-    with_tile(c->tile_k[HETDDFT_TUNE_PCC], [&](auto sh) {
+    with_shape(c->tile_k[HETDDFT_TUNE_PCC], c->pf_k[HETDDFT_TUNE_PCC], [&](auto sh, auto pf) {
       using SH = decltype(sh);
-      PLAUNCH(pscorr, (k_pc_correct25<T, SH>), grid25<SH>(c, HETDDFT_TUNE_PCC), dim3(SH::TX, SH::TY), 0, hc,
+      constexpr bool PF = decltype(pf)::value;
+      PLAUNCH(pscorr, (k_pc_correct25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_PCC), dim3(SH::TX, SH::TY), 0, hc,
               (const c64*)c->psi,
               (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
               (const c64*)c->hold[H[0]], (const c64*)c->hold[H[1]], c->pc, c->spare,
@@ -1620,14 +1638,16 @@ void tuning_of(hetddft_ctx* c, hetddft_tuning* t) {
   for (int k = 0; k < HETDDFT_TUNE_NSTENCIL; ++k) {
     t->tile[k] = c->tile_k[k];
     t->zc[k] = c->zc_k[k];
+    t->prefetch[k] = c->pf_k[k];
   }
   t->fft_z_tile = fft_z_tile(c);
 }
 
 int apply_tuning(hetddft_ctx* c, const hetddft_tuning* t) {
   for (int k = 0; k < HETDDFT_TUNE_NSTENCIL; ++k)
-    if (t->tile[k] < 0 || t->tile[k] > 3 || t->zc[k] < 1)
-      return fail(c, HETDDFT_E_ARG, "tuning: class %d has tile %d, zc %d", k, t->tile[k], t->zc[k]);
+    if (t->tile[k] < 0 || t->tile[k] > 3 || t->zc[k] < 1 || (t->prefetch[k] != 0 && t->prefetch[k] != 1))
+      return fail(c, HETDDFT_E_ARG, "tuning: class %d has tile %d, zc %d, prefetch %d", k, t->tile[k], t->zc[k],
+                  t->prefetch[k]);
   {
     // the corrector's errHe partials (one per workgroup) must fit the
     // partial-sum buffer
@@ -1645,6 +1665,7 @@ int apply_tuning(hetddft_ctx* c, const hetddft_tuning* t) {
   for (int k = 0; k < HETDDFT_TUNE_NSTENCIL; ++k) {
     c->tile_k[k] = t->tile[k];
     c->zc_k[k] = std::min(t->zc[k], c->g.nz);
+    c->pf_k[k] = t->prefetch[k];
   }
   clear_graphs(c);  // captured steps hold the old launch shapes
   return HETDDFT_OK;
@@ -2647,11 +2668,13 @@ int hetddft_autotune(hetddft_ctx* c, double dt, int steps, hetddft_tuning* out) 
     double best_ms[4] = {1e300, 1e300, 1e300, 1e300};
     if (c->st25) {
       for (int tile = 0; tile < 4; ++tile)
-        for (int zc : zc_candidates(c, tile)) {
+        for (int zc : zc_candidates(c, tile))
+          for (int pf = 1; pf >= 0; --pf) {
           hetddft_tuning t = best;
           for (int k = 0; k < HETDDFT_TUNE_NSTENCIL; ++k) {
             t.tile[k] = tile;
             t.zc[k] = zc;
+            t.prefetch[k] = pf;
           }
           RET(apply_tuning(c, &t));
           double ms[HETDDFT_PROF_N];
@@ -2661,10 +2684,12 @@ int hetddft_autotune(hetddft_ctx* c, double dt, int steps, hetddft_tuning* out) 
               best_ms[j] = ms[cls_prof[j]];
               best.tile[cls_tune[j]] = tile;
               best.zc[cls_tune[j]] = zc;
+              best.prefetch[cls_tune[j]] = pf;
             }
-        }
+          }
       best.tile[HETDDFT_TUNE_RK] = best.tile[HETDDFT_TUNE_PCP];
       best.zc[HETDDFT_TUNE_RK] = best.zc[HETDDFT_TUNE_PCP];
+      best.prefetch[HETDDFT_TUNE_RK] = best.prefetch[HETDDFT_TUNE_PCP];
     }
     RET(apply_tuning(c, &best));
     // z transform tile width

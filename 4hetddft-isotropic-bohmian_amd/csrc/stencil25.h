@@ -226,10 +226,12 @@ __device__ inline c64 ham25(const Ham25& a, const c64* tile, const ZQueue<c64, c
   return cmul(a.timec_z[z], t);
 }
 
-// Hamming predictor + modifier (steppc.f90:46-56).  Software-pipelined: the
-// tile, queue entry and centre values of plane z+1 are loaded while plane z
-// is computed.
-template <class R, class SH>
+// Hamming predictor + modifier (steppc.f90:46-56).  PF (software pipeline):
+// the tile, queue entry and centre values of plane z+1 are loaded while plane
+// z is computed.  Without PF each plane's loads are issued in its own
+// iteration, which needs about 40 fewer VGPRs (one more resident wave per
+// SIMD) and gives the same results.
+template <class R, class SH, bool PF = true>
 __global__ void __launch_bounds__(SH::N) k_pc_predict25(Ham25 a, const c64* __restrict__ psi,
                                                              const R* __restrict__ pot4, c64* old3,
                                                              const c64* __restrict__ h1, c64* h2,
@@ -256,14 +258,19 @@ __global__ void __launch_bounds__(SH::N) k_pc_predict25(Ham25 a, const c64* __re
     a2n = h2[j];
     pcn = pc[j];
   };
-  if (z0 < z1) prefetch(z0);
+  if constexpr (PF) {
+    if (z0 < z1) prefetch(z0);
+  }
   for (int z = z0; z < z1; ++z) {
+    if constexpr (!PF) prefetch(z);
     q.q[12] = qn;
     plane_store<SH>(tile, pr, c, q.q[6]);
     const double pv = pvn;
     const c64 o3 = o3n, a1 = a1n, a2 = a2n, pci = pcn;
     __syncthreads();
-    if (z + 1 < z1) prefetch(z + 1);
+    if constexpr (PF) {
+      if (z + 1 < z1) prefetch(z + 1);
+    }
     if (c.in) {
       const int i = gidx(g, c.x, c.y, z);
       const c64 s4 = ham25<SH, std::is_same<R, float>::value>(a, tile, q, c, pv, z);
@@ -282,7 +289,7 @@ __global__ void __launch_bounds__(SH::N) k_pc_predict25(Ham25 a, const c64* __re
 
 // Hamming corrector + final value + errHe partial sums (steppc.f90:97-112),
 // software-pipelined like the predictor.
-template <class R, class SH>
+template <class R, class SH, bool PF = true>
 __global__ void __launch_bounds__(SH::N) k_pc_correct25(Ham25 a, const c64* __restrict__ psi,
                                                              const R* __restrict__ pot4,
                                                              const c64* __restrict__ o3p,
@@ -315,15 +322,20 @@ __global__ void __launch_bounds__(SH::N) k_pc_correct25(Ham25 a, const c64* __re
     a2n = h2p[j];
     pcn = pc[j];
   };
-  if (z0 < z1) prefetch(z0);
+  if constexpr (PF) {
+    if (z0 < z1) prefetch(z0);
+  }
   double e = 0.0;
   for (int z = z0; z < z1; ++z) {
+    if constexpr (!PF) prefetch(z);
     q.q[12] = qn;
     plane_store<SH>(tile, pr, c, q.q[6]);
     const double pv = pvn;
     const c64 v3 = v3n, v2 = v2n, a1 = a1n, a2 = a2n, pci = pcn;
     __syncthreads();
-    if (z + 1 < z1) prefetch(z + 1);
+    if constexpr (PF) {
+      if (z + 1 < z1) prefetch(z + 1);
+    }
     if (c.in) {
       const int i = gidx(g, c.x, c.y, z);
       const c64 s4 = ham25<SH, std::is_same<R, float>::value>(a, tile, q, c, pv, z);
@@ -344,7 +356,7 @@ __global__ void __launch_bounds__(SH::N) k_pc_correct25(Ham25 a, const c64* __re
 }
 
 // RK-Gill stage (steprk.f90:56-74), software-pipelined like the predictor.
-template <class R, class SH>
+template <class R, class SH, bool PF = true>
 __global__ void __launch_bounds__(SH::N) k_rk_stage25(Ham25 a, const c64* __restrict__ psi,
                                                            const R* __restrict__ pot4, c64* __restrict__ qacc,
                                                            c64* __restrict__ psi_out, R* __restrict__ den,
@@ -368,14 +380,19 @@ __global__ void __launch_bounds__(SH::N) k_rk_stage25(Ham25 a, const c64* __rest
     pvn = (double)pot4[j];
     qin = qacc[j];
   };
-  if (z0 < z1) prefetch(z0);
+  if constexpr (PF) {
+    if (z0 < z1) prefetch(z0);
+  }
   for (int z = z0; z < z1; ++z) {
+    if constexpr (!PF) prefetch(z);
     q.q[12] = qn;
     plane_store<SH>(tile, pr, c, q.q[6]);
     const double pv = pvn;
     const c64 qi = qin;
     __syncthreads();
-    if (z + 1 < z1) prefetch(z + 1);
+    if constexpr (PF) {
+      if (z + 1 < z1) prefetch(z + 1);
+    }
     if (c.in) {
       const int i = gidx(g, c.x, c.y, z);
       const c64 s4 = ham25<SH, std::is_same<R, float>::value>(a, tile, q, c, pv, z);
@@ -396,7 +413,7 @@ __global__ void __launch_bounds__(SH::N) k_rk_stage25(Ham25 a, const c64* __rest
 // software-pipelined like the predictor.  The arithmetic type is the field
 // type: FP64 in the FP64 path, FP32 (weights rounded to FP32) on the FP32
 // potential side of the mixed path.
-template <class R, class SH>
+template <class R, class SH, bool PF = true>
 __global__ void __launch_bounds__(SH::N) k_grad3_25(Grid g, int zc, int xcd, const R* __restrict__ f,
                                                          StencilA<R> wx, StencilA<R> wy, StencilA<R> wz,
                                                          R* __restrict__ fx, R* __restrict__ fy,
@@ -413,12 +430,17 @@ __global__ void __launch_bounds__(SH::N) k_grad3_25(Grid g, int zc, int xcd, con
     qn = q.next(f, g, c, z);
     plane_load<SH>(pr, f, g, c, z);
   };
-  if (z0 < z1) prefetch(z0);
+  if constexpr (PF) {
+    if (z0 < z1) prefetch(z0);
+  }
   for (int z = z0; z < z1; ++z) {
+    if constexpr (!PF) prefetch(z);
     q.q[12] = qn;
     plane_store<SH>(tile, pr, c, q.q[6]);
     __syncthreads();
-    if (z + 1 < z1) prefetch(z + 1);
+    if constexpr (PF) {
+      if (z + 1 < z1) prefetch(z + 1);
+    }
     if (c.in) {
       const int i = gidx(g, c.x, c.y, z);
       constexpr bool FU = std::is_same<R, float>::value;
@@ -468,7 +490,7 @@ __device__ inline float penalty(float dn, float C, float beta, float den_m) {
   return C * (sg + beta * dn * (sg * (2.0f - sg)));
 }
 
-template <class R, class SH>
+template <class R, class SH, bool PF = true>
 __global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25T<R> a, const R* __restrict__ den,
                                                             const R* __restrict__ dencg,
                                                             const R* __restrict__ delj4,
@@ -542,8 +564,11 @@ __global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25T<R> a, const R* __r
     s1n = sto1c[i];
     uin = uimp[i];
   };
-  if (z0 < z1) prefetch(z0);
+  if constexpr (PF) {
+    if (z0 < z1) prefetch(z0);
+  }
   for (int z = z0; z < z1; ++z) {
+    if constexpr (!PF) prefetch(z);
     if (a.otc) {
       q[12] = okz ? (A)fzn * (A)izn : A(0);
 #pragma unroll
@@ -560,7 +585,9 @@ __global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25T<R> a, const R* __r
     const A dn = (A)dnn, dcg = (A)dcgn, dlj = (A)dljn, s1 = (A)s1n;
     const A ui = (A)uin, uac = (A)uan;
     __syncthreads();
-    if (z + 1 < z1) prefetch(z + 1);
+    if constexpr (PF) {
+      if (z + 1 < z1) prefetch(z + 1);
+    }
     if (c.in) {
       const int i = gidx(g, c.x, c.y, z);
       A p4 = (dlj + dcg * dcg * (a.a0 + a.a1 * dcg)) + s1;

@@ -265,7 +265,9 @@ long long hetddft_device_bytes(const hetddft_ctx* ctx);
  * loops have no launch shapes).  Each stencil kernel class below runs on a
  * z-marching engine with a tile of TX x TY columns per workgroup
  * (tile 0 = 64x4, 1 = 32x8, 2 = 32x16, 3 = 64x8) and zc z planes per
- * workgroup; the register-resident z transform of 64-, 128- and 256-point
+ * workgroup, with (prefetch = 1) or without (0) the next plane's loads in
+ * flight while a plane is computed (without: fewer registers, more resident
+ * waves); the register-resident z transform of 64-, 128- and 256-point
  * z lines takes fft_z_tile consecutive ky per work item (8, 16 or 32; 0 when
  * the grid has no such choice).  Every choice computes each output value with
  * the same operations in the same order, so the state after any step is the
@@ -275,8 +277,8 @@ long long hetddft_device_bytes(const hetddft_ctx* ctx);
  * hetddft_autotune times each candidate on the context's own state: it saves
  * a checkpoint (hetddft_checkpoint_save), runs `steps` profiled PC iterations
  * (hetddft_iterate with deltat) per candidate -- every tile shape with z chunks giving
- * about 512, 768, 1024 and 2048 workgroups, then each z-transform tile
- * width -- keeps the fastest shape per class, restores the checkpoint
+ * about 512, 768, 1024 and 2048 workgroups, with and without prefetch, then
+ * each z-transform tile width -- keeps the fastest shape per class, restores the checkpoint
  * (hetddft_checkpoint_load) and applies the choice.  It needs
  * hetddft_start first.  A later call in the same process for the same grid,
  * precision, N_par and core reuses the result without timing.  out (may be
@@ -292,13 +294,14 @@ long long hetddft_device_bytes(const hetddft_ctx* ctx);
 typedef struct hetddft_tuning {
   int tile[HETDDFT_TUNE_NSTENCIL];
   int zc[HETDDFT_TUNE_NSTENCIL];
+  int prefetch[HETDDFT_TUNE_NSTENCIL];
   int fft_z_tile;
   double ms_per_step;
 } hetddft_tuning;
 int hetddft_autotune(hetddft_ctx* ctx, double deltat, int steps, hetddft_tuning* out);
 int hetddft_get_tuning(hetddft_ctx* ctx, hetddft_tuning* out);
 /* Applies a tuning (from hetddft_get_tuning or a saved autotune result);
- * tile outside 0..3, zc < 1, or an fft_z_tile this grid does not offer
+ * tile outside 0..3, zc < 1, prefetch other than 0 or 1, or an fft_z_tile this grid does not offer
  * (other than 0, which keeps the current one) is HETDDFT_E_ARG. */
 int hetddft_set_tuning(hetddft_ctx* ctx, const hetddft_tuning* t);
 

@@ -43,11 +43,12 @@ def test_autotune_restores_state_and_keeps_results(hip, name, precision):
     t = a.autotune(wl.deltat, steps=2)
     assert all(0 <= t.tile[c] <= 3 for c in range(5))
     assert all(1 <= t.zc[c] <= wl.n for c in range(5))
+    assert all(t.prefetch[c] in (0, 1) for c in range(5))
     assert t.ms_per_step > 0.0
     assert t.fft_z_tile in ((8, 16, 32) if wl.n in (64, 128, 256) else (0,))
     got = a.get_tuning()
     assert list(got.tile) == list(t.tile) and list(got.zc) == list(t.zc)
-    assert got.fft_z_tile == t.fft_z_tile
+    assert list(got.prefetch) == list(t.prefetch) and got.fft_z_tile == t.fft_z_tile
     _same_state(a, b, "after autotune")
     for it in range(4):
         want = it == 3
@@ -65,20 +66,23 @@ def test_autotune_restores_state_and_keeps_results(hip, name, precision):
 
 @pytest.mark.parametrize("tile", [0, 1, 2, 3])
 @pytest.mark.parametrize("zc", [1, 5, 24])
-def test_every_stencil_shape_gives_the_same_steps(hip, tile, zc):
+@pytest.mark.parametrize("pf", [1, 0])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_every_stencil_shape_gives_the_same_steps(hip, tile, zc, pf, precision):
     """24^3: z chunks that do not divide nz, tiles wider than the grid; RK
     start-up and PC steps under one shape against the default shapes."""
     import inputs
     wl = inputs.WORKLOADS["t24"]
-    print(f"shape tile={tile} zc={zc}", flush=True)
-    p = inputs.params_for(wl, precision=0)
+    print(f"shape tile={tile} zc={zc} prefetch={pf} precision={precision}", flush=True)
+    p = inputs.params_for(wl, precision=precision)
     k = hip.make_kernels(p)
-    a = _make(hip, wl, 0, k)
-    b = _make(hip, wl, 0, k)
+    a = _make(hip, wl, precision, k)
+    b = _make(hip, wl, precision, k)
     t = b.get_tuning()
     for c in range(5):
         t.tile[c] = tile
         t.zc[c] = zc
+        t.prefetch[c] = pf
     b.set_tuning(t)
     for it in range(3 + 3):
         a.iterate(wl.deltat, want_errors=False)
@@ -97,7 +101,7 @@ def test_set_tuning_rejects_what_the_grid_does_not_offer(hip):
         g = hip.Instance(p)
         base = g.get_tuning()
         assert base.fft_z_tile == (32 if z_ok else 0)
-        for field, val in (("tile", 4), ("tile", -1), ("zc", 0)):
+        for field, val in (("tile", 4), ("tile", -1), ("zc", 0), ("prefetch", 2)):
             t = g.get_tuning()
             getattr(t, field)[2] = val
             with pytest.raises(hip.HetddftError):
