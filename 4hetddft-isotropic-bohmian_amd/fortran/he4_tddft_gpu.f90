@@ -13,7 +13,9 @@
 !   nx ny nz xmax ymax zmax N_par mimpur ironx core4 l lambdah tzmean tzsurf
 !   nq deltat deltatps niter pener r_cutoff_gs umax_gs
 ! plus the cored Morse parameters d_morse re_morse alpha_morse, the precision
-! (0 FP64, 1 mixed) and the files:
+! (0 FP64, 1 mixed), tune_steps (> 0: run the launch-shape autotuner with that
+! many PC iterations per candidate after the start; results are unchanged)
+! and the files:
 !   filein   stream file: psi(nx,ny,nz) complex(8), rimp(N_par,3), vimp(N_par,3)
 !   fileout  stream file: psi, rimp, vimp, aimp after the last iteration
 !            (with dry_run: fvlj4, wcgk, kalfs, vq from hetddft_make_kernels)
@@ -33,17 +35,18 @@ program he4_tddft_gpu
   real(c_double) :: deltat, deltatps, r_cutoff_gs, umax_gs, d_morse, re_morse, alpha_morse
   real(c_double) :: time0
   character(len=3) :: core4
-  integer :: niter, pener
+  integer :: niter, pener, tune_steps
   logical :: dry_run
   character(len=512) :: filein, fileout, filelog
 
   namelist /input/ nx, ny, nz, xmax, ymax, zmax, n_par, mimpur, ironx, core4, l, &
                    lambdah, tzmean, tzsurf, nq, deltat, deltatps, niter, pener, &
                    r_cutoff_gs, umax_gs, d_morse, re_morse, alpha_morse, precision, &
-                   time0, filein, fileout, filelog, dry_run
+                   time0, filein, fileout, filelog, dry_run, tune_steps
 
   type(hetddft_params) :: p
   type(hetddft_morse) :: morse
+  type(hetddft_tuning) :: tuning
   type(c_ptr) :: ctx
   complex(c_double_complex), allocatable :: psi(:,:,:)
   real(c_double), allocatable :: rimp(:,:), vimp(:,:), aimp(:,:)
@@ -64,6 +67,7 @@ program he4_tddft_gpu
   nq = 1500; deltat = 0.d0; deltatps = 5.d-4; niter = 10; pener = 5; time0 = 0.d0
   d_morse = 20.d0; re_morse = 5.d0; alpha_morse = 0.8d0; r_cutoff_gs = 2.d0; umax_gs = 0.d0
   filein = 'state.in'; fileout = 'state.out'; filelog = 'steps.log'; dry_run = .false.
+  tune_steps = 0
 
   narg = command_argument_count()
   if (narg >= 1) then
@@ -101,7 +105,8 @@ program he4_tddft_gpu
                      'hetddft_make_kernels')
 
   if (dry_run) then
-    write(*, '("hetddft_params bytes ",i0,", hetddft_morse bytes ",i0)') c_sizeof(p), c_sizeof(morse)
+    write(*, '("hetddft_params bytes ",i0,", hetddft_morse bytes ",i0,", hetddft_tuning bytes ",i0)') &
+        c_sizeof(p), c_sizeof(morse), c_sizeof(tuning)
     open(newunit=u, file=trim(fileout), access='stream', form='unformatted', status='replace')
     write(u) fvlj4, wcgk, kalfs, vq
     close(u)
@@ -126,6 +131,8 @@ program he4_tddft_gpu
 
   ! den = conjg(psi)*psi; potenimp; poten; forceimp; zeroed histories
   call hetddft_check(hetddft_start(ctx), ctx, 'hetddft_start')
+  if (tune_steps > 0) call hetddft_check(hetddft_autotune(ctx, deltat, int(tune_steps, c_int), tuning), ctx, &
+                                         'hetddft_autotune')
   call hetddft_check(hetddft_particle_means(ctx, means), ctx, 'hetddft_particle_means')
   write(ulog, '("means ",i8,13es26.17e3)') 0, time0, 0.d0, 0.d0, 0.d0, means
   call gpu_energy(ctx, e)

@@ -54,6 +54,13 @@ module hetddft_mod
     integer(c_int) :: nq
   end type hetddft_morse
 
+  ! Mirrors struct hetddft_tuning (launch shapes; index 1..5 = HETDDFT_TUNE_GRAD..RK + 1).
+  type, bind(C), public :: hetddft_tuning
+    integer(c_int) :: tile(5), zc(5), prefetch(5)
+    integer(c_int) :: fft_z_tile
+    real(c_double) :: ms_per_step
+  end type hetddft_tuning
+
   public :: hetddft_default_params, hetddft_create, hetddft_destroy, hetddft_last_error, &
             hetddft_make_kernels, hetddft_set_kernels, hetddft_set_state, hetddft_start, &
             hetddft_potential, hetddft_step_rk, hetddft_step_pc, hetddft_last_errors, &
@@ -61,7 +68,9 @@ module hetddft_mod
             hetddft_get_field, hetddft_convolve, hetddft_profile, hetddft_profile_read, &
             hetddft_sync, hetddft_device_bytes, hetddft_graph_info, hetddft_diagnostics, &
             hetddft_current, hetddft_checkpoint_size, hetddft_checkpoint_save, hetddft_checkpoint_load, &
-            hetddft_sample_particles, hetddft_init_kernels, hetddft_get_kernels
+            hetddft_sample_particles, hetddft_init_kernels, hetddft_get_kernels, &
+            hetddft_autotune, hetddft_get_tuning, hetddft_set_tuning, &
+            hetddft_batch_create, hetddft_batch_iterate, hetddft_batch_last_error, hetddft_batch_destroy
   public :: hetddft_message, hetddft_check, gpu_steprk, gpu_steppc, gpu_veval, gpu_energy
 
   interface
@@ -259,6 +268,54 @@ module hetddft_mod
     integer(c_long_long) function hetddft_device_bytes(ctx) bind(C, name="hetddft_device_bytes")
       import :: c_long_long, c_ptr
       type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function hetddft_autotune(ctx, deltat, steps, out) bind(C, name="hetddft_autotune")
+      import :: c_int, c_ptr, c_double, hetddft_tuning
+      type(c_ptr), value :: ctx
+      real(c_double), value :: deltat
+      integer(c_int), value :: steps
+      type(hetddft_tuning), intent(out) :: out
+    end function
+
+    integer(c_int) function hetddft_get_tuning(ctx, out) bind(C, name="hetddft_get_tuning")
+      import :: c_int, c_ptr, hetddft_tuning
+      type(c_ptr), value :: ctx
+      type(hetddft_tuning), intent(out) :: out
+    end function
+
+    integer(c_int) function hetddft_set_tuning(ctx, t) bind(C, name="hetddft_set_tuning")
+      import :: c_int, c_ptr, hetddft_tuning
+      type(c_ptr), value :: ctx
+      type(hetddft_tuning), intent(in) :: t
+    end function
+
+    ! ctxs(i) are the instances' contexts; deltat(i) their time steps.
+    integer(c_int) function hetddft_batch_create(ctxs, n, nthreads, batch) bind(C, name="hetddft_batch_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(in) :: ctxs(*)
+      integer(c_int), value :: n, nthreads
+      type(c_ptr), intent(out) :: batch
+    end function
+
+    integer(c_int) function hetddft_batch_iterate(batch, deltat, steps, err, failed) &
+        bind(C, name="hetddft_batch_iterate")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: batch
+      real(c_double), intent(in) :: deltat(*)
+      integer(c_int), value :: steps
+      real(c_double), intent(out) :: err(3, *)
+      integer(c_int), intent(out) :: failed
+    end function
+
+    type(c_ptr) function hetddft_batch_last_error(batch) bind(C, name="hetddft_batch_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: batch
+    end function
+
+    integer(c_int) function hetddft_batch_destroy(batch) bind(C, name="hetddft_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
     end function
 
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")

@@ -58,7 +58,7 @@ def test_fortran_struct_layout_and_kernels(tmp_path):
     res = subprocess.run([str(exe), "in.nml"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stderr
     sizes = [int(t) for t in res.stdout.replace(",", " ").split() if t.isdigit()]
-    assert sizes[:2] == [ctypes.sizeof(hetddft.Params), ctypes.sizeof(hetddft.Morse)]
+    assert sizes[:3] == [ctypes.sizeof(hetddft.Params), ctypes.sizeof(hetddft.Morse), ctypes.sizeof(hetddft.Tuning)]
     p = inputs.params_for(wl)
     k = hetddft.make_kernels(p, hetddft.Morse(20.0, 5.0, 0.8, 2.0, 0.0, 1500))
     raw = np.fromfile(tmp_path / "k.bin", dtype=np.float64).reshape((4,) + p.hshape)
@@ -123,7 +123,11 @@ def _run_python(wl, precision, niter, pener):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", [0, 1])
-def test_fortran_host_loop_matches(tmp_path, precision):
+@pytest.mark.parametrize("tune_steps", [0, 2])
+def test_fortran_host_loop_matches(tmp_path, precision, tune_steps):
+    """With tune_steps > 0 the Fortran host runs the autotuner after the start;
+    the state stays bit-identical to the untuned ctypes run, and the PC error
+    estimates (sums of per-workgroup partials) agree to rounding."""
     exe = _driver()
     wl = inputs.WORKLOADS["t24"]
     niter, pener = 8, 4
@@ -133,8 +137,8 @@ def test_fortran_host_loop_matches(tmp_path, precision):
         r.astype(np.float64).tofile(f)
         v.astype(np.float64).tofile(f)
     _namelist(tmp_path / "in.nml", wl, niter=niter, pener=pener, precision=precision,
-              filein="state.in", fileout="state.out", filelog="steps.log")
-    print(f"fortran host loop: t24, precision {precision}", flush=True)
+              filein="state.in", fileout="state.out", filelog="steps.log", tune_steps=tune_steps)
+    print(f"fortran host loop: t24, precision {precision}, tune_steps {tune_steps}", flush=True)
     res = subprocess.run([str(exe), "in.nml"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert res.returncode == 0, res.stderr[-2000:]
 
@@ -150,7 +154,10 @@ def test_fortran_host_loop_matches(tmp_path, precision):
     assert np.array_equal(rest[0], gr) and np.array_equal(rest[1], gv) and np.array_equal(rest[2], ga)
     assert sorted(fmeans) == sorted(gmeans) and sorted(fenergy) == sorted(genergy)
     for it in gmeans:
-        np.testing.assert_allclose(fmeans[it], gmeans[it], rtol=1e-16, atol=0)
+        np.testing.assert_allclose(fmeans[it][[0] + list(range(4, 13))], gmeans[it][[0] + list(range(4, 13))],
+                                   rtol=1e-16, atol=0)
+        np.testing.assert_allclose(fmeans[it][1:4], gmeans[it][1:4], rtol=1e-16 if tune_steps == 0 else 1e-12,
+                                   atol=0)
     for it in genergy:
         np.testing.assert_allclose(fenergy[it], genergy[it], rtol=1e-16, atol=0)
     assert sorted(fdiag) == sorted(gdiag) and len(gdiag) > 0
