@@ -138,9 +138,34 @@ struct ZQueue {
 // The same tile in two steps, so that the loads of the next plane can be in
 // flight while the current plane is computed: plane_load issues the halo
 // loads into registers, plane_store writes them (and the centre) to LDS.
+// Only the star halo is loaded: 6 rows above and below the tile's TX
+// columns and 6 columns left and right of its TY rows.  The interior comes
+// from the z queue and the corners of the LX x LY tile are never read.
+template <class SH>
+struct Halo {
+  static constexpr int NYH = 12 * SH::TX;    // y halo: rows 0..5 and TY+6..TY+11
+  static constexpr int NH = NYH + 12 * SH::TY;
+  static constexpr int NE = (NH + SH::N - 1) / SH::N;
+};
+
+// Tile position (lx, ly) of halo element h < NH.
+template <class SH>
+__device__ inline void halo_pos(int h, int& lx, int& ly) {
+  if (h < Halo<SH>::NYH) {
+    const int r = h / SH::TX;
+    lx = 6 + (h - r * SH::TX);
+    ly = r < 6 ? r : r + SH::TY;
+  } else {
+    const int hx = h - Halo<SH>::NYH;
+    const int r = hx / 12, k = hx - r * 12;
+    ly = 6 + r;
+    lx = k < 6 ? k : k + SH::TX;
+  }
+}
+
 template <class SH, class V>
 struct PlaneRegs {
-  static constexpr int NE = (SH::LX * SH::LY + SH::N - 1) / SH::N;
+  static constexpr int NE = Halo<SH>::NE;
   V v[NE];
 };
 
@@ -149,8 +174,8 @@ __device__ inline void plane_load(PlaneRegs<SH, V>& pr, const R* f, const Grid& 
   const int x0 = c.bx * SH::TX - 6, y0 = c.by * SH::TY - 6;
 #pragma unroll
   for (int k = 0; k < PlaneRegs<SH, V>::NE; ++k) {
-    const int e = c.tid + k * SH::N;
-    const int ly = e / SH::LX, lx = e - ly * SH::LX;
+    int lx, ly;
+    halo_pos<SH>(min(c.tid + k * SH::N, Halo<SH>::NH - 1), lx, ly);
     const int x = x0 + lx, y = y0 + ly;
     const bool inb = x >= 0 && x < g.nx && y >= 0 && y < g.ny;
     const int xc = min(max(x, 0), g.nx - 1), yc = min(max(y, 0), g.ny - 1);
@@ -164,10 +189,12 @@ __device__ inline void plane_store(V* tile, const PlaneRegs<SH, V>& pr, const Co
   tile[(c.ty + 6) * SH::LX + c.tx + 6] = centre;
 #pragma unroll
   for (int k = 0; k < PlaneRegs<SH, V>::NE; ++k) {
-    const int e = c.tid + k * SH::N;
-    const int ly = e / SH::LX, lx = e - ly * SH::LX;
-    const bool centre_region = ly >= 6 && ly < 6 + SH::TY && lx >= 6 && lx < 6 + SH::TX;
-    if (e < SH::LX * SH::LY && !centre_region) tile[e] = pr.v[k];
+    const int h = c.tid + k * SH::N;
+    if (h < Halo<SH>::NH) {
+      int lx, ly;
+      halo_pos<SH>(h, lx, ly);
+      tile[ly * SH::LX + lx] = pr.v[k];
+    }
   }
 }
 
