@@ -74,6 +74,35 @@ struct StencilOf<float> {
 template <class A>
 using StencilA = typename StencilOf<A>::type;
 
+// A 13-point stencil stored by its centre symmetry: u[0] is the centre
+// weight, u[j] the weight of offset +j, and offset -j has u[j] (SYM, the
+// second derivative) or -u[j] (antisymmetric, the first derivative).  The
+// first- and second-derivative tables are exactly (anti)symmetric, and
+// (-u)*f = -(u*f) exactly, so sums over at(m) equal sums over the full table
+// bit for bit.  7 values instead of 13 keep three axes' weights in scalar
+// registers.
+template <class A, bool SYM>
+struct Stencil7 {
+  A u[7];
+  __host__ __device__ constexpr A at(int m) const {  // offset m - 6, m = 0..12
+    return m > 6 ? u[m - 6] : (m < 6 ? (SYM ? u[6 - m] : -u[6 - m]) : u[0]);
+  }
+};
+template <class A, bool SYM>
+inline Stencil7<A, SYM> compact(const Stencil13& s) {
+  Stencil7<A, SYM> r;
+  r.u[0] = (A)s.w[6];
+  for (int j = 1; j <= 6; ++j) r.u[j] = (A)s.w[6 + j];
+  return r;
+}
+
+// Weight m (offset m - 6) of any stencil form.
+__host__ __device__ inline double wat(const Stencil13& s, int m) { return s.w[m]; }
+template <class A, bool SYM>
+__host__ __device__ inline A wat(const Stencil7<A, SYM>& s, int m) {
+  return s.at(m);
+}
+
 inline Stencil13f to_f32(const Stencil13& s) {
   Stencil13f r;
   for (int m = 0; m < 13; ++m) r.w[m] = (float)s.w[m];

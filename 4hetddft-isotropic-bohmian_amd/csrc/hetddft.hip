@@ -727,12 +727,12 @@ int conv_batch(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin,
   return HETDDFT_OK;
 }
 
-HamArgs ham_direct(const Ham25& h) {
+HamArgs ham_direct(const hetddft_ctx* c, const Ham25& h) {
   HamArgs a{};
   a.g = h.g;
-  a.d2x = h.d2x;
-  a.d2y = h.d2y;
-  a.d2z = h.d2z;
+  a.d2x = c->d2x;
+  a.d2y = c->d2y;
+  a.d2z = c->d2z;
   a.h2o2m4 = h.h2o2m4;
   a.timec_z = h.timec_z;
   return a;
@@ -890,8 +890,8 @@ int veval(hetddft_ctx* c) {
       constexpr bool PF = decltype(pf)::value;
       PLAUNCH(psg, (k_grad3_25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_GRAD), dim3(SH::TX, SH::TY), 0, g,
               c->zc_k[HETDDFT_TUNE_GRAD], c->xcd,
-              (const T*)F(hetddft_ctx::F_DEN), stencil_as<T>(c->d1x), stencil_as<T>(c->d1y),
-              stencil_as<T>(c->d1z), F(hetddft_ctx::F_DX),
+              (const T*)F(hetddft_ctx::F_DEN), compact<T, false>(c->d1x), compact<T, false>(c->d1y),
+              compact<T, false>(c->d1z), F(hetddft_ctx::F_DX),
               F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ));
       return 0;
     });
@@ -1016,9 +1016,9 @@ int veval(hetddft_ctx* c) {
   if (c->st25) {
     Pot25T<T> p5{};
     p5.g = g;
-    p5.d1x = stencil_as<T>(c->d1x);
-    p5.d1y = stencil_as<T>(c->d1y);
-    p5.d1z = stencil_as<T>(c->d1z);
+    p5.d1x = compact<T, false>(c->d1x);
+    p5.d1y = compact<T, false>(c->d1y);
+    p5.d1z = compact<T, false>(c->d1z);
     p5.a0 = (T)pa.a0;
     p5.a1 = (T)pa.a1;
     p5.den0s = (T)pa.den0s;
@@ -1063,9 +1063,9 @@ int run_veval(hetddft_ctx* c) { return c->mixed ? veval<float>(c) : veval<double
 Ham25 ham_args(const hetddft_ctx* c) {
   Ham25 a{};
   a.g = c->g;
-  a.d2x = c->d2x;
-  a.d2y = c->d2y;
-  a.d2z = c->d2z;
+  a.d2x = compact<double, true>(c->d2x);
+  a.d2y = compact<double, true>(c->d2y);
+  a.d2z = compact<double, true>(c->d2z);
   a.h2o2m4 = c->p.h2o2m4;
   a.timec_z = c->timec;
   a.zc = c->zc;
@@ -1130,7 +1130,7 @@ int step_rk_t(hetddft_ctx* c, double dt) {
         return 0;
       });
     else
-      SLAUNCH((k_rk_stage<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, ham_direct(ha),
+      SLAUNCH((k_rk_stage<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, ham_direct(c, ha),
                          (const c64*)c->psi, (const T*)c->fld[hetddft_ctx::F_POT4], c->q, out,
                          (T*)c->fld[hetddft_ctx::F_DEN], h1o, arun[j], brun[j], crun[j], dt);
     LAUNCHCHK(c);
@@ -1195,7 +1195,7 @@ int step_pc_t(hetddft_ctx* c, double dt) {
       return 0;
     });
   else
-    PLAUNCH(pspred, (k_pc_predict<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, ham_direct(hp),
+    PLAUNCH(pspred, (k_pc_predict<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, ham_direct(c, hp),
                        (const c64*)x, (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]],
                        c->hold[H[1]], c->pc, (T*)c->fld[hetddft_ctx::F_DEN], c4o3 * dt, c112);
   }
@@ -1249,7 +1249,7 @@ int step_pc_t(hetddft_ctx* c, double dt) {
       return 0;
     });
   else
-    PLAUNCH(pscorr, (k_pc_correct<T>), dim3(nb), dim3(kBlock), 0, ham_direct(hc), (const c64*)c->psi,
+    PLAUNCH(pscorr, (k_pc_correct<T>), dim3(nb), dim3(kBlock), 0, ham_direct(c, hc), (const c64*)c->psi,
                        (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
                        (const c64*)c->hold[H[0]], (const c64*)c->hold[H[1]], c->pc, c->spare,
                        (T*)c->fld[hetddft_ctx::F_DEN], c->part, 3.0 * dt, c9);

@@ -130,7 +130,7 @@ struct ZQueue {
   __device__ inline V dz(const W& w) const {
     V acc = vzero(V());
 #pragma unroll
-    for (int m = 0; m < 13; ++m) acc = madd<FU>(acc, w.w[m], q[m]);
+    for (int m = 0; m < 13; ++m) acc = madd<FU>(acc, wat(w, m), q[m]);
     return acc;
   }
 };
@@ -203,7 +203,7 @@ __device__ inline V tile_dx(const V* tile, const Col& c, const W& w) {
   const V* row = tile + (c.ty + 6) * SH::LX + c.tx;
   V acc = vzero(V());
 #pragma unroll
-  for (int m = 0; m < 13; ++m) acc = madd<FU>(acc, w.w[m], row[m]);
+  for (int m = 0; m < 13; ++m) acc = madd<FU>(acc, wat(w, m), row[m]);
   return acc;
 }
 
@@ -212,7 +212,7 @@ __device__ inline V tile_dy(const V* tile, const Col& c, const W& w) {
   const V* colp = tile + c.ty * SH::LX + c.tx + 6;
   V acc = vzero(V());
 #pragma unroll
-  for (int m = 0; m < 13; ++m) acc = madd<FU>(acc, w.w[m], colp[m * SH::LX]);
+  for (int m = 0; m < 13; ++m) acc = madd<FU>(acc, wat(w, m), colp[m * SH::LX]);
   return acc;
 }
 
@@ -232,7 +232,7 @@ __device__ inline double block_sum25(double v, double* sh, int tid) {
 // ---- psi-side kernels on the engine ------------------------------------
 struct Ham25 {
   Grid g;
-  Stencil13 d2x, d2y, d2z;
+  Stencil7<double, true> d2x, d2y, d2z;
   double h2o2m4;
   const c64* timec_z;
   int zc;   // z planes per workgroup
@@ -442,7 +442,8 @@ __global__ void __launch_bounds__(SH::N) k_rk_stage25(Ham25 a, const c64* __rest
 // potential side of the mixed path.
 template <class R, class SH, bool PF = true>
 __global__ void __launch_bounds__(SH::N) k_grad3_25(Grid g, int zc, int xcd, const R* __restrict__ f,
-                                                         StencilA<R> wx, StencilA<R> wy, StencilA<R> wz,
+                                                         Stencil7<R, false> wx, Stencil7<R, false> wy,
+                                                         Stencil7<R, false> wz,
                                                          R* __restrict__ fx, R* __restrict__ fy,
                                                          R* __restrict__ fz) {
   __shared__ R tile[SH::LX * SH::LY];
@@ -493,7 +494,7 @@ namespace hetd {
 template <class A>
 struct Pot25T {
   Grid g;
-  StencilA<A> d1x, d1y, d1z;
+  Stencil7<A, false> d1x, d1y, d1z;
   A a0, a1;                 // cp4/2, cpp4/3
   A den0s, h2o2m4, alphas;
   A C, beta, den_m;
@@ -624,11 +625,11 @@ __global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25T<R> a, const R* __r
         const A* cy = ty_tile + c.tx + c.ty * SH::TX;
         constexpr bool FU = std::is_same<R, float>::value;
 #pragma unroll
-        for (int m = 0; m < 13; ++m) s4 = madd<FU>(s4, a.d1x.w[m], rx[m]);
+        for (int m = 0; m < 13; ++m) s4 = madd<FU>(s4, a.d1x.at(m), rx[m]);
 #pragma unroll
-        for (int m = 0; m < 13; ++m) s5 = madd<FU>(s5, a.d1y.w[m], cy[m * SH::TX]);
+        for (int m = 0; m < 13; ++m) s5 = madd<FU>(s5, a.d1y.at(m), cy[m * SH::TX]);
 #pragma unroll
-        for (int m = 0; m < 13; ++m) s6 = madd<FU>(s6, a.d1z.w[m], q[m]);
+        for (int m = 0; m < 13; ++m) s6 = madd<FU>(s6, a.d1z.at(m), q[m]);
         const A ua = uac / a.den0s;
         const A ual = (((ua + s4) + s5) + s6) * a.h2o2m4 * a.alphas;
         p4 = p4 + ual;
