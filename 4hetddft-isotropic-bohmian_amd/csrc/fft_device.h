@@ -20,6 +20,21 @@ struct RadixConst {
   static constexpr T r2 = T(0.70710678118654752440084436210485);   // 1/sqrt(2)
 };
 
+// Twiddle products of the transforms with fused multiply-adds.  The FFT
+// layer has no reference operation order to keep (the reference calls
+// FFTW/MKL, fft.f90:43-65, whose results are pinned by the DFT definition),
+// and one rounding per component is more accurate than two.
+__device__ inline float fmad(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ inline double fmad(double a, double b, double c) { return fma(a, b, c); }
+template <class T>
+__device__ inline cplx<T> tmul(cplx<T> a, cplx<T> b) {
+  return mk(fmad(a.x, b.x, -(a.y * b.y)), fmad(a.x, b.y, a.y * b.x));
+}
+template <class T>
+__device__ inline cplx<T> tmulc(cplx<T> a, cplx<T> b) {  // a * conj(b)
+  return mk(fmad(a.x, b.x, a.y * b.y), fmad(a.y, b.x, -(a.x * b.y)));
+}
+
 // multiply by -i (forward) or +i (inverse)
 template <class T, bool INV>
 __device__ inline cplx<T> rot90(cplx<T> a) {
@@ -115,15 +130,15 @@ __device__ inline void bfly16(cplx<T>* v) {
   const cplx<T> w3 = mk(s1, sg * c1);    // W16^3
   const cplx<T> w6 = mk(-r, sg * r);     // W16^6
   const cplx<T> w9 = mk(-c1, -sg * s1);  // W16^9
-  y[1][1] = cmul(y[1][1], w1);
-  y[1][2] = cmul(y[1][2], w2);
-  y[1][3] = cmul(y[1][3], w3);
-  y[2][1] = cmul(y[2][1], w2);
+  y[1][1] = tmul(y[1][1], w1);
+  y[1][2] = tmul(y[1][2], w2);
+  y[1][3] = tmul(y[1][3], w3);
+  y[2][1] = tmul(y[2][1], w2);
   y[2][2] = rot90<T, INV>(y[2][2]);
-  y[2][3] = cmul(y[2][3], w6);
-  y[3][1] = cmul(y[3][1], w3);
-  y[3][2] = cmul(y[3][2], w6);
-  y[3][3] = cmul(y[3][3], w9);
+  y[2][3] = tmul(y[2][3], w6);
+  y[3][1] = tmul(y[3][1], w3);
+  y[3][2] = tmul(y[3][2], w6);
+  y[3][3] = tmul(y[3][3], w9);
 #pragma unroll
   for (int k1 = 0; k1 < 4; ++k1) {
     cplx<T> z[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
@@ -154,7 +169,7 @@ __device__ inline void stage_r(const cplx<T>* __restrict__ src, cplx<T>* __restr
 #pragma unroll
       for (int t = 1; t < R; ++t) {
         cplx<T> w = tw[t * k * twstep];
-        v[t] = INV ? cmulc(v[t], w) : cmul(v[t], w);
+        v[t] = INV ? tmulc(v[t], w) : tmul(v[t], w);
       }
     }
     if constexpr (R == 2) bfly2<T, INV>(v);

@@ -77,7 +77,7 @@ __device__ inline void stage_ip(cplx<T>* buf, int ls, const cplx<T>* __restrict_
 #pragma unroll
         for (int t = 1; t < R; ++t) {
           const cplx<T> w = tw[t * k * TWS];
-          v[q][t] = INV ? cmulc(v[q][t], w) : cmul(v[q][t], w);
+          v[q][t] = INV ? tmulc(v[q][t], w) : tmul(v[q][t], w);
         }
       }
       if constexpr (R == 2) bfly2<T, INV>(v[q]);
@@ -207,14 +207,14 @@ __global__ void __launch_bounds__(kThreads) k_fft_xf(XfArgs<R> a, Grid g, FftPla
       const cplx<T> E = scale(za + zc, T(0.5));
       const cplx<T> D = za - zc;
       const cplx<T> O = mk(T(0.5) * D.y, T(-0.5) * D.x);
-      z[k] = E + cmul(twh[k], O);
+      z[k] = E + tmul(twh[k], O);
     }
     if (k2 != k) {
       const cplx<T> zc = conj(za);
       const cplx<T> E = scale(zb + zc, T(0.5));
       const cplx<T> D = zb - zc;
       const cplx<T> O = mk(T(0.5) * D.y, T(-0.5) * D.x);
-      z[k2] = E + cmul(twh[k2], O);
+      z[k2] = E + tmul(twh[k2], O);
     }
   }
   __syncthreads();
@@ -275,13 +275,13 @@ __global__ void __launch_bounds__(kThreads) k_fft_xi(XiArgs<T, R> a, Grid g, Fft
     {
       const cplx<T> xc = conj(xb);
       const cplx<T> s = xa + xc;
-      const cplx<T> d = cmulc(xa - xc, twh[k]);
+      const cplx<T> d = tmulc(xa - xc, twh[k]);
       x[k] = mk(s.x - d.y, s.y + d.x);
     }
     if (k != 0 && k2 != k) {
       const cplx<T> xc = conj(xa);
       const cplx<T> s = xb + xc;
-      const cplx<T> d = cmulc(xb - xc, twh[k2]);
+      const cplx<T> d = tmulc(xb - xc, twh[k2]);
       x[k2] = mk(s.x - d.y, s.y + d.x);
     }
   }
@@ -596,7 +596,7 @@ __global__ void __launch_bounds__(kThreads) k_fft_zreg(ZArgs<T> a, int njobs, Gr
       if (NB % kThreads == 0 || b < NB) {
         const int l = b % TY, j = b / TY;
 #pragma unroll
-        for (int t = 1; t < R2; ++t) u[q][t] = cmul(u[q][t], tw[t * j]);
+        for (int t = 1; t < R2; ++t) u[q][t] = tmul(u[q][t], tw[t * j]);
         bfly<T, false, R2>(u[q]);
 #pragma unroll
         for (int t = 0; t < R2; ++t) {
@@ -620,7 +620,7 @@ __global__ void __launch_bounds__(kThreads) k_fft_zreg(ZArgs<T> a, int njobs, Gr
 #pragma unroll
         for (int t = 0; t < R1; ++t) v[t] = tile[l * ls + j + R2 * t];
 #pragma unroll
-        for (int t = 1; t < R1; ++t) v[t] = cmulc(v[t], tw[t * j]);
+        for (int t = 1; t < R1; ++t) v[t] = tmulc(v[t], tw[t * j]);
         bfly<T, true, R1>(v);
         if (it.ky0 + l < g.ny) {
 #pragma unroll
@@ -671,7 +671,7 @@ __device__ inline void stage_g(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
 #pragma unroll
         for (int t = 1; t < R; ++t) {
           const cplx<T> w = tw[t * k * TWS];
-          v[q][t] = INV ? cmulc(v[q][t], w) : cmul(v[q][t], w);
+          v[q][t] = INV ? tmulc(v[q][t], w) : tmul(v[q][t], w);
         }
       }
       if constexpr (R == 2) bfly2<T, INV>(v[q]);
@@ -819,14 +819,14 @@ __global__ void __launch_bounds__(kPlaneThreads) k_fft_xyf(XfArgs<R> a, Grid g, 
       const cplx<T> E = scale(za + zc, T(0.5));
       const cplx<T> D = za - zc;
       const cplx<T> O = mk(T(0.5) * D.y, T(-0.5) * D.x);
-      z[k] = E + cmul(twh[k], O);
+      z[k] = E + tmul(twh[k], O);
     }
     if (k2 != k) {
       const cplx<T> zc = conj(za);
       const cplx<T> E = scale(zb + zc, T(0.5));
       const cplx<T> D = zb - zc;
       const cplx<T> O = mk(T(0.5) * D.y, T(-0.5) * D.x);
-      z[k2] = E + cmul(twh[k2], O);
+      z[k2] = E + tmul(twh[k2], O);
     }
     if (k == QH) {
       const cplx<T> z0 = z[0];
@@ -910,20 +910,20 @@ __global__ void __launch_bounds__(kPlaneThreads) k_fft_xyi(XiArgs<T, R> a, Grid 
     {
       const cplx<T> xc = conj(xb);
       const cplx<T> s = xa + xc;
-      const cplx<T> d = cmulc(xa - xc, twh[k]);
+      const cplx<T> d = tmulc(xa - xc, twh[k]);
       x[k] = mk(s.x - d.y, s.y + d.x);
     }
     if (k2 != k) {
       const cplx<T> xc = conj(xa);
       const cplx<T> s = xb + xc;
-      const cplx<T> d = cmulc(xb - xc, twh[k2]);
+      const cplx<T> d = tmulc(xb - xc, twh[k2]);
       x[k2] = mk(s.x - d.y, s.y + d.x);
     }
     if (k == QH) {
       const cplx<T> w = x[0];
       const cplx<T> x0 = mk(w.x, T(0)), xm = mk(w.y, T(0));
       const cplx<T> s = x0 + xm;
-      const cplx<T> d = cmulc(x0 - xm, twh[0]);
+      const cplx<T> d = tmulc(x0 - xm, twh[0]);
       x[0] = mk(s.x - d.y, s.y + d.x);
     }
   }
@@ -1077,7 +1077,7 @@ __device__ inline void p128_load_row(const XfArgs<R>& a, int f, long long row0, 
       for (int t = 0; t < 8; ++t) u[t] = ld(0, t), w[t] = ld(1, t);
 #pragma unroll
       for (int t = 0; t < 8; ++t)
-        z[t] = mk((T)((double)u[t].x * (double)w[t].x), (T)((double)u[t].y * (double)w[t].y));
+        z[t] = mk((T)u[t].x * (T)w[t].x, (T)u[t].y * (T)w[t].y);  // the correctly rounded product
       break;
     }
     case XF_GRAD_DOT: {
@@ -1137,7 +1137,7 @@ __global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyf128(XfArgs<R
 #pragma unroll
   for (int j = 0; j < 8; ++j) v[j] = xe[p128_xe(yl, j, k1)];
 #pragma unroll
-  for (int j = 1; j < 8; ++j) v[j] = cmul(v[j], tw[2 * j * k1]);
+  for (int j = 1; j < 8; ++j) v[j] = tmul(v[j], tw[2 * j * k1]);
   bfly<T, false, 8>(v);  // v[k2] = X[k1 + 8 k2]
   // Hermitian split Z[k] = E + W_128^k O with X[64 - k] from the partner lane
   {
@@ -1153,7 +1153,7 @@ __global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyf128(XfArgs<R
       const cplx<T> E = scale(za + zc, T(0.5));
       const cplx<T> D = za - zc;
       const cplx<T> O = mk(T(0.5) * D.y, T(-0.5) * D.x);
-      zk[k2] = E + cmul(tw[k1 + 8 * k2], O);
+      zk[k2] = E + tmul(tw[k1 + 8 * k2], O);
       // k = 0: DC + i Nyquist, both real
       if (k2 == 0 && k1 == 0) zk[0] = mk(za.x + za.y, za.x - za.y);
     }
@@ -1168,7 +1168,7 @@ __global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyf128(XfArgs<R
   __syncthreads();  // every plane read precedes the column exchanges below
   bfly<T, false, 8>(v);  // v[m1]
 #pragma unroll
-  for (int m = 1; m < 8; ++m) v[m] = cmul(v[m], tws[q * m]);
+  for (int m = 1; m < 8; ++m) v[m] = tmul(v[m], tws[q * m]);
   cplx<T>* ye = P + w * 512;
 #pragma unroll
   for (int m = 0; m < 8; ++m) ye[p128_ye(c, q, m)] = v[m];
@@ -1178,7 +1178,7 @@ __global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyf128(XfArgs<R
   for (int i1 = 0; i1 < 8; ++i1) v[i1] = ye[p128_ye(c, i0 + 2 * i1, m1)];
   bfly<T, false, 8>(v);  // B_i0[r]
 #pragma unroll
-  for (int r = 0; r < 8; ++r) v[r] = cmul(v[r], tws[8 * r * i0]);
+  for (int r = 0; r < 8; ++r) v[r] = tmul(v[r], tws[8 * r * i0]);
   {
     cplx<T> o[8];
 #pragma unroll
@@ -1241,7 +1241,7 @@ __global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyi128(XiArgs<T
   if (tid < 128) tws[tid] = tw[tid];
   bfly<T, true, 8>(v);
 #pragma unroll
-  for (int m = 1; m < 8; ++m) v[m] = cmulc(v[m], tw[q * m]);
+  for (int m = 1; m < 8; ++m) v[m] = tmulc(v[m], tw[q * m]);
   cplx<T>* ex = P + w * 512;
 #pragma unroll
   for (int m = 0; m < 8; ++m) ex[p128_ye(c, q, m)] = v[m];
@@ -1252,7 +1252,7 @@ __global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyi128(XiArgs<T
     for (int i1 = 0; i1 < 8; ++i1) v[i1] = ex[p128_ye(c, i0 + 2 * i1, m1)];
     bfly<T, true, 8>(v);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = cmulc(v[r], tw[8 * r * i0]);
+    for (int r = 0; r < 8; ++r) v[r] = tmulc(v[r], tw[8 * r * i0]);
     cplx<T> o[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) o[r] = lane_swap1(v[r]);
@@ -1285,7 +1285,7 @@ __global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyi128(XiArgs<T
         xc = mk(xa.y, T(0));
       }
       const cplx<T> sm = xd + xc;
-      const cplx<T> d = cmulc(xd - xc, tws[k1 + 8 * k2]);
+      const cplx<T> d = tmulc(xd - xc, tws[k1 + 8 * k2]);
       xk[k2] = mk(sm.x - d.y, sm.y + d.x);
     }
 #pragma unroll
@@ -1293,7 +1293,7 @@ __global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyi128(XiArgs<T
   }
   bfly<T, true, 8>(v);  // v[j] = G[k1][j]
 #pragma unroll
-  for (int j = 1; j < 8; ++j) v[j] = cmulc(v[j], tws[2 * j * k1]);
+  for (int j = 1; j < 8; ++j) v[j] = tmulc(v[j], tws[2 * j * k1]);
   cplx<T>* xe = P + w * 512;
 #pragma unroll
   for (int j = 0; j < 8; ++j) xe[p128_xe(yl, j, k1)] = v[j];

@@ -34,9 +34,9 @@ __device__ inline Idx3 idx3(long long i, const Grid& g) {
 // The sum runs over offsets -6..6 in order, adding w*0 for out-of-box
 // points.  Every load is issued unconditionally from an in-box address (the
 // centre for out-of-box offsets) so all 13 can be in flight together.
-template <class R>
+template <class R, class W = Stencil13>
 __device__ inline double d13(const R* __restrict__ f, long long i, int pos, int n, long long stride,
-                             const Stencil13& w) {
+                             const W& w) {
   double v[13];
 #pragma unroll
   for (int m = -6; m <= 6; ++m) {
@@ -46,7 +46,7 @@ __device__ inline double d13(const R* __restrict__ f, long long i, int pos, int 
   }
   double acc = 0.0;
 #pragma unroll
-  for (int m = 0; m < 13; ++m) acc = acc + w.w[m] * v[m];
+  for (int m = 0; m < 13; ++m) acc = acc + wat(w, m) * v[m];
   return acc;
 }
 
@@ -426,7 +426,8 @@ __global__ void __launch_bounds__(kBlock) k_iron(Grid g, const int* __restrict__
 // hpsix = -h2o2mx*(d2x+d2y+d2z)psix (denpart.f90:42-46)
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_hpsix(Grid g, const int* __restrict__ reg, const R* __restrict__ psix,
-                                                   Stencil13 wx, Stencil13 wy, Stencil13 wz, double h2o2mx,
+                                                   Stencil7<double, true> wx, Stencil7<double, true> wy,
+                                                   Stencil7<double, true> wz, double h2o2mx,
                                                    R* __restrict__ hpsix) {
   const RegionIter it = region_iter(reg + kRegWork);
   for (int vv = blockIdx.x * blockDim.x + threadIdx.x; vv < it.vol; vv += gridDim.x * blockDim.x) {

@@ -839,8 +839,9 @@ int issue_chain(hetddft_ctx* c) {
     std::swap(cur, oth);
   }
   S.denx = cur;
-  S2LAUNCH((k_hpsix<T>), dim3(region_blocks()), dim3(kBlock), 0, g, (const int*)S.reg, (const T*)psix, c->d2x,
-           c->d2y, c->d2z, c->h2o2mx, (T*)S.hpsix);
+  S2LAUNCH((k_hpsix<T>), dim3(region_blocks()), dim3(kBlock), 0, g, (const int*)S.reg, (const T*)psix,
+           compact<double, true>(c->d2x), compact<double, true>(c->d2y), compact<double, true>(c->d2z), c->h2o2mx,
+           (T*)S.hpsix);
   LAUNCHCHK(c);
   if (!c->dry && !c->one_stream) HIPCHK(c, hipEventRecord(c->ev_chain, c->st2));
   c->chain_pending = true;

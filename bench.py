@@ -65,7 +65,7 @@ def run_gpu(wl, precision, steps, warmup, prof_steps=0, tune=True):
     tuning = None
     if tune:                                # launch shapes picked by the autotuner (state restored)
         ta = time.perf_counter()
-        tuning = g.autotune(wl.deltat, steps=4).as_dict()
+        tuning = g.autotune(wl.deltat, steps=8).as_dict()
         tuning["seconds"] = time.perf_counter() - ta
     g.sync()
     t0 = time.perf_counter()
