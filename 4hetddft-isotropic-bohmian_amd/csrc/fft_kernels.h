@@ -632,6 +632,83 @@ __global__ void __launch_bounds__(kThreads) k_fft_zreg(ZArgs<T> a, int njobs, Gr
   }
 }
 
+// Y pass with the transform held in registers around one LDS exchange, for
+// lengths N = R1 * R2 (R1, R2 in {8, 16}).  A workgroup takes kLinesY
+// contiguous lines (line q = kz*mx + kx, length ny, elements contiguous):
+//   A  radix-R1 stage straight from HBM (elements j + R2 t), written to LDS
+//      at j (R1+1) + t (the +1 keeps consecutive j in different banks);
+//   B  radix-R2 stage from LDS (elements j + R1 t) with twiddles W_N^(t j)
+//      (conjugate for the inverse); the thread then holds X[j + R1 t] and
+//      stores it straight to HBM.
+// Threads take consecutive j first, so every HBM access moves runs of R2 (A)
+// or R1 (B) consecutive complex values of a line.  In place: a workgroup
+// reads all of its lines before it writes any.  grid = (ceil(mx*nz/kLinesY),
+// 1, nfields).
+template <class T, int N, int R1, int R2>
+__host__ __device__ constexpr size_t yreg_bytes() {
+  return (size_t)(kLinesY * R2 * (R1 + 1) + N) * sizeof(cplx<T>);
+}
+
+template <class T, bool INV, int N, int R1, int R2>
+__global__ void __launch_bounds__(kThreads) k_fft_yreg(YArgs<T> a, Grid g, FftPlan pl,
+                                                       const cplx<T>* __restrict__ twg) {
+  static_assert(R1 * R2 == N && (R1 == 8 || R1 == 16) && (R2 == 8 || R2 == 16), "N = R1 * R2");
+  constexpr int TL = kLinesY;
+  constexpr int LS = R2 * (R1 + 1);  // LDS words of a line, in complex values
+  constexpr int NA = TL * R2;        // stage-A butterflies
+  constexpr int NB = TL * R1;        // stage-B butterflies
+  constexpr int PA = (NA + kThreads - 1) / kThreads;
+  constexpr int PB = (NB + kThreads - 1) / kThreads;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* tw = tile + TL * LS;
+  const int tid = threadIdx.x;
+  (void)pl;
+  for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
+  const int nlines = g.mx * g.nz;
+  const int q0 = blockIdx.x * TL;
+  cplx<T>* base = reinterpret_cast<cplx<T>*>(a.buf[blockIdx.z]) + (long long)q0 * N;
+  // A: loads of every line of the workgroup, radix R1, to LDS
+  cplx<T> v[PA][R1];
+#pragma unroll
+  for (int q = 0; q < PA; ++q) {
+    const int b = tid + q * kThreads;
+    const int j = b % R2, l = b / R2;
+    const bool ok = (NA % kThreads == 0 || b < NA) && q0 + l < nlines;
+#pragma unroll
+    for (int t = 0; t < R1; ++t) v[q][t] = ok ? base[(long long)l * N + j + R2 * t] : mk(T(0), T(0));
+  }
+#pragma unroll
+  for (int q = 0; q < PA; ++q) {
+    const int b = tid + q * kThreads;
+    if (NA % kThreads == 0 || b < NA) {
+      const int j = b % R2, l = b / R2;
+      bfly<T, INV, R1>(v[q]);
+#pragma unroll
+      for (int t = 0; t < R1; ++t) tile[l * LS + j * (R1 + 1) + t] = v[q][t];
+    }
+  }
+  __syncthreads();
+  // B: twiddles, radix R2, to HBM
+#pragma unroll
+  for (int q = 0; q < PB; ++q) {
+    const int b = tid + q * kThreads;
+    if (NB % kThreads == 0 || b < NB) {
+      const int j = b % R1, l = b / R1;
+      cplx<T> u[R2];
+#pragma unroll
+      for (int t = 0; t < R2; ++t) u[t] = tile[l * LS + t * (R1 + 1) + j];
+#pragma unroll
+      for (int t = 1; t < R2; ++t) u[t] = INV ? tmulc(u[t], tw[t * j]) : tmul(u[t], tw[t * j]);
+      bfly<T, INV, R2>(u);
+      if (q0 + l < nlines) {
+#pragma unroll
+        for (int t = 0; t < R2; ++t) base[(long long)l * N + j + R1 * t] = u[t];
+      }
+    }
+  }
+}
+
 }  // namespace hetd
 
 namespace hetd {

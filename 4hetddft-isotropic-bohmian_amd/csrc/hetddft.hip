@@ -423,6 +423,18 @@ void pick_x(FftKernels<T>& k, int m) {
 }
 template <class T, int N>
 void pick_y(FftKernels<T>& k, int n) {
+  if constexpr (N == 64 || N == 128 || N == 256) {
+    // register-resident y pass (one LDS exchange)
+    const char* e = std::getenv("HETDDFT_YREG");
+    if (!(e && std::atoi(e) == 0)) {
+      constexpr int R1 = N == 64 ? 8 : 16;
+      constexpr int R2 = N / R1;
+      k.yf = k_fft_yreg<T, false, N, R1, R2>;
+      k.yi = k_fft_yreg<T, true, N, R1, R2>;
+      k.lds_y = yreg_bytes<T, N, R1, R2>();
+      return;
+    }
+  }
   k.yf = k_fft_y<T, false, N>;
   k.yi = k_fft_y<T, true, N>;
   k.lds_y = (N > 0 ? 1 : 2) * tile_bytes<T>(kLinesY, n);
