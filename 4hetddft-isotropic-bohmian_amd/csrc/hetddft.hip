@@ -1654,6 +1654,7 @@ void tuning_of(hetddft_ctx* c, hetddft_tuning* t) {
     t->prefetch[k] = c->pf_k[k];
   }
   t->fft_z_tile = fft_z_tile(c);
+  t->fft_z_wgs = c->zp_per_cu;
 }
 
 int apply_tuning(hetddft_ctx* c, const hetddft_tuning* t) {
@@ -1671,6 +1672,8 @@ int apply_tuning(hetddft_ctx* c, const hetddft_tuning* t) {
     if ((long long)d.x * d.y * d.z > (long long)kDiagSlots * blocks_for(c->g.n))
       return fail(c, HETDDFT_E_ARG, "tuning: corrector grid of %u workgroups is too large", d.x * d.y * d.z);
   }
+  if (t->fft_z_wgs < 0 || t->fft_z_wgs > 8)
+    return fail(c, HETDDFT_E_ARG, "tuning: fft_z_wgs %d outside 0..8", t->fft_z_wgs);
   if (t->fft_z_tile != 0) {
     const bool ok = c->mixed ? select_fft_z_tile<float>(c, t->fft_z_tile) : select_fft_z_tile<double>(c, t->fft_z_tile);
     if (!ok) return fail(c, HETDDFT_E_ARG, "tuning: no z transform with ky tile %d on this grid", t->fft_z_tile);
@@ -1680,6 +1683,7 @@ int apply_tuning(hetddft_ctx* c, const hetddft_tuning* t) {
     c->zc_k[k] = std::min(t->zc[k], c->g.nz);
     c->pf_k[k] = t->prefetch[k];
   }
+  if (t->fft_z_wgs > 0) c->zp_per_cu = t->fft_z_wgs;
   clear_graphs(c);  // captured steps hold the old launch shapes
   return HETDDFT_OK;
 }
@@ -1762,6 +1766,7 @@ int hetddft_destroy(hetddft_ctx* c) {
   clear_graphs(c);
   for (hipEvent_t e : {c->ev_fork, c->ev_chain, c->ev_b3, c->ev_join, c->ev_force})
     if (e) hipEventDestroy(e);
+
   if (c->st2) hipStreamDestroy(c->st2);
   for (auto& r : c->prof_pending) {
     if (r.a) hipEventDestroy(r.a);
@@ -2705,20 +2710,23 @@ int hetddft_autotune(hetddft_ctx* c, double dt, int steps, hetddft_tuning* out) 
       best.prefetch[HETDDFT_TUNE_RK] = best.prefetch[HETDDFT_TUNE_PCP];
     }
     RET(apply_tuning(c, &best));
-    // z transform tile width
+    // z transform: tile width x persistent workgroups per CU
     if (best.fft_z_tile != 0) {
       double zbest = 1e300;
-      for (int w : {8, 16, 32}) {
-        hetddft_tuning t = best;
-        t.fft_z_tile = w;
-        if (apply_tuning(c, &t) != HETDDFT_OK) continue;
-        double ms[HETDDFT_PROF_N];
-        RET(time_classes(c, dt, steps, ms));
-        if (ms[HETDDFT_PROF_Z] > 0.0 && ms[HETDDFT_PROF_Z] < zbest) {
-          zbest = ms[HETDDFT_PROF_Z];
-          best.fft_z_tile = w;
+      for (int w : {8, 16, 32})
+        for (int o = 1; o <= 4; ++o) {
+          hetddft_tuning t = best;
+          t.fft_z_tile = w;
+          t.fft_z_wgs = o;
+          if (apply_tuning(c, &t) != HETDDFT_OK) continue;
+          double ms[HETDDFT_PROF_N];
+          RET(time_classes(c, dt, steps, ms));
+          if (ms[HETDDFT_PROF_Z] > 0.0 && ms[HETDDFT_PROF_Z] < zbest) {
+            zbest = ms[HETDDFT_PROF_Z];
+            best.fft_z_tile = w;
+            best.fft_z_wgs = o;
+          }
         }
-      }
       RET(apply_tuning(c, &best));
     }
     // wall time of the chosen shapes

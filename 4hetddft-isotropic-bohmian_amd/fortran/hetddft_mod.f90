@@ -57,7 +57,7 @@ module hetddft_mod
   ! Mirrors struct hetddft_tuning (launch shapes; index 1..5 = HETDDFT_TUNE_GRAD..RK + 1).
   type, bind(C), public :: hetddft_tuning
     integer(c_int) :: tile(5), zc(5), prefetch(5)
-    integer(c_int) :: fft_z_tile
+    integer(c_int) :: fft_z_tile, fft_z_wgs
     real(c_double) :: ms_per_step
   end type hetddft_tuning
 

@@ -82,13 +82,14 @@ TILES = {0: (64, 4), 1: (32, 8), 2: (32, 16), 3: (64, 8)}
 class Tuning(ctypes.Structure):
     """Mirror of ``hetddft_tuning``: launch shape per stencil class and the z-transform tile."""
     _fields_ = [("tile", ctypes.c_int * 5), ("zc", ctypes.c_int * 5), ("prefetch", ctypes.c_int * 5),
-                ("fft_z_tile", ctypes.c_int), ("ms_per_step", ctypes.c_double)]
+                ("fft_z_tile", ctypes.c_int), ("fft_z_wgs", ctypes.c_int), ("ms_per_step", ctypes.c_double)]
 
     def as_dict(self):
         d = {name: {"tile": "x".join(map(str, TILES[self.tile[k]])), "zc": self.zc[k],
                     "prefetch": self.prefetch[k]}
              for name, k in TUNE_CLASSES.items()}
         d["fft_z_tile"] = self.fft_z_tile
+        d["fft_z_wgs"] = self.fft_z_wgs
         d["ms_per_step"] = self.ms_per_step
         return d
 

@@ -7,7 +7,12 @@ every machine sees the same inputs:
 
 * He density: Woods-Saxon rho(r) = 0.0218/(1+exp((r-R)/0.7)), R = 2.22 N^(1/3),
   rescaled so that sum(rho)*dxyz = N; optional surface ripple of spherical
-  harmonics l <= 4;
+  harmonics l <= 4; optional cavity around the impurity, rho times
+  1/(1+exp(-(r-r_b)/0.3)).  The benchmark workloads take r_b = 4.2 A, where
+  the cored Morse potential (D = 20 K, r_e = 5 A, alpha = 0.8/A) changes sign,
+  as a ground-state start has: with He inside the 2000 K core the density is
+  expelled into a thin shell at the core edge and the explicit time step
+  diverges after about 400 PC steps at 128^3;
 * psi = sqrt(rho) exp(i phi), phi = 0 or a per-point N(0, sigma_phi^2) phase;
 * particles: isotropic Gaussian cloud at the origin, common velocity v0 along
   a unit vector, converted from A/ps to A*K with 7.63822291
@@ -40,6 +45,7 @@ class Workload:
     n_pc: int = 100
     mimpur: float = 40.0
     ironx: int = 3
+    cavity: float = 0.0    # radius of the He cavity around the impurity (A); 0: none
 
     @property
     def xmax(self):
@@ -53,11 +59,11 @@ class Workload:
 
 WORKLOADS = {
     # reference's own CPU-runnable case: 64^3, dx = 0.5 A, N = 100
-    "c1": Workload("c1", 64, 0.5, 100, 1000, 0.5, seed=1, n_pc=100),
+    "c1": Workload("c1", 64, 0.5, 100, 1000, 0.5, seed=1, n_pc=100, cavity=4.2),
     # the metric case: 128^3, dx = 0.4 A, N = 500, 1% phase noise, 10^4 particles
-    "c2": Workload("c2", 128, 0.4, 500, 10000, 0.8, v0=1.0, phase_noise=0.01, seed=2, n_pc=1000),
+    "c2": Workload("c2", 128, 0.4, 500, 10000, 0.8, v0=1.0, phase_noise=0.01, seed=2, n_pc=1000, cavity=4.2),
     # large droplet: 256^3, dx = 0.3 A, N = 2000, rippled surface, 10^5 particles
-    "c5": Workload("c5", 256, 0.3, 2000, 100000, 1.0, v0=2.0, ripple=0.3, seed=5, n_pc=2000),
+    "c5": Workload("c5", 256, 0.3, 2000, 100000, 1.0, v0=2.0, ripple=0.3, seed=5, n_pc=2000, cavity=4.2),
     # small parity cases
     "t32": Workload("t32", 32, 0.5, 20, 256, 0.5, v0=1.0, phase_noise=0.01, seed=7, n_pc=5),
     "t24": Workload("t24", 24, 0.6, 12, 128, 0.5, v0=0.5, phase_noise=0.01, seed=8, n_pc=5),
@@ -88,7 +94,7 @@ def ensemble_c4(count=32, seed=4, n_pc=500):
         d = g.standard_normal(3)
         d /= np.linalg.norm(d)
         out.append(Workload(f"c4-{i}", n, h, n_he, 2000, 0.5, v0=speed, vdir=tuple(float(x) for x in d),
-                            seed=400 + i, n_pc=n_pc))
+                            seed=400 + i, n_pc=n_pc, cavity=4.2))
     return out
 
 
@@ -116,6 +122,8 @@ def woods_saxon(wl: Workload, shape=None):
         surf /= max(np.abs(surf).max(), 1e-300)
         R = R + wl.ripple * surf
     rho = 0.0218 / (1.0 + np.exp((r - R) / 0.7))
+    if wl.cavity > 0.0:
+        rho = rho / (1.0 + np.exp(-(r - wl.cavity) / 0.3))
     dxyz = wl.h ** 3
     rho *= wl.n_he / (rho.sum() * dxyz)
     return rho

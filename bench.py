@@ -60,13 +60,13 @@ def run_gpu(wl, precision, steps, warmup, prof_steps=0, tune=True):
     g.start()
     for _ in range(3):                      # RK-Gill start-up (DFT4He3d.f90:851)
         g.iterate(wl.deltat, want_errors=False)
-    for _ in range(warmup):
-        g.iterate(wl.deltat, want_errors=False)
     tuning = None
     if tune:                                # launch shapes picked by the autotuner (state restored)
         ta = time.perf_counter()
         tuning = g.autotune(wl.deltat, steps=8).as_dict()
         tuning["seconds"] = time.perf_counter() - ta
+    for _ in range(warmup):
+        g.iterate(wl.deltat, want_errors=False)
     g.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -113,7 +113,7 @@ def run_ensemble(wls, precision, steps, warmup, threads=8, tune=True, executor="
         g.start()
         insts.append(g)
     dts = [wl.deltat for wl in wls]
-    for _ in range(3 + warmup):
+    for _ in range(3):
         for g, dt in zip(insts, dts):
             g.iterate(dt, want_errors=False)
     ta = time.perf_counter()
@@ -125,6 +125,7 @@ def run_ensemble(wls, precision, steps, warmup, threads=8, tune=True, executor="
         g.sync()
     if executor == "library":
         b = hetddft.Batch(insts, dts, nthreads=threads)
+        b.iterate(warmup)                   # the tuned shapes, steady state
         t0 = time.perf_counter()
         b.iterate(steps)
         t1 = time.perf_counter()
@@ -135,6 +136,11 @@ def run_ensemble(wls, precision, steps, warmup, threads=8, tune=True, executor="
             g.close()
         return dict(seconds=t1 - t0, instance_steps_per_s=len(insts) * steps / (t1 - t0), energies=energies,
                     device_bytes=dev, tune_seconds=tune_s)
+    for _ in range(warmup):
+        for g, dt in zip(insts, dts):
+            g.iterate(dt, want_errors=False)
+    for g in insts:
+        g.sync()
     groups = [list(range(t, len(insts), threads)) for t in range(min(threads, len(insts)))]
     errors = []
 

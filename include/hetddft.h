@@ -269,7 +269,8 @@ long long hetddft_device_bytes(const hetddft_ctx* ctx);
  * flight while a plane is computed (without: fewer registers, more resident
  * waves); the register-resident z transform of 64-, 128- and 256-point
  * z lines takes fft_z_tile consecutive ky per work item (8, 16 or 32; 0 when
- * the grid has no such choice).  Every choice computes each output value with
+ * the grid has no such choice) and runs fft_z_wgs persistent workgroups per
+ * CU (1..8) over the work items.  Every choice computes each output value with
  * the same operations in the same order, so the state after any step is the
  * same bit for bit whatever the tuning (the errHe estimate sums per-workgroup
  * partials, so its last bits follow the corrector's workgroup count).
@@ -278,7 +279,7 @@ long long hetddft_device_bytes(const hetddft_ctx* ctx);
  * a checkpoint (hetddft_checkpoint_save), runs `steps` profiled PC iterations
  * (hetddft_iterate with deltat) per candidate -- every tile shape with z chunks giving
  * about 512, 768, 1024 and 2048 workgroups, with and without prefetch, then
- * each z-transform tile width -- keeps the fastest shape per class, restores the checkpoint
+ * each z-transform tile width with 1..4 workgroups per CU -- keeps the fastest shape per class, restores the checkpoint
  * (hetddft_checkpoint_load) and applies the choice.  It needs
  * hetddft_start first.  A later call in the same process for the same grid,
  * precision, N_par and core reuses the result without timing.  out (may be
@@ -296,12 +297,14 @@ typedef struct hetddft_tuning {
   int zc[HETDDFT_TUNE_NSTENCIL];
   int prefetch[HETDDFT_TUNE_NSTENCIL];
   int fft_z_tile;
+  int fft_z_wgs;
   double ms_per_step;
 } hetddft_tuning;
 int hetddft_autotune(hetddft_ctx* ctx, double deltat, int steps, hetddft_tuning* out);
 int hetddft_get_tuning(hetddft_ctx* ctx, hetddft_tuning* out);
 /* Applies a tuning (from hetddft_get_tuning or a saved autotune result);
- * tile outside 0..3, zc < 1, prefetch other than 0 or 1, or an fft_z_tile this grid does not offer
+ * tile outside 0..3, zc < 1, prefetch other than 0 or 1, fft_z_wgs outside
+ * 0..8 (0 keeps the current one), or an fft_z_tile this grid does not offer
  * (other than 0, which keeps the current one) is HETDDFT_E_ARG. */
 int hetddft_set_tuning(hetddft_ctx* ctx, const hetddft_tuning* t);
 

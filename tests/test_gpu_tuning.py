@@ -46,9 +46,11 @@ def test_autotune_restores_state_and_keeps_results(hip, name, precision):
     assert all(t.prefetch[c] in (0, 1) for c in range(5))
     assert t.ms_per_step > 0.0
     assert t.fft_z_tile in ((8, 16, 32) if wl.n in (64, 128, 256) else (0,))
+    assert 1 <= t.fft_z_wgs <= 8
     got = a.get_tuning()
     assert list(got.tile) == list(t.tile) and list(got.zc) == list(t.zc)
     assert list(got.prefetch) == list(t.prefetch) and got.fft_z_tile == t.fft_z_tile
+    assert got.fft_z_wgs == t.fft_z_wgs
     _same_state(a, b, "after autotune")
     for it in range(4):
         want = it == 3
@@ -106,6 +108,15 @@ def test_set_tuning_rejects_what_the_grid_does_not_offer(hip):
             getattr(t, field)[2] = val
             with pytest.raises(hip.HetddftError):
                 g.set_tuning(t)
+        for wgs, ok in ((9, False), (-1, False), (2, True)):
+            t = g.get_tuning()
+            t.fft_z_wgs = wgs
+            if ok:
+                g.set_tuning(t)
+                assert g.get_tuning().fft_z_wgs == wgs
+            else:
+                with pytest.raises(hip.HetddftError):
+                    g.set_tuning(t)
         for w in (8, 12, 16, 32):
             t = g.get_tuning()
             t.fft_z_tile = w
