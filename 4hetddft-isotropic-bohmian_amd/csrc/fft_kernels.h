@@ -709,6 +709,280 @@ __global__ void __launch_bounds__(kThreads) k_fft_yreg(YArgs<T> a, Grid g, FftPl
   }
 }
 
+// Producer pairs z[t] = (f[2n], f[2n+1]), n = j + STRIDE t (t < CNT), of one
+// row of the x pass: the XF_* producer of field f, formed as in xf_value.
+template <class T, class R, int CNT, int STRIDE>
+__device__ inline void xrow_pairs(const XfArgs<R>& a, int f, long long row0, int j, cplx<T>* z) {
+  auto ld = [&](int k, int t) { return reinterpret_cast<const cplx<R>*>(a.in[f][k] + row0)[j + STRIDE * t]; };
+  switch (a.op[f]) {
+    case XF_CORREL: {
+      const double c0 = a.c[f][0], c1 = a.c[f][1];
+      auto fn = [&](R den, R dcg) {
+        const double d = (double)den, g = (double)dcg;
+        return (T)(d * g * (c0 + c1 * g));
+      };
+#pragma unroll
+      for (int t = 0; t < CNT; ++t) {
+        const cplx<R> u = ld(0, t), w = ld(1, t);
+        z[t] = mk(fn(u.x, w.x), fn(u.y, w.y));
+      }
+      break;
+    }
+    case XF_MUL: {
+#pragma unroll
+      for (int t = 0; t < CNT; ++t) {
+        const cplx<R> u = ld(0, t), w = ld(1, t);
+        z[t] = mk((T)u.x * (T)w.x, (T)u.y * (T)w.y);  // the correctly rounded product
+      }
+      break;
+    }
+    case XF_GRAD_DOT: {
+#pragma unroll
+      for (int t = 0; t < CNT; ++t) {
+        cplx<R> v[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v[k] = ld(k, t);
+        const double sx = (double)v[0].x * (double)v[3].x + (double)v[1].x * (double)v[4].x;
+        const double sy = (double)v[0].y * (double)v[3].y + (double)v[1].y * (double)v[4].y;
+        z[t] = mk((T)(sx + (double)v[2].x * (double)v[5].x), (T)(sy + (double)v[2].y * (double)v[5].y));
+      }
+      break;
+    }
+    default: {
+#pragma unroll
+      for (int t = 0; t < CNT; ++t) {
+        const cplx<R> u = ld(0, t);
+        z[t] = mk((T)u.x, (T)u.y);
+      }
+      break;
+    }
+  }
+}
+
+// X pass (r2c along x, nx = 2 MH, MH = R1 * R2) with the half-length
+// transform in registers, for kRowsX consecutive y rows of one z plane:
+//   A  radix R1 on the producer pairs z[n], n = j + R2 t, straight from HBM,
+//      to LDS at j (R1+1) + t;
+//   B  radix R2 from LDS (elements j + R1 t) with twiddles W_MH^(t j): the
+//      thread holds Z[j + R1 t] and writes it to LDS in natural order;
+// then the Hermitian split and the transposed store of k_fft_xf.  grid =
+// (ceil(ny/kRowsX), nz, nfields); tw = exp(-2 pi i k/MH), twh = exp(-2 pi i k/nx).
+template <class T, int MH, int R1, int R2>
+__host__ __device__ constexpr size_t xreg_bytes() {
+  constexpr int a = kRowsX * R2 * (R1 + 1), b = kRowsX * (MH + 1);
+  return (size_t)(a > b ? a : b) * sizeof(cplx<T>);
+}
+
+template <class T, class R, int MH, int R1, int R2>
+__global__ void __launch_bounds__(kThreads) k_fft_xreg(XfArgs<R> a, Grid g, FftPlan pl,
+                                                       const cplx<T>* __restrict__ tw,
+                                                       const cplx<T>* __restrict__ twh) {
+  static_assert(R1 * R2 == MH && (R1 == 8 || R1 == 16) && (R2 == 8 || R2 == 16), "MH = R1 * R2");
+  constexpr int ROWS = kRowsX, NX = 2 * MH, M = MH + 1, ls = MH + 1;
+  constexpr int LS = R2 * (R1 + 1);
+  constexpr int NA = ROWS * R2, NB = ROWS * R1;
+  constexpr int PA = (NA + kThreads - 1) / kThreads, PB = (NB + kThreads - 1) / kThreads;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem_raw);
+  (void)pl;
+  const int tid = threadIdx.x;
+  const int f = blockIdx.z;
+  const int y0 = blockIdx.x * ROWS;
+  const int iz = blockIdx.y;
+  const int nrow = (g.ny - y0) < ROWS ? (g.ny - y0) : ROWS;
+  // A
+  cplx<T> v[PA][R1];
+#pragma unroll
+  for (int q = 0; q < PA; ++q) {
+    const int b = tid + q * kThreads;
+    const int j = b % R2, r = b / R2;
+    if ((NA % kThreads == 0 || b < NA) && r < nrow) {
+      xrow_pairs<T, R, R1, R2>(a, f, ((long long)iz * g.ny + y0 + r) * NX, j, v[q]);
+    } else {
+#pragma unroll
+      for (int t = 0; t < R1; ++t) v[q][t] = mk(T(0), T(0));
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PA; ++q) {
+    const int b = tid + q * kThreads;
+    if (NA % kThreads == 0 || b < NA) {
+      const int j = b % R2, r = b / R2;
+      bfly<T, false, R1>(v[q]);
+#pragma unroll
+      for (int t = 0; t < R1; ++t) tile[r * LS + j * (R1 + 1) + t] = v[q][t];
+    }
+  }
+  __syncthreads();
+  // B
+  cplx<T> u[PB][R2];
+#pragma unroll
+  for (int q = 0; q < PB; ++q) {
+    const int b = tid + q * kThreads;
+    if (NB % kThreads == 0 || b < NB) {
+      const int j = b % R1, r = b / R1;
+#pragma unroll
+      for (int t = 0; t < R2; ++t) u[q][t] = tile[r * LS + t * (R1 + 1) + j];
+#pragma unroll
+      for (int t = 1; t < R2; ++t) u[q][t] = tmul(u[q][t], tw[t * j]);
+      bfly<T, false, R2>(u[q]);
+    }
+  }
+  __syncthreads();  // every stage-B read precedes the natural-order writes below
+  cplx<T>* Z = tile;
+#pragma unroll
+  for (int q = 0; q < PB; ++q) {
+    const int b = tid + q * kThreads;
+    if (NB % kThreads == 0 || b < NB) {
+      const int j = b % R1, r = b / R1;
+#pragma unroll
+      for (int t = 0; t < R2; ++t) Z[r * ls + j + R1 * t] = u[q][t];
+    }
+  }
+  __syncthreads();
+  // Hermitian split in place, one thread per (k, MH-k) pair (as k_fft_xf)
+  constexpr int half = MH / 2 + 1;
+  for (int e = tid; e < ROWS * half; e += kThreads) {
+    const int r = e / half, k = e - r * half;
+    cplx<T>* z = Z + r * ls;
+    const int k2 = MH - k;  // == MH for k == 0
+    const cplx<T> za = z[k];
+    const cplx<T> zb = z[k == 0 ? 0 : k2];
+    {
+      const cplx<T> zc = conj(zb);
+      const cplx<T> E = scale(za + zc, T(0.5));
+      const cplx<T> D = za - zc;
+      const cplx<T> O = mk(T(0.5) * D.y, T(-0.5) * D.x);
+      z[k] = E + tmul(twh[k], O);
+    }
+    if (k2 != k) {
+      const cplx<T> zc = conj(za);
+      const cplx<T> E = scale(zb + zc, T(0.5));
+      const cplx<T> D = zb - zc;
+      const cplx<T> O = mk(T(0.5) * D.y, T(-0.5) * D.x);
+      z[k2] = E + tmul(twh[k2], O);
+    }
+  }
+  __syncthreads();
+  // transposed store: S[iz][kx][y0 + r]
+  cplx<T>* out = reinterpret_cast<cplx<T>*>(a.out[f]) + (long long)iz * g.mx * g.ny + y0;
+  for (int e = tid; e < ROWS * M; e += kThreads) {
+    const int kx = e / ROWS, r = e - kx * ROWS;
+    if (r < nrow) out[(long long)kx * g.ny + r] = Z[r * ls + kx];
+  }
+}
+
+// Inverse x pass (c2r, nx = 2 MH) with the half-length transform in
+// registers: the transposed read and Hermitian merge of k_fft_xi, then
+//   A  inverse radix R1 from LDS (elements j + R2 t), written back over the
+//      merged rows at j (R1+1) + t (one LDS buffer: more workgroups per CU);
+//   B  conjugate twiddles W_MH^(-t j), inverse radix R2: the thread holds
+//      z[j + R1 t] = (x[2n], x[2n+1]) and stores it straight to HBM, times
+//      renor, with the XI epilogue.
+template <class T, int MH, int R1, int R2>
+__host__ __device__ constexpr size_t xireg_bytes() {
+  constexpr int a = kRowsX * (MH + 1), b = kRowsX * R2 * (R1 + 1);
+  return (size_t)(a > b ? a : b) * sizeof(cplx<T>);
+}
+
+template <class T, class R, int MH, int R1, int R2>
+__global__ void __launch_bounds__(kThreads) k_fft_xireg(XiArgs<T, R> a, Grid g, FftPlan pl,
+                                                        const cplx<T>* __restrict__ tw,
+                                                        const cplx<T>* __restrict__ twh) {
+  static_assert(R1 * R2 == MH && (R1 == 8 || R1 == 16) && (R2 == 8 || R2 == 16), "MH = R1 * R2");
+  constexpr int ROWS = kRowsX, NX = 2 * MH, M = MH + 1, ls = MH + 1;
+  constexpr int LS = R2 * (R1 + 1);
+  constexpr int NA = ROWS * R2, NB = ROWS * R1;
+  constexpr int PA = (NA + kThreads - 1) / kThreads, PB = (NB + kThreads - 1) / kThreads;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* b0 = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* tile = b0;
+  (void)pl;
+  const int tid = threadIdx.x;
+  const int f = blockIdx.z;
+  const int y0 = blockIdx.x * ROWS;
+  const int iz = blockIdx.y;
+  const int nrow = (g.ny - y0) < ROWS ? (g.ny - y0) : ROWS;
+  const cplx<T>* in = reinterpret_cast<const cplx<T>*>(a.in[f]) + (long long)iz * g.mx * g.ny + y0;
+  for (int e = tid; e < ROWS * M; e += kThreads) {
+    const int kx = e / ROWS, r = e - kx * ROWS;
+    b0[r * ls + kx] = (r < nrow) ? in[(long long)kx * g.ny + r] : mk(T(0), T(0));
+  }
+  __syncthreads();
+  // Hermitian merge in place (as k_fft_xi)
+  constexpr int half = MH / 2 + 1;
+  for (int e = tid; e < ROWS * half; e += kThreads) {
+    const int r = e / half, k = e - r * half;
+    cplx<T>* x = b0 + r * ls;
+    const int k2 = MH - k;
+    const cplx<T> xa = x[k], xb = x[k2];
+    {
+      const cplx<T> xc = conj(xb);
+      const cplx<T> s = xa + xc;
+      const cplx<T> d = tmulc(xa - xc, twh[k]);
+      x[k] = mk(s.x - d.y, s.y + d.x);
+    }
+    if (k != 0 && k2 != k) {
+      const cplx<T> xc = conj(xa);
+      const cplx<T> s = xb + xc;
+      const cplx<T> d = tmulc(xb - xc, twh[k2]);
+      x[k2] = mk(s.x - d.y, s.y + d.x);
+    }
+  }
+  __syncthreads();
+  // A
+  cplx<T> v[PA][R1];
+#pragma unroll
+  for (int q = 0; q < PA; ++q) {
+    const int b = tid + q * kThreads;
+    if (NA % kThreads == 0 || b < NA) {
+      const int j = b % R2, r = b / R2;
+#pragma unroll
+      for (int t = 0; t < R1; ++t) v[q][t] = b0[r * ls + j + R2 * t];
+      bfly<T, true, R1>(v[q]);
+    }
+  }
+  __syncthreads();  // every merged value is read before the tile overwrites the rows
+#pragma unroll
+  for (int q = 0; q < PA; ++q) {
+    const int b = tid + q * kThreads;
+    if (NA % kThreads == 0 || b < NA) {
+      const int j = b % R2, r = b / R2;
+#pragma unroll
+      for (int t = 0; t < R1; ++t) tile[r * LS + j * (R1 + 1) + t] = v[q][t];
+    }
+  }
+  __syncthreads();
+  // B -> HBM
+  const T renor = (T)a.renor;
+  const bool omd = a.epi[f] == XI_ONE_MINUS_DIV;
+  const double ec = a.ec[f];
+#pragma unroll
+  for (int q = 0; q < PB; ++q) {
+    const int b = tid + q * kThreads;
+    if (NB % kThreads == 0 || b < NB) {
+      const int j = b % R1, r = b / R1;
+      cplx<T> u[R2];
+#pragma unroll
+      for (int t = 0; t < R2; ++t) u[t] = tile[r * LS + t * (R1 + 1) + j];
+#pragma unroll
+      for (int t = 1; t < R2; ++t) u[t] = tmulc(u[t], tw[t * j]);
+      bfly<T, true, R2>(u);
+      if (r < nrow) {
+        cplx<R>* dst = reinterpret_cast<cplx<R>*>(a.out[f] + ((long long)iz * g.ny + y0 + r) * NX);
+#pragma unroll
+        for (int t = 0; t < R2; ++t) {
+          const T vx = u[t].x * renor, vy = u[t].y * renor;
+          cplx<R> o;
+          o.x = omd ? (R)(1.0 - (double)(R)vx / ec) : (R)vx;
+          o.y = omd ? (R)(1.0 - (double)(R)vy / ec) : (R)vy;
+          dst[j + R1 * t] = o;
+        }
+      }
+    }
+  }
+}
+
 }  // namespace hetd
 
 namespace hetd {

@@ -417,6 +417,19 @@ using FftKernels = FftKernelsT<T>;
 
 template <class T, int MH>
 void pick_x(FftKernels<T>& k, int m) {
+  if constexpr (MH == 64 || MH == 128) {
+    // half-length transform in registers (one LDS exchange)
+    const char* e = std::getenv("HETDDFT_XREG");
+    if (!(e && std::atoi(e) == 0)) {
+      constexpr int F1 = 8, F2 = MH / 8;             // forward: stage A full, 16-wide HBM rows
+      constexpr int I1 = MH / 8, I2 = 8;             // inverse: stage B stores runs of MH/8 pairs
+      k.xf = k_fft_xreg<T, T, MH, F1, F2>;
+      k.xi = k_fft_xireg<T, T, MH, I1, I2>;
+      const size_t bf = xreg_bytes<T, MH, F1, F2>(), bi = xireg_bytes<T, MH, I1, I2>();
+      k.lds_x = bf > bi ? bf : bi;
+      return;
+    }
+  }
   k.xf = k_fft_xf<T, T, MH>;
   k.xi = k_fft_xi<T, T, MH>;
   k.lds_x = (MH > 0 ? 1 : 2) * tile_bytes<T>(kRowsX, m);
