@@ -291,10 +291,36 @@ struct BoxArgs {
   int npar;
 };
 
+// Cell of a particle for the trilinear gather (interpola3,
+// potenimp.f90:91-106): ix = int((r - x1)/h + 1); valid when 1 <= ix < n.
+struct GatherCell {
+  int ix, iy, iz;
+  double r, s, t;
+  bool ok;
+};
+
+__device__ inline GatherCell gather_cell(const BoxArgs& b, const double* __restrict__ rp, int i) {
+  GatherCell q;
+  const int np = b.npar;
+  const double ax = (rp[i] - b.xmin) / b.hx + 1;
+  const double ay = (rp[np + i] - b.ymin) / b.hy + 1;
+  const double az = (rp[2 * np + i] - b.zmin) / b.hz + 1;
+  q.ix = (int)ax;
+  q.iy = (int)ay;
+  q.iz = (int)az;
+  q.r = ax - q.ix;
+  q.s = ay - q.iy;
+  q.t = az - q.iz;
+  q.ok = !(q.ix >= b.g.nx || q.iy >= b.g.ny || q.iz >= b.g.nz || q.ix <= 0 || q.iy <= 0 || q.iz <= 0);
+  return q;
+}
+
 // reg layout: [0..5] bbox of occupied cells (imin,jmin,kmin,imax,jmax,kmax);
 // [6..11] region that may hold non-zero denx/psix/hpsix; [12..17] region
-// recomputed by this evaluation.  Regions are inclusive 0-based boxes.
-constexpr int kRegBox = 0, kRegLive = 6, kRegWork = 12, kRegLen = 18;
+// recomputed by this evaluation; [18..23] bbox of the trilinear-gather cell
+// corners (interpola3) of the particles inside the box.  Regions are
+// inclusive 0-based boxes.
+constexpr int kRegBox = 0, kRegLive = 6, kRegWork = 12, kRegGather = 18, kRegLen = 24;
 
 __global__ void k_hist(BoxArgs b, const double* __restrict__ r, unsigned int* __restrict__ cnt,
                        int* __restrict__ reg) {
@@ -310,17 +336,29 @@ __global__ void k_hist(BoxArgs b, const double* __restrict__ r, unsigned int* __
     const long long j = (ix - 1) + (long long)b.g.nx * ((iy - 1) + (long long)b.g.ny * (iz - 1));
     atomicAdd(&cnt[j], 1u);
   }
-  // bounding box: reduce across the wavefront first, one atomic per wave
-  int v[6] = {in ? ix - 1 : 0x7fffffff, in ? iy - 1 : 0x7fffffff, in ? iz - 1 : 0x7fffffff,
-              in ? ix - 1 : -1, in ? iy - 1 : -1, in ? iz - 1 : -1};
+  // the gather cell's corners are (ix-1 .. ix) 0-based in each direction
+  GatherCell q{};
+  q.ok = false;
+  if (i < b.npar) q = gather_cell(b, r, i);
+  // bounding boxes: reduce across the wavefront first, one atomic per wave
+  int v[12] = {in ? ix - 1 : 0x7fffffff, in ? iy - 1 : 0x7fffffff, in ? iz - 1 : 0x7fffffff,
+               in ? ix - 1 : -1, in ? iy - 1 : -1, in ? iz - 1 : -1,
+               q.ok ? q.ix - 1 : 0x7fffffff, q.ok ? q.iy - 1 : 0x7fffffff, q.ok ? q.iz - 1 : 0x7fffffff,
+               q.ok ? q.ix : -1, q.ok ? q.iy : -1, q.ok ? q.iz : -1};
   for (int off = 32; off > 0; off >>= 1)
-    for (int d = 0; d < 6; ++d) {
+    for (int d = 0; d < 12; ++d) {
       const int o = __shfl_down(v[d], off, 64);
-      v[d] = d < 3 ? min(v[d], o) : max(v[d], o);
+      v[d] = d % 6 < 3 ? min(v[d], o) : max(v[d], o);
     }
-  if ((threadIdx.x & 63) == 0 && v[3] >= 0) {
-    for (int d = 0; d < 3; ++d) atomicMin(&reg[d], v[d]);
-    for (int d = 3; d < 6; ++d) atomicMax(&reg[d], v[d]);
+  if ((threadIdx.x & 63) == 0) {
+    if (v[3] >= 0) {
+      for (int d = 0; d < 3; ++d) atomicMin(&reg[d], v[d]);
+      for (int d = 3; d < 6; ++d) atomicMax(&reg[d], v[d]);
+    }
+    if (v[9] >= 0) {
+      for (int d = 6; d < 9; ++d) atomicMin(&reg[kRegGather - 6 + d], v[d]);
+      for (int d = 9; d < 12; ++d) atomicMax(&reg[kRegGather - 6 + d], v[d]);
+    }
   }
 }
 
@@ -371,13 +409,15 @@ __device__ inline Idx3 region_point(const RegionIter& it, int v) {
   return p;
 }
 
-// Clears the counts of the live region and resets the occupied-cell box
-// (reg[0..5]) for the histogram that follows.
+// Clears the counts of the live region and resets the occupied-cell and
+// gather-corner boxes for the histogram that follows.
 __global__ void k_clear_counts(Grid g, int* __restrict__ reg, unsigned int* __restrict__ cnt) {
   const RegionIter it = region_iter(reg + kRegLive);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     reg[0] = reg[1] = reg[2] = 0x7fffffff;
     reg[3] = reg[4] = reg[5] = -1;
+    reg[kRegGather] = reg[kRegGather + 1] = reg[kRegGather + 2] = 0x7fffffff;
+    reg[kRegGather + 3] = reg[kRegGather + 4] = reg[kRegGather + 5] = -1;
   }
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < it.vol; v += gridDim.x * blockDim.x) {
     const Idx3 p = region_point(it, v);
@@ -464,73 +504,71 @@ __device__ inline double force_at(const GatherArgs& a, int c, const R* __restric
   return ((double)fconv[j] + sto7 * (s1 * (double)hpsix[j] - (double)psix[j] * s4)) / a.m_imp;
 }
 
-// Cell of a particle for the trilinear gather (interpola3,
-// potenimp.f90:91-106): ix = int((r - x1)/h + 1); valid when 1 <= ix < n.
-struct GatherCell {
-  int ix, iy, iz;
-  double r, s, t;
-  bool ok;
-};
-
-__device__ inline GatherCell gather_cell(const BoxArgs& b, const double* __restrict__ rp, int i) {
-  GatherCell q;
-  const int np = b.npar;
-  const double ax = (rp[i] - b.xmin) / b.hx + 1;
-  const double ay = (rp[np + i] - b.ymin) / b.hy + 1;
-  const double az = (rp[2 * np + i] - b.zmin) / b.hz + 1;
-  q.ix = (int)ax;
-  q.iy = (int)ay;
-  q.iz = (int)az;
-  q.r = ax - q.ix;
-  q.s = ay - q.iy;
-  q.t = az - q.iz;
-  q.ok = !(q.ix >= b.g.nx || q.iy >= b.g.ny || q.iz >= b.g.nz || q.ix <= 0 || q.iy <= 0 || q.iz <= 0);
-  return q;
-}
-
-// Force component c at the 8 corners of each particle's cell: one thread per
-// (particle, corner, component); corner bits (dx,dy,dz) = (bit2,bit1,bit0)
-// in the order x000,x001,...,x111 of blend_103.  out[(c*8+corner)*N_par+i].
+// The force field F_c on the gather box (the union of the particles' cell
+// corners): one thread per (component, box point), out[c*vol + local], local
+// x fastest.  Evaluated once per grid point instead of once per (particle,
+// corner), which shares the stencils of the corners that several particles'
+// cells have in common.  Skipped when the box holds more than 8 points per
+// particle (3 vol > 24 N_par): k_force_blend then evaluates the corners
+// itself.  Either way each corner value is the same force_at result.
 template <class R>
-__global__ void __launch_bounds__(kBlock) k_force_corners(GatherArgs a, const double* __restrict__ rp,
-                                                           const R* __restrict__ fcx, const R* __restrict__ fcy,
-                                                           const R* __restrict__ fcz, const R* __restrict__ denx,
-                                                           const R* __restrict__ psix,
-                                                           const R* __restrict__ hpsix,
-                                                           double* __restrict__ out) {
-  const int np = a.b.npar;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 24 * np) return;
-  const int slot = t / np;
-  const int i = t - slot * np;
-  const int c = slot >> 3, corner = slot & 7;
-  const GatherCell q = gather_cell(a.b, rp, i);
-  double v = 0.0;
-  if (q.ok) {
+__global__ void __launch_bounds__(kBlock) k_force_grid(GatherArgs a, const int* __restrict__ reg,
+                                                       const R* __restrict__ fcx, const R* __restrict__ fcy,
+                                                       const R* __restrict__ fcz, const R* __restrict__ denx,
+                                                       const R* __restrict__ psix,
+                                                       const R* __restrict__ hpsix,
+                                                       double* __restrict__ out) {
+  const RegionIter it = region_iter(reg + kRegGather);
+  if (it.vol > 8 * a.b.npar) return;
+  const int n = 3 * it.vol;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const int c = v / it.vol;
+    const Idx3 p = region_point(it, v - c * it.vol);
     const R* fc = c == 0 ? fcx : (c == 1 ? fcy : fcz);
-    v = force_at(a, c, fc, denx, psix, hpsix, q.ix - 1 + ((corner >> 2) & 1), q.iy - 1 + ((corner >> 1) & 1),
-                 q.iz - 1 + (corner & 1));
+    out[v] = force_at(a, c, fc, denx, psix, hpsix, p.ix, p.iy, p.iz);
   }
-  out[t] = v;
 }
 
-// Trilinear blend at the particles (blend_103, potenimp.f90:278-286); zero
-// outside [1, n-1] (potenimp.f90:104-105).  One thread per (particle, comp).
-__global__ void __launch_bounds__(kBlock) k_blend(BoxArgs b, const double* __restrict__ rp,
-                                                   const double* __restrict__ fcorner, double* __restrict__ acc) {
-  const int np = b.npar;
+// Trilinear blend of F_c at the particles (blend_103, potenimp.f90:278-286)
+// from the 8 corners x000..x111 of each particle's cell (corner bits
+// (dx,dy,dz)); zero outside [1, n-1] (potenimp.f90:104-105).  One thread
+// per (particle, component); corners from k_force_grid's box, or evaluated
+// here when that box was too large.
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_force_blend(GatherArgs a, const int* __restrict__ reg,
+                                                        const double* __restrict__ rp,
+                                                        const R* __restrict__ fcx, const R* __restrict__ fcy,
+                                                        const R* __restrict__ fcz, const R* __restrict__ denx,
+                                                        const R* __restrict__ psix,
+                                                        const R* __restrict__ hpsix,
+                                                        const double* __restrict__ fgrid,
+                                                        double* __restrict__ acc) {
+  const int np = a.b.npar;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 3 * np) return;
   const int c = t / np;
   const int i = t - c * np;
-  const GatherCell q = gather_cell(b, rp, i);
+  const GatherCell q = gather_cell(a.b, rp, i);
   if (!q.ok) {
     acc[t] = 0.0;
     return;
   }
-  const double* f = fcorner + (long long)c * 8 * np + i;
-  const double x000 = f[0], x001 = f[np], x010 = f[2 * np], x011 = f[3 * np];
-  const double x100 = f[4 * np], x101 = f[5 * np], x110 = f[6 * np], x111 = f[7 * np];
+  const RegionIter it = region_iter(reg + kRegGather);
+  double x[8];
+  if (it.vol <= 8 * np) {
+    const double* f = fgrid + (long long)c * it.vol + (q.ix - 1 - it.x0) +
+                      it.nxr * ((q.iy - 1 - it.y0) + it.nyr * (q.iz - 1 - it.z0));
+    const int sy = it.nxr, sz = it.nxr * it.nyr;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = f[((k >> 2) & 1) + sy * ((k >> 1) & 1) + sz * (k & 1)];
+  } else {
+    const R* fc = c == 0 ? fcx : (c == 1 ? fcy : fcz);
+    for (int k = 0; k < 8; ++k)
+      x[k] = force_at(a, c, fc, denx, psix, hpsix, q.ix - 1 + ((k >> 2) & 1), q.iy - 1 + ((k >> 1) & 1),
+                      q.iz - 1 + (k & 1));
+  }
+  const double x000 = x[0], x001 = x[1], x010 = x[2], x011 = x[3];
+  const double x100 = x[4], x101 = x[5], x110 = x[6], x111 = x[7];
   const double rr = q.r, ss = q.s, tt = q.t;
   acc[t] = 1.0 * (+x000) + rr * (-x000 + x100) + ss * (-x000 + x010) + tt * (-x000 + x001) +
            rr * ss * (+x000 - x100 - x010 + x110) + rr * tt * (+x000 - x100 - x001 + x101) +

@@ -194,7 +194,7 @@ struct hetddft_ctx {
   void* denx = nullptr;  // whichever of DENXA/DENXB holds the ironed histogram
   unsigned int* counts = nullptr;
   int* reg = nullptr;           // occupied-region bookkeeping (field_kernels.h)
-  double* fcorner = nullptr;    // force at the 8 cell corners of each particle
+  double* fgather = nullptr;    // force field on the gather box (3 x 8 N_par doubles)
   // The particle density chain (histogram -> ironing -> psix -> hpsix) has
   // two buffer sets, each with its own counts and live/work regions.  A chain
   // queued on st2 writes the set that is not current; the potential
@@ -1015,13 +1015,15 @@ int veval(hetddft_ctx* c) {
     ga.d1[1] = c->d1y;
     ga.d1[2] = c->d1z;
     ga.m_imp = c->m_imp;
-    S2LAUNCH((k_force_corners<T>), dim3(blocks_for(24LL * c->p.n_par)), dim3(kBlock), 0, ga,
-             c->rimp, (const T*)F(hetddft_ctx::F_FCX), (const T*)F(hetddft_ctx::F_FCY),
+    S2LAUNCH((k_force_grid<T>), dim3(region_blocks()), dim3(kBlock), 0, ga, (const int*)c->reg,
+             (const T*)F(hetddft_ctx::F_FCX), (const T*)F(hetddft_ctx::F_FCY),
              (const T*)F(hetddft_ctx::F_FCZ), (const T*)cur, (const T*)psix,
-             (const T*)F(hetddft_ctx::F_HPSIX), c->fcorner);
+             (const T*)F(hetddft_ctx::F_HPSIX), c->fgather);
     LAUNCHCHK(c);
-    S2LAUNCH(k_blend, dim3(blocks_for(3LL * c->p.n_par)), dim3(kBlock), 0, b, c->rimp,
-             (const double*)c->fcorner, c->aimp);
+    S2LAUNCH((k_force_blend<T>), dim3(blocks_for(3LL * c->p.n_par)), dim3(kBlock), 0, ga,
+             (const int*)c->reg, c->rimp, (const T*)F(hetddft_ctx::F_FCX), (const T*)F(hetddft_ctx::F_FCY),
+             (const T*)F(hetddft_ctx::F_FCZ), (const T*)cur, (const T*)psix,
+             (const T*)F(hetddft_ctx::F_HPSIX), (const double*)c->fgather, c->aimp);
     LAUNCHCHK(c);
   }
   // pot4 assembly with the alpha_s divergence (term_alfa.f90:85-113, poten.f90:80-124)
@@ -1915,7 +1917,7 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
   for (int k = 0; k < hetddft_ctx::F_COUNT; ++k) want.push_back({&c->fld[k], n * c->tsz});
   want.push_back({(void**)&c->counts, n * 4});
   want.push_back({(void**)&c->reg, kRegLen * sizeof(int)});
-  want.push_back({(void**)&c->fcorner, (size_t)24 * std::max(p->n_par, 1) * 8});
+  want.push_back({(void**)&c->fgather, (size_t)24 * std::max(p->n_par, 1) * 8});
   for (int k = 0; k < kSpectra; ++k) want.push_back({&c->fs.spec[k], h * 2 * c->tsz});
   want.push_back({&c->fs.kfvlj4, h * c->tsz});
   want.push_back({&c->fs.kwcgk, h * c->tsz});
@@ -1969,8 +1971,8 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
     if (rc) return bad(rc);
     s1.denx = s1.denxa;
     // occupied region: nothing live yet
-    const int r0[kRegLen] = {0x7fffffff, 0x7fffffff, 0x7fffffff, -1, -1, -1, 0x7fffffff, 0x7fffffff, 0x7fffffff,
-                             -1, -1, -1, 0x7fffffff, 0x7fffffff, 0x7fffffff, -1, -1, -1};
+    int r0[kRegLen];
+    for (int k = 0; k < kRegLen; ++k) r0[k] = k % 6 < 3 ? 0x7fffffff : -1;
     for (int* rg : {s0.reg, s1.reg})
       if (hipMemcpy(rg, r0, sizeof r0, hipMemcpyHostToDevice) != hipSuccess) {
         fail(c, HETDDFT_E_HIP, "region init failed");

@@ -133,3 +133,47 @@ def test_trajectory_tile_shapes(hip, name, tile, zc, xcd, monkeypatch):
         assert rel_max(r, o.rimp) < TOL, it
         assert rel_max(g.get_field("pot4"), o.pot4) < TOL, it
     g.close()
+
+
+@pytest.mark.parametrize("cloud", ["cluster_and_strays", "uniform"])
+def test_force_gather_any_cloud(hip, cloud):
+    """The force gather evaluates F_c once per point of the box spanned by the
+    particles' cell corners when that box holds at most 8 points per
+    particle, and at each particle's own corners otherwise.  A tight cluster
+    with a few particles outside the grid (zero force, potenimp.f90:104-105)
+    takes the first path, a cloud spread over the whole box the second; both
+    match the oracle over start-up and PC steps."""
+    import inputs
+    from oracle import hetddft_oracle as orc
+    wl = inputs.WORKLOADS["t32"]
+    p = inputs.params_for(wl)
+    k = hip.make_kernels(p)
+    psi, r, v = inputs.initial_state(wl)
+    rng = np.random.default_rng(11)
+    if cloud == "uniform":
+        r = rng.uniform(-0.95 * wl.xmax, 0.95 * wl.xmax, size=r.shape)
+    else:
+        r = 0.4 * rng.standard_normal(r.shape)
+        r[0, :3] = 1.2 * wl.xmax      # beyond the last cell in x
+        r[1, 3] = -1.1 * wl.xmax      # before the first cell in y
+        r[2, 4] = wl.xmax - 0.5 * wl.h  # past the last grid point: outside [1, n-1]
+    o = orc.Oracle(oracle_params(wl), k)
+    o.set_state(psi, r, v)
+    g = hip.Instance(p)
+    g.set_kernels(k)
+    g.set_state(psi, r, v)
+    print(f"force gather {cloud}", flush=True)
+    o.start()
+    g.start()
+    _, _, _, a = g.get_state()
+    assert rel_max(a, o.aimp) < TOL
+    if cloud != "uniform":
+        assert np.all(a[:, :5] == 0.0)
+    for it in range(3 + 2):
+        o.iterate(it, wl.deltat)
+        g.iterate(wl.deltat)
+        psi_g, r_g, v_g, a = g.get_state()
+        assert rel_max(a, o.aimp) < TOL, it
+        assert rel_max(r_g, o.rimp) < TOL, it
+        assert rel_max(psi_g, o.psi) < TOL, it
+    g.close()
