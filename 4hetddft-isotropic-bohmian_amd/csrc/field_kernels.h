@@ -322,8 +322,23 @@ __device__ inline GatherCell gather_cell(const BoxArgs& b, const double* __restr
 // inclusive 0-based boxes.
 constexpr int kRegBox = 0, kRegLive = 6, kRegWork = 12, kRegGather = 18, kRegLen = 24;
 
-__global__ void k_hist(BoxArgs b, const double* __restrict__ r, unsigned int* __restrict__ cnt,
-                       int* __restrict__ reg) {
+// Partial boxes of k_hist: per workgroup, the occupied-cell box then the
+// gather-corner box, (min x,y,z, max x,y,z) each; empty: min 0x7fffffff, max -1.
+constexpr int kBoxPart = 12;
+
+__device__ inline void box_reduce_wave(int (&v)[kBoxPart]) {
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int d = 0; d < kBoxPart; ++d) {
+      const int o = __shfl_down(v[d], off, 64);
+      v[d] = d % 6 < 3 ? min(v[d], o) : max(v[d], o);
+    }
+}
+
+// NGP histogram (denpart.f90:20-31) and the workgroup's partial boxes; the
+// boxes are combined by k_reg_union, so no atomics contend on them.
+__global__ void __launch_bounds__(kBlock) k_hist(BoxArgs b, const double* __restrict__ r,
+                                                 unsigned int* __restrict__ cnt, int* __restrict__ part) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int ix = 0, iy = 0, iz = 0;
   if (i < b.npar) {
@@ -340,37 +355,51 @@ __global__ void k_hist(BoxArgs b, const double* __restrict__ r, unsigned int* __
   GatherCell q{};
   q.ok = false;
   if (i < b.npar) q = gather_cell(b, r, i);
-  // bounding boxes: reduce across the wavefront first, one atomic per wave
-  int v[12] = {in ? ix - 1 : 0x7fffffff, in ? iy - 1 : 0x7fffffff, in ? iz - 1 : 0x7fffffff,
-               in ? ix - 1 : -1, in ? iy - 1 : -1, in ? iz - 1 : -1,
-               q.ok ? q.ix - 1 : 0x7fffffff, q.ok ? q.iy - 1 : 0x7fffffff, q.ok ? q.iz - 1 : 0x7fffffff,
-               q.ok ? q.ix : -1, q.ok ? q.iy : -1, q.ok ? q.iz : -1};
-  for (int off = 32; off > 0; off >>= 1)
-    for (int d = 0; d < 12; ++d) {
-      const int o = __shfl_down(v[d], off, 64);
-      v[d] = d % 6 < 3 ? min(v[d], o) : max(v[d], o);
-    }
-  if ((threadIdx.x & 63) == 0) {
-    if (v[3] >= 0) {
-      for (int d = 0; d < 3; ++d) atomicMin(&reg[d], v[d]);
-      for (int d = 3; d < 6; ++d) atomicMax(&reg[d], v[d]);
-    }
-    if (v[9] >= 0) {
-      for (int d = 6; d < 9; ++d) atomicMin(&reg[kRegGather - 6 + d], v[d]);
-      for (int d = 9; d < 12; ++d) atomicMax(&reg[kRegGather - 6 + d], v[d]);
-    }
+  int v[kBoxPart] = {in ? ix - 1 : 0x7fffffff, in ? iy - 1 : 0x7fffffff, in ? iz - 1 : 0x7fffffff,
+                     in ? ix - 1 : -1, in ? iy - 1 : -1, in ? iz - 1 : -1,
+                     q.ok ? q.ix - 1 : 0x7fffffff, q.ok ? q.iy - 1 : 0x7fffffff, q.ok ? q.iz - 1 : 0x7fffffff,
+                     q.ok ? q.ix : -1, q.ok ? q.iy : -1, q.ok ? q.iz : -1};
+  box_reduce_wave(v);
+  constexpr int W = kBlock / 64;
+  __shared__ int sw[W][kBoxPart];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int d = 0; d < kBoxPart; ++d) sw[w][d] = v[d];
+  __syncthreads();
+  if (threadIdx.x < kBoxPart) {
+    const int d = threadIdx.x;
+    int m = sw[0][d];
+    for (int k = 1; k < W; ++k) m = d % 6 < 3 ? min(m, sw[k][d]) : max(m, sw[k][d]);
+    part[blockIdx.x * kBoxPart + d] = m;
   }
 }
 
+// Combines the nb partial boxes of k_hist into the occupied-cell box
+// (reg[0..5]) and the gather-corner box (reg[kRegGather..]); then
 // work = live U expand(bbox, margin); live := expand(bbox, margin).
-__global__ void k_reg_union(Grid g, int* __restrict__ reg, int margin) {
+// One wavefront.
+__global__ void k_reg_union(Grid g, int* __restrict__ reg, int margin, const int* __restrict__ part, int nb) {
+  int v[kBoxPart];
+#pragma unroll
+  for (int d = 0; d < kBoxPart; ++d) v[d] = d % 6 < 3 ? 0x7fffffff : -1;
+  for (int k = threadIdx.x; k < nb; k += 64)
+#pragma unroll
+    for (int d = 0; d < kBoxPart; ++d) {
+      const int o = part[k * kBoxPart + d];
+      v[d] = d % 6 < 3 ? min(v[d], o) : max(v[d], o);
+    }
+  box_reduce_wave(v);
   if (threadIdx.x != 0) return;
+  for (int d = 0; d < 6; ++d) {
+    reg[kRegBox + d] = v[d];
+    reg[kRegGather + d] = v[6 + d];
+  }
   const int n[3] = {g.nx, g.ny, g.nz};
   int e[6];
-  const bool empty = reg[3] < 0;
+  const bool empty = v[3] < 0;
   for (int d = 0; d < 3; ++d) {
-    e[d] = empty ? 0x7fffffff : max(reg[d] - margin, 0);
-    e[3 + d] = empty ? -1 : min(reg[3 + d] + margin, n[d] - 1);
+    e[d] = empty ? 0x7fffffff : max(v[d] - margin, 0);
+    e[3 + d] = empty ? -1 : min(v[3 + d] + margin, n[d] - 1);
   }
   for (int d = 0; d < 3; ++d) {
     const bool lempty = reg[kRegLive + 3] < 0;
@@ -409,16 +438,9 @@ __device__ inline Idx3 region_point(const RegionIter& it, int v) {
   return p;
 }
 
-// Clears the counts of the live region and resets the occupied-cell and
-// gather-corner boxes for the histogram that follows.
+// Clears the counts of the live region for the histogram that follows.
 __global__ void k_clear_counts(Grid g, int* __restrict__ reg, unsigned int* __restrict__ cnt) {
   const RegionIter it = region_iter(reg + kRegLive);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    reg[0] = reg[1] = reg[2] = 0x7fffffff;
-    reg[3] = reg[4] = reg[5] = -1;
-    reg[kRegGather] = reg[kRegGather + 1] = reg[kRegGather + 2] = 0x7fffffff;
-    reg[kRegGather + 3] = reg[kRegGather + 4] = reg[kRegGather + 5] = -1;
-  }
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < it.vol; v += gridDim.x * blockDim.x) {
     const Idx3 p = region_point(it, v);
     cnt[p.ix + (long long)g.nx * (p.iy + (long long)g.ny * p.iz)] = 0u;
@@ -486,12 +508,12 @@ struct GatherArgs {
   double m_imp;
 };
 
-// F_c at grid point j (0-based) of component c (potenimp.f90:52-69):
-// (fconv + sto7*(d_c psix*hpsix - psix*d_c hpsix))/m, sto7 = 1/denx if > 0.
+// Quantum part of F_c at grid point (ix,iy,iz), 0-based (potenimp.f90:52-69):
+// sto7*(d_c psix*hpsix - psix*d_c hpsix), sto7 = 1/denx if > 0.
 template <class R>
-__device__ inline double force_at(const GatherArgs& a, int c, const R* __restrict__ fconv,
-                                  const R* __restrict__ denx, const R* __restrict__ psix,
-                                  const R* __restrict__ hpsix, int ix, int iy, int iz) {
+__device__ inline double force_q(const GatherArgs& a, int c, const R* __restrict__ denx,
+                                 const R* __restrict__ psix, const R* __restrict__ hpsix, int ix, int iy,
+                                 int iz) {
   const Grid& g = a.b.g;
   const long long j = ix + (long long)g.nx * (iy + (long long)g.ny * iz);
   const double dx = (double)denx[j];
@@ -501,47 +523,62 @@ __device__ inline double force_at(const GatherArgs& a, int c, const R* __restric
   const long long st = c == 0 ? 1 : (c == 1 ? (long long)g.nx : (long long)g.nx * g.ny);
   const double s1 = d13(psix, j, pos, n, st, a.d1[c]);
   const double s4 = d13(hpsix, j, pos, n, st, a.d1[c]);
-  return ((double)fconv[j] + sto7 * (s1 * (double)hpsix[j] - (double)psix[j] * s4)) / a.m_imp;
+  return sto7 * (s1 * (double)hpsix[j] - (double)psix[j] * s4);
 }
 
-// The force field F_c on the gather box (the union of the particles' cell
-// corners): one thread per (component, box point), out[c*vol + local], local
-// x fastest.  Evaluated once per grid point instead of once per (particle,
-// corner), which shares the stencils of the corners that several particles'
-// cells have in common.  Skipped when the box holds more than 8 points per
-// particle (3 vol > 24 N_par): k_force_blend then evaluates the corners
-// itself.  Either way each corner value is the same force_at result.
+// F_c = (fconv + quantum part)/m_imp at grid point j.
 template <class R>
-__global__ void __launch_bounds__(kBlock) k_force_grid(GatherArgs a, const int* __restrict__ reg,
-                                                       const R* __restrict__ fcx, const R* __restrict__ fcy,
-                                                       const R* __restrict__ fcz, const R* __restrict__ denx,
-                                                       const R* __restrict__ psix,
-                                                       const R* __restrict__ hpsix,
-                                                       double* __restrict__ out) {
+__device__ inline double force_of(const GatherArgs& a, const R* __restrict__ fconv, long long j, double q) {
+  return ((double)fconv[j] + q) / a.m_imp;
+}
+
+// The quantum part of F_c wherever the trilinear gather reads it.  When the
+// gather box (the union of the particles' cell corners) holds at most 8
+// points per particle, once per (component, box point): out[c*vol + local],
+// local x fastest; evaluated once per grid point instead of once per
+// (particle, corner), it shares the stencils of the corners that several
+// particles' cells have in common.  Otherwise (a sparse cloud) once per
+// (component, corner, particle): out[(c*8 + corner)*N_par + i], corner bits
+// (dx,dy,dz) = (bit2,bit1,bit0) in the order x000..x111 of blend_103.  The
+// buffer holds 24 N_par doubles either way.
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_force_q(GatherArgs a, const int* __restrict__ reg,
+                                                    const double* __restrict__ rp, const R* __restrict__ denx,
+                                                    const R* __restrict__ psix, const R* __restrict__ hpsix,
+                                                    double* __restrict__ out) {
   const RegionIter it = region_iter(reg + kRegGather);
-  if (it.vol > 8 * a.b.npar) return;
-  const int n = 3 * it.vol;
-  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    const int c = v / it.vol;
-    const Idx3 p = region_point(it, v - c * it.vol);
-    const R* fc = c == 0 ? fcx : (c == 1 ? fcy : fcz);
-    out[v] = force_at(a, c, fc, denx, psix, hpsix, p.ix, p.iy, p.iz);
+  const int np = a.b.npar;
+  if (it.vol <= 8 * np) {
+    const int n = 3 * it.vol;
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+      const int c = v / it.vol;
+      const Idx3 p = region_point(it, v - c * it.vol);
+      out[v] = force_q(a, c, denx, psix, hpsix, p.ix, p.iy, p.iz);
+    }
+  } else {
+    const int n = 24 * np;
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+      const int slot = v / np;
+      const int i = v - slot * np;
+      const int c = slot >> 3, k = slot & 7;
+      const GatherCell q = gather_cell(a.b, rp, i);
+      out[v] = q.ok ? force_q(a, c, denx, psix, hpsix, q.ix - 1 + ((k >> 2) & 1), q.iy - 1 + ((k >> 1) & 1),
+                              q.iz - 1 + (k & 1))
+                    : 0.0;
+    }
   }
 }
 
-// Trilinear blend of F_c at the particles (blend_103, potenimp.f90:278-286)
-// from the 8 corners x000..x111 of each particle's cell (corner bits
-// (dx,dy,dz)); zero outside [1, n-1] (potenimp.f90:104-105).  One thread
-// per (particle, component); corners from k_force_grid's box, or evaluated
-// here when that box was too large.
+// Trilinear blend of F_c = (fconv + quantum part)/m_imp at the particles
+// (blend_103, potenimp.f90:278-286) from the 8 corners x000..x111 of each
+// particle's cell; zero outside [1, n-1] (potenimp.f90:104-105).  One
+// thread per (particle, component); the quantum parts in either layout of
+// k_force_q.
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_force_blend(GatherArgs a, const int* __restrict__ reg,
                                                         const double* __restrict__ rp,
                                                         const R* __restrict__ fcx, const R* __restrict__ fcy,
-                                                        const R* __restrict__ fcz, const R* __restrict__ denx,
-                                                        const R* __restrict__ psix,
-                                                        const R* __restrict__ hpsix,
-                                                        const double* __restrict__ fgrid,
+                                                        const R* __restrict__ fcz, const double* __restrict__ fq,
                                                         double* __restrict__ acc) {
   const int np = a.b.npar;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -554,18 +591,20 @@ __global__ void __launch_bounds__(kBlock) k_force_blend(GatherArgs a, const int*
     return;
   }
   const RegionIter it = region_iter(reg + kRegGather);
+  const R* fc = c == 0 ? fcx : (c == 1 ? fcy : fcz);
+  const Grid& g = a.b.g;
+  const long long j0 = (q.ix - 1) + (long long)g.nx * ((q.iy - 1) + (long long)g.ny * (q.iz - 1));
+  const long long gy = g.nx, gz = (long long)g.nx * g.ny;
+  const bool box = it.vol <= 8 * np;
+  const double* f = box ? fq + (long long)c * it.vol + (q.ix - 1 - it.x0) +
+                              it.nxr * ((q.iy - 1 - it.y0) + it.nyr * (q.iz - 1 - it.z0))
+                        : fq + (long long)c * 8 * np + i;
+  const int sx = box ? 1 : 4 * np, sy = box ? it.nxr : 2 * np, sz = box ? it.nxr * it.nyr : np;
   double x[8];
-  if (it.vol <= 8 * np) {
-    const double* f = fgrid + (long long)c * it.vol + (q.ix - 1 - it.x0) +
-                      it.nxr * ((q.iy - 1 - it.y0) + it.nyr * (q.iz - 1 - it.z0));
-    const int sy = it.nxr, sz = it.nxr * it.nyr;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) x[k] = f[((k >> 2) & 1) + sy * ((k >> 1) & 1) + sz * (k & 1)];
-  } else {
-    const R* fc = c == 0 ? fcx : (c == 1 ? fcy : fcz);
-    for (int k = 0; k < 8; ++k)
-      x[k] = force_at(a, c, fc, denx, psix, hpsix, q.ix - 1 + ((k >> 2) & 1), q.iy - 1 + ((k >> 1) & 1),
-                      q.iz - 1 + (k & 1));
+  for (int k = 0; k < 8; ++k) {
+    const int dx = (k >> 2) & 1, dy = (k >> 1) & 1, dz = k & 1;
+    x[k] = force_of(a, fc, j0 + dx + gy * dy + gz * dz, f[sx * dx + sy * dy + sz * dz]);
   }
   const double x000 = x[0], x001 = x[1], x010 = x[2], x011 = x[3];
   const double x100 = x[4], x101 = x[5], x110 = x[6], x111 = x[7];

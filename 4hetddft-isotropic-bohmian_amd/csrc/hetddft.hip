@@ -194,7 +194,8 @@ struct hetddft_ctx {
   void* denx = nullptr;  // whichever of DENXA/DENXB holds the ironed histogram
   unsigned int* counts = nullptr;
   int* reg = nullptr;           // occupied-region bookkeeping (field_kernels.h)
-  double* fgather = nullptr;    // force field on the gather box (3 x 8 N_par doubles)
+  double* fgather = nullptr;    // quantum part of the force at the gather corners (24 N_par doubles)
+  int* hpart = nullptr;         // k_hist's partial boxes (chains run in order on one stream)
   // The particle density chain (histogram -> ironing -> psix -> hpsix) has
   // two buffer sets, each with its own counts and live/work regions.  A chain
   // queued on st2 writes the set that is not current; the potential
@@ -821,6 +822,16 @@ BoxArgs box_args(const hetddft_ctx* c) {
   return b;
 }
 
+GatherArgs gather_args(const hetddft_ctx* c) {
+  GatherArgs ga{};
+  ga.b = box_args(c);
+  ga.d1[0] = c->d1x;
+  ga.d1[1] = c->d1y;
+  ga.d1[2] = c->d1z;
+  ga.m_imp = c->m_imp;
+  return ga;
+}
+
 // Grid-stride launches over a device-side region.  The chain runs beside
 // the field kernels, so it takes few CUs (HETDDFT_REGION_BLOCKS overrides).
 unsigned region_blocks() {
@@ -843,10 +854,11 @@ int issue_chain(hetddft_ctx* c) {
   S2LAUNCH(k_clear_counts, dim3(region_blocks()), dim3(kBlock), 0, g, S.reg, S.counts);
   LAUNCHCHK(c);
   if (c->p.n_par > 0) {
-    S2LAUNCH(k_hist, dim3(blocks_for(c->p.n_par)), dim3(kBlock), 0, b, c->rimp, S.counts, S.reg);
+    S2LAUNCH(k_hist, dim3(blocks_for(c->p.n_par)), dim3(kBlock), 0, b, c->rimp, S.counts, c->hpart);
     LAUNCHCHK(c);
   }
-  S2LAUNCH(k_reg_union, dim3(1), dim3(64), 0, g, S.reg, c->p.ironx + 6);
+  S2LAUNCH(k_reg_union, dim3(1), dim3(64), 0, g, S.reg, c->p.ironx + 6, (const int*)c->hpart,
+           c->p.n_par > 0 ? (int)blocks_for(c->p.n_par) : 0);
   LAUNCHCHK(c);
   const double hxyz = c->hx * c->hy * c->hz;
   const double npar = (double)c->p.n_par;
@@ -1009,21 +1021,12 @@ int veval(hetddft_ctx* c) {
   // pot4 respectively; st waits for the gather after the assembly.
   if (c->p.n_par > 0) {
     RET(fork_side(c));
-    GatherArgs ga{};
-    ga.b = b;
-    ga.d1[0] = c->d1x;
-    ga.d1[1] = c->d1y;
-    ga.d1[2] = c->d1z;
-    ga.m_imp = c->m_imp;
-    S2LAUNCH((k_force_grid<T>), dim3(region_blocks()), dim3(kBlock), 0, ga, (const int*)c->reg,
-             (const T*)F(hetddft_ctx::F_FCX), (const T*)F(hetddft_ctx::F_FCY),
-             (const T*)F(hetddft_ctx::F_FCZ), (const T*)cur, (const T*)psix,
-             (const T*)F(hetddft_ctx::F_HPSIX), c->fgather);
+    S2LAUNCH((k_force_q<T>), dim3(region_blocks()), dim3(kBlock), 0, gather_args(c), (const int*)c->reg,
+             c->rimp, (const T*)cur, (const T*)psix, (const T*)F(hetddft_ctx::F_HPSIX), c->fgather);
     LAUNCHCHK(c);
-    S2LAUNCH((k_force_blend<T>), dim3(blocks_for(3LL * c->p.n_par)), dim3(kBlock), 0, ga,
+    S2LAUNCH((k_force_blend<T>), dim3(blocks_for(3LL * c->p.n_par)), dim3(kBlock), 0, gather_args(c),
              (const int*)c->reg, c->rimp, (const T*)F(hetddft_ctx::F_FCX), (const T*)F(hetddft_ctx::F_FCY),
-             (const T*)F(hetddft_ctx::F_FCZ), (const T*)cur, (const T*)psix,
-             (const T*)F(hetddft_ctx::F_HPSIX), (const double*)c->fgather, c->aimp);
+             (const T*)F(hetddft_ctx::F_FCZ), (const double*)c->fgather, c->aimp);
     LAUNCHCHK(c);
   }
   // pot4 assembly with the alpha_s divergence (term_alfa.f90:85-113, poten.f90:80-124)
@@ -1918,6 +1921,7 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
   want.push_back({(void**)&c->counts, n * 4});
   want.push_back({(void**)&c->reg, kRegLen * sizeof(int)});
   want.push_back({(void**)&c->fgather, (size_t)24 * std::max(p->n_par, 1) * 8});
+  want.push_back({(void**)&c->hpart, (size_t)kBoxPart * blocks_for(std::max(p->n_par, 1)) * sizeof(int)});
   for (int k = 0; k < kSpectra; ++k) want.push_back({&c->fs.spec[k], h * 2 * c->tsz});
   want.push_back({&c->fs.kfvlj4, h * c->tsz});
   want.push_back({&c->fs.kwcgk, h * c->tsz});
