@@ -438,15 +438,6 @@ __device__ inline Idx3 region_point(const RegionIter& it, int v) {
   return p;
 }
 
-// Clears the counts of the live region for the histogram that follows.
-__global__ void k_clear_counts(Grid g, int* __restrict__ reg, unsigned int* __restrict__ cnt) {
-  const RegionIter it = region_iter(reg + kRegLive);
-  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < it.vol; v += gridDim.x * blockDim.x) {
-    const Idx3 p = region_point(it, v);
-    cnt[p.ix + (long long)g.nx * (p.iy + (long long)g.ny * p.iz)] = 0u;
-  }
-}
-
 // denx = count/N_par/hxyz (denpart.f90:33); optionally the first ironing
 // pass (ironing.f90:16-28) is applied on the fly.
 template <class R>
@@ -485,16 +476,20 @@ __global__ void __launch_bounds__(kBlock) k_iron(Grid g, const int* __restrict__
   }
 }
 
-// hpsix = -h2o2mx*(d2x+d2y+d2z)psix (denpart.f90:42-46)
+// hpsix = -h2o2mx*(d2x+d2y+d2z)psix (denpart.f90:42-46).  It also clears
+// the NGP counts over the work region (which holds every occupied cell), so
+// the counts are zero everywhere between chains and the next histogram needs
+// no clearing pass.
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_hpsix(Grid g, const int* __restrict__ reg, const R* __restrict__ psix,
                                                    Stencil7<double, true> wx, Stencil7<double, true> wy,
                                                    Stencil7<double, true> wz, double h2o2mx,
-                                                   R* __restrict__ hpsix) {
+                                                   R* __restrict__ hpsix, unsigned int* __restrict__ cnt) {
   const RegionIter it = region_iter(reg + kRegWork);
   for (int vv = blockIdx.x * blockDim.x + threadIdx.x; vv < it.vol; vv += gridDim.x * blockDim.x) {
   const Idx3 p = region_point(it, vv);
   const long long i = p.ix + (long long)g.nx * (p.iy + (long long)g.ny * p.iz);
+  cnt[i] = 0u;  // the first ironing pass has read the counts
   const double sx = d13(psix, i, p.ix, g.nx, 1, wx);
   const double sy = d13(psix, i, p.iy, g.ny, g.nx, wy);
   const double sz = d13(psix, i, p.iz, g.nz, (long long)g.nx * g.ny, wz);

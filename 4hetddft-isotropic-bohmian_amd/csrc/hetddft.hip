@@ -851,8 +851,6 @@ int issue_chain(hetddft_ctx* c) {
   // ---- potenimp: denpart (denpart.f90:5-49) on the occupied region only,
   // queued on the side stream into the set that is not current
   BoxArgs b = box_args(c);
-  S2LAUNCH(k_clear_counts, dim3(region_blocks()), dim3(kBlock), 0, g, S.reg, S.counts);
-  LAUNCHCHK(c);
   if (c->p.n_par > 0) {
     S2LAUNCH(k_hist, dim3(blocks_for(c->p.n_par)), dim3(kBlock), 0, b, c->rimp, S.counts, c->hpart);
     LAUNCHCHK(c);
@@ -878,7 +876,7 @@ int issue_chain(hetddft_ctx* c) {
   S.denx = cur;
   S2LAUNCH((k_hpsix<T>), dim3(region_blocks()), dim3(kBlock), 0, g, (const int*)S.reg, (const T*)psix,
            compact<double, true>(c->d2x), compact<double, true>(c->d2y), compact<double, true>(c->d2z), c->h2o2mx,
-           (T*)S.hpsix);
+           (T*)S.hpsix, S.counts);
   LAUNCHCHK(c);
   if (!c->dry && !c->one_stream) HIPCHK(c, hipEventRecord(c->ev_chain, c->st2));
   c->chain_pending = true;
