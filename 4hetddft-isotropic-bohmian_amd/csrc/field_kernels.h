@@ -335,8 +335,35 @@ __device__ inline void box_reduce_wave(int (&v)[kBoxPart]) {
     }
 }
 
-// NGP histogram (denpart.f90:20-31) and the workgroup's partial boxes; the
-// boxes are combined by k_reg_union, so no atomics contend on them.
+// Occupied-cell box v[0..5] and gather-corner box v[6..11] -> reg; then
+// work = live U expand(bbox, margin); live := expand(bbox, margin).
+__device__ inline void reg_union_apply(Grid g, int* __restrict__ reg, int margin, const int (&v)[kBoxPart]) {
+  for (int d = 0; d < 6; ++d) {
+    reg[kRegBox + d] = v[d];
+    reg[kRegGather + d] = v[6 + d];
+  }
+  const int n[3] = {g.nx, g.ny, g.nz};
+  int e[6];
+  const bool empty = v[3] < 0;
+  for (int d = 0; d < 3; ++d) {
+    e[d] = empty ? 0x7fffffff : max(v[d] - margin, 0);
+    e[3 + d] = empty ? -1 : min(v[3 + d] + margin, n[d] - 1);
+  }
+  for (int d = 0; d < 3; ++d) {
+    const bool lempty = reg[kRegLive + 3] < 0;
+    const int lo = lempty ? e[d] : min(reg[kRegLive + d], e[d]);
+    const int hi = lempty ? e[3 + d] : max(reg[kRegLive + 3 + d], e[3 + d]);
+    reg[kRegWork + d] = lo;
+    reg[kRegWork + 3 + d] = hi;
+  }
+  for (int d = 0; d < 6; ++d) reg[kRegLive + d] = e[d];
+}
+
+// NGP histogram (denpart.f90:20-31).  Each workgroup reduces its particles'
+// boxes in registers and LDS and writes one partial, which k_reg_union
+// combines, so no atomics contend on the boxes.  (Combining them in the
+// last workgroup to finish, behind a device-scope fence in every workgroup,
+// measured 3-4% slower per step at 128^3.)
 __global__ void __launch_bounds__(kBlock) k_hist(BoxArgs b, const double* __restrict__ r,
                                                  unsigned int* __restrict__ cnt, int* __restrict__ part) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -374,10 +401,8 @@ __global__ void __launch_bounds__(kBlock) k_hist(BoxArgs b, const double* __rest
   }
 }
 
-// Combines the nb partial boxes of k_hist into the occupied-cell box
-// (reg[0..5]) and the gather-corner box (reg[kRegGather..]); then
-// work = live U expand(bbox, margin); live := expand(bbox, margin).
-// One wavefront.
+// Combines the nb partial boxes of k_hist (none without particles) and
+// updates the regions.  One wavefront.
 __global__ void k_reg_union(Grid g, int* __restrict__ reg, int margin, const int* __restrict__ part, int nb) {
   int v[kBoxPart];
 #pragma unroll
@@ -389,26 +414,7 @@ __global__ void k_reg_union(Grid g, int* __restrict__ reg, int margin, const int
       v[d] = d % 6 < 3 ? min(v[d], o) : max(v[d], o);
     }
   box_reduce_wave(v);
-  if (threadIdx.x != 0) return;
-  for (int d = 0; d < 6; ++d) {
-    reg[kRegBox + d] = v[d];
-    reg[kRegGather + d] = v[6 + d];
-  }
-  const int n[3] = {g.nx, g.ny, g.nz};
-  int e[6];
-  const bool empty = v[3] < 0;
-  for (int d = 0; d < 3; ++d) {
-    e[d] = empty ? 0x7fffffff : max(v[d] - margin, 0);
-    e[3 + d] = empty ? -1 : min(v[3 + d] + margin, n[d] - 1);
-  }
-  for (int d = 0; d < 3; ++d) {
-    const bool lempty = reg[kRegLive + 3] < 0;
-    const int lo = lempty ? e[d] : min(reg[kRegLive + d], e[d]);
-    const int hi = lempty ? e[3 + d] : max(reg[kRegLive + 3 + d], e[3 + d]);
-    reg[kRegWork + d] = lo;
-    reg[kRegWork + 3 + d] = hi;
-  }
-  for (int d = 0; d < 6; ++d) reg[kRegLive + d] = e[d];
+  if (threadIdx.x == 0) reg_union_apply(g, reg, margin, v);
 }
 
 // Grid-stride iteration over an inclusive region box.
@@ -473,6 +479,96 @@ __global__ void __launch_bounds__(kBlock) k_iron(Grid g, const int* __restrict__
   }
   dst[i] = (R)v;
   if (psix_out) psix_out[i] = (R)sqrt((double)(R)v);
+  }
+}
+
+// denx and psix of NP ironing passes (1 <= NP <= 3) in one launch: each
+// workgroup takes 8^3-point tiles of the work region and keeps the levels in
+// LDS -- the histogram densities on the tile grown by NP, pass k on the tile
+// grown by NP - k -- so the halo is recomputed instead of exchanged through
+// HBM between launches.  Every value is formed exactly as in k_iron (same
+// operand order, level >= 1 rounded to R); points outside the grid are
+// never read by points inside it, and points outside the work region hold
+// zero at every level (the region covers the histogram grown by more than
+// NP), so the results equal NP launches of k_iron bit for bit.
+template <class R, int NP>
+__global__ void __launch_bounds__(kBlock) k_iron_fused(Grid g, const int* __restrict__ reg,
+                                                        const unsigned int* __restrict__ cnt, R* __restrict__ dst,
+                                                        R* __restrict__ psix, double npar, double hxyz) {
+  constexpr int TB = 8;
+  constexpr int L0 = TB + 2 * NP;
+  constexpr int L1 = NP >= 2 ? TB + 2 * (NP - 1) : 1;
+  constexpr int L2 = NP >= 3 ? TB + 2 * (NP - 2) : 1;
+  __shared__ double lv0[L0 * L0 * L0];
+  __shared__ R lv1[L1 * L1 * L1];
+  __shared__ R lv2[L2 * L2 * L2];
+  const int* w = reg + kRegWork;
+  const RegionIter it = region_iter(w);
+  if (it.vol == 0) return;
+  const int nzr = w[5] - w[2] + 1;
+  const int ntx = (it.nxr + TB - 1) / TB, nty = (it.nyr + TB - 1) / TB, ntz = (nzr + TB - 1) / TB;
+  const int ntiles = ntx * nty * ntz;
+  const int tid = threadIdx.x;
+  auto interior = [&](int x, int y, int z) {
+    return !(x == 0 || y == 0 || z == 0 || x == g.nx - 1 || y == g.ny - 1 || z == g.nz - 1);
+  };
+  auto ingrid = [&](int x, int y, int z) {
+    return x >= 0 && y >= 0 && z >= 0 && x < g.nx && y < g.ny && z < g.nz;
+  };
+  // one smoothing pass from a level of side S (values as double via ld) to
+  // the level of side S - 2 whose origin is one point further in
+  auto smooth = [&](auto ld, int S, int e, int ox, int oy, int oz) -> double {
+    const int D = S - 2;
+    const int dz = e / (D * D), rem = e - dz * D * D, dy = rem / D, dx = rem - dy * D;
+    const int x = ox + dx, y = oy + dy, z = oz + dz;
+    const int sc = (dx + 1) + S * ((dy + 1) + S * (dz + 1));
+    const double c = ld(sc);
+    if (!ingrid(x, y, z)) return 0.0;
+    if (!interior(x, y, z)) return c;
+    double s = ld(sc - 1) + ld(sc + 1);
+    s = s + ld(sc - S);
+    s = s + ld(sc + S);
+    s = s + ld(sc - S * S);
+    s = s + ld(sc + S * S);
+    return 8.333333333333333e-2 * s + 0.5 * c;
+  };
+  auto ld0 = [&](int k) { return lv0[k]; };
+  auto ld1 = [&](int k) { return (double)lv1[k]; };
+  auto ld2 = [&](int k) { return (double)lv2[k]; };
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int tz = t / (ntx * nty), tr = t - tz * ntx * nty, ty = tr / ntx, tx = tr - ty * ntx;
+    const int ox = it.x0 + tx * TB, oy = it.y0 + ty * TB, oz = it.z0 + tz * TB;
+    for (int e = tid; e < L0 * L0 * L0; e += blockDim.x) {
+      const int lz = e / (L0 * L0), rem = e - lz * L0 * L0, ly = rem / L0, lx = rem - ly * L0;
+      const int x = ox - NP + lx, y = oy - NP + ly, z = oz - NP + lz;
+      lv0[e] = ingrid(x, y, z) ? ((double)cnt[x + (long long)g.nx * (y + (long long)g.ny * z)] / npar) / hxyz
+                               : 0.0;
+    }
+    __syncthreads();
+    if constexpr (NP >= 2) {
+      for (int e = tid; e < L1 * L1 * L1; e += blockDim.x)
+        lv1[e] = (R)smooth(ld0, L0, e, ox - (NP - 1), oy - (NP - 1), oz - (NP - 1));
+      __syncthreads();
+    }
+    if constexpr (NP >= 3) {
+      for (int e = tid; e < L2 * L2 * L2; e += blockDim.x)
+        lv2[e] = (R)smooth(ld1, L1, e, ox - (NP - 2), oy - (NP - 2), oz - (NP - 2));
+      __syncthreads();
+    }
+    for (int e = tid; e < TB * TB * TB; e += blockDim.x) {
+      double v;
+      if constexpr (NP == 1) v = smooth(ld0, L0, e, ox, oy, oz);
+      else if constexpr (NP == 2) v = smooth(ld1, L1, e, ox, oy, oz);
+      else v = smooth(ld2, L2, e, ox, oy, oz);
+      const int dz = e / (TB * TB), rem = e - dz * TB * TB, dy = rem / TB, dx = rem - dy * TB;
+      const int x = ox + dx, y = oy + dy, z = oz + dz;
+      if (x <= w[3] && y <= w[4] && z <= w[5]) {
+        const long long i = x + (long long)g.nx * (y + (long long)g.ny * z);
+        dst[i] = (R)v;
+        psix[i] = (R)sqrt((double)(R)v);
+      }
+    }
+    __syncthreads();
   }
 }
 

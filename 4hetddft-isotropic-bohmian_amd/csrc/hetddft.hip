@@ -196,6 +196,7 @@ struct hetddft_ctx {
   int* reg = nullptr;           // occupied-region bookkeeping (field_kernels.h)
   double* fgather = nullptr;    // quantum part of the force at the gather corners (24 N_par doubles)
   int* hpart = nullptr;         // k_hist's partial boxes (chains run in order on one stream)
+  bool iron_fused = true;       // ironing passes in one launch (HETDDFT_IRON_FUSED=0: one per pass)
   // The particle density chain (histogram -> ironing -> psix -> hpsix) has
   // two buffer sets, each with its own counts and live/work regions.  A chain
   // queued on st2 writes the set that is not current; the potential
@@ -864,14 +865,27 @@ int issue_chain(hetddft_ctx* c) {
   T* oth = (T*)S.denxb;
   T* psix = (T*)S.psix;
   const int npass = c->p.ironx > 0 ? c->p.ironx : 1;
-  S2LAUNCH((k_iron<T>), dim3(region_blocks()), dim3(kBlock), 0, g, (const int*)S.reg, S.counts,
-           (const T*)nullptr, cur, npass == 1 ? psix : (T*)nullptr, npar, hxyz, c->p.ironx > 0 ? 1 : 0);
-  LAUNCHCHK(c);
-  for (int it = 1; it < npass; ++it) {
-    S2LAUNCH((k_iron<T>), dim3(region_blocks()), dim3(kBlock), 0, g, (const int*)S.reg,
-             (const unsigned int*)nullptr, (const T*)cur, oth, it == npass - 1 ? psix : (T*)nullptr, npar, hxyz, 1);
+  if (c->iron_fused && c->p.ironx >= 1 && c->p.ironx <= 3) {
+    // all passes in one launch
+    auto fused = [&](auto np) {
+      S2LAUNCH((k_iron_fused<T, decltype(np)::value>), dim3(region_blocks()), dim3(kBlock), 0, g,
+               (const int*)S.reg, (const unsigned int*)S.counts, cur, psix, npar, hxyz);
+    };
+    if (c->p.ironx == 1) fused(std::integral_constant<int, 1>{});
+    else if (c->p.ironx == 2) fused(std::integral_constant<int, 2>{});
+    else fused(std::integral_constant<int, 3>{});
     LAUNCHCHK(c);
-    std::swap(cur, oth);
+  } else {
+    S2LAUNCH((k_iron<T>), dim3(region_blocks()), dim3(kBlock), 0, g, (const int*)S.reg, S.counts,
+             (const T*)nullptr, cur, npass == 1 ? psix : (T*)nullptr, npar, hxyz, c->p.ironx > 0 ? 1 : 0);
+    LAUNCHCHK(c);
+    for (int it = 1; it < npass; ++it) {
+      S2LAUNCH((k_iron<T>), dim3(region_blocks()), dim3(kBlock), 0, g, (const int*)S.reg,
+               (const unsigned int*)nullptr, (const T*)cur, oth, it == npass - 1 ? psix : (T*)nullptr, npar,
+               hxyz, 1);
+      LAUNCHCHK(c);
+      std::swap(cur, oth);
+    }
   }
   S.denx = cur;
   S2LAUNCH((k_hpsix<T>), dim3(region_blocks()), dim3(kBlock), 0, g, (const int*)S.reg, (const T*)psix,
@@ -1858,6 +1872,8 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
     if (z && std::atoi(z) > 0) c->zc = std::atoi(z);
     const char* x = std::getenv("HETDDFT_XCD");
     if (x) c->xcd = std::atoi(x) != 0;
+    const char* fi = std::getenv("HETDDFT_IRON_FUSED");
+    if (fi) c->iron_fused = std::atoi(fi) != 0;
     const char* gph = std::getenv("HETDDFT_GRAPH");
     c->graph_on = gph && std::atoi(gph) != 0;
     c->graph_one_stream = gph && std::atoi(gph) == 2;

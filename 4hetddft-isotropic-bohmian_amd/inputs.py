@@ -150,5 +150,6 @@ def initial_state(wl: Workload):
 def params_for(wl: Workload, precision=0, **kw):
     """hetddft_params for a workload (loads the library)."""
     from hetddft import default_params  # noqa: E402  (package-local import)
+    kw.setdefault("ironx", wl.ironx)
     return default_params(nx=wl.n, ny=wl.n, nz=wl.n, xmax=wl.xmax, ymax=wl.xmax, zmax=wl.xmax,
-                          n_par=wl.n_par, mimpur=wl.mimpur, ironx=wl.ironx, precision=precision, **kw)
+                          n_par=wl.n_par, mimpur=wl.mimpur, precision=precision, **kw)

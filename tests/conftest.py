@@ -24,8 +24,9 @@ def hip():
 
 def oracle_params(wl, **kw):
     from oracle import hetddft_oracle as orc
+    kw.setdefault("ironx", wl.ironx)
     return orc.Params(nx=wl.n, ny=wl.n, nz=wl.n, xmax=wl.xmax, ymax=wl.xmax, zmax=wl.xmax,
-                      n_par=wl.n_par, mimpur=wl.mimpur, ironx=wl.ironx, **kw)
+                      n_par=wl.n_par, mimpur=wl.mimpur, **kw)
 
 
 def rel_max(a, b):

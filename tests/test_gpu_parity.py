@@ -177,3 +177,50 @@ def test_force_gather_any_cloud(hip, cloud):
         assert rel_max(r_g, o.rimp) < TOL, it
         assert rel_max(psi_g, o.psi) < TOL, it
     g.close()
+
+
+@pytest.mark.parametrize("ironx", [0, 1, 2, 3, 4])
+def test_ironing_passes(hip, ironx):
+    """ironx smoothing passes of the particle histogram (ironing.f90): the
+    fields of the particle chain and the force match the oracle for every
+    pass count (1-3 run in one launch, 0 and 4 one launch per pass)."""
+    wl, o, g = _setup(hip, "t32", gkw=dict(ironx=ironx), okw=dict(ironx=ironx))
+    print(f"ironing ironx={ironx}", flush=True)
+    o.start()
+    g.start()
+    for f in ("denx", "psix", "hpsix", "uimp"):
+        assert rel_max(g.get_field(f), getattr(o, f)) < TOL, f
+    for it in range(3 + 1):
+        o.iterate(it, wl.deltat)
+        g.iterate(wl.deltat)
+    psi, r, v, a = g.get_state()
+    assert rel_max(psi, o.psi) < TOL
+    assert rel_max(a, o.aimp) < TOL
+    g.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("ironx", [1, 2, 3])
+def test_fused_ironing_equals_one_launch_per_pass(hip, monkeypatch, precision, ironx):
+    """The fused ironing launch gives the same bits as one launch per pass,
+    in both precisions, over the start-up and a PC step (the work region
+    moves with the particles)."""
+    import inputs
+    wl = inputs.WORKLOADS["t32"]
+    psi, r, v = inputs.initial_state(wl)
+    p = inputs.params_for(wl, precision=precision, ironx=ironx)
+    k = hip.make_kernels(p)
+    out = []
+    for fused in ("1", "0"):
+        monkeypatch.setenv("HETDDFT_IRON_FUSED", fused)
+        print(f"ironing fused={fused} ironx={ironx} precision={precision}", flush=True)
+        g = hip.Instance(p)
+        g.set_kernels(k)
+        g.set_state(psi, r, v)
+        g.start()
+        for _ in range(3 + 2):
+            g.iterate(wl.deltat, want_errors=False)
+        out.append([g.get_field(f) for f in ("denx", "psix", "hpsix")] + list(g.get_state()))
+        g.close()
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
