@@ -337,26 +337,52 @@ __device__ inline void box_reduce_wave(int (&v)[kBoxPart]) {
 
 // Occupied-cell box v[0..5] and gather-corner box v[6..11] -> reg; then
 // work = live U expand(bbox, margin); live := expand(bbox, margin).
+// live := expand(box, margin), clamped to the grid (empty box: empty).
+__device__ inline void box_expand(Grid g, const int* box, int margin, int (&e)[6]) {
+  const int n[3] = {g.nx, g.ny, g.nz};
+  const bool empty = box[3] < 0;
+  for (int d = 0; d < 3; ++d) {
+    e[d] = empty ? 0x7fffffff : max(box[d] - margin, 0);
+    e[3 + d] = empty ? -1 : min(box[3 + d] + margin, n[d] - 1);
+  }
+}
+
+// work = live U e, from the live region before this evaluation.
+__device__ inline void work_region(const int* __restrict__ reg, const int (&e)[6], int (&w)[6]) {
+  const bool lempty = reg[kRegLive + 3] < 0;
+  for (int d = 0; d < 3; ++d) {
+    w[d] = lempty ? e[d] : min(reg[kRegLive + d], e[d]);
+    w[3 + d] = lempty ? e[3 + d] : max(reg[kRegLive + 3 + d], e[3 + d]);
+  }
+}
+
+// Combines nb partial boxes of k_hist over one wavefront.
+__device__ inline void box_reduce_parts(const int* __restrict__ part, int nb, int (&v)[kBoxPart]) {
+#pragma unroll
+  for (int d = 0; d < kBoxPart; ++d) v[d] = d % 6 < 3 ? 0x7fffffff : -1;
+  for (int k = (int)(threadIdx.x & 63); k < nb; k += 64)
+#pragma unroll
+    for (int d = 0; d < kBoxPart; ++d) {
+      const int o = part[k * kBoxPart + d];
+      v[d] = d % 6 < 3 ? min(v[d], o) : max(v[d], o);
+    }
+  box_reduce_wave(v);
+}
+
+// Occupied-cell box v[0..5] and gather-corner box v[6..11] -> reg; then
+// work = live U expand(bbox, margin); live := expand(bbox, margin).
 __device__ inline void reg_union_apply(Grid g, int* __restrict__ reg, int margin, const int (&v)[kBoxPart]) {
   for (int d = 0; d < 6; ++d) {
     reg[kRegBox + d] = v[d];
     reg[kRegGather + d] = v[6 + d];
   }
-  const int n[3] = {g.nx, g.ny, g.nz};
-  int e[6];
-  const bool empty = v[3] < 0;
-  for (int d = 0; d < 3; ++d) {
-    e[d] = empty ? 0x7fffffff : max(v[d] - margin, 0);
-    e[3 + d] = empty ? -1 : min(v[3 + d] + margin, n[d] - 1);
+  int e[6], w[6];
+  box_expand(g, v, margin, e);
+  work_region(reg, e, w);
+  for (int d = 0; d < 6; ++d) {
+    reg[kRegWork + d] = w[d];
+    reg[kRegLive + d] = e[d];
   }
-  for (int d = 0; d < 3; ++d) {
-    const bool lempty = reg[kRegLive + 3] < 0;
-    const int lo = lempty ? e[d] : min(reg[kRegLive + d], e[d]);
-    const int hi = lempty ? e[3 + d] : max(reg[kRegLive + 3 + d], e[3 + d]);
-    reg[kRegWork + d] = lo;
-    reg[kRegWork + 3 + d] = hi;
-  }
-  for (int d = 0; d < 6; ++d) reg[kRegLive + d] = e[d];
 }
 
 // NGP histogram (denpart.f90:20-31).  Each workgroup reduces its particles'
@@ -405,15 +431,7 @@ __global__ void __launch_bounds__(kBlock) k_hist(BoxArgs b, const double* __rest
 // updates the regions.  One wavefront.
 __global__ void k_reg_union(Grid g, int* __restrict__ reg, int margin, const int* __restrict__ part, int nb) {
   int v[kBoxPart];
-#pragma unroll
-  for (int d = 0; d < kBoxPart; ++d) v[d] = d % 6 < 3 ? 0x7fffffff : -1;
-  for (int k = threadIdx.x; k < nb; k += 64)
-#pragma unroll
-    for (int d = 0; d < kBoxPart; ++d) {
-      const int o = part[k * kBoxPart + d];
-      v[d] = d % 6 < 3 ? min(v[d], o) : max(v[d], o);
-    }
-  box_reduce_wave(v);
+  box_reduce_parts(part, nb, v);
   if (threadIdx.x == 0) reg_union_apply(g, reg, margin, v);
 }
 
@@ -482,7 +500,12 @@ __global__ void __launch_bounds__(kBlock) k_iron(Grid g, const int* __restrict__
   }
 }
 
-// denx and psix of NP ironing passes (1 <= NP <= 3) in one launch: each
+// denx and psix of NP ironing passes (1 <= NP <= 3) in one launch, with the
+// region update of k_reg_union folded in: every workgroup combines the
+// histogram's partial boxes and forms the work region itself (the live
+// region it reads is not written in this launch); workgroup 0 stores the
+// boxes and the work region, and k_hpsix, which runs after every workgroup
+// here has read the old live region, stores the new one.  Each
 // workgroup takes 8^3-point tiles of the work region and keeps the levels in
 // LDS -- the histogram densities on the tile grown by NP, pass k on the tile
 // grown by NP - k -- so the halo is recomputed instead of exchanged through
@@ -492,7 +515,8 @@ __global__ void __launch_bounds__(kBlock) k_iron(Grid g, const int* __restrict__
 // zero at every level (the region covers the histogram grown by more than
 // NP), so the results equal NP launches of k_iron bit for bit.
 template <class R, int NP>
-__global__ void __launch_bounds__(kBlock) k_iron_fused(Grid g, const int* __restrict__ reg,
+__global__ void __launch_bounds__(kBlock) k_iron_fused(Grid g, int* __restrict__ reg,
+                                                        const int* __restrict__ part, int nb, int margin,
                                                         const unsigned int* __restrict__ cnt, R* __restrict__ dst,
                                                         R* __restrict__ psix, double npar, double hxyz) {
   constexpr int TB = 8;
@@ -502,7 +526,24 @@ __global__ void __launch_bounds__(kBlock) k_iron_fused(Grid g, const int* __rest
   __shared__ double lv0[L0 * L0 * L0];
   __shared__ R lv1[L1 * L1 * L1];
   __shared__ R lv2[L2 * L2 * L2];
-  const int* w = reg + kRegWork;
+  __shared__ int w[6];
+  if (threadIdx.x < 64) {
+    int v[kBoxPart];
+    box_reduce_parts(part, nb, v);
+    if (threadIdx.x == 0) {
+      int e[6], wr[6];
+      box_expand(g, v, margin, e);
+      work_region(reg, e, wr);
+      for (int d = 0; d < 6; ++d) w[d] = wr[d];
+      if (blockIdx.x == 0)
+        for (int d = 0; d < 6; ++d) {
+          reg[kRegBox + d] = v[d];
+          reg[kRegGather + d] = v[6 + d];
+          reg[kRegWork + d] = wr[d];
+        }
+    }
+  }
+  __syncthreads();
   const RegionIter it = region_iter(w);
   if (it.vol == 0) return;
   const int nzr = w[5] - w[2] + 1;
@@ -575,13 +616,20 @@ __global__ void __launch_bounds__(kBlock) k_iron_fused(Grid g, const int* __rest
 // hpsix = -h2o2mx*(d2x+d2y+d2z)psix (denpart.f90:42-46).  It also clears
 // the NGP counts over the work region (which holds every occupied cell), so
 // the counts are zero everywhere between chains and the next histogram needs
-// no clearing pass.
+// no clearing pass, and stores live := expand(box, margin) (the value it
+// already has when k_reg_union ran).
 template <class R>
-__global__ void __launch_bounds__(kBlock) k_hpsix(Grid g, const int* __restrict__ reg, const R* __restrict__ psix,
+__global__ void __launch_bounds__(kBlock) k_hpsix(Grid g, int* __restrict__ reg, const R* __restrict__ psix,
                                                    Stencil7<double, true> wx, Stencil7<double, true> wy,
                                                    Stencil7<double, true> wz, double h2o2mx,
-                                                   R* __restrict__ hpsix, unsigned int* __restrict__ cnt) {
+                                                   R* __restrict__ hpsix, unsigned int* __restrict__ cnt,
+                                                   int margin) {
   const RegionIter it = region_iter(reg + kRegWork);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    int e[6];
+    box_expand(g, reg + kRegBox, margin, e);
+    for (int d = 0; d < 6; ++d) reg[kRegLive + d] = e[d];
+  }
   for (int vv = blockIdx.x * blockDim.x + threadIdx.x; vv < it.vol; vv += gridDim.x * blockDim.x) {
   const Idx3 p = region_point(it, vv);
   const long long i = p.ix + (long long)g.nx * (p.iy + (long long)g.ny * p.iz);

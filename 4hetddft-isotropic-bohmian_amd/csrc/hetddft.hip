@@ -856,20 +856,24 @@ int issue_chain(hetddft_ctx* c) {
     S2LAUNCH(k_hist, dim3(blocks_for(c->p.n_par)), dim3(kBlock), 0, b, c->rimp, S.counts, c->hpart);
     LAUNCHCHK(c);
   }
-  S2LAUNCH(k_reg_union, dim3(1), dim3(64), 0, g, S.reg, c->p.ironx + 6, (const int*)c->hpart,
-           c->p.n_par > 0 ? (int)blocks_for(c->p.n_par) : 0);
-  LAUNCHCHK(c);
+  const int nb = c->p.n_par > 0 ? (int)blocks_for(c->p.n_par) : 0;
+  const int margin = c->p.ironx + 6;
+  const bool fused_iron = c->iron_fused && c->p.ironx >= 1 && c->p.ironx <= 3;
+  if (!fused_iron) {
+    S2LAUNCH(k_reg_union, dim3(1), dim3(64), 0, g, S.reg, margin, (const int*)c->hpart, nb);
+    LAUNCHCHK(c);
+  }
   const double hxyz = c->hx * c->hy * c->hz;
   const double npar = (double)c->p.n_par;
   T* cur = (T*)S.denxa;
   T* oth = (T*)S.denxb;
   T* psix = (T*)S.psix;
   const int npass = c->p.ironx > 0 ? c->p.ironx : 1;
-  if (c->iron_fused && c->p.ironx >= 1 && c->p.ironx <= 3) {
-    // all passes in one launch
+  if (fused_iron) {
+    // all passes (and the region update) in one launch
     auto fused = [&](auto np) {
-      S2LAUNCH((k_iron_fused<T, decltype(np)::value>), dim3(region_blocks()), dim3(kBlock), 0, g,
-               (const int*)S.reg, (const unsigned int*)S.counts, cur, psix, npar, hxyz);
+      S2LAUNCH((k_iron_fused<T, decltype(np)::value>), dim3(region_blocks()), dim3(kBlock), 0, g, S.reg,
+               (const int*)c->hpart, nb, margin, (const unsigned int*)S.counts, cur, psix, npar, hxyz);
     };
     if (c->p.ironx == 1) fused(std::integral_constant<int, 1>{});
     else if (c->p.ironx == 2) fused(std::integral_constant<int, 2>{});
@@ -888,9 +892,9 @@ int issue_chain(hetddft_ctx* c) {
     }
   }
   S.denx = cur;
-  S2LAUNCH((k_hpsix<T>), dim3(region_blocks()), dim3(kBlock), 0, g, (const int*)S.reg, (const T*)psix,
+  S2LAUNCH((k_hpsix<T>), dim3(region_blocks()), dim3(kBlock), 0, g, S.reg, (const T*)psix,
            compact<double, true>(c->d2x), compact<double, true>(c->d2y), compact<double, true>(c->d2z), c->h2o2mx,
-           (T*)S.hpsix, S.counts);
+           (T*)S.hpsix, S.counts, margin);
   LAUNCHCHK(c);
   if (!c->dry && !c->one_stream) HIPCHK(c, hipEventRecord(c->ev_chain, c->st2));
   c->chain_pending = true;
