@@ -197,6 +197,7 @@ struct hetddft_ctx {
   double* fgather = nullptr;    // quantum part of the force at the gather corners (24 N_par doubles)
   int* hpart = nullptr;         // k_hist's partial boxes (chains run in order on one stream)
   bool iron_fused = true;       // ironing passes in one launch (HETDDFT_IRON_FUSED=0: one per pass)
+  bool hpsix_late = true;       // H psi_X after the point the field side waits for (HETDDFT_HPSIX_LATE)
   // The particle density chain (histogram -> ironing -> psix -> hpsix) has
   // two buffer sets, each with its own counts and live/work regions.  A chain
   // queued on st2 writes the set that is not current; the potential
@@ -892,11 +893,14 @@ int issue_chain(hetddft_ctx* c) {
     }
   }
   S.denx = cur;
+  // the field side needs denx (batch 1); H psi_X enters only the force
+  // gather after batch 3, so with hpsix_late it stays off the point st waits for
+  if (c->hpsix_late && !c->dry && !c->one_stream) HIPCHK(c, hipEventRecord(c->ev_chain, c->st2));
   S2LAUNCH((k_hpsix<T>), dim3(region_blocks()), dim3(kBlock), 0, g, S.reg, (const T*)psix,
            compact<double, true>(c->d2x), compact<double, true>(c->d2y), compact<double, true>(c->d2z), c->h2o2mx,
            (T*)S.hpsix, S.counts, margin);
   LAUNCHCHK(c);
-  if (!c->dry && !c->one_stream) HIPCHK(c, hipEventRecord(c->ev_chain, c->st2));
+  if (!c->hpsix_late && !c->dry && !c->one_stream) HIPCHK(c, hipEventRecord(c->ev_chain, c->st2));
   c->chain_pending = true;
   c->side_pending = true;
   return HETDDFT_OK;
@@ -1098,7 +1102,7 @@ int veval(hetddft_ctx* c) {
                        (const T*)F(hetddft_ctx::F_IZ), (const T*)F(hetddft_ctx::F_UACONV), F(hetddft_ctx::F_POT4));
   }
   LAUNCHCHK(c);
-  if (c->p.n_par > 0) {
+  if (c->p.n_par > 0 || c->hpsix_late) {
     RET(join_side(c, c->ev_force));
     RET(prof_mark(c));
   }
@@ -1876,6 +1880,8 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
     if (z && std::atoi(z) > 0) c->zc = std::atoi(z);
     const char* x = std::getenv("HETDDFT_XCD");
     if (x) c->xcd = std::atoi(x) != 0;
+    const char* hl = std::getenv("HETDDFT_HPSIX_LATE");
+    if (hl) c->hpsix_late = std::atoi(hl) != 0;
     const char* fi = std::getenv("HETDDFT_IRON_FUSED");
     if (fi) c->iron_fused = std::atoi(fi) != 0;
     const char* gph = std::getenv("HETDDFT_GRAPH");
