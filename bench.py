@@ -176,7 +176,10 @@ def ensemble_main(args):
     wls = inputs.ensemble_c4(n_pc=args.steps)
     r64 = run_ensemble(wls, 0, args.steps, args.warmup, args.threads, tune=not args.no_tune, executor=args.executor)
     r32 = run_ensemble(wls, 1, args.steps, args.warmup, args.threads, tune=not args.no_tune, executor=args.executor)
-    rel = max(abs(a - b) / abs(a) for a, b in zip(r64["energies"], r32["energies"]))
+    rels = [abs(a - b) / abs(a) for a, b in zip(r64["energies"], r32["energies"])]
+    rel = max(rels)
+    worst = rels.index(rel)
+    abs_err = max(abs(a - b) for a, b in zip(r64["energies"], r32["energies"]))
     sizes = {}
     for wl in wls:
         sizes[f"{wl.n}^3"] = sizes.get(f"{wl.n}^3", 0) + 1
@@ -192,6 +195,10 @@ def ensemble_main(args):
         "mixed_instance_steps_per_s": r32["instance_steps_per_s"],
         "mixed_vs_fp64_speedup": r32["instance_steps_per_s"] / r64["instance_steps_per_s"],
         "mixed_max_rel_energy_err": rel,
+        # the relative error is largest where the total energy nearly cancels
+        "mixed_max_rel_energy_err_instance": {"name": wls[worst].name, "n_he": wls[worst].n_he,
+                                              "etot_fp64": r64["energies"][worst]},
+        "mixed_max_abs_energy_err_K": abs_err,
         "model_gb_per_instance_step_mixed": mb / 1e9,
         "model_hbm_gbs": mb * r32["instance_steps_per_s"] / 1e9,
         "device_bytes_fp64": r64["device_bytes"], "device_bytes_mixed": r32["device_bytes"],
