@@ -904,9 +904,23 @@ __global__ void __launch_bounds__(kThreads) k_fft_xireg(XiArgs<T, R> a, Grid g, 
   const int iz = blockIdx.y;
   const int nrow = (g.ny - y0) < ROWS ? (g.ny - y0) : ROWS;
   const cplx<T>* in = reinterpret_cast<const cplx<T>*>(a.in[f]) + (long long)iz * g.mx * g.ny + y0;
-  for (int e = tid; e < ROWS * M; e += kThreads) {
-    const int kx = e / ROWS, r = e - kx * ROWS;
-    b0[r * ls + kx] = (r < nrow) ? in[(long long)kx * g.ny + r] : mk(T(0), T(0));
+  {
+    // transposed read: every load of the thread is issued before the first
+    // LDS store, so the reads of a row block are in flight together
+    constexpr int NL = (ROWS * M + kThreads - 1) / kThreads;
+    cplx<T> ld[NL];
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+      const int e = tid + q * kThreads;
+      const int kx = e / ROWS, r = e - kx * ROWS;
+      ld[q] = (e < ROWS * M && r < nrow) ? in[(long long)kx * g.ny + r] : mk(T(0), T(0));
+    }
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+      const int e = tid + q * kThreads;
+      const int kx = e / ROWS, r = e - kx * ROWS;
+      if (e < ROWS * M) b0[r * ls + kx] = ld[q];
+    }
   }
   __syncthreads();
   // Hermitian merge in place (as k_fft_xi)
