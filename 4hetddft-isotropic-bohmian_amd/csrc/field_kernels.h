@@ -579,11 +579,22 @@ __global__ void __launch_bounds__(kBlock) k_iron_fused(Grid g, int* __restrict__
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int tz = t / (ntx * nty), tr = t - tz * ntx * nty, ty = tr / ntx, tx = tr - ty * ntx;
     const int ox = it.x0 + tx * TB, oy = it.y0 + ty * TB, oz = it.z0 + tz * TB;
-    for (int e = tid; e < L0 * L0 * L0; e += blockDim.x) {
-      const int lz = e / (L0 * L0), rem = e - lz * L0 * L0, ly = rem / L0, lx = rem - ly * L0;
-      const int x = ox - NP + lx, y = oy - NP + ly, z = oz - NP + lz;
-      lv0[e] = ingrid(x, y, z) ? ((double)cnt[x + (long long)g.nx * (y + (long long)g.ny * z)] / npar) / hxyz
-                               : 0.0;
+    {
+      // every count load of the thread is issued before the first LDS store
+      constexpr int NL = (L0 * L0 * L0 + kBlock - 1) / kBlock;
+      unsigned int cv[NL];
+#pragma unroll
+      for (int q = 0; q < NL; ++q) {
+        const int e = tid + q * kBlock;
+        const int lz = e / (L0 * L0), rem = e - lz * L0 * L0, ly = rem / L0, lx = rem - ly * L0;
+        const int x = ox - NP + lx, y = oy - NP + ly, z = oz - NP + lz;
+        cv[q] = (e < L0 * L0 * L0 && ingrid(x, y, z)) ? cnt[x + (long long)g.nx * (y + (long long)g.ny * z)] : 0u;
+      }
+#pragma unroll
+      for (int q = 0; q < NL; ++q) {
+        const int e = tid + q * kBlock;
+        if (e < L0 * L0 * L0) lv0[e] = ((double)cv[q] / npar) / hxyz;
+      }
     }
     __syncthreads();
     if constexpr (NP >= 2) {
