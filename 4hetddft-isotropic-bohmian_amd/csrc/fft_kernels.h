@@ -1446,7 +1446,8 @@ __device__ inline void p128_load_row(const XfArgs<R>& a, int f, long long row0, 
       break;
     }
     case XF_GRAD_DOT: {
-      constexpr int CH = 2;
+      // values per chunk: FP32 planes keep to 64 VGPRs; FP64 planes have room for 4
+      constexpr int CH = sizeof(T) == 8 ? 4 : 2;
 #pragma unroll
       for (int t0 = 0; t0 < 8; t0 += CH) {
         cplx<R> v[6][CH];
