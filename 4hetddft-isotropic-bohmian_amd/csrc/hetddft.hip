@@ -197,6 +197,11 @@ struct hetddft_ctx {
   double* fgather = nullptr;    // quantum part of the force at the gather corners (24 N_par doubles)
   int* hpart = nullptr;         // k_hist's partial boxes (chains run in order on one stream)
   bool iron_fused = true;       // ironing passes in one launch (HETDDFT_IRON_FUSED=0: one per pass)
+  // The PC error partials of the last step (part[0..err_nb) for psi,
+  // part[err_nb..+2 err_pb) for the particles) are reduced into res only when
+  // asked for, before anything else reuses part, or dropped by the next step.
+  bool err_pending = false;
+  int err_nb = 0, err_pb = 0;
   bool hpsix_late = true;       // H psi_X after the point the field side waits for (HETDDFT_HPSIX_LATE)
   // The particle density chain (histogram -> ironing -> psix -> hpsix) has
   // two buffer sets, each with its own counts and live/work regions.  A chain
@@ -1207,6 +1212,7 @@ int step_rk_t(hetddft_ctx* c, double dt) {
 template <class T>
 int step_pc_t(hetddft_ctx* c, double dt) {
   const double c112 = 112.0 / 121.0, c9 = 9.0 / 121.0, c4o3 = 4.0 / 3.0;
+  c->err_pending = false;  // the previous step's estimates were not asked for
   const Grid& g = c->g;
   const Ham25 hp = ham_args(c, HETDDFT_TUNE_PCP);
   const Ham25 hc = ham_args(c, HETDDFT_TUNE_PCC);
@@ -1266,11 +1272,6 @@ int step_pc_t(hetddft_ctx* c, double dt) {
                        (const double*)c->vold[V[0]], (const double*)c->vold[V[1]], c->vold[V[2]],
                        (const double*)c->aold[A[0]], (const double*)c->aold[A[1]], c->pcr, c->pcv, pp, 3.0 * dt, c9);
     LAUNCHCHK(c);
-    SLAUNCH(k_finish_sums, dim3(2), dim3(kBlock), 0, (const double*)pp, (int)pb, 2, c->res + 1);
-    LAUNCHCHK(c);
-  } else {
-    SLAUNCH(k_fill_zero, dim3(1), dim3(64), 0, c->res + 1, 2);
-    LAUNCHCHK(c);
   }
   {
     int t = R[2];
@@ -1307,7 +1308,9 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   }
   LAUNCHCHK(c);
   std::swap(c->psi, c->spare);
-  SLAUNCH(k_finish_sums, dim3(1), dim3(kBlock), 0, (const double*)c->part, (int)nb, 1, c->res);
+  c->err_pending = true;
+  c->err_nb = (int)nb;
+  c->err_pb = n3 > 0 ? (int)blocks_for(n3) : 0;
   LAUNCHCHK(c);
   {
     const int t = P[2];
@@ -1318,8 +1321,23 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   return HETDDFT_OK;
 }
 
+// Reduces the last PC step's error partials into res (see err_pending).
+int flush_errors(hetddft_ctx* c) {
+  if (!c->err_pending) return HETDDFT_OK;
+  c->err_pending = false;
+  SLAUNCH(k_finish_sums, dim3(1), dim3(kBlock), 0, (const double*)c->part, c->err_nb, 1, c->res);
+  LAUNCHCHK(c);
+  if (c->err_pb > 0)
+    SLAUNCH(k_finish_sums, dim3(2), dim3(kBlock), 0, (const double*)(c->part + c->err_nb), c->err_pb, 2, c->res + 1);
+  else
+    SLAUNCH(k_fill_zero, dim3(1), dim3(64), 0, c->res + 1, 2);
+  LAUNCHCHK(c);
+  return HETDDFT_OK;
+}
+
 int fetch_errors(hetddft_ctx* c, double err[3]) {
   RET(settle(c));
+  RET(flush_errors(c));
   HIPCHK(c, hipMemcpyAsync(c->hres, c->res, 3 * sizeof(double), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   const double c9 = 9.0 / 121.0;
@@ -2339,6 +2357,7 @@ int hetddft_iterate(hetddft_ctx* c, double dt, double err[3]) {
 int hetddft_particle_means(hetddft_ctx* c, double out[9]) {
   if (!c || !out) return HETDDFT_E_ARG;
   RET(settle(c));
+  RET(flush_errors(c));
   const int np = c->p.n_par;
   if (np == 0) {
     std::memset(out, 0, 9 * sizeof(double));
@@ -2367,6 +2386,7 @@ int hetddft_diagnostics(hetddft_ctx* c, double out[HETDDFT_DIAG_N]) {
   if (!c || !out) return HETDDFT_E_ARG;
   if (!c->started) return fail(c, HETDDFT_E_STATE, "hetddft_start not called");
   RET(settle(c));
+  RET(flush_errors(c));
   const Grid& g = c->g;
   const DiagArgs da = diag_args(c);
   const unsigned nb = blocks_for(g.n);
@@ -2566,6 +2586,7 @@ int hetddft_energy(hetddft_ctx* c, double out[12]) {
   if (!c || !out) return HETDDFT_E_ARG;
   RET(settle(c));
   if (!c->started) return fail(c, HETDDFT_E_STATE, "hetddft_start not called");
+  RET(flush_errors(c));
   return c->mixed ? energy_t<float>(c, out) : energy_t<double>(c, out);
 }
 
