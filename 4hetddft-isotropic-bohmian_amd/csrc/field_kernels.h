@@ -386,8 +386,9 @@ __device__ inline void reg_union_apply(Grid g, int* __restrict__ reg, int margin
 }
 
 // NGP histogram (denpart.f90:20-31).  Each workgroup reduces its particles'
-// boxes in registers and LDS and writes one partial, which k_reg_union
-// combines, so no atomics contend on the boxes.  (Combining them in the
+// boxes in registers and LDS and writes one partial, which the next kernel of
+// the chain (k_iron_fused, or k_reg_union) combines, so no atomics contend on
+// the boxes.  (Combining them in the
 // last workgroup to finish, behind a device-scope fence in every workgroup,
 // measured 3-4% slower per step at 128^3.)
 __global__ void __launch_bounds__(kBlock) k_hist(BoxArgs b, const double* __restrict__ r,
