@@ -224,3 +224,36 @@ def test_fused_ironing_equals_one_launch_per_pass(hip, monkeypatch, precision, i
         g.close()
     for x, y in zip(*out):
         assert np.array_equal(x, y)
+
+
+def test_error_estimates_after_other_calls(hip):
+    """The PC error estimates are reduced from device partials only when
+    asked for.  Energy, diagnostics and particle-means calls made after a
+    step reuse the partials buffer; the estimates read after them equal
+    those returned by the step itself (a twin instance asks every step)."""
+    import inputs
+    wl = inputs.WORKLOADS["t24"]
+    p = inputs.params_for(wl)
+    k = hip.make_kernels(p)
+    psi, r, v = inputs.initial_state(wl)
+    inst = []
+    for _ in range(2):
+        g = hip.Instance(p)
+        g.set_kernels(k)
+        g.set_state(psi, r, v)
+        g.start()
+        inst.append(g)
+    a, b = inst
+    print("error estimates after other calls", flush=True)
+    for it in range(3 + 4):
+        ea = a.iterate(wl.deltat, want_errors=True)
+        b.iterate(wl.deltat, want_errors=False)
+        if it >= 3:
+            b.energy()
+            b.diagnostics()
+            b.particle_means()
+            eb = b.last_errors()
+            assert np.array_equal(np.asarray(ea), np.asarray(eb)), (it, ea, eb)
+            assert np.array_equal(np.asarray(b.last_errors()), np.asarray(eb))
+    a.close()
+    b.close()
