@@ -1305,47 +1305,6 @@ __global__ void __launch_bounds__(kPlaneThreads) k_fft_xyi(XiArgs<T, R> a, Grid 
   }
 }
 
-// z pass with a wide tile: TY consecutive ky (rows of TY complex values),
-// THR threads, one job per workgroup.  grid = (ceil(ny/TY), mx, njobs).
-template <class T, int N, int TY, int THR>
-__global__ void __launch_bounds__(THR) k_fft_zwide(ZArgs<T> a, Grid g, const cplx<T>* __restrict__ tw) {
-  extern __shared__ unsigned char smem_raw[];
-  constexpr int LS = N + 1;
-  cplx<T>* b0 = reinterpret_cast<cplx<T>*>(smem_raw);
-  const long long pstride = (long long)g.mx * g.ny;
-  const int ky0 = blockIdx.x * TY;
-  const int nty = (g.ny - ky0) < TY ? (g.ny - ky0) : TY;
-  const int kx = blockIdx.y;
-  const int jb = blockIdx.z;
-  const long long off = (long long)kx * g.ny + ky0;
-  const cplx<T>* src = reinterpret_cast<const cplx<T>*>(a.in[jb]) + off;
-  for (int e = threadIdx.x; e < TY * N; e += THR) {
-    const int j = e / TY, l = e - j * TY;
-    b0[l * LS + j] = (l < nty) ? src[(long long)j * pstride + l] : mk(T(0), T(0));
-  }
-  __syncthreads();
-  stages_g<T, false, N, 1, N, TY, THR, LS, 1>(b0, tw);
-  const T* kern = a.kern[jb] + off;
-  const T sg = a.ksign[jb];
-  for (int e = threadIdx.x; e < TY * N; e += THR) {
-    const int j = e / TY, l = e - j * TY;
-    if (l < nty) {
-      const cplx<T> v = b0[l * LS + j];
-      const T kv = kern[(long long)j * pstride + l];
-      const cplx<T> w = mk(v.x * kv, v.y * kv);
-      b0[l * LS + j] = sg < T(0) ? mk(-w.x, -w.y) : w;
-    }
-  }
-  __syncthreads();
-  stages_g<T, true, N, 1, N, TY, THR, LS, 1>(b0, tw);
-  cplx<T>* dst = reinterpret_cast<cplx<T>*>(a.out[jb]) + off;
-  for (int e = threadIdx.x; e < TY * N; e += THR) {
-    const int j = e / TY, l = e - j * TY;
-    if (l < nty) dst[(long long)j * pstride + l] = b0[l * LS + j];
-  }
-}
-
-
 // ---- register-resident kernels for 128 x 128 planes ------------------------
 // The same r2c / c2r of one z plane as k_fft_xyf / k_fft_xyi, with every
 // radix stage in registers.  Per plane (1024 threads, 16 wavefronts):

@@ -68,28 +68,6 @@ __device__ inline c64 d13c(const c64* __restrict__ f, long long i, int pos, int 
   return mk(ax, ay);
 }
 
-struct HamArgs {
-  Grid g;
-  Stencil13 d2x, d2y, d2z;
-  double h2o2m4;
-  const c64* timec_z;  // per-z time constant, (Lambda(1+tanh((z-zm)/zs)), 1)
-};
-
-// Sto4c = timec*((d2x+d2y+d2z)psi*h2o2m4 - pot4*psi)  (steppc.f90:36-46)
-template <class R>
-__device__ inline c64 hpsi_at(const HamArgs& a, const c64* __restrict__ psi, const R* __restrict__ pot4,
-                              long long i, Idx3 p) {
-  const Grid& g = a.g;
-  const c64 sx = d13c(psi, i, p.ix, g.nx, 1, a.d2x);
-  const c64 sy = d13c(psi, i, p.iy, g.ny, g.nx, a.d2y);
-  const c64 sz = d13c(psi, i, p.iz, g.nz, (long long)g.nx * g.ny, a.d2z);
-  const c64 lap = (sx + sy) + sz;
-  const c64 ps = psi[i];
-  const double pv = (double)pot4[i];
-  const c64 t = mk(lap.x * a.h2o2m4 - pv * ps.x, lap.y * a.h2o2m4 - pv * ps.y);
-  return cmul(a.timec_z[p.iz], t);
-}
-
 // ---- block reduction (sum of doubles), fixed tree order ------------------
 __device__ inline double block_sum(double v, double* sh) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -123,82 +101,6 @@ __global__ void k_finish_sums(const double* __restrict__ part, int nblocks, int 
   if (threadIdx.x == 0) out[s] = v;
 }
 
-// ---- psi-side fused kernels -------------------------------------------
-// RK-Gill stage (steprk.f90:56-74)
-template <class R>
-__global__ void __launch_bounds__(kBlock) k_rk_stage(HamArgs a, const c64* __restrict__ psi,
-                                                      const R* __restrict__ pot4, c64* __restrict__ q,
-                                                      c64* __restrict__ psi_out, R* __restrict__ den,
-                                                      c64* __restrict__ h1_out, double ca, double cb,
-                                                      double cc, double dt) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.g.n) return;
-  const Idx3 p = idx3(i, a.g);
-  const c64 s4 = hpsi_at(a, psi, pot4, i, p);
-  const c64 qi = q[i];
-  const c64 s1 = mk(ca * (s4.x - cb * qi.x), ca * (s4.y - cb * qi.y));
-  q[i] = mk((qi.x + 3.0 * s1.x) - cc * s4.x, (qi.y + 3.0 * s1.y) - cc * s4.y);
-  if (h1_out) h1_out[i] = s4;
-  const c64 ps = psi[i];
-  const c64 np = mk(ps.x + dt * s1.x, ps.y + dt * s1.y);
-  psi_out[i] = np;
-  den[i] = (R)(np.x * np.x + np.y * np.y);
-}
-
-// Hamming predictor + modifier (steppc.f90:46-56).  `old3` holds
-// psiold(ioldp(3)) on entry and receives the modified psi; `h2` holds
-// hpsiold(ioldh(2)) on entry and receives Sto4c.
-template <class R>
-__global__ void __launch_bounds__(kBlock) k_pc_predict(HamArgs a, const c64* __restrict__ psi,
-                                                        const R* __restrict__ pot4, c64* old3,
-                                                        const c64* __restrict__ h1, c64* h2,
-                                                        c64* __restrict__ pc, R* __restrict__ den,
-                                                        double c4o3dt, double c112) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.g.n) return;
-  const Idx3 p = idx3(i, a.g);
-  const c64 s4 = hpsi_at(a, psi, pot4, i, p);
-  const c64 o3 = old3[i], a1 = h1[i], a2 = h2[i], pci = pc[i];
-  const c64 s1 = mk(o3.x + c4o3dt * ((2.0 * s4.x - a1.x) + 2.0 * a2.x),
-                    o3.y + c4o3dt * ((2.0 * s4.y - a1.y) + 2.0 * a2.y));
-  const c64 np = mk(s1.x - c112 * pci.x, s1.y - c112 * pci.y);
-  old3[i] = np;
-  pc[i] = s1;
-  h2[i] = s4;
-  den[i] = (R)(np.x * np.x + np.y * np.y);
-}
-
-// Hamming corrector + final value + errHe partial sums (steppc.f90:97-112).
-template <class R>
-__global__ void __launch_bounds__(kBlock) k_pc_correct(HamArgs a, const c64* __restrict__ psi,
-                                                        const R* __restrict__ pot4,
-                                                        const c64* __restrict__ o3,
-                                                        const c64* __restrict__ o2,
-                                                        const c64* __restrict__ h1,
-                                                        const c64* __restrict__ h2, c64* __restrict__ pc,
-                                                        c64* __restrict__ psi_out, R* __restrict__ den,
-                                                        double* __restrict__ part, double dt3, double c9) {
-  __shared__ double sh[16];
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  double e = 0.0;
-  if (i < a.g.n) {
-    const Idx3 p = idx3(i, a.g);
-    const c64 s4 = hpsi_at(a, psi, pot4, i, p);
-    const c64 v3 = o3[i], v2 = o2[i], a1 = h1[i], a2 = h2[i];
-    const c64 s5 = mk(0.125 * ((9.0 * v3.x - v2.x) + dt3 * ((s4.x + 2.0 * a1.x) - a2.x)),
-                      0.125 * ((9.0 * v3.y - v2.y) + dt3 * ((s4.y + 2.0 * a1.y) - a2.y)));
-    const c64 pci = pc[i];
-    const c64 npc = mk(pci.x - s5.x, pci.y - s5.y);
-    pc[i] = npc;
-    const c64 np = mk(s5.x + c9 * npc.x, s5.y + c9 * npc.y);
-    psi_out[i] = np;
-    den[i] = (R)(np.x * np.x + np.y * np.y);
-    e = hypot(npc.x, npc.y);
-  }
-  const double s = block_sum(e, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-
 // den = |psi|^2 (DFT4He3d.f90:630 with conjg(psi)*psi)
 template <class R>
 __global__ void k_density(const c64* __restrict__ psi, R* __restrict__ den, long long n) {
@@ -206,79 +108,6 @@ __global__ void k_density(const c64* __restrict__ psi, R* __restrict__ den, long
   if (i >= n) return;
   const c64 v = psi[i];
   den[i] = (R)(v.x * v.x + v.y * v.y);
-}
-
-// ---- potential side -------------------------------------------------------
-// grad(den) with the first-derivative stencil (derden.f90:26-28)
-template <class R>
-__global__ void __launch_bounds__(kBlock) k_grad3(Grid g, const R* __restrict__ f, Stencil13 wx,
-                                                   Stencil13 wy, Stencil13 wz, R* __restrict__ fx,
-                                                   R* __restrict__ fy, R* __restrict__ fz) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g.n) return;
-  const Idx3 p = idx3(i, g);
-  fx[i] = (R)d13(f, i, p.ix, g.nx, 1, wx);
-  fy[i] = (R)d13(f, i, p.iy, g.ny, g.nx, wy);
-  fz[i] = (R)d13(f, i, p.iz, g.nz, (long long)g.nx * g.ny, wz);
-}
-
-struct PotArgs {
-  Grid g;
-  Stencil13 d1x, d1y, d1z;
-  double a0, a1;                 // cp4/2, cpp4/3
-  double den0s, h2o2m4, alphas;
-  double C, beta, den_m;
-  int otc;
-};
-
-// Alpha_s divergence, alpha_s field and final pot4 (term_alfa.f90:85-113,
-// poten.f90:80-124).  The products falfs*int are formed at each neighbour.
-template <class R>
-__device__ inline double dfint(const R* __restrict__ falfs, const R* __restrict__ in, long long i, int pos,
-                               int n, long long stride, const Stencil13& w) {
-  double v[13];
-#pragma unroll
-  for (int m = -6; m <= 6; ++m) {
-    const bool inb = (unsigned)(pos + m) < (unsigned)n;
-    const long long j = i + (inb ? m * stride : 0);
-    const double t = (double)falfs[j] * (double)in[j];
-    v[m + 6] = inb ? t : 0.0;
-  }
-  double acc = 0.0;
-#pragma unroll
-  for (int m = 0; m < 13; ++m) acc = acc + w.w[m] * v[m];
-  return acc;
-}
-
-template <class R>
-__global__ void __launch_bounds__(kBlock) k_pot_final(PotArgs a, const R* __restrict__ den,
-                                                       const R* __restrict__ dencg,
-                                                       const R* __restrict__ delj4,
-                                                       const R* __restrict__ sto1c,
-                                                       const R* __restrict__ uimp,
-                                                       const R* __restrict__ falfs,
-                                                       const R* __restrict__ intx,
-                                                       const R* __restrict__ inty,
-                                                       const R* __restrict__ intz,
-                                                       const R* __restrict__ uaconv, R* __restrict__ pot4) {
-  const Grid& g = a.g;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g.n) return;
-  const double dn = (double)den[i];
-  const double dcg = (double)dencg[i];
-  double p4 = ((double)delj4[i] + dcg * dcg * (a.a0 + a.a1 * dcg)) + (double)sto1c[i];
-  if (a.otc) {
-    const Idx3 p = idx3(i, g);
-    const double s4 = dfint(falfs, intx, i, p.ix, g.nx, 1, a.d1x);
-    const double s5 = dfint(falfs, inty, i, p.iy, g.ny, g.nx, a.d1y);
-    const double s6 = dfint(falfs, intz, i, p.iz, g.nz, (long long)g.nx * g.ny, a.d1z);
-    const double ua = (double)uaconv[i] / a.den0s;
-    const double ual = (((ua + s4) + s5) + s6) * a.h2o2m4 * a.alphas;
-    p4 = p4 + ual;
-  }
-  const double t = tanh(a.beta * (dn - a.den_m));
-  const double pen = a.C * (1.0 + t + a.beta * dn * (1.0 - t * t));
-  pot4[i] = (R)((p4 + (double)uimp[i]) + pen);
 }
 
 // ---- impurity side ------------------------------------------------------
@@ -305,13 +134,16 @@ __device__ inline GatherCell gather_cell(const BoxArgs& b, const double* __restr
   const double ax = (rp[i] - b.xmin) / b.hx + 1;
   const double ay = (rp[np + i] - b.ymin) / b.hy + 1;
   const double az = (rp[2 * np + i] - b.zmin) / b.hz + 1;
-  q.ix = (int)ax;
-  q.iy = (int)ay;
-  q.iz = (int)az;
+  // 1 <= int(a) <= n-1  <=>  1 <= a < n; the comparisons are false for a
+  // non-finite position, which is then outside (and never converted)
+  q.ok = ax >= 1.0 && ax < (double)b.g.nx && ay >= 1.0 && ay < (double)b.g.ny && az >= 1.0 &&
+         az < (double)b.g.nz;
+  q.ix = q.ok ? (int)ax : 0;
+  q.iy = q.ok ? (int)ay : 0;
+  q.iz = q.ok ? (int)az : 0;
   q.r = ax - q.ix;
   q.s = ay - q.iy;
   q.t = az - q.iz;
-  q.ok = !(q.ix >= b.g.nx || q.iy >= b.g.ny || q.iz >= b.g.nz || q.ix <= 0 || q.iy <= 0 || q.iz <= 0);
   return q;
 }
 
@@ -395,12 +227,20 @@ __global__ void __launch_bounds__(kBlock) k_hist(BoxArgs b, const double* __rest
                                                  unsigned int* __restrict__ cnt, int* __restrict__ part) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int ix = 0, iy = 0, iz = 0;
+  bool in = false;
   if (i < b.npar) {
-    ix = (int)(1.5 + (r[i] - b.xmin) / b.hx);
-    iy = (int)(1.5 + (r[b.npar + i] - b.ymin) / b.hy);
-    iz = (int)(1.5 + (r[2 * b.npar + i] - b.zmin) / b.hz);
+    const double fx = 1.5 + (r[i] - b.xmin) / b.hx;
+    const double fy = 1.5 + (r[b.npar + i] - b.ymin) / b.hy;
+    const double fz = 1.5 + (r[2 * b.npar + i] - b.zmin) / b.hz;
+    // 1 <= int(f) <= n  <=>  1 <= f < n+1; false for a non-finite position,
+    // which is dropped like one outside the box (and never converted)
+    in = fx >= 1.0 && fx < b.g.nx + 1.0 && fy >= 1.0 && fy < b.g.ny + 1.0 && fz >= 1.0 && fz < b.g.nz + 1.0;
+    if (in) {
+      ix = (int)fx;
+      iy = (int)fy;
+      iz = (int)fz;
+    }
   }
-  const bool in = iz >= 1 && iz <= b.g.nz && iy >= 1 && iy <= b.g.ny && ix >= 1 && ix <= b.g.nx;
   if (in) {
     const long long j = (ix - 1) + (long long)b.g.nx * ((iy - 1) + (long long)b.g.ny * (iz - 1));
     atomicAdd(&cnt[j], 1u);

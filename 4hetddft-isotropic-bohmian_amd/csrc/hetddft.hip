@@ -111,12 +111,10 @@ struct FftKernelsT {
   void (*zpv[3])(ZArgs<T>, int, Grid, const cplx<T>*) = {};
   size_t lds_zpv[3] = {};
   int typv[3] = {};
-  // whole-plane x+y kernels (when a plane fits in LDS) and the wide z pass
+  // whole-plane x+y kernels (when a plane fits in LDS)
   void (*xyf)(XfArgs<T>, Grid, const cplx<T>*, const cplx<T>*, const cplx<T>*) = nullptr;
   void (*xyi)(XiArgs<T, T>, Grid, const cplx<T>*, const cplx<T>*, const cplx<T>*) = nullptr;
-  void (*zw)(ZArgs<T>, Grid, const cplx<T>*) = nullptr;
-  size_t lds_xy = 0, lds_zw = 0;
-  int tyw = 0, thrw = 0;
+  size_t lds_xy = 0;
 };
 using FftKernels64 = FftKernelsT<double>;
 using FftKernels32 = FftKernelsT<float>;
@@ -169,7 +167,6 @@ struct hetddft_ctx {
   int pf_k[HETDDFT_TUNE_NSTENCIL] = {1, 1, 1, 1, 1};
   int cus = 256;      // compute units of the device
   int zp_per_cu = 4;  // resident workgroups per CU of the persistent z pass
-  bool st25 = true;  // z-marching stencil engine (false: direct neighbour loads)
 
   // device memory
   std::vector<void*> allocs;
@@ -202,7 +199,6 @@ struct hetddft_ctx {
   // asked for, before anything else reuses part, or dropped by the next step.
   bool err_pending = false;
   int err_nb = 0, err_pb = 0;
-  bool hpsix_late = true;       // H psi_X after the point the field side waits for (HETDDFT_HPSIX_LATE)
   // The particle density chain (histogram -> ironing -> psix -> hpsix) has
   // two buffer sets, each with its own counts and live/work regions.  A chain
   // queued on st2 writes the set that is not current; the potential
@@ -427,99 +423,62 @@ template <class T, int MH>
 void pick_x(FftKernels<T>& k, int m) {
   if constexpr (MH == 64 || MH == 128) {
     // half-length transform in registers (one LDS exchange)
-    const char* e = std::getenv("HETDDFT_XREG");
-    if (!(e && std::atoi(e) == 0)) {
-      constexpr int F1 = 8, F2 = MH / 8;             // forward: stage A full, 16-wide HBM rows
-      constexpr int I1 = MH / 8, I2 = 8;             // inverse: stage B stores runs of MH/8 pairs
-      k.xf = k_fft_xreg<T, T, MH, F1, F2>;
-      k.xi = k_fft_xireg<T, T, MH, I1, I2>;
-      const size_t bf = xreg_bytes<T, MH, F1, F2>(), bi = xireg_bytes<T, MH, I1, I2>();
-      k.lds_x = bf > bi ? bf : bi;
-      return;
-    }
+    constexpr int F1 = 8, F2 = MH / 8;             // forward: stage A full, 16-wide HBM rows
+    constexpr int I1 = MH / 8, I2 = 8;             // inverse: stage B stores runs of MH/8 pairs
+    k.xf = k_fft_xreg<T, T, MH, F1, F2>;
+    k.xi = k_fft_xireg<T, T, MH, I1, I2>;
+    const size_t bf = xreg_bytes<T, MH, F1, F2>(), bi = xireg_bytes<T, MH, I1, I2>();
+    k.lds_x = bf > bi ? bf : bi;
+  } else {
+    k.xf = k_fft_xf<T, T, MH>;
+    k.xi = k_fft_xi<T, T, MH>;
+    k.lds_x = (MH > 0 ? 1 : 2) * tile_bytes<T>(kRowsX, m);
   }
-  k.xf = k_fft_xf<T, T, MH>;
-  k.xi = k_fft_xi<T, T, MH>;
-  k.lds_x = (MH > 0 ? 1 : 2) * tile_bytes<T>(kRowsX, m);
 }
 template <class T, int N>
 void pick_y(FftKernels<T>& k, int n) {
   if constexpr (N == 64 || N == 128 || N == 256) {
     // register-resident y pass (one LDS exchange)
-    const char* e = std::getenv("HETDDFT_YREG");
-    if (!(e && std::atoi(e) == 0)) {
-      constexpr int R1 = N == 64 ? 8 : 16;
-      constexpr int R2 = N / R1;
-      k.yf = k_fft_yreg<T, false, N, R1, R2>;
-      k.yi = k_fft_yreg<T, true, N, R1, R2>;
-      k.lds_y = yreg_bytes<T, N, R1, R2>();
-      return;
-    }
+    constexpr int R1 = N == 64 ? 8 : 16;
+    constexpr int R2 = N / R1;
+    k.yf = k_fft_yreg<T, false, N, R1, R2>;
+    k.yi = k_fft_yreg<T, true, N, R1, R2>;
+    k.lds_y = yreg_bytes<T, N, R1, R2>();
+  } else {
+    k.yf = k_fft_y<T, false, N>;
+    k.yi = k_fft_y<T, true, N>;
+    k.lds_y = (N > 0 ? 1 : 2) * tile_bytes<T>(kLinesY, n);
   }
-  k.yf = k_fft_y<T, false, N>;
-  k.yi = k_fft_y<T, true, N>;
-  k.lds_y = (N > 0 ? 1 : 2) * tile_bytes<T>(kLinesY, n);
 }
 template <class T, int N>
 void pick_z(FftKernels<T>& k, int n) {
   constexpr int TY = tile_z<T, N>();
   k.zc = k_fft_zconv<T, N, TY>;
   k.tyz = TY;
-  if constexpr (N == 128) {
-    // alternative tile widths (HETDDFT_TYZ) for measurement
-    const char* e = std::getenv("HETDDFT_TYZ");
-    const int want = e ? std::atoi(e) : 0;
-    if (want == TY / 2) {
-      k.zc = k_fft_zconv<T, N, TY / 2>;
-      k.tyz = TY / 2;
-    } else if (want == TY * 2) {
-      k.zc = k_fft_zconv<T, N, TY * 2>;
-      k.tyz = TY * 2;
-    }
-  }
   k.lds_z = (N > 0 ? 1 : 2) * tile_bytes<T>(k.tyz, n);
   if constexpr (N == 64 || N == 128 || N == 256) {
-    // register-resident z pass (two LDS exchanges per transform pair)
-    const char* r = std::getenv("HETDDFT_ZREG");
-    if (!(r && std::atoi(r) == 0)) {
-      constexpr int R1 = N == 256 ? 16 : 8;
-      constexpr int R2 = N / R1;
-      k.nzv = 3;
-      k.zpv[0] = k_fft_zreg<T, N, 8, R1, R2>;
-      k.lds_zpv[0] = zreg_bytes<T, N, 8, R1, R2>();
-      k.typv[0] = 8;
-      k.zpv[1] = k_fft_zreg<T, N, 16, R1, R2>;
-      k.lds_zpv[1] = zreg_bytes<T, N, 16, R1, R2>();
-      k.typv[1] = 16;
-      k.zpv[2] = k_fft_zreg<T, N, 32, R1, R2>;
-      k.lds_zpv[2] = zreg_bytes<T, N, 32, R1, R2>();
-      k.typv[2] = 32;
-      const char* t = std::getenv("HETDDFT_ZRTY");
-      const int want = t ? std::atoi(t) : 32;
-      const int v = want == 8 ? 0 : want == 16 ? 1 : 2;
-      k.zp = k.zpv[v];
-      k.lds_zp = k.lds_zpv[v];
-      k.typ = k.typv[v];
-      return;
-    }
-  }
-  if constexpr (N > 0) {
-    const char* e = std::getenv("HETDDFT_ZPIPE");
-    if (!(e && std::atoi(e) == 0)) {
-      const char* t = std::getenv("HETDDFT_ZPTY");
-      const int want = t ? std::atoi(t) : 0;
-      if constexpr (N <= 128) {
-        if (want == 2 * TY) {
-          k.zp = k_fft_zpipe<T, N, 2 * TY>;
-          k.lds_zp = zpipe_bytes<T, N, 2 * TY>();
-          k.typ = 2 * TY;
-          return;
-        }
-      }
-      k.zp = k_fft_zpipe<T, N, TY>;
-      k.lds_zp = zpipe_bytes<T, N, TY>();
-      k.typ = TY;
-    }
+    // register-resident z pass (two LDS exchanges per transform pair), at
+    // ky-tile widths 8, 16 and 32 for the autotuner; 32 by default
+    constexpr int R1 = N == 256 ? 16 : 8;
+    constexpr int R2 = N / R1;
+    k.nzv = 3;
+    k.zpv[0] = k_fft_zreg<T, N, 8, R1, R2>;
+    k.lds_zpv[0] = zreg_bytes<T, N, 8, R1, R2>();
+    k.typv[0] = 8;
+    k.zpv[1] = k_fft_zreg<T, N, 16, R1, R2>;
+    k.lds_zpv[1] = zreg_bytes<T, N, 16, R1, R2>();
+    k.typv[1] = 16;
+    k.zpv[2] = k_fft_zreg<T, N, 32, R1, R2>;
+    k.lds_zpv[2] = zreg_bytes<T, N, 32, R1, R2>();
+    k.typv[2] = 32;
+    k.zp = k.zpv[2];
+    k.lds_zp = k.lds_zpv[2];
+    k.typ = k.typv[2];
+  } else if constexpr (N > 0) {
+    // persistent z pass with the next item's first-stage rows prefetched
+    k.zp = k_fft_zpipe<T, N, TY>;
+    k.lds_zp = zpipe_bytes<T, N, TY>();
+    k.typ = TY;
   }
 }
 
@@ -527,27 +486,14 @@ template <class T, int MH, int NY>
 void pick_xy(FftKernels<T>& k) {
   constexpr size_t b = plane_bytes<T>(MH, NY);
   if constexpr (MH == 64 && NY == 128) {
-    const char* e = std::getenv("HETDDFT_XYREG");
-    if (!(e && std::atoi(e) == 0)) {
-      k.xyf = k_fft_xyf128<T, T>;
-      k.xyi = k_fft_xyi128<T, T>;
-      k.lds_xy = kPlane128Lds * sizeof(cplx<T>);
-      return;
-    }
-  }
-  if constexpr (b <= 160 * 1024) {
+    k.xyf = k_fft_xyf128<T, T>;
+    k.xyi = k_fft_xyi128<T, T>;
+    k.lds_xy = kPlane128Lds * sizeof(cplx<T>);
+  } else if constexpr (b <= 160 * 1024) {
     k.xyf = k_fft_xyf<T, T, MH, NY>;
     k.xyi = k_fft_xyi<T, T, MH, NY>;
     k.lds_xy = b;
   }
-}
-template <class T, int N>
-void pick_zw(FftKernels<T>& k) {
-  constexpr int TY = 64, THR = 1024;
-  k.zw = k_fft_zwide<T, N, TY, THR>;
-  k.lds_zw = tile_bytes<T>(TY, N);
-  k.tyw = TY;
-  k.thrw = THR;
 }
 
 template <class T>
@@ -564,21 +510,6 @@ FftKernels<T> select_fft(int m, int ny, int nz) {
       case 64: pick_xy<T, 64, 128>(k); break;
       default: break;
     }
-  }
-  switch (nz) {
-    case 32: pick_zw<T, 32>(k); break;
-    case 48: pick_zw<T, 48>(k); break;
-    case 64: pick_zw<T, 64>(k); break;
-    case 80: pick_zw<T, 80>(k); break;
-    case 96: pick_zw<T, 96>(k); break;
-    case 128: pick_zw<T, 128>(k); break;
-    default: break;
-  }
-  {
-    const char* e = std::getenv("HETDDFT_XY");
-    if (e && std::atoi(e) == 0) k.xyf = nullptr, k.xyi = nullptr;
-    const char* w = std::getenv("HETDDFT_ZWIDE");
-    if (!(w && std::atoi(w) == 1)) k.zw = nullptr;
   }
 #define HETD_X(M) \
   case M: pick_x<T, M>(k, m); break;
@@ -639,7 +570,6 @@ int fft_alloc_tables(hetddft_ctx* c, FftSet& fs) {
     HIPCHK(c, hipFuncSetAttribute((const void*)k.xyf, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
     HIPCHK(c, hipFuncSetAttribute((const void*)k.xyi, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   }
-  if (k.zw) HIPCHK(c, hipFuncSetAttribute((const void*)k.zw, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   if (k.zp) HIPCHK(c, hipFuncSetAttribute((const void*)k.zp, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   for (int v = 0; v < k.nzv; ++v)
     HIPCHK(c, hipFuncSetAttribute((const void*)k.zpv[v], hipFuncAttributeMaxDynamicSharedMemorySize, lim));
@@ -730,10 +660,6 @@ int conv_batch(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin,
     const unsigned grid = (unsigned)std::min<long long>(items, (long long)c->cus * c->zp_per_cu);
     ProfScope ps(c, HETDDFT_PROF_Z, nout * (2.0 * Sb + Kb));
     PLAUNCH(ps, K.zp, dim3(grid), dim3(kThreads), K.lds_zp, za, nout, g, (const cplx<T>*)fs.tw_z);
-  } else if (K.zw) {
-    ProfScope ps(c, HETDDFT_PROF_Z, nout * (2.0 * Sb + Kb));
-    PLAUNCH(ps, K.zw, dim3((g.ny + K.tyw - 1) / K.tyw, g.mx, nout), dim3(K.thrw), K.lds_zw, za, g,
-                                                                                         (const cplx<T>*)fs.tw_z);
   } else {
     ProfScope ps(c, HETDDFT_PROF_Z, nout * (2.0 * Sb + Kb));
     PLAUNCH(ps, K.zc, dim3((g.ny + K.tyz - 1) / K.tyz, g.mx, nout), dim3(kThreads), fs.lds_z,
@@ -758,17 +684,6 @@ int conv_batch(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin,
     LAUNCHCHK(c);
   }
   return HETDDFT_OK;
-}
-
-HamArgs ham_direct(const hetddft_ctx* c, const Ham25& h) {
-  HamArgs a{};
-  a.g = h.g;
-  a.d2x = c->d2x;
-  a.d2y = c->d2y;
-  a.d2z = c->d2z;
-  a.h2o2m4 = h.h2o2m4;
-  a.timec_z = h.timec_z;
-  return a;
 }
 
 // Tile shapes of the z-marching stencil engine (c->tile selects one).
@@ -840,15 +755,8 @@ GatherArgs gather_args(const hetddft_ctx* c) {
 }
 
 // Grid-stride launches over a device-side region.  The chain runs beside
-// the field kernels, so it takes few CUs (HETDDFT_REGION_BLOCKS overrides).
-unsigned region_blocks() {
-  static const unsigned n = [] {
-    const char* e = std::getenv("HETDDFT_REGION_BLOCKS");
-    const int v = e ? std::atoi(e) : 0;
-    return v > 0 ? (unsigned)v : 256u;
-  }();
-  return n;
-}
+// the field kernels, so it takes few CUs.
+unsigned region_blocks() { return 256u; }
 
 // Particle density chain of potenimp (denpart.f90, ironing.f90).
 template <class T>
@@ -899,13 +807,12 @@ int issue_chain(hetddft_ctx* c) {
   }
   S.denx = cur;
   // the field side needs denx (batch 1); H psi_X enters only the force
-  // gather after batch 3, so with hpsix_late it stays off the point st waits for
-  if (c->hpsix_late && !c->dry && !c->one_stream) HIPCHK(c, hipEventRecord(c->ev_chain, c->st2));
+  // gather after batch 3, so it stays off the point st waits for
+  if (!c->dry && !c->one_stream) HIPCHK(c, hipEventRecord(c->ev_chain, c->st2));
   S2LAUNCH((k_hpsix<T>), dim3(region_blocks()), dim3(kBlock), 0, g, S.reg, (const T*)psix,
            compact<double, true>(c->d2x), compact<double, true>(c->d2y), compact<double, true>(c->d2z), c->h2o2mx,
            (T*)S.hpsix, S.counts, margin);
   LAUNCHCHK(c);
-  if (!c->hpsix_late && !c->dry && !c->one_stream) HIPCHK(c, hipEventRecord(c->ev_chain, c->st2));
   c->chain_pending = true;
   c->side_pending = true;
   return HETDDFT_OK;
@@ -947,21 +854,16 @@ int veval(hetddft_ctx* c) {
   // ---- poten: derden (derden.f90:26-28)
   {
   ProfScope psg(c, HETDDFT_PROF_GRAD, 4.0 * (double)g.n * sizeof(T));
-  if (c->st25)
-    with_shape(c->tile_k[HETDDFT_TUNE_GRAD], c->pf_k[HETDDFT_TUNE_GRAD], [&](auto sh, auto pf) {
-      using SH = decltype(sh);
-      constexpr bool PF = decltype(pf)::value;
-      PLAUNCH(psg, (k_grad3_25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_GRAD), dim3(SH::TX, SH::TY), 0, g,
-              c->zc_k[HETDDFT_TUNE_GRAD], c->xcd,
-              (const T*)F(hetddft_ctx::F_DEN), compact<T, false>(c->d1x), compact<T, false>(c->d1y),
-              compact<T, false>(c->d1z), F(hetddft_ctx::F_DX),
-              F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ));
-      return 0;
-    });
-  else
-    PLAUNCH(psg, (k_grad3<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, g,
-                       (const T*)F(hetddft_ctx::F_DEN), c->d1x, c->d1y, c->d1z, F(hetddft_ctx::F_DX),
-                       F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ));
+  with_shape(c->tile_k[HETDDFT_TUNE_GRAD], c->pf_k[HETDDFT_TUNE_GRAD], [&](auto sh, auto pf) {
+    using SH = decltype(sh);
+    constexpr bool PF = decltype(pf)::value;
+    PLAUNCH(psg, (k_grad3_25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_GRAD), dim3(SH::TX, SH::TY), 0, g,
+            c->zc_k[HETDDFT_TUNE_GRAD], c->xcd,
+            (const T*)F(hetddft_ctx::F_DEN), compact<T, false>(c->d1x), compact<T, false>(c->d1y),
+            compact<T, false>(c->d1z), F(hetddft_ctx::F_DX),
+            F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ));
+    return 0;
+  });
   }
   LAUNCHCHK(c);
   // the particle density chain enters here: its set becomes current
@@ -1055,35 +957,21 @@ int veval(hetddft_ctx* c) {
     LAUNCHCHK(c);
   }
   // pot4 assembly with the alpha_s divergence (term_alfa.f90:85-113, poten.f90:80-124)
-  PotArgs pa{};
-  pa.g = g;
-  pa.d1x = c->d1x;
-  pa.d1y = c->d1y;
-  pa.d1z = c->d1z;
-  pa.a0 = c->p.cp4 / 2.0;
-  pa.a1 = c->p.cpp4 / 3.0;
-  pa.den0s = c->p.den0s;
-  pa.h2o2m4 = c->p.h2o2m4;
-  pa.alphas = c->p.alphas;
-  pa.C = c->p.C;
-  pa.beta = c->p.beta;
-  pa.den_m = c->p.den_m;
-  pa.otc = otc ? 1 : 0;
-  if (c->st25) {
+  {
     Pot25T<T> p5{};
     p5.g = g;
     p5.d1x = compact<T, false>(c->d1x);
     p5.d1y = compact<T, false>(c->d1y);
     p5.d1z = compact<T, false>(c->d1z);
-    p5.a0 = (T)pa.a0;
-    p5.a1 = (T)pa.a1;
-    p5.den0s = (T)pa.den0s;
-    p5.h2o2m4 = (T)pa.h2o2m4;
-    p5.alphas = (T)pa.alphas;
-    p5.C = (T)pa.C;
-    p5.beta = (T)pa.beta;
-    p5.den_m = (T)pa.den_m;
-    p5.otc = pa.otc;
+    p5.a0 = (T)(c->p.cp4 / 2.0);
+    p5.a1 = (T)(c->p.cpp4 / 3.0);
+    p5.den0s = (T)c->p.den0s;
+    p5.h2o2m4 = (T)c->p.h2o2m4;
+    p5.alphas = (T)c->p.alphas;
+    p5.C = (T)c->p.C;
+    p5.beta = (T)c->p.beta;
+    p5.den_m = (T)c->p.den_m;
+    p5.otc = otc ? 1 : 0;
     p5.zc = c->zc_k[HETDDFT_TUNE_POT];
     p5.xcd = c->xcd;
     ProfScope ps(c, HETDDFT_PROF_POT, (otc ? 11.0 : 6.0) * (double)g.n * sizeof(T));
@@ -1098,16 +986,10 @@ int veval(hetddft_ctx* c) {
               (const T*)F(hetddft_ctx::F_IZ), (const T*)F(hetddft_ctx::F_UACONV), F(hetddft_ctx::F_POT4));
       return 0;
     });
-  } else {
-    SLAUNCH((k_pot_final<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, pa,
-                       (const T*)F(hetddft_ctx::F_DEN), (const T*)F(hetddft_ctx::F_DENCG),
-                       (const T*)F(hetddft_ctx::F_DELJ4), (const T*)F(hetddft_ctx::F_STO1),
-                       (const T*)F(hetddft_ctx::F_UIMP), (const T*)F(hetddft_ctx::F_FALFS),
-                       (const T*)F(hetddft_ctx::F_IX), (const T*)F(hetddft_ctx::F_IY),
-                       (const T*)F(hetddft_ctx::F_IZ), (const T*)F(hetddft_ctx::F_UACONV), F(hetddft_ctx::F_POT4));
   }
   LAUNCHCHK(c);
-  if (c->p.n_par > 0 || c->hpsix_late) {
+  {
+    // H psi_X (and the gather, with particles) ran on the side stream
     RET(join_side(c, c->ev_force));
     RET(prof_mark(c));
   }
@@ -1176,19 +1058,14 @@ int step_rk_t(hetddft_ctx* c, double dt) {
     } else {
       out = c->spare;
     }
-    if (c->st25)
-      with_shape(c->tile_k[HETDDFT_TUNE_RK], c->pf_k[HETDDFT_TUNE_RK], [&](auto sh, auto pf) {
-        using SH = decltype(sh);
-        constexpr bool PF = decltype(pf)::value;
-        SLAUNCH((k_rk_stage25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_RK), dim3(SH::TX, SH::TY), 0, ha,
-                           (const c64*)c->psi, (const T*)c->fld[hetddft_ctx::F_POT4], c->q, out,
-                           (T*)c->fld[hetddft_ctx::F_DEN], h1o, arun[j], brun[j], crun[j], dt);
-        return 0;
-      });
-    else
-      SLAUNCH((k_rk_stage<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, ham_direct(c, ha),
-                         (const c64*)c->psi, (const T*)c->fld[hetddft_ctx::F_POT4], c->q, out,
-                         (T*)c->fld[hetddft_ctx::F_DEN], h1o, arun[j], brun[j], crun[j], dt);
+    with_shape(c->tile_k[HETDDFT_TUNE_RK], c->pf_k[HETDDFT_TUNE_RK], [&](auto sh, auto pf) {
+      using SH = decltype(sh);
+      constexpr bool PF = decltype(pf)::value;
+      SLAUNCH((k_rk_stage25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_RK), dim3(SH::TX, SH::TY), 0, ha,
+              (const c64*)c->psi, (const T*)c->fld[hetddft_ctx::F_POT4], c->q, out,
+              (T*)c->fld[hetddft_ctx::F_DEN], h1o, arun[j], brun[j], crun[j], dt);
+      return 0;
+    });
     LAUNCHCHK(c);
     if (j == 0) {
       c->psi = out;
@@ -1223,7 +1100,7 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   int* A = c->iolda;
   const int n3 = 3 * c->p.n_par;
   // errHe partial sums: one per workgroup of the corrector
-  const unsigned nb = c->st25 ? (unsigned)blocks25(c, HETDDFT_TUNE_PCC) : blocks_for(g.n);
+  const unsigned nb = (unsigned)blocks25(c, HETDDFT_TUNE_PCC);
   // Particle predictor (steppc.f90:64-87), then the density chain of the
   // predicted positions on the side stream, beside the psi predictor.
   if (n3 > 0) {
@@ -1241,20 +1118,15 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   c64* y = c->old[P[2]];
   {
   ProfScope pspred(c, HETDDFT_PROF_PCP, (double)g.n * (16.0 * 8 + 2.0 * sizeof(T)));
-  if (c->st25)
-    with_shape(c->tile_k[HETDDFT_TUNE_PCP], c->pf_k[HETDDFT_TUNE_PCP], [&](auto sh, auto pf) {
-      using SH = decltype(sh);
-      constexpr bool PF = decltype(pf)::value;
-      PLAUNCH(pspred, (k_pc_predict25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_PCP), dim3(SH::TX, SH::TY), 0, hp,
-              (const c64*)x,
-              (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]], c->hold[H[1]], c->pc,
-              (T*)c->fld[hetddft_ctx::F_DEN], c4o3 * dt, c112);
-      return 0;
-    });
-  else
-    PLAUNCH(pspred, (k_pc_predict<T>), dim3(blocks_for(g.n)), dim3(kBlock), 0, ham_direct(c, hp),
-                       (const c64*)x, (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]],
-                       c->hold[H[1]], c->pc, (T*)c->fld[hetddft_ctx::F_DEN], c4o3 * dt, c112);
+  with_shape(c->tile_k[HETDDFT_TUNE_PCP], c->pf_k[HETDDFT_TUNE_PCP], [&](auto sh, auto pf) {
+    using SH = decltype(sh);
+    constexpr bool PF = decltype(pf)::value;
+    PLAUNCH(pspred, (k_pc_predict25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_PCP), dim3(SH::TX, SH::TY), 0, hp,
+            (const c64*)x,
+            (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]], c->hold[H[1]], c->pc,
+            (T*)c->fld[hetddft_ctx::F_DEN], c4o3 * dt, c112);
+    return 0;
+  });
   }
   LAUNCHCHK(c);
   c->old[P[2]] = x;
@@ -1289,22 +1161,16 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   // until fetch_errors asks for them
   {
   ProfScope pscorr(c, HETDDFT_PROF_PCC, (double)g.n * (16.0 * 10 + 2.0 * sizeof(T)));
-  if (c->st25)
-    with_shape(c->tile_k[HETDDFT_TUNE_PCC], c->pf_k[HETDDFT_TUNE_PCC], [&](auto sh, auto pf) {
-      using SH = decltype(sh);
-      constexpr bool PF = decltype(pf)::value;
-      PLAUNCH(pscorr, (k_pc_correct25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_PCC), dim3(SH::TX, SH::TY), 0, hc,
-              (const c64*)c->psi,
-              (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
-              (const c64*)c->hold[H[0]], (const c64*)c->hold[H[1]], c->pc, c->spare,
-              (T*)c->fld[hetddft_ctx::F_DEN], c->part, 3.0 * dt, c9);
-      return 0;
-    });
-  else
-    PLAUNCH(pscorr, (k_pc_correct<T>), dim3(nb), dim3(kBlock), 0, ham_direct(c, hc), (const c64*)c->psi,
-                       (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
-                       (const c64*)c->hold[H[0]], (const c64*)c->hold[H[1]], c->pc, c->spare,
-                       (T*)c->fld[hetddft_ctx::F_DEN], c->part, 3.0 * dt, c9);
+  with_shape(c->tile_k[HETDDFT_TUNE_PCC], c->pf_k[HETDDFT_TUNE_PCC], [&](auto sh, auto pf) {
+    using SH = decltype(sh);
+    constexpr bool PF = decltype(pf)::value;
+    PLAUNCH(pscorr, (k_pc_correct25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_PCC), dim3(SH::TX, SH::TY), 0, hc,
+            (const c64*)c->psi,
+            (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
+            (const c64*)c->hold[H[0]], (const c64*)c->hold[H[1]], c->pc, c->spare,
+            (T*)c->fld[hetddft_ctx::F_DEN], c->part, 3.0 * dt, c9);
+    return 0;
+  });
   }
   LAUNCHCHK(c);
   std::swap(c->psi, c->spare);
@@ -1884,22 +1750,17 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
     // (4..32 planes each)
     const char* t = std::getenv("HETDDFT_TILE");
     if (t) c->tile = std::max(0, std::min(3, std::atoi(t)));
-    const char* zb = std::getenv("HETDDFT_ZBLOCKS");
-    const long long target = zb && std::atoll(zb) > 0 ? std::atoll(zb) : 1024;
+    const long long target = 1024;
     const long long cols = with_tile(c->tile, [&](auto sh) {
       using SH = decltype(sh);
       return (int)(((p->nx + SH::TX - 1) / SH::TX) * ((p->ny + SH::TY - 1) / SH::TY));
     });
     long long zc = cols * p->nz / target;
     c->zc = (int)std::max(4LL, std::min(32LL, zc));
-    const char* e = std::getenv("HETDDFT_STENCIL");
-    if (e && std::strcmp(e, "direct") == 0) c->st25 = false;
     const char* z = std::getenv("HETDDFT_ZC");
     if (z && std::atoi(z) > 0) c->zc = std::atoi(z);
     const char* x = std::getenv("HETDDFT_XCD");
     if (x) c->xcd = std::atoi(x) != 0;
-    const char* hl = std::getenv("HETDDFT_HPSIX_LATE");
-    if (hl) c->hpsix_late = std::atoi(hl) != 0;
     const char* fi = std::getenv("HETDDFT_IRON_FUSED");
     if (fi) c->iron_fused = std::atoi(fi) != 0;
     const char* gph = std::getenv("HETDDFT_GRAPH");
@@ -1927,12 +1788,8 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
   }
   // Cross-stream ordering between the two streams of this device: a
   // device-scope release is enough (no system-scope cache writeback per
-  // record).  HETDDFT_SYSFENCE=1 restores the system-scope events.
-  unsigned evflags = hipEventDisableTiming | hipEventDisableSystemFence;
-  {
-    const char* sf = std::getenv("HETDDFT_SYSFENCE");
-    if (sf && std::atoi(sf) == 1) evflags = hipEventDisableTiming;
-  }
+  // record).
+  const unsigned evflags = hipEventDisableTiming | hipEventDisableSystemFence;
   for (hipEvent_t* e : {&c->ev_fork, &c->ev_chain, &c->ev_b3, &c->ev_join, &c->ev_force})
     if (hipEventCreateWithFlags(e, evflags) != hipSuccess) {
       fail(c, HETDDFT_E_HIP, "hipEventCreate failed");
@@ -1943,8 +1800,6 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
       c->cus = cus;
-    const char* e = std::getenv("HETDDFT_ZOCC");
-    if (e && std::atoi(e) > 0) c->zp_per_cu = std::atoi(e);
   }
   int rc = c->mixed ? fft_alloc_tables<float>(c, c->fs) : fft_alloc_tables<double>(c, c->fs);
   if (rc) return bad(rc);
@@ -2291,11 +2146,24 @@ int hetddft_start(hetddft_ctx* c) {
                        (double*)c->fld[hetddft_ctx::F_DEN], g.n);
   LAUNCHCHK(c);
   RET(run_veval(c));
+  // every accumulator starts at zero, as after allocation in the reference
+  // (dimen.f90:78-79,131-134; DFT4He3d.f90:841-845), also on a context that
+  // has run before: pc and q of psi, pcr and pcv of the particles, the
+  // particle history slots and the RK accumulators qr, qv
   const size_t np3 = (size_t)3 * std::max(c->p.n_par, 1);
-  for (double* pp : {c->qr, c->qv, c->rold[0], c->rold[1], c->rold[2], c->vold[0], c->vold[1], c->vold[2],
-                     c->aold[0], c->aold[1]})
+  for (double* pp : {c->qr, c->qv, c->pcr, c->pcv, c->rold[0], c->rold[1], c->rold[2], c->vold[0], c->vold[1],
+                     c->vold[2], c->aold[0], c->aold[1]})
     HIPCHK(c, hipMemsetAsync(pp, 0, np3 * 8, c->st));
+  HIPCHK(c, hipMemsetAsync(c->pc, 0, (size_t)g.n * 16, c->st));
+  HIPCHK(c, hipMemsetAsync(c->q, 0, (size_t)g.n * 16, c->st));
+  HIPCHK(c, hipMemsetAsync(c->res, 0, 3 * sizeof(double), c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
+  c->err_pending = false;
+  std::memset(c->last_err, 0, sizeof c->last_err);
+  for (int k = 0; k < 3; ++k) c->ioldp[k] = c->ioldr[k] = c->ioldv[k] = k;
+  c->ioldh[0] = c->iolda[0] = 0;
+  c->ioldh[1] = c->iolda[1] = 1;
+  clear_graphs(c);
   c->started = true;
   c->iter_in_run = 0;
   return HETDDFT_OK;
@@ -2492,6 +2360,10 @@ int hetddft_checkpoint_load(hetddft_ctx* c, const void* buf) {
   std::copy(h.iolda, h.iolda + 2, c->iolda);
   c->iter_in_run = h.iter_in_run;
   c->chain_pending = false;
+  // the error estimates of steps run before the load belong to another state
+  c->err_pending = false;
+  std::memset(c->last_err, 0, sizeof c->last_err);
+  HIPCHK(c, hipMemsetAsync(c->res, 0, 3 * sizeof(double), c->st));
   clear_graphs(c);
   // den and the potential fields of the restored state (the fields the saved
   // context held after its last potential evaluation)
@@ -2698,6 +2570,9 @@ int hetddft_graph_info(hetddft_ctx* c, int* count, int* enabled) {
 
 int hetddft_sync(hetddft_ctx* c) {
   if (!c) return HETDDFT_E_ARG;
+  // a PC step whose error estimates nobody fetched: reduce and check them
+  // here, so a run driven with err = NULL still reports a diverged state
+  if (c->err_pending) return fetch_errors(c, nullptr);
   RET(settle(c));
   HIPCHK(c, hipStreamSynchronize(c->st));
   return HETDDFT_OK;
@@ -2748,7 +2623,7 @@ int hetddft_autotune(hetddft_ctx* c, double dt, int steps, hetddft_tuning* out) 
     const int cls_prof[4] = {HETDDFT_PROF_GRAD, HETDDFT_PROF_POT, HETDDFT_PROF_PCP, HETDDFT_PROF_PCC};
     const int cls_tune[4] = {HETDDFT_TUNE_GRAD, HETDDFT_TUNE_POT, HETDDFT_TUNE_PCP, HETDDFT_TUNE_PCC};
     double best_ms[4] = {1e300, 1e300, 1e300, 1e300};
-    if (c->st25) {
+    {
       for (int tile = 0; tile < 4; ++tile)
         for (int zc : zc_candidates(c, tile))
           for (int pf = 1; pf >= 0; --pf) {

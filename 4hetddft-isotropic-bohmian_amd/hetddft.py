@@ -41,8 +41,20 @@ KTOPS = 7.63822291          # DFT4He3d.f90:110
 PSTOK = 1.0 / 7.63822291
 
 
+E_ARG, E_HIP, E_NOMEM, E_STATE, E_NONFINITE = -1, -2, -3, -4, -5   # HETDDFT_E_*
+
+
 class HetddftError(RuntimeError):
-    pass
+    """A failed library call: args = (message[, HETDDFT_E_* code[, instance]]);
+    ``code`` and ``instance`` name the last two (None when absent)."""
+
+    def __init__(self, msg, *extra):
+        super().__init__(msg, *extra)
+        self.code = extra[0] if extra else None
+        self.instance = extra[1] if len(extra) > 1 else None
+
+    def __str__(self):
+        return str(self.args[0])
 
 
 class Params(ctypes.Structure):
@@ -202,13 +214,13 @@ class Instance:
         rc = self.lib.hetddft_create(ctypes.byref(p), ctypes.byref(self._c))
         if rc != 0:
             raise HetddftError(f"hetddft_create failed ({rc}): "
-                               f"{self.lib.hetddft_last_error(None).decode()}")
+                               f"{self.lib.hetddft_last_error(None).decode()}", rc)
 
     # -- plumbing --
     def _check(self, rc, what):
         if rc != 0:
             msg = self.lib.hetddft_last_error(self._c).decode()
-            raise HetddftError(f"{what} failed ({rc}): {msg}")
+            raise HetddftError(f"{what} failed ({rc}): {msg}", rc)
 
     def close(self):
         if self._c:

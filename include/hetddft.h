@@ -89,8 +89,9 @@ int hetddft_set_kernels(hetddft_ctx* ctx, const double* fvlj4, const double* wcg
 /* Uploads psi (complex, interleaved), rimp and vimp (Fortran (N_par,3)). */
 int hetddft_set_state(hetddft_ctx* ctx, const double* psi, const double* rimp, const double* vimp);
 
-/* den = conjg(psi)*psi, first potential evaluation, zeroed particle history
- * (DFT4He3d.f90:630-634, 841-845). */
+/* den = conjg(psi)*psi, first potential evaluation, zeroed histories and
+ * accumulators (DFT4He3d.f90:630-634, 841-845; dimen.f90:78-79, 131-134):
+ * a context that has run before restarts exactly like a new one. */
 int hetddft_start(hetddft_ctx* ctx);
 
 /* potenimp(); poten(); forceimp()  (potenimp.f90:1-73, poten.f90:5-129). */
@@ -101,7 +102,9 @@ int hetddft_step_rk(hetddft_ctx* ctx, double deltat);
 
 /* steppc(deltat, errHe, errimp, errvimp)  (steppc.f90:1-155).  err may be
  * NULL: the step then runs without a host synchronisation and the error
- * estimates stay on the device (hetddft_last_errors fetches them). */
+ * estimates stay on the device (hetddft_last_errors fetches them, and
+ * hetddft_sync checks them).  A non-finite estimate is HETDDFT_E_NONFINITE
+ * (the reference prints errHe every step, steppc.f90:109-112). */
 int hetddft_step_pc(hetddft_ctx* ctx, double deltat, double err[3]);
 
 int hetddft_last_errors(hetddft_ctx* ctx, double err[3]);
@@ -245,7 +248,10 @@ int hetddft_checkpoint_load(hetddft_ctx* ctx, const void* buf);
  * field behind HETDDFT_DIAG_JTOT. */
 int hetddft_current(hetddft_ctx* ctx, double* jx, double* jy, double* jz);
 
-/* Waits for all work queued on the context's stream. */
+/* Waits for all work queued on the context's stream.  When the last call was
+ * a PC step (or an iteration ending in one) whose error estimates were not
+ * fetched, it reduces them first and returns HETDDFT_E_NONFINITE if one is
+ * not finite: a run driven with err = NULL learns of a diverged state here. */
 int hetddft_sync(hetddft_ctx* ctx);
 
 /* With HETDDFT_GRAPH=1 (both streams) or 2 (particle chain on the main
@@ -321,7 +327,9 @@ int hetddft_set_tuning(hetddft_ctx* ctx, const hetddft_tuning* t);
  * thread submits one iteration of each of its instances in turn, with no
  * synchronisation between iterations, and the call returns when every
  * instance has finished.  err (may be NULL) receives 3 doubles per instance,
- * the PC error estimates of its last iteration.  On a failure the other
+ * the PC error estimates of its last iteration; they are reduced and checked
+ * for every instance in any case, so a diverged instance fails the call with
+ * HETDDFT_E_NONFINITE.  On a failure the other
  * threads stop after their current iteration, and the return code and the
  * message (hetddft_batch_last_error) are those of the first failing instance,
  * whose index goes to *failed (may be NULL; -1 when all succeed). */

@@ -805,6 +805,12 @@ class Oracle:
         denx = cnt / p.n_par / hxyz
         for _ in range(p.ironx):
             denx = ironing(denx)
+        self.set_denx(denx)
+
+    def set_denx(self, denx):
+        """psix = sqrt(denx), hpsix = -h2o2mx lap psix from a given impurity
+        density (denpart.f90:40-46)."""
+        p = self.p
         self.denx = denx
         self.psix = np.sqrt(denx)
         s1 = deriv(self.psix, "x", 2, p.hx)
