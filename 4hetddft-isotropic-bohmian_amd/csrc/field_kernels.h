@@ -687,6 +687,17 @@ __global__ void k_part_sums(int np, const double* __restrict__ r, const double* 
   if (threadIdx.x == 0) part[comp * gridDim.x + blockIdx.x] = s;
 }
 
+// Means from the 9 particle sums, scaled as hetddft_particle_means scales
+// them on the host: r in A, v in A/ps, a in A/ps^2 (DFT4He3d.f90:864-878).
+__global__ void k_part_means_scale(const double* __restrict__ sums, int np, double pstok, double* __restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= 9) return;
+  double v = sums[k] / np;
+  if (k >= 3 && k < 6) v = v * pstok;
+  if (k >= 6) v = v * pstok * pstok;
+  out[k] = v;
+}
+
 // ---- energy (energy.f90:32-124) ----------------------------------------
 struct EnergyArgs {
   Grid g;

@@ -1353,6 +1353,50 @@ int energy_t(hetddft_ctx* c, double out[12]) {
 
 }  // namespace
 
+// ---- allocation-free particle means -------------------------------------
+// Device scratch of hetddft_particle_means_dev, the entry point that
+// allocates nothing: kNoAllocParts partial sums for each of the 9 components,
+// then the 9 sums.  Every GPU holds its own copy; host code reaches it only
+// through the address hipGetSymbolAddress returns on that GPU.
+constexpr int kNoAllocParts = 64;
+constexpr int kScratchDoubles = 1024;
+static_assert(9 * kNoAllocParts + 9 <= kScratchDoubles, "particle-means scratch too small");
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// Address of this GPU's scratch, looked up once per device (current device).
+int scratch_of_current_device(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    g_create_error = std::string("hipGetDevice: ") + hipGetErrorString(e);
+    return HETDDFT_E_HIP;
+  }
+  if (dev < 0 || dev >= kMaxDevices) {
+    g_create_error = "device ordinal beyond the scratch table";
+    return HETDDFT_E_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      g_create_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return HETDDFT_E_HIP;
+    }
+    g_scratch_dev[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_dev[dev];
+  return HETDDFT_OK;
+}
+
+}  // namespace
+
 // ============================ C-ABI ======================================
 // Host state that a PC step and the following potential evaluation advance:
 // the rotating psi / H psi / particle slots, the chain set, the flags, and
@@ -2246,6 +2290,40 @@ int hetddft_particle_means(hetddft_ctx* c, double out[9]) {
     if (k >= 3 && k < 6) v = v * pstok;
     if (k >= 6) v = v * pstok * pstok;
     out[k] = v;
+  }
+  return HETDDFT_OK;
+}
+
+int hetddft_particle_means_dev(const double* rimp, const double* vimp, const double* aimp, int n_par,
+                               double* out, void* stream) {
+  auto bad = [](int code, const char* msg) {
+    g_create_error = msg;
+    return code;
+  };
+  if (n_par < 0 || n_par > HETDDFT_MEANS_DEV_MAX_PAR)
+    return bad(HETDDFT_E_ARG, "hetddft_particle_means_dev: n_par outside 0..2^30");
+  if (!out || (n_par > 0 && (!rimp || !vimp || !aimp)))
+    return bad(HETDDFT_E_ARG, "hetddft_particle_means_dev: null array");
+  double* scr = nullptr;
+  const int rc = scratch_of_current_device(&scr);
+  if (rc != HETDDFT_OK) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (n_par == 0) {
+    hipLaunchKernelGGL(k_fill_zero, dim3(1), dim3(64), 0, st, out, 9);
+  } else {
+    // the kernels and block counts of hetddft_particle_means, so the sums
+    // are the same bit for bit
+    const unsigned nb = std::min(blocks_for(n_par), (unsigned)kNoAllocParts);
+    hipLaunchKernelGGL(k_part_sums, dim3(nb, 9), dim3(kBlock), 0, st, n_par, rimp, vimp, aimp, scr);
+    hipLaunchKernelGGL(k_finish_sums, dim3(9), dim3(kBlock), 0, st, (const double*)scr, (int)nb, 9,
+                       scr + 9 * kNoAllocParts);
+    hipLaunchKernelGGL(k_part_means_scale, dim3(1), dim3(64), 0, st, (const double*)(scr + 9 * kNoAllocParts),
+                       n_par, 1.0 / 7.63822291, out);
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    g_create_error = std::string("hetddft_particle_means_dev: ") + hipGetErrorString(e);
+    return HETDDFT_E_HIP;
   }
   return HETDDFT_OK;
 }

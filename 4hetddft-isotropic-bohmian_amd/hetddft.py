@@ -144,6 +144,7 @@ def load_library():
         "hetddft_last_errors": ([C, _VP], ctypes.c_int),
         "hetddft_iterate": ([C, ctypes.c_double, _VP], ctypes.c_int),
         "hetddft_particle_means": ([C, _VP], ctypes.c_int),
+        "hetddft_particle_means_dev": ([_VP, _VP, _VP, ctypes.c_int, _VP, _VP], ctypes.c_int),
         "hetddft_energy": ([C, _VP], ctypes.c_int),
         "hetddft_get_state": ([C, _VP, _VP, _VP, _VP], ctypes.c_int),
         "hetddft_get_field": ([C, ctypes.c_int, _VP], ctypes.c_int),

@@ -71,7 +71,7 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out);
 int hetddft_destroy(hetddft_ctx* ctx);
 
 /* Message of the last failure on ctx; with ctx == NULL, the message of the
- * last failed hetddft_create on this thread. */
+ * last failed hetddft_create or hetddft_particle_means_dev on this thread. */
 const char* hetddft_last_error(const hetddft_ctx* ctx);
 
 /* Host-side generator of the k-space kernels on the half grid:
@@ -117,6 +117,21 @@ int hetddft_iterate(hetddft_ctx* ctx, double deltat, double err[3]);
 
 /* Means of rimp (A), vimp (A/ps) and aimp (A/ps^2) (DFT4He3d.f90:864-878). */
 int hetddft_particle_means(hetddft_ctx* ctx, double out[9]);
+
+/* The same means with no context and no allocation: rimp, vimp and aimp are
+ * DEVICE arrays (N_par,3) of the current GPU, out is a DEVICE array of 9
+ * doubles written as hetddft_particle_means writes its host array (same sums,
+ * bit for bit), and the work is queued on `stream` (a hipStream_t; NULL is
+ * the null stream) without a host synchronisation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * partial sums live in a fixed 8 KB device array of the library (one per GPU,
+ * held until the process ends), so n_par is limited to
+ * HETDDFT_MEANS_DEV_MAX_PAR; calls that overlap on one GPU share that array
+ * and race.  Errors: hetddft_last_error(NULL). */
+#define HETDDFT_MEANS_DEV_MAX_PAR (1 << 30)
+int hetddft_particle_means_dev(const double* rimp, const double* vimp, const double* aimp, int n_par,
+                               double* out, void* stream);
 
 /* energy() (energy.f90:12-129) plus the norm sum(den)*dxyz:
  * out = {etot, etot4, ekin4, elj4, ealphas, esolid, ecor4, eimpu, ekinx,
