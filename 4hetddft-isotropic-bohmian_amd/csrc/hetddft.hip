@@ -612,13 +612,17 @@ int conv_batch(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin,
   const dim3 gx((unsigned)((g.ny + kRowsX - 1) / kRowsX), (unsigned)g.nz, (unsigned)nin);
   const unsigned ylines = (unsigned)(((long long)g.mx * g.nz + kLinesY - 1) / kLinesY);
   const double Rb = (double)g.n * sizeof(T), Sb = 2.0 * (double)g.h * sizeof(T), Kb = (double)g.h * sizeof(T);
-  double in_bytes = 0.0;
+  // algorithmic bytes of the x(+y) pass: each distinct input field read once
+  // (a field that several producers use, e.g. falfs or the density gradient,
+  // counts once), each spectrum written once
+  std::vector<const void*> fields;
   for (int i = 0; i < nin; ++i) {
-    int nsrc = 1;
-    if (xa.op[i] == XF_CORREL || xa.op[i] == XF_MUL) nsrc = 2;
-    if (xa.op[i] == XF_GRAD_DOT) nsrc = 6;
-    in_bytes += nsrc * Rb;
+    const int nsrc = xa.op[i] == XF_GRAD_DOT ? 6 : (xa.op[i] == XF_CORREL || xa.op[i] == XF_MUL) ? 2 : 1;
+    for (int k = 0; k < nsrc; ++k)
+      if (std::find(fields.begin(), fields.end(), (const void*)xa.in[i][k]) == fields.end())
+        fields.push_back(xa.in[i][k]);
   }
+  const double in_bytes = (double)fields.size() * Rb;
   if (K.xyf) {
     ProfScope ps(c, HETDDFT_PROF_XYF, in_bytes + nin * Sb);
     PLAUNCH(ps, K.xyf, dim3(g.nz, nin), dim3(kPlaneThreads), K.lds_xy,
@@ -654,14 +658,23 @@ int conv_batch(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin,
     xi.epi[k] = j.epi;
     xi.ec[k] = j.ec;
   }
+  // z pass: each distinct input spectrum and k-space kernel read once, each
+  // job's output spectrum written once
+  std::vector<const void*> kerns;
+  std::vector<int> zin;
+  for (const ConvJob<T>& j : jobs) {
+    if (std::find(kerns.begin(), kerns.end(), j.kern) == kerns.end()) kerns.push_back(j.kern);
+    if (std::find(zin.begin(), zin.end(), j.input) == zin.end()) zin.push_back(j.input);
+  }
+  const double z_bytes = ((double)zin.size() + nout) * Sb + (double)kerns.size() * Kb;
   if (K.zp) {
     // persistent grid: a few workgroups per CU, each walking the work items
     const long long items = (long long)((g.ny + K.typ - 1) / K.typ) * g.mx * nout;
     const unsigned grid = (unsigned)std::min<long long>(items, (long long)c->cus * c->zp_per_cu);
-    ProfScope ps(c, HETDDFT_PROF_Z, nout * (2.0 * Sb + Kb));
+    ProfScope ps(c, HETDDFT_PROF_Z, z_bytes);
     PLAUNCH(ps, K.zp, dim3(grid), dim3(kThreads), K.lds_zp, za, nout, g, (const cplx<T>*)fs.tw_z);
   } else {
-    ProfScope ps(c, HETDDFT_PROF_Z, nout * (2.0 * Sb + Kb));
+    ProfScope ps(c, HETDDFT_PROF_Z, z_bytes);
     PLAUNCH(ps, K.zc, dim3((g.ny + K.tyz - 1) / K.tyz, g.mx, nout), dim3(kThreads), fs.lds_z,
         za, g, c->pl_z, (const cplx<T>*)fs.tw_z);
   }

@@ -48,7 +48,35 @@ def model_bytes(n, n_par, precision, ironx=3):
     return 2 * veval + psi_side + parts
 
 
-def run_gpu(wl, precision, steps, warmup, prof_steps=0, tune=True):
+def _load_tuning(path, precision):
+    """A tuning saved by --save-tuning for this precision, as a hetddft.Tuning, or None."""
+    import hetddft
+    if not path:
+        return None
+    rec = json.loads(Path(path).read_text()).get("fp64" if precision == 0 else "mixed")
+    if not rec:
+        return None
+    t = hetddft.Tuning()
+    for k in range(5):
+        t.tile[k], t.zc[k], t.prefetch[k] = rec["tile"][k], rec["zc"][k], rec["prefetch"][k]
+    t.fft_z_tile, t.fft_z_wgs = rec["fft_z_tile"], rec["fft_z_wgs"]
+    return t
+
+
+def _tuning_record(t):
+    return {"tile": list(t.tile), "zc": list(t.zc), "prefetch": list(t.prefetch),
+            "fft_z_tile": t.fft_z_tile, "fft_z_wgs": t.fft_z_wgs}
+
+
+def run_gpu(wl, precision, steps, warmup, prof_steps=0, tune=True, tuning_file=None, dropin_steps=0):
+    """Time `steps` iterations of the time loop after the RK start-up.
+
+    Resident loop (the headline): hetddft_iterate with err = NULL, so the
+    error estimates stay on the device; the closing hetddft_sync reduces and
+    checks the last step's estimates (HETDDFT_E_NONFINITE on divergence).
+    Drop-in loop (dropin_steps > 0): the reference's loop as a Fortran host
+    drives it, with the three error estimates and the nine particle means
+    fetched to the host after every step (DFT4He3d.f90:855-878)."""
     import hetddft
     import inputs
     p = inputs.params_for(wl, precision=precision)
@@ -61,10 +89,16 @@ def run_gpu(wl, precision, steps, warmup, prof_steps=0, tune=True):
     for _ in range(3):                      # RK-Gill start-up (DFT4He3d.f90:851)
         g.iterate(wl.deltat, want_errors=False)
     tuning = None
-    if tune:                                # launch shapes picked by the autotuner (state restored)
+    saved = _load_tuning(tuning_file, precision)
+    if saved is not None:                   # a tuning of an earlier run (no trial launches)
+        g.set_tuning(saved)
+        tuning = g.get_tuning().as_dict()
+        tuning["from"] = str(tuning_file)
+    elif tune:                              # launch shapes picked by the autotuner (state restored)
         ta = time.perf_counter()
         tuning = g.autotune(wl.deltat, steps=8).as_dict()
         tuning["seconds"] = time.perf_counter() - ta
+    tuning_rec = _tuning_record(g.get_tuning())
     for _ in range(warmup):
         g.iterate(wl.deltat, want_errors=False)
     g.sync()
@@ -74,6 +108,14 @@ def run_gpu(wl, precision, steps, warmup, prof_steps=0, tune=True):
     g.sync()
     t1 = time.perf_counter()
     errs = g.last_errors()
+    dropin = None
+    if dropin_steps:
+        t2 = time.perf_counter()
+        for _ in range(dropin_steps):
+            g.iterate(wl.deltat, want_errors=True)
+            g.particle_means()
+        t3 = time.perf_counter()
+        dropin = dropin_steps / (t3 - t2)
     energy = g.energy()
     den = g.get_field("den")
     prof = None
@@ -84,7 +126,8 @@ def run_gpu(wl, precision, steps, warmup, prof_steps=0, tune=True):
         prof = g.profile_read()
         g.profile(False)
     out = dict(seconds=t1 - t0, steps_per_s=steps / (t1 - t0), energy=energy, errs=errs, den=den,
-               device_bytes=g.device_bytes, prof=prof, tuning=tuning)
+               device_bytes=g.device_bytes, prof=prof, tuning=tuning, tuning_rec=tuning_rec,
+               dropin_steps_per_s=dropin)
     g.close()
     return out
 
@@ -248,41 +291,97 @@ def copy_calibration(nbytes=1 << 30, reps=5):
             "GB_per_s": 2 * nbytes * reps / (t1 - t0) / 1e9}
 
 
-def cpu_baseline(wl):
-    """Oracle (numpy, one core) on one PC step's work at the same size.
+def _cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    Sample: after start(), the work of one PC step (two potential
-    evaluations and two Hamiltonian applications) repeated until at least
-    10 s have passed; steps/s = repetitions / elapsed.
-    """
+
+class _ThreadedOracle:
+    """The oracle's transforms and stencils spread over `workers` threads:
+    scipy.fft (pocketfft, the same algorithm as numpy.fft) with `workers`
+    threads, and each 13-point stencil pass cut into slabs along an axis it
+    does not differentiate (numpy releases the GIL), so every value is formed
+    as in the oracle.  The remaining whole-array expressions stay numpy's
+    single-threaded ones."""
+
+    def __init__(self, orc, workers):
+        import scipy.fft
+        from concurrent.futures import ThreadPoolExecutor
+        self.orc, self.workers = orc, workers
+        self.pool = ThreadPoolExecutor(workers)
+        self.saved = (orc.r2c, orc.c2r, orc.deriv)
+        plain = orc.deriv
+
+        def r2c(f):
+            return scipy.fft.rfftn(f, workers=workers)
+
+        def c2r(F, shape, renor):
+            return scipy.fft.irfftn(F, s=shape, axes=(0, 1, 2), norm="forward", workers=workers) * renor
+
+        def deriv(f, axis, order, h):
+            split = 0 if orc._AX[axis] != 0 else 1
+            n = f.shape[split]
+            out = np.empty_like(f)
+
+            def run(lo, hi):
+                sl = [slice(None)] * 3
+                sl[split] = slice(lo, hi)
+                out[tuple(sl)] = plain(f[tuple(sl)], axis, order, h)
+            cuts = [(i * n // workers, (i + 1) * n // workers) for i in range(workers)]
+            list(self.pool.map(lambda c: run(*c), [c for c in cuts if c[1] > c[0]]))
+            return out
+        self.patched = (r2c, c2r, deriv)
+
+    def __enter__(self):
+        self.orc.r2c, self.orc.c2r, self.orc.deriv = self.patched
+        return self
+
+    def __exit__(self, *exc):
+        self.orc.r2c, self.orc.c2r, self.orc.deriv = self.saved
+        self.pool.shutdown()
+
+
+def cpu_baseline(workloads, samples=5):
+    """The oracle on the host cores: PC steps/s as the median of `samples`
+    timed PC steps (steppc + the driver's potential evaluation, as the GPU
+    loop counts them) after start() and the 3 RK steps, per workload."""
     import hetddft
     import inputs
     from oracle import hetddft_oracle as orc
     sys.path.insert(0, str(ROOT))
-    p = inputs.params_for(wl)
-    k = hetddft.make_kernels(p)
-    psi, r, v = inputs.initial_state(wl)
-    prm = orc.Params(nx=wl.n, ny=wl.n, nz=wl.n, xmax=wl.xmax, ymax=wl.xmax, zmax=wl.xmax,
-                     n_par=wl.n_par, mimpur=wl.mimpur, ironx=wl.ironx)
-    o = orc.Oracle(prm, k)
-    o.set_state(psi, r, v)
-    o.start()
-    from threadpoolctl import threadpool_limits
-    reps = 0
-    with threadpool_limits(limits=1):      # one core: numpy's BLAS/OpenMP pools held to one thread
-        t0 = time.perf_counter()
-        while True:
-            o.hpsi()
-            o.veval()
-            o.hpsi()
-            o.veval()
-            reps += 1
-            t1 = time.perf_counter()
-            if t1 - t0 >= 10.0:
-                break
-    return {"value": reps / (t1 - t0), "unit": "PC steps/s", "cores": 1, "kind": "port",
-            "sample": f"numpy oracle, {reps} x the work of one PC step (2 potential evaluations + 2 H psi) "
-                      f"at {wl.n}^3, N_par={wl.n_par}, after start(); {t1 - t0:.1f} s"}
+    cores = len(os.sched_getaffinity(0))
+    out = {}
+    with _ThreadedOracle(orc, cores):
+        for wl in workloads:
+            p = inputs.params_for(wl)
+            k = hetddft.make_kernels(p)
+            psi, r, v = inputs.initial_state(wl)
+            prm = orc.Params(nx=wl.n, ny=wl.n, nz=wl.n, xmax=wl.xmax, ymax=wl.xmax, zmax=wl.xmax,
+                             n_par=wl.n_par, mimpur=wl.mimpur, ironx=wl.ironx)
+            o = orc.Oracle(prm, k)
+            o.set_state(psi, r, v)
+            o.start()
+            for it in range(3):
+                o.iterate(it, wl.deltat)
+            times = []
+            for it in range(samples):
+                t0 = time.perf_counter()
+                o.iterate(3 + it, wl.deltat)
+                times.append(time.perf_counter() - t0)
+            out[wl.name] = {"steps_per_s": 1.0 / float(np.median(times)),
+                            "seconds": [round(t, 4) for t in times]}
+    main = workloads[0].name
+    return {"value": out[main]["steps_per_s"], "unit": "PC steps/s", "cores": cores, "kind": "port",
+            "cpu": _cpu_model(),
+            "sample": f"numpy oracle on {cores} threads (transforms and stencils threaded, whole-array "
+                      f"expressions single-threaded): median of {samples} PC steps after start + 3 RK "
+                      f"steps; value is {main}",
+            "per_workload": out}
 
 
 def main():
@@ -298,6 +397,10 @@ def main():
     ap.add_argument("--threads", type=int, default=8, help="host threads driving the ensemble (c4)")
     ap.add_argument("--only", choices=["fp64", "mixed", "copy"], default=None,
                     help="time one precision only, or run the copy calibration (profiling runs)")
+    ap.add_argument("--tuning", default=None, help="apply a tuning saved by --save-tuning (no autotuner)")
+    ap.add_argument("--save-tuning", default=None, help="write the launch shapes used to this JSON file")
+    ap.add_argument("--dropin-steps", type=int, default=100,
+                    help="steps of the drop-in loop (errors and particle means fetched every step)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the path has no multi-GPU mode "
@@ -313,15 +416,19 @@ def main():
         return
     if args.only:
         res = run_gpu(wl, 0 if args.only == "fp64" else 1, args.steps, args.warmup, prof_steps=5,
-                      tune=not args.no_tune)
+                      tune=not args.no_tune, tuning_file=args.tuning)
         kernels = {k: {"avg_us": round(ms * 1e3 / n, 2), "launches": n,
                        "GB_per_s": round(b / (ms * 1e-3) / 1e9, 1)}
                    for k, (ms, n, b) in res["prof"].items() if n}
         print(json.dumps({"only": args.only, "steps_per_s": res["steps_per_s"],
                           "ms_per_step": 1000.0 / res["steps_per_s"], "kernels": kernels}))
         return
-    res64 = run_gpu(wl, 0, args.steps, args.warmup, prof_steps=20, tune=not args.no_tune)
-    res32 = run_gpu(wl, 1, args.steps, args.warmup, prof_steps=20, tune=not args.no_tune)
+    res64 = run_gpu(wl, 0, args.steps, args.warmup, prof_steps=20, tune=not args.no_tune,
+                    tuning_file=args.tuning, dropin_steps=args.dropin_steps)
+    res32 = run_gpu(wl, 1, args.steps, args.warmup, prof_steps=20, tune=not args.no_tune,
+                    tuning_file=args.tuning, dropin_steps=args.dropin_steps)
+    if args.save_tuning:
+        Path(args.save_tuning).write_text(json.dumps({"fp64": res64["tuning_rec"], "mixed": res32["tuning_rec"]}))
     e64 = res64["energy"]["etot"]
     e32 = res32["energy"]["etot"]
     rel_e = abs(e32 - e64) / abs(e64)
@@ -344,12 +451,19 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64" if use_mixed else "f64",
         "data": "synthetic (seeded Woods-Saxon droplet, Gaussian particle cloud)",
-        "config": {"workload": f"{wl.name}: {wl.n}^3 grid dx={wl.h} A, N={wl.n_he} He, "
-                               f"N_par={wl.n_par}, dt={wl.dt_ps * 1000:.1f} fs, OTC, ironx={wl.ironx}",
-                   "precision": "mixed" if use_mixed else "fp64"},
+        "config": {"workload": f"{wl.name}: {wl.n}^3 grid dx={wl.h} A, N={wl.n_he} He Woods-Saxon droplet "
+                               f"with a {wl.cavity} A cavity at the impurity, {wl.phase_noise:g} rad phase "
+                               f"noise, N_par={wl.n_par} (sigma {wl.sigma} A), v0={wl.v0 * 100:g} m/s, "
+                               f"Morse D=20 K, dt={wl.dt_ps * 1000:.1f} fs, OTC, ironx={wl.ironx}",
+                   "precision": "mixed" if use_mixed else "fp64",
+                   "loop": "resident: iterate with the error estimates on the device, checked at the end "
+                           "of the timed region (hetddft_sync)"},
         "roofline": roofline(best["prof"], wl.name, "mixed" if use_mixed else "fp64"),
         "fp64_steps_per_s": res64["steps_per_s"],
         "mixed_steps_per_s": res32["steps_per_s"],
+        # the reference's loop: errors and particle means to the host every step
+        "dropin_loop_steps_per_s": {"fp64": res64["dropin_steps_per_s"], "mixed": res32["dropin_steps_per_s"],
+                                    "steps": args.dropin_steps},
         "mixed_vs_fp64_speedup": res32["steps_per_s"] / res64["steps_per_s"],
         "mixed_rel_energy_err": rel_e,
         "mixed_density_rel_l2_err": den_l2,
@@ -362,7 +476,8 @@ def main():
         "device_bytes": best["device_bytes"],
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(wl)
+        others = [inputs.WORKLOADS[n] for n in ("c1", "c2") if n != wl.name]
+        line["cpu_baseline"] = cpu_baseline([wl] + others)
     print(json.dumps(line))
 
 
