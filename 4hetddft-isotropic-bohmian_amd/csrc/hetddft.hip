@@ -216,6 +216,9 @@ struct hetddft_ctx {
   };
   ChainSet cset[2];
   int cs = 0;
+  // PC steps keep the particle integrators on st2 beside the field work
+  // (false with HIP graphs, whose captures fork and join st2 within a step)
+  bool lean = true;
   bool chain_pending = false;  // a chain for the next potential evaluation is queued on st2
   bool side_pending = false;   // st2 holds work that st has not waited for
   // particles
@@ -854,8 +857,12 @@ int settle(hetddft_ctx* c) {
 // potenimp(); poten(); forceimp().  The particle density chain runs on the
 // side stream (issued here, or by the caller when chain_issued) beside the
 // density-side work on st, which waits for it only where denx enters.
+// With leave_side (the PC step's layout), the force gather stays on st2
+// without a join: st does not read particle state, and the next potential
+// evaluation's wait for the particle chain (queued on st2 after the gather)
+// orders it after the gather.
 template <class T>
-int veval(hetddft_ctx* c) {
+int veval(hetddft_ctx* c, bool leave_side = false) {
   const Grid& g = c->g;
   auto F = [&](int id) { return (T*)c->fld[id]; };
   const bool otc = c->p.core_otc != 0;
@@ -1001,7 +1008,9 @@ int veval(hetddft_ctx* c) {
     });
   }
   LAUNCHCHK(c);
-  {
+  if (leave_side) {
+    c->side_pending = true;
+  } else {
     // H psi_X (and the gather, with particles) ran on the side stream
     RET(join_side(c, c->ev_force));
     RET(prof_mark(c));
@@ -1116,15 +1125,25 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   const unsigned nb = (unsigned)blocks25(c, HETDDFT_TUNE_PCC);
   // Particle predictor (steppc.f90:64-87), then the density chain of the
   // predicted positions on the side stream, beside the psi predictor.
+  // In the lean layout the particle predictor, the chain, the force gather
+  // and the particle corrector all run on st2, in that order, and st2 waits
+  // for st only before each gather (the force fields of batch 3).
+  const bool lean = c->lean && !c->one_stream;
   if (n3 > 0) {
-    SLAUNCH(k_part_predict, dim3(blocks_for(n3)), dim3(kBlock), 0, n3, c->rimp,
-                       (const double*)c->vimp, (const double*)c->aimp, c->rold[R[2]], (const double*)c->vold[V[0]],
-                       (const double*)c->vold[V[1]], c->vold[V[2]], (const double*)c->aold[A[0]], c->aold[A[1]],
-                       c->pcr, c->pcv, c4o3 * dt, c112);
+    if (lean)
+      S2LAUNCH(k_part_predict, dim3(blocks_for(n3)), dim3(kBlock), 0, n3, c->rimp,
+               (const double*)c->vimp, (const double*)c->aimp, c->rold[R[2]], (const double*)c->vold[V[0]],
+               (const double*)c->vold[V[1]], c->vold[V[2]], (const double*)c->aold[A[0]], c->aold[A[1]],
+               c->pcr, c->pcv, c4o3 * dt, c112);
+    else
+      SLAUNCH(k_part_predict, dim3(blocks_for(n3)), dim3(kBlock), 0, n3, c->rimp,
+              (const double*)c->vimp, (const double*)c->aimp, c->rold[R[2]], (const double*)c->vold[V[0]],
+              (const double*)c->vold[V[1]], c->vold[V[2]], (const double*)c->aold[A[0]], c->aold[A[1]],
+              c->pcr, c->pcv, c4o3 * dt, c112);
     LAUNCHCHK(c);
   }
   std::swap(A[0], A[1]);
-  RET(fork_side(c));
+  if (!lean) RET(fork_side(c));
   RET(issue_chain<T>(c));
   // psi predictor + modifier (steppc.f90:36-62)
   c64* x = c->psi;
@@ -1145,17 +1164,23 @@ int step_pc_t(hetddft_ctx* c, double dt) {
   c->old[P[2]] = x;
   c->psi = y;
   std::swap(H[0], H[1]);
-  RET(veval<T>(c));
+  RET(veval<T>(c, lean));
   // Particle corrector (steppc.f90:122-151), then the density chain of the
   // corrected positions on the side stream beside the psi corrector; the
   // driver's potential evaluation (DFT4He3d.f90:861) consumes that chain.
   if (n3 > 0) {
     const unsigned pb = blocks_for(n3);
-    double* pp = c->part + nb;  // after the psi partials
-    SLAUNCH(k_part_correct, dim3(pb), dim3(kBlock), 0, n3, c->rimp, c->vimp,
-                       (const double*)c->aimp, (const double*)c->rold[R[2]], (const double*)c->rold[R[1]],
-                       (const double*)c->vold[V[0]], (const double*)c->vold[V[1]], c->vold[V[2]],
-                       (const double*)c->aold[A[0]], (const double*)c->aold[A[1]], c->pcr, c->pcv, pp, 3.0 * dt, c9);
+    double* pp = c->part + nb;  // after the psi partials (disjoint from the psi corrector's)
+    if (lean)
+      S2LAUNCH(k_part_correct, dim3(pb), dim3(kBlock), 0, n3, c->rimp, c->vimp,
+               (const double*)c->aimp, (const double*)c->rold[R[2]], (const double*)c->rold[R[1]],
+               (const double*)c->vold[V[0]], (const double*)c->vold[V[1]], c->vold[V[2]],
+               (const double*)c->aold[A[0]], (const double*)c->aold[A[1]], c->pcr, c->pcv, pp, 3.0 * dt, c9);
+    else
+      SLAUNCH(k_part_correct, dim3(pb), dim3(kBlock), 0, n3, c->rimp, c->vimp,
+              (const double*)c->aimp, (const double*)c->rold[R[2]], (const double*)c->rold[R[1]],
+              (const double*)c->vold[V[0]], (const double*)c->vold[V[1]], c->vold[V[2]],
+              (const double*)c->aold[A[0]], (const double*)c->aold[A[1]], c->pcr, c->pcv, pp, 3.0 * dt, c9);
     LAUNCHCHK(c);
   }
   {
@@ -1168,7 +1193,7 @@ int step_pc_t(hetddft_ctx* c, double dt) {
     V[1] = V[0];
     V[0] = t;
   }
-  RET(fork_side(c));
+  if (!lean) RET(fork_side(c));
   RET(issue_chain<T>(c));
   // psi corrector (steppc.f90:97-120); the error sums stay on the device
   // until fetch_errors asks for them
@@ -1486,7 +1511,7 @@ void clear_graphs(hetddft_ctx* c) {
 template <class T>
 int pc_iteration(hetddft_ctx* c, double dt) {
   RET(step_pc_t<T>(c, dt));
-  return veval<T>(c);
+  return veval<T>(c, c->lean && !c->one_stream);
 }
 
 // One PC step and the driver's potential evaluation.  With graphs on
@@ -1823,6 +1848,7 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
     const char* gph = std::getenv("HETDDFT_GRAPH");
     c->graph_on = gph && std::atoi(gph) != 0;
     c->graph_one_stream = gph && std::atoi(gph) == 2;
+    c->lean = !c->graph_on;
     for (int k = 0; k < HETDDFT_TUNE_NSTENCIL; ++k) {
       c->tile_k[k] = c->tile;
       c->zc_k[k] = c->zc;
