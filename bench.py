@@ -301,51 +301,6 @@ def _cpu_model():
     return "unknown"
 
 
-class _ThreadedOracle:
-    """The oracle's transforms and stencils spread over `workers` threads:
-    scipy.fft (pocketfft, the same algorithm as numpy.fft) with `workers`
-    threads, and each 13-point stencil pass cut into slabs along an axis it
-    does not differentiate (numpy releases the GIL), so every value is formed
-    as in the oracle.  The remaining whole-array expressions stay numpy's
-    single-threaded ones."""
-
-    def __init__(self, orc, workers):
-        import scipy.fft
-        from concurrent.futures import ThreadPoolExecutor
-        self.orc, self.workers = orc, workers
-        self.pool = ThreadPoolExecutor(workers)
-        self.saved = (orc.r2c, orc.c2r, orc.deriv)
-        plain = orc.deriv
-
-        def r2c(f):
-            return scipy.fft.rfftn(f, workers=workers)
-
-        def c2r(F, shape, renor):
-            return scipy.fft.irfftn(F, s=shape, axes=(0, 1, 2), norm="forward", workers=workers) * renor
-
-        def deriv(f, axis, order, h):
-            split = 0 if orc._AX[axis] != 0 else 1
-            n = f.shape[split]
-            out = np.empty_like(f)
-
-            def run(lo, hi):
-                sl = [slice(None)] * 3
-                sl[split] = slice(lo, hi)
-                out[tuple(sl)] = plain(f[tuple(sl)], axis, order, h)
-            cuts = [(i * n // workers, (i + 1) * n // workers) for i in range(workers)]
-            list(self.pool.map(lambda c: run(*c), [c for c in cuts if c[1] > c[0]]))
-            return out
-        self.patched = (r2c, c2r, deriv)
-
-    def __enter__(self):
-        self.orc.r2c, self.orc.c2r, self.orc.deriv = self.patched
-        return self
-
-    def __exit__(self, *exc):
-        self.orc.r2c, self.orc.c2r, self.orc.deriv = self.saved
-        self.pool.shutdown()
-
-
 def cpu_baseline(workloads, samples=5):
     """The oracle on the host cores: PC steps/s as the median of `samples`
     timed PC steps (steppc + the driver's potential evaluation, as the GPU
@@ -356,7 +311,8 @@ def cpu_baseline(workloads, samples=5):
     sys.path.insert(0, str(ROOT))
     cores = len(os.sched_getaffinity(0))
     out = {}
-    with _ThreadedOracle(orc, cores):
+    from oracle.threaded import ThreadedOracle
+    with ThreadedOracle(orc, cores):
         for wl in workloads:
             p = inputs.params_for(wl)
             k = hetddft.make_kernels(p)

@@ -159,3 +159,25 @@ def test_kinetic_energy_of_gaussian():
     ekin = prm.h2o2m4 * sum(np.sum(np.abs(d) ** 2) for d in g) * prm.dxyz
     want = prm.h2o2m4 * 1.5 * math.pi ** 1.5 * s
     assert abs(ekin - want) < 2e-9 * want   # 13-point truncation error ~ (h/s)^12
+
+
+def test_threaded_oracle_forms_the_same_values():
+    """oracle/threaded.py (the multi-core oracle of the size-scale tests and
+    the CPU baseline) gives the single-threaded oracle's stencils bit for bit
+    and its transforms to rounding."""
+    from oracle.threaded import ThreadedOracle
+    g = np.random.default_rng(3)
+    f = g.standard_normal((16, 20, 24))
+    c = f + 1j * g.standard_normal(f.shape)
+    plain = [orc.deriv(f, a, k, 0.3) for a in "xyz" for k in (1, 2)] + [orc.deriv(c, "y", 2, 0.4)]
+    F = orc.r2c(f)
+    back = orc.c2r(F, f.shape, 1.0 / f.size)
+    with ThreadedOracle(orc, 3):
+        threaded = [orc.deriv(f, a, k, 0.3) for a in "xyz" for k in (1, 2)] + [orc.deriv(c, "y", 2, 0.4)]
+        F2 = orc.r2c(f)
+        back2 = orc.c2r(F2, f.shape, 1.0 / f.size)
+    for x, y in zip(plain, threaded):
+        assert np.array_equal(x, y)
+    assert np.abs(F2 - F).max() <= 1e-13 * np.abs(F).max()
+    assert np.abs(back2 - back).max() <= 1e-14
+    assert orc.deriv is not None and orc.r2c.__module__ == orc.__name__
