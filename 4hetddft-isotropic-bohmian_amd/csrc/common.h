@@ -127,4 +127,8 @@ struct Grid {
   long long h;      // mp*ny*nz (padded half spectrum)
 };
 
+// Instances of one grid that a launch carries at most (the member tables of
+// the kernels hold one pointer per instance).
+constexpr int kMaxMembers = 8;
+
 }  // namespace hetd

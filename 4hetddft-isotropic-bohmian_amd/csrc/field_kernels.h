@@ -223,7 +223,7 @@ __device__ inline void reg_union_apply(Grid g, int* __restrict__ reg, int margin
 // the boxes.  (Combining them in the
 // last workgroup to finish, behind a device-scope fence in every workgroup,
 // measured 3-4% slower per step at 128^3.)
-__global__ void __launch_bounds__(kBlock) k_hist(BoxArgs b, const double* __restrict__ r,
+__device__ __forceinline__ void hist_body(const BoxArgs& b, const double* __restrict__ r,
                                                  unsigned int* __restrict__ cnt, int* __restrict__ part) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int ix = 0, iy = 0, iz = 0;
@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(kBlock) k_iron(Grid g, const int* __restrict__
 // zero at every level (the region covers the histogram grown by more than
 // NP), so the results equal NP launches of k_iron bit for bit.
 template <class R, int NP>
-__global__ void __launch_bounds__(kBlock) k_iron_fused(Grid g, int* __restrict__ reg,
+__device__ __forceinline__ void iron_fused_body(const Grid& g, int* __restrict__ reg,
                                                         const int* __restrict__ part, int nb, int margin,
                                                         const unsigned int* __restrict__ cnt, R* __restrict__ dst,
                                                         R* __restrict__ psix, double npar, double hxyz) {
@@ -471,9 +471,9 @@ __global__ void __launch_bounds__(kBlock) k_iron_fused(Grid g, int* __restrict__
 // no clearing pass, and stores live := expand(box, margin) (the value it
 // already has when k_reg_union ran).
 template <class R>
-__global__ void __launch_bounds__(kBlock) k_hpsix(Grid g, int* __restrict__ reg, const R* __restrict__ psix,
-                                                   Stencil7<double, true> wx, Stencil7<double, true> wy,
-                                                   Stencil7<double, true> wz, double h2o2mx,
+__device__ __forceinline__ void hpsix_body(const Grid& g, int* __restrict__ reg, const R* __restrict__ psix,
+                                                   const Stencil7<double, true>& wx, const Stencil7<double, true>& wy,
+                                                   const Stencil7<double, true>& wz, double h2o2mx,
                                                    R* __restrict__ hpsix, unsigned int* __restrict__ cnt,
                                                    int margin) {
   const RegionIter it = region_iter(reg + kRegWork);
@@ -500,21 +500,33 @@ struct GatherArgs {
 };
 
 // Quantum part of F_c at grid point (ix,iy,iz), 0-based (potenimp.f90:52-69):
-// sto7*(d_c psix*hpsix - psix*d_c hpsix), sto7 = 1/denx if > 0.
-template <class R>
-__device__ inline double force_q(const GatherArgs& a, int c, const R* __restrict__ denx,
-                                 const R* __restrict__ psix, const R* __restrict__ hpsix, int ix, int iy,
-                                 int iz) {
+// sto7*(d_c psix*hpsix - psix*d_c hpsix), sto7 = 1/denx if > 0.  The
+// component is a template argument so that every read of the kernel's
+// argument block has a fixed offset; a run-time choice among its fields
+// makes the compiler copy the whole block into private memory.
+template <int C, class R>
+__device__ __forceinline__ double force_q_c(const GatherArgs& a, const R* __restrict__ denx,
+                                            const R* __restrict__ psix, const R* __restrict__ hpsix, int ix,
+                                            int iy, int iz) {
   const Grid& g = a.b.g;
   const long long j = ix + (long long)g.nx * (iy + (long long)g.ny * iz);
   const double dx = (double)denx[j];
   const double sto7 = dx > 0.0 ? 1.0 / dx : 0.0;
-  const int pos = c == 0 ? ix : (c == 1 ? iy : iz);
-  const int n = c == 0 ? g.nx : (c == 1 ? g.ny : g.nz);
-  const long long st = c == 0 ? 1 : (c == 1 ? (long long)g.nx : (long long)g.nx * g.ny);
-  const double s1 = d13(psix, j, pos, n, st, a.d1[c]);
-  const double s4 = d13(hpsix, j, pos, n, st, a.d1[c]);
+  const int pos = C == 0 ? ix : (C == 1 ? iy : iz);
+  const int n = C == 0 ? g.nx : (C == 1 ? g.ny : g.nz);
+  const long long st = C == 0 ? 1 : (C == 1 ? (long long)g.nx : (long long)g.nx * g.ny);
+  const double s1 = d13(psix, j, pos, n, st, a.d1[C]);
+  const double s4 = d13(hpsix, j, pos, n, st, a.d1[C]);
   return sto7 * (s1 * (double)hpsix[j] - (double)psix[j] * s4);
+}
+
+template <class R>
+__device__ __forceinline__ double force_q(const GatherArgs& a, int c, const R* __restrict__ denx,
+                                          const R* __restrict__ psix, const R* __restrict__ hpsix, int ix, int iy,
+                                          int iz) {
+  if (c == 0) return force_q_c<0>(a, denx, psix, hpsix, ix, iy, iz);
+  if (c == 1) return force_q_c<1>(a, denx, psix, hpsix, ix, iy, iz);
+  return force_q_c<2>(a, denx, psix, hpsix, ix, iy, iz);
 }
 
 // F_c = (fconv + quantum part)/m_imp at grid point j.
@@ -533,7 +545,7 @@ __device__ inline double force_of(const GatherArgs& a, const R* __restrict__ fco
 // (dx,dy,dz) = (bit2,bit1,bit0) in the order x000..x111 of blend_103.  The
 // buffer holds 24 N_par doubles either way.
 template <class R>
-__global__ void __launch_bounds__(kBlock) k_force_q(GatherArgs a, const int* __restrict__ reg,
+__device__ __forceinline__ void force_q_body(const GatherArgs& a, const int* __restrict__ reg,
                                                     const double* __restrict__ rp, const R* __restrict__ denx,
                                                     const R* __restrict__ psix, const R* __restrict__ hpsix,
                                                     double* __restrict__ out) {
@@ -566,7 +578,7 @@ __global__ void __launch_bounds__(kBlock) k_force_q(GatherArgs a, const int* __r
 // thread per (particle, component); the quantum parts in either layout of
 // k_force_q.
 template <class R>
-__global__ void __launch_bounds__(kBlock) k_force_blend(GatherArgs a, const int* __restrict__ reg,
+__device__ __forceinline__ void force_blend_body(const GatherArgs& a, const int* __restrict__ reg,
                                                         const double* __restrict__ rp,
                                                         const R* __restrict__ fcx, const R* __restrict__ fcy,
                                                         const R* __restrict__ fcz, const double* __restrict__ fq,
@@ -608,7 +620,7 @@ __global__ void __launch_bounds__(kBlock) k_force_blend(GatherArgs a, const int*
 
 // ---- particle integrators (steprk.f90:78-105, steppc.f90:64-87,122-151) --
 // All particle arrays are (N_par,3) column-major: entry c*N_par + i.
-__global__ void k_part_rk(int n3, double* __restrict__ r, double* __restrict__ v,
+__device__ __forceinline__ void part_rk_body(int n3, double* __restrict__ r, double* __restrict__ v,
                           const double* __restrict__ acc, double* __restrict__ qr, double* __restrict__ qv,
                           double* __restrict__ r1_out, double* __restrict__ v1_out,
                           double* __restrict__ a1_out, double ca, double cb, double cc, double dt) {
@@ -626,7 +638,7 @@ __global__ void k_part_rk(int n3, double* __restrict__ r, double* __restrict__ v
   v[i] = vi + dt * st;
 }
 
-__global__ void k_part_predict(int n3, double* __restrict__ r, const double* __restrict__ v,
+__device__ __forceinline__ void part_predict_body(int n3, double* __restrict__ r, const double* __restrict__ v,
                                const double* __restrict__ acc, double* r3, const double* __restrict__ v1,
                                const double* __restrict__ v2, double* v3, const double* __restrict__ a1,
                                double* a2, double* __restrict__ pcr, double* __restrict__ pcv,
@@ -643,7 +655,7 @@ __global__ void k_part_predict(int n3, double* __restrict__ r, const double* __r
   a2[i] = acc[i];
 }
 
-__global__ void k_part_correct(int n3, double* __restrict__ r, double* __restrict__ v,
+__device__ __forceinline__ void part_correct_body(int n3, double* __restrict__ r, double* __restrict__ v,
                                const double* __restrict__ acc, const double* __restrict__ r3,
                                const double* __restrict__ r2, const double* __restrict__ v1,
                                const double* __restrict__ v2, double* v3, const double* __restrict__ a1,
@@ -947,6 +959,135 @@ __global__ void __launch_bounds__(kBlock) k_sorteo(SortArgs a, int np, const dou
     }
   }
   atomicOr(failed, 1);
+}
+
+// ---- particle-side launches carrying several members -----------------------
+// blockIdx.y is the member; each member's work uses blockIdx.x and gridDim.x
+// exactly as a launch of its own.  All members share the box and step.
+struct HistTab {
+  const double* r[kMaxMembers];
+  unsigned int* cnt[kMaxMembers];
+  int* part[kMaxMembers];
+};
+__global__ void __launch_bounds__(kBlock) k_hist(BoxArgs b, HistTab t) {
+  const int m = blockIdx.y;
+  hist_body(b, t.r[m], t.cnt[m], t.part[m]);
+}
+
+template <class R>
+struct IronTab {
+  int* reg[kMaxMembers];
+  const int* part[kMaxMembers];
+  const unsigned int* cnt[kMaxMembers];
+  R* dst[kMaxMembers];
+  R* psix[kMaxMembers];
+};
+template <class R, int NP>
+__global__ void __launch_bounds__(kBlock) k_iron_fused(Grid g, IronTab<R> t, int nb, int margin, double npar,
+                                                        double hxyz) {
+  const int m = blockIdx.y;
+  iron_fused_body<R, NP>(g, t.reg[m], t.part[m], nb, margin, t.cnt[m], t.dst[m], t.psix[m], npar, hxyz);
+}
+
+template <class R>
+struct HpsixTab {
+  int* reg[kMaxMembers];
+  const R* psix[kMaxMembers];
+  R* hpsix[kMaxMembers];
+  unsigned int* cnt[kMaxMembers];
+};
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_hpsix(Grid g, HpsixTab<R> t, Stencil7<double, true> wx,
+                                                   Stencil7<double, true> wy, Stencil7<double, true> wz,
+                                                   double h2o2mx, int margin) {
+  const int m = blockIdx.y;
+  hpsix_body<R>(g, t.reg[m], t.psix[m], wx, wy, wz, h2o2mx, t.hpsix[m], t.cnt[m], margin);
+}
+
+template <class R>
+struct ForceQTab {
+  const int* reg[kMaxMembers];
+  const double* rp[kMaxMembers];
+  const R* denx[kMaxMembers];
+  const R* psix[kMaxMembers];
+  const R* hpsix[kMaxMembers];
+  double* out[kMaxMembers];
+};
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_force_q(GatherArgs a, ForceQTab<R> t) {
+  const int m = blockIdx.y;
+  force_q_body<R>(a, t.reg[m], t.rp[m], t.denx[m], t.psix[m], t.hpsix[m], t.out[m]);
+}
+
+template <class R>
+struct BlendTab {
+  const int* reg[kMaxMembers];
+  const double* rp[kMaxMembers];
+  const R* fcx[kMaxMembers];
+  const R* fcy[kMaxMembers];
+  const R* fcz[kMaxMembers];
+  const double* fq[kMaxMembers];
+  double* acc[kMaxMembers];
+};
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_force_blend(GatherArgs a, BlendTab<R> t) {
+  const int m = blockIdx.y;
+  force_blend_body<R>(a, t.reg[m], t.rp[m], t.fcx[m], t.fcy[m], t.fcz[m], t.fq[m], t.acc[m]);
+}
+
+struct PartRkTab {
+  double* r[kMaxMembers];
+  double* v[kMaxMembers];
+  const double* acc[kMaxMembers];
+  double* qr[kMaxMembers];
+  double* qv[kMaxMembers];
+  double* r1[kMaxMembers];
+  double* v1[kMaxMembers];
+  double* a1[kMaxMembers];
+};
+__global__ void k_part_rk(int n3, PartRkTab t, double ca, double cb, double cc, double dt) {
+  const int m = blockIdx.y;
+  part_rk_body(n3, t.r[m], t.v[m], t.acc[m], t.qr[m], t.qv[m], t.r1[m], t.v1[m], t.a1[m], ca, cb, cc, dt);
+}
+
+struct PartPredTab {
+  double* r[kMaxMembers];
+  const double* v[kMaxMembers];
+  const double* acc[kMaxMembers];
+  double* r3[kMaxMembers];
+  const double* v1[kMaxMembers];
+  const double* v2[kMaxMembers];
+  double* v3[kMaxMembers];
+  const double* a1[kMaxMembers];
+  double* a2[kMaxMembers];
+  double* pcr[kMaxMembers];
+  double* pcv[kMaxMembers];
+};
+__global__ void k_part_predict(int n3, PartPredTab t, double c4o3dt, double c112) {
+  const int m = blockIdx.y;
+  part_predict_body(n3, t.r[m], t.v[m], t.acc[m], t.r3[m], t.v1[m], t.v2[m], t.v3[m], t.a1[m], t.a2[m], t.pcr[m],
+                    t.pcv[m], c4o3dt, c112);
+}
+
+struct PartCorrTab {
+  double* r[kMaxMembers];
+  double* v[kMaxMembers];
+  const double* acc[kMaxMembers];
+  const double* r3[kMaxMembers];
+  const double* r2[kMaxMembers];
+  const double* v1[kMaxMembers];
+  const double* v2[kMaxMembers];
+  double* v3[kMaxMembers];
+  const double* a1[kMaxMembers];
+  const double* a2[kMaxMembers];
+  double* pcr[kMaxMembers];
+  double* pcv[kMaxMembers];
+  double* part[kMaxMembers];
+};
+__global__ void k_part_correct(int n3, PartCorrTab t, double dt3, double c9) {
+  const int m = blockIdx.y;
+  part_correct_body(n3, t.r[m], t.v[m], t.acc[m], t.r3[m], t.r2[m], t.v1[m], t.v2[m], t.v3[m], t.a1[m], t.a2[m],
+                    t.pcr[m], t.pcv[m], t.part[m], dt3, c9);
 }
 
 }  // namespace hetd

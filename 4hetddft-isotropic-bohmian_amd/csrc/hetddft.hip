@@ -595,22 +595,54 @@ struct ConvJob {
 // each input feeds exactly one job, the z pass works in place (a work item
 // reads its whole column tile before writing it back, and distinct items
 // cover disjoint columns), so a batch needs only nin spectrum buffers.
+// Spectrum buffers of one instance's batch: input i in spec[i]; job k in
+// place in its input's buffer when every input feeds exactly one job, else in
+// spec[nin + k].  False when the batch does not fit the instance's buffers.
+template <class T>
+bool batch_buffers(const FftSet& fs, int nin, const std::vector<ConvJob<T>>& jobs, std::vector<void*>& sin,
+                   std::vector<void*>& sout) {
+  const int nout = (int)jobs.size();
+  int uses[kMaxBatch] = {};
+  for (const ConvJob<T>& j : jobs) {
+    if (j.input < 0 || j.input >= nin || nin > kMaxBatch) return false;
+    uses[j.input]++;
+  }
+  bool inplace = nout == nin;
+  for (int i = 0; i < nin && inplace; ++i) inplace = uses[i] == 1;
+  if (nin < 1 || nout < 1 || (!inplace && nin + nout > kSpectra)) return false;
+  for (int i = 0; i < nin; ++i) sin.push_back(fs.spec[i]);
+  for (int k = 0; k < nout; ++k) sout.push_back(inplace ? fs.spec[jobs[k].input] : fs.spec[nin + k]);
+  return true;
+}
+
+template <class T>
+int conv_batch_in(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin, const std::vector<ConvJob<T>>& jobs,
+                  void* const* spec_in, void* const* spec_out);
+
 template <class T>
 int conv_batch(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin,
                const std::vector<ConvJob<T>>& jobs) {
+  std::vector<void*> sin, sout;
+  if (!batch_buffers<T>(fs, nin, jobs, sin, sout))
+    return fail(c, HETDDFT_E_ARG, "conv_batch: %d in / %d out", nin, (int)jobs.size());
+  return conv_batch_in<T>(c, fs, xf_in, nin, jobs, sin.data(), sout.data());
+}
+
+// One launch per pass for all inputs and jobs given: input i's spectrum in
+// spec_in[i], job k's in spec_out[k] (its input's buffer when in place).
+// fs supplies the twiddle tables.  The jobs of several instances of one grid
+// go into one batch this way.
+template <class T>
+int conv_batch_in(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin, const std::vector<ConvJob<T>>& jobs,
+                  void* const* spec_in, void* const* spec_out) {
   const Grid& g = c->g;
   const int nout = (int)jobs.size();
-  int uses[kMaxBatch] = {};
-  for (const ConvJob<T>& j : jobs)
-    if (j.input >= 0 && j.input < kMaxBatch) uses[j.input]++;
-  bool inplace = nout == nin;
-  for (int i = 0; i < nin && inplace; ++i) inplace = uses[i] == 1;
-  if (nin < 1 || nout < 1 || nin > kMaxBatch || nout > kMaxBatch || (!inplace && nin + nout > kSpectra))
+  if (nin < 1 || nout < 1 || nin > kMaxBatch || nout > kMaxBatch || nout > kMaxJobs)
     return fail(c, HETDDFT_E_ARG, "conv_batch: %d in / %d out", nin, nout);
   for (const ConvJob<T>& j : jobs)
     if (j.input < 0 || j.input >= nin) return fail(c, HETDDFT_E_ARG, "conv_batch: job input %d", j.input);
   XfArgs<T> xa = xf_in;
-  for (int i = 0; i < nin; ++i) xa.out[i] = fs.spec[i];
+  for (int i = 0; i < nin; ++i) xa.out[i] = spec_in[i];
   FftKernels<T>& K = fft_of<T>(c);
   const dim3 gx((unsigned)((g.ny + kRowsX - 1) / kRowsX), (unsigned)g.nz, (unsigned)nin);
   const unsigned ylines = (unsigned)(((long long)g.mx * g.nz + kLinesY - 1) / kLinesY);
@@ -639,7 +671,7 @@ int conv_batch(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin,
     }
     LAUNCHCHK(c);
     YArgs<T> ya{};
-    for (int i = 0; i < nin; ++i) ya.buf[i] = fs.spec[i];
+    for (int i = 0; i < nin; ++i) ya.buf[i] = spec_in[i];
     ProfScope ps(c, HETDDFT_PROF_YF, nin * 2.0 * Sb);
     PLAUNCH(ps, K.yf, dim3(ylines, 1, nin), dim3(kThreads), fs.lds_y, ya, g, c->pl_y, (const cplx<T>*)fs.tw_y);
     LAUNCHCHK(c);
@@ -650,10 +682,10 @@ int conv_batch(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin,
   xi.renor = c->renor;
   for (int k = 0; k < nout; ++k) {
     const ConvJob<T>& j = jobs[k];
-    za.in[k] = fs.spec[j.input];
+    za.in[k] = spec_in[j.input];
     za.kern[k] = (const T*)j.kern;
     za.ksign[k] = (T)j.sign;
-    void* os = inplace ? fs.spec[j.input] : fs.spec[nin + k];
+    void* os = spec_out[k];
     za.out[k] = os;
     yb.buf[k] = os;
     xi.in[k] = os;
@@ -741,6 +773,13 @@ template <class SH>
 dim3 grid25(const hetddft_ctx* c, int k) {
   return grid25z<SH>(c->g, c->zc_k[k]);
 }
+// ... for nm members (member-major in z)
+template <class SH>
+dim3 grid25m(const hetddft_ctx* c, int k, int nm) {
+  dim3 d = grid25<SH>(c, k);
+  d.z *= (unsigned)nm;
+  return d;
+}
 // Workgroup count of class k's launch.
 long long blocks25(const hetddft_ctx* c, int k) {
   const dim3 d = with_tile(c->tile_k[k], [&](auto sh) { return grid25<decltype(sh)>(c, k); });
@@ -774,63 +813,113 @@ GatherArgs gather_args(const hetddft_ctx* c) {
 // the field kernels, so it takes few CUs.
 unsigned region_blocks() { return 256u; }
 
-// Particle density chain of potenimp (denpart.f90, ironing.f90).
+// Instances advanced together: every launch goes to the streams of the
+// first (lead) instance and carries the data of all.  The members share
+// grid, parameters, launch shapes, step counter and slot state, so each step's
+// host logic is the same for all of them.  A single instance is a group of
+// one.
+struct Group {
+  hetddft_ctx* m[kMaxMembers] = {};
+  int n = 0;
+};
+inline Group solo(hetddft_ctx* c) {
+  Group g;
+  g.m[0] = c;
+  g.n = 1;
+  return g;
+}
+
+// Particle density chain of potenimp (denpart.f90, ironing.f90) of every
+// member, launched on the lead c's side stream: histogram, ironing and psi_X
+// (one launch each for all members), then H psi_X.
 template <class T>
-int issue_chain(hetddft_ctx* c) {
+int issue_chain(Group& G) {
+  hetddft_ctx* c = G.m[0];
   const Grid& g = c->g;
-  hetddft_ctx::ChainSet& S = c->cset[c->cs ^ 1];
   // ---- potenimp: denpart (denpart.f90:5-49) on the occupied region only,
   // queued on the side stream into the set that is not current
-  BoxArgs b = box_args(c);
+  const BoxArgs b = box_args(c);
   if (c->p.n_par > 0) {
-    S2LAUNCH(k_hist, dim3(blocks_for(c->p.n_par)), dim3(kBlock), 0, b, c->rimp, S.counts, c->hpart);
+    HistTab t{};
+    for (int i = 0; i < G.n; ++i) {
+      hetddft_ctx* m = G.m[i];
+      t.r[i] = m->rimp;
+      t.cnt[i] = m->cset[m->cs ^ 1].counts;
+      t.part[i] = m->hpart;
+    }
+    S2LAUNCH(k_hist, dim3(blocks_for(c->p.n_par), G.n), dim3(kBlock), 0, b, t);
     LAUNCHCHK(c);
   }
   const int nb = c->p.n_par > 0 ? (int)blocks_for(c->p.n_par) : 0;
   const int margin = c->p.ironx + 6;
   const bool fused_iron = c->iron_fused && c->p.ironx >= 1 && c->p.ironx <= 3;
-  if (!fused_iron) {
-    S2LAUNCH(k_reg_union, dim3(1), dim3(64), 0, g, S.reg, margin, (const int*)c->hpart, nb);
-    LAUNCHCHK(c);
-  }
   const double hxyz = c->hx * c->hy * c->hz;
   const double npar = (double)c->p.n_par;
-  T* cur = (T*)S.denxa;
-  T* oth = (T*)S.denxb;
-  T* psix = (T*)S.psix;
-  const int npass = c->p.ironx > 0 ? c->p.ironx : 1;
   if (fused_iron) {
     // all passes (and the region update) in one launch
+    IronTab<T> t{};
+    for (int i = 0; i < G.n; ++i) {
+      hetddft_ctx* m = G.m[i];
+      hetddft_ctx::ChainSet& S = m->cset[m->cs ^ 1];
+      t.reg[i] = S.reg;
+      t.part[i] = m->hpart;
+      t.cnt[i] = S.counts;
+      t.dst[i] = (T*)S.denxa;
+      t.psix[i] = (T*)S.psix;
+      S.denx = S.denxa;
+    }
     auto fused = [&](auto np) {
-      S2LAUNCH((k_iron_fused<T, decltype(np)::value>), dim3(region_blocks()), dim3(kBlock), 0, g, S.reg,
-               (const int*)c->hpart, nb, margin, (const unsigned int*)S.counts, cur, psix, npar, hxyz);
+      S2LAUNCH((k_iron_fused<T, decltype(np)::value>), dim3(region_blocks(), G.n), dim3(kBlock), 0, g, t, nb,
+               margin, npar, hxyz);
     };
     if (c->p.ironx == 1) fused(std::integral_constant<int, 1>{});
     else if (c->p.ironx == 2) fused(std::integral_constant<int, 2>{});
     else fused(std::integral_constant<int, 3>{});
     LAUNCHCHK(c);
   } else {
-    S2LAUNCH((k_iron<T>), dim3(region_blocks()), dim3(kBlock), 0, g, (const int*)S.reg, S.counts,
-             (const T*)nullptr, cur, npass == 1 ? psix : (T*)nullptr, npar, hxyz, c->p.ironx > 0 ? 1 : 0);
-    LAUNCHCHK(c);
-    for (int it = 1; it < npass; ++it) {
-      S2LAUNCH((k_iron<T>), dim3(region_blocks()), dim3(kBlock), 0, g, (const int*)S.reg,
-               (const unsigned int*)nullptr, (const T*)cur, oth, it == npass - 1 ? psix : (T*)nullptr, npar,
-               hxyz, 1);
+    const int npass = c->p.ironx > 0 ? c->p.ironx : 1;
+    for (int i = 0; i < G.n; ++i) {
+      hetddft_ctx* m = G.m[i];
+      hetddft_ctx::ChainSet& S = m->cset[m->cs ^ 1];
+      S2LAUNCH(k_reg_union, dim3(1), dim3(64), 0, g, S.reg, margin, (const int*)m->hpart, nb);
       LAUNCHCHK(c);
-      std::swap(cur, oth);
+      T* cur = (T*)S.denxa;
+      T* oth = (T*)S.denxb;
+      T* psix = (T*)S.psix;
+      S2LAUNCH((k_iron<T>), dim3(region_blocks()), dim3(kBlock), 0, g, (const int*)S.reg, S.counts,
+               (const T*)nullptr, cur, npass == 1 ? psix : (T*)nullptr, npar, hxyz, c->p.ironx > 0 ? 1 : 0);
+      LAUNCHCHK(c);
+      for (int it = 1; it < npass; ++it) {
+        S2LAUNCH((k_iron<T>), dim3(region_blocks()), dim3(kBlock), 0, g, (const int*)S.reg,
+                 (const unsigned int*)nullptr, (const T*)cur, oth, it == npass - 1 ? psix : (T*)nullptr, npar,
+                 hxyz, 1);
+        LAUNCHCHK(c);
+        std::swap(cur, oth);
+      }
+      S.denx = cur;
     }
   }
-  S.denx = cur;
-  // the field side needs denx (batch 1); H psi_X enters only the force
-  // gather after batch 3, so it stays off the point st waits for
+  // the field side needs denx (batch 1); H psi_X (denpart.f90:42-46) enters
+  // only the force gather after batch 3, so it stays off the point st waits for
   if (!c->dry && !c->one_stream) HIPCHK(c, hipEventRecord(c->ev_chain, c->st2));
-  S2LAUNCH((k_hpsix<T>), dim3(region_blocks()), dim3(kBlock), 0, g, S.reg, (const T*)psix,
-           compact<double, true>(c->d2x), compact<double, true>(c->d2y), compact<double, true>(c->d2z), c->h2o2mx,
-           (T*)S.hpsix, S.counts, margin);
-  LAUNCHCHK(c);
-  c->chain_pending = true;
-  c->side_pending = true;
+  {
+    HpsixTab<T> t{};
+    for (int i = 0; i < G.n; ++i) {
+      hetddft_ctx* m = G.m[i];
+      hetddft_ctx::ChainSet& S = m->cset[m->cs ^ 1];
+      t.reg[i] = S.reg;
+      t.psix[i] = (const T*)S.psix;
+      t.hpsix[i] = (T*)S.hpsix;
+      t.cnt[i] = S.counts;
+    }
+    S2LAUNCH((k_hpsix<T>), dim3(region_blocks(), G.n), dim3(kBlock), 0, g, t, compact<double, true>(c->d2x),
+             compact<double, true>(c->d2y), compact<double, true>(c->d2z), c->h2o2mx, margin);
+    LAUNCHCHK(c);
+  }
+  for (int i = 0; i < G.n; ++i) {
+    G.m[i]->chain_pending = true;
+    G.m[i]->side_pending = true;
+  }
   return HETDDFT_OK;
 }
 
@@ -862,121 +951,177 @@ int settle(hetddft_ctx* c) {
 // evaluation's wait for the particle chain (queued on st2 after the gather)
 // orders it after the gather.
 template <class T>
-int veval(hetddft_ctx* c, bool leave_side = false) {
+int veval(Group& G, bool leave_side = false) {
+  hetddft_ctx* c = G.m[0];
   const Grid& g = c->g;
-  auto F = [&](int id) { return (T*)c->fld[id]; };
+  auto F = [&](hetddft_ctx* m, int id) { return (T*)m->fld[id]; };
   const bool otc = c->p.core_otc != 0;
   if (!c->chain_pending) {
     RET(fork_side(c));
-    RET(issue_chain<T>(c));
+    RET(issue_chain<T>(G));
   }
-  BoxArgs b = box_args(c);
-  // ---- poten: derden (derden.f90:26-28)
+  // ---- poten: derden (derden.f90:26-28), every member in one launch
   {
-  ProfScope psg(c, HETDDFT_PROF_GRAD, 4.0 * (double)g.n * sizeof(T));
-  with_shape(c->tile_k[HETDDFT_TUNE_GRAD], c->pf_k[HETDDFT_TUNE_GRAD], [&](auto sh, auto pf) {
-    using SH = decltype(sh);
-    constexpr bool PF = decltype(pf)::value;
-    PLAUNCH(psg, (k_grad3_25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_GRAD), dim3(SH::TX, SH::TY), 0, g,
-            c->zc_k[HETDDFT_TUNE_GRAD], c->xcd,
-            (const T*)F(hetddft_ctx::F_DEN), compact<T, false>(c->d1x), compact<T, false>(c->d1y),
-            compact<T, false>(c->d1z), F(hetddft_ctx::F_DX),
-            F(hetddft_ctx::F_DY), F(hetddft_ctx::F_DZ));
-    return 0;
-  });
+    ProfScope psg(c, HETDDFT_PROF_GRAD, 4.0 * (double)g.n * sizeof(T));
+    with_shape(c->tile_k[HETDDFT_TUNE_GRAD], c->pf_k[HETDDFT_TUNE_GRAD], [&](auto sh, auto pf) {
+      using SH = decltype(sh);
+      constexpr bool PF = decltype(pf)::value;
+      GradTab<T> t{};
+      t.nzb = (int)grid25<SH>(c, HETDDFT_TUNE_GRAD).z;
+      for (int i = 0; i < G.n; ++i) {
+        hetddft_ctx* m = G.m[i];
+        t.f[i] = F(m, hetddft_ctx::F_DEN);
+        t.fx[i] = F(m, hetddft_ctx::F_DX);
+        t.fy[i] = F(m, hetddft_ctx::F_DY);
+        t.fz[i] = F(m, hetddft_ctx::F_DZ);
+      }
+      PLAUNCH(psg, (k_grad3_25<T, SH, PF>), grid25m<SH>(c, HETDDFT_TUNE_GRAD, G.n), dim3(SH::TX, SH::TY), 0, g,
+              c->zc_k[HETDDFT_TUNE_GRAD], c->xcd, t, compact<T, false>(c->d1x), compact<T, false>(c->d1y),
+              compact<T, false>(c->d1z));
+      return 0;
+    });
+    LAUNCHCHK(c);
   }
-  LAUNCHCHK(c);
   // the particle density chain enters here: its set becomes current
   if (!c->dry && !c->one_stream) HIPCHK(c, hipStreamWaitEvent(c->st, c->ev_chain, 0));
   RET(prof_mark(c));
-  c->cs ^= 1;
-  use_chain_set(c);
-  c->chain_pending = false;
-  c->side_pending = false;
-  T* cur = (T*)c->denx;
-  T* psix = F(hetddft_ctx::F_PSIX);
+  for (int i = 0; i < G.n; ++i) {
+    hetddft_ctx* m = G.m[i];
+    m->cs ^= 1;
+    use_chain_set(m);
+    m->chain_pending = false;
+    m->side_pending = false;
+  }
+  // Each batch: the members' inputs and jobs in one launch per pass, as
+  // many members per launch as the argument arrays hold (kMaxBatch inputs and
+  // jobs: larger kernel arguments cost every launch several microseconds).
+  auto run_batch = [&](auto&& build) -> int {
+    XfArgs<T> xa{};
+    int nin = 0;
+    std::vector<ConvJob<T>> jobs;
+    std::vector<void*> sin, sout;
+    for (int i = 0; i < G.n; ++i) {
+      hetddft_ctx* m = G.m[i];
+      XfArgs<T> xm{};
+      std::vector<ConvJob<T>> jm;
+      const int nm = build(m, xm, jm);
+      if (nin > 0 && (nin + nm > kMaxBatch || (int)(jobs.size() + jm.size()) > kMaxBatch)) {
+        RET(conv_batch_in<T>(c, c->fs, xa, nin, jobs, sin.data(), sout.data()));
+        xa = XfArgs<T>{};
+        nin = 0;
+        jobs.clear();
+        sin.clear();
+        sout.clear();
+      }
+      if (nm > kMaxBatch || (int)jm.size() > kMaxBatch || !batch_buffers<T>(m->fs, nm, jm, sin, sout))
+        return fail(c, HETDDFT_E_ARG, "potential batch does not fit");
+      for (int k = 0; k < nm; ++k) {
+        xa.op[nin + k] = xm.op[k];
+        for (int q = 0; q < 6; ++q) xa.in[nin + k][q] = xm.in[k][q];
+        xa.c[nin + k][0] = xm.c[k][0];
+        xa.c[nin + k][1] = xm.c[k][1];
+      }
+      for (ConvJob<T> j : jm) {
+        j.input += nin;
+        jobs.push_back(j);
+      }
+      nin += nm;
+    }
+    return conv_batch_in<T>(c, c->fs, xa, nin, jobs, sin.data(), sout.data());
+  };
   // batch 1: denx*Vq -> uimp (potenimp.f90:15-17); den*{wcgk, fvlj4, kalfs}
   //          -> dencg, delj4, falfs = 1 - denalf/den0s (poten.f90:27-50,94-99;
   //          term_alfa.f90:28-30)
-  {
-    XfArgs<T> xa{};
+  RET(run_batch([&](hetddft_ctx* m, XfArgs<T>& xa, std::vector<ConvJob<T>>& jobs) {
     xa.op[0] = XF_COPY;
-    xa.in[0][0] = cur;
+    xa.in[0][0] = (T*)m->denx;
     xa.op[1] = XF_COPY;
-    xa.in[1][0] = F(hetddft_ctx::F_DEN);
-    std::vector<ConvJob<T>> jobs;
-    jobs.push_back({0, c->fs.kvq, 1.0, F(hetddft_ctx::F_UIMP)});
-    jobs.push_back({1, c->fs.kwcgk, 1.0, F(hetddft_ctx::F_DENCG)});
-    jobs.push_back({1, c->fs.kfvlj4, 1.0, F(hetddft_ctx::F_DELJ4)});
-    if (otc) jobs.push_back({1, c->fs.kkalfs, 1.0, F(hetddft_ctx::F_FALFS), XI_ONE_MINUS_DIV, c->p.den0s});
-    RET(conv_batch<T>(c, c->fs, xa, 2, jobs));
-  }
+    xa.in[1][0] = F(m, hetddft_ctx::F_DEN);
+    jobs.push_back({0, m->fs.kvq, 1.0, F(m, hetddft_ctx::F_UIMP)});
+    jobs.push_back({1, m->fs.kwcgk, 1.0, F(m, hetddft_ctx::F_DENCG)});
+    jobs.push_back({1, m->fs.kfvlj4, 1.0, F(m, hetddft_ctx::F_DELJ4)});
+    if (otc) jobs.push_back({1, m->fs.kkalfs, 1.0, F(m, hetddft_ctx::F_FALFS), XI_ONE_MINUS_DIV, m->p.den0s});
+    return 2;
+  }));
   // batch 2: correlation term (poten.f90:62-74); alpha_s integrals
   //          int_c = G*(falfs d_c den) (term_alfa.f90:32-52)
-  {
-    XfArgs<T> xa{};
+  RET(run_batch([&](hetddft_ctx* m, XfArgs<T>& xa, std::vector<ConvJob<T>>& jobs) {
     xa.op[0] = XF_CORREL;
-    xa.in[0][0] = F(hetddft_ctx::F_DEN);
-    xa.in[0][1] = F(hetddft_ctx::F_DENCG);
-    xa.c[0][0] = c->p.cp4;
-    xa.c[0][1] = c->p.cpp4;
-    std::vector<ConvJob<T>> jobs;
-    jobs.push_back({0, c->fs.kwcgk, 1.0, F(hetddft_ctx::F_STO1)});
+    xa.in[0][0] = F(m, hetddft_ctx::F_DEN);
+    xa.in[0][1] = F(m, hetddft_ctx::F_DENCG);
+    xa.c[0][0] = m->p.cp4;
+    xa.c[0][1] = m->p.cpp4;
+    jobs.push_back({0, m->fs.kwcgk, 1.0, F(m, hetddft_ctx::F_STO1)});
     int nin = 1;
     if (otc) {
       const int d[3] = {hetddft_ctx::F_DX, hetddft_ctx::F_DY, hetddft_ctx::F_DZ};
       const int o[3] = {hetddft_ctx::F_IX, hetddft_ctx::F_IY, hetddft_ctx::F_IZ};
       for (int k = 0; k < 3; ++k) {
         xa.op[1 + k] = XF_MUL;
-        xa.in[1 + k][0] = F(hetddft_ctx::F_FALFS);
-        xa.in[1 + k][1] = F(d[k]);
-        jobs.push_back({1 + k, c->fs.kkalfs, 1.0, F(o[k])});
+        xa.in[1 + k][0] = F(m, hetddft_ctx::F_FALFS);
+        xa.in[1 + k][1] = F(m, d[k]);
+        jobs.push_back({1 + k, m->fs.kkalfs, 1.0, F(m, o[k])});
       }
       nin = 4;
     }
-    RET(conv_batch<T>(c, c->fs, xa, nin, jobs));
-  }
+    return nin;
+  }));
   // batch 3: grad(den).int -> ualphas convolution (term_alfa.f90:65-80);
   //          -Vq*FFT(d_c den) -> force convolutions (potenimp.f90:39-50)
-  {
-    XfArgs<T> xa{};
-    std::vector<ConvJob<T>> jobs;
+  RET(run_batch([&](hetddft_ctx* m, XfArgs<T>& xa, std::vector<ConvJob<T>>& jobs) {
     int nin = 0;
     if (otc) {
       xa.op[0] = XF_GRAD_DOT;
-      xa.in[0][0] = F(hetddft_ctx::F_DX);
-      xa.in[0][1] = F(hetddft_ctx::F_DY);
-      xa.in[0][2] = F(hetddft_ctx::F_DZ);
-      xa.in[0][3] = F(hetddft_ctx::F_IX);
-      xa.in[0][4] = F(hetddft_ctx::F_IY);
-      xa.in[0][5] = F(hetddft_ctx::F_IZ);
-      jobs.push_back({0, c->fs.kkalfs, 1.0, F(hetddft_ctx::F_UACONV)});
+      xa.in[0][0] = F(m, hetddft_ctx::F_DX);
+      xa.in[0][1] = F(m, hetddft_ctx::F_DY);
+      xa.in[0][2] = F(m, hetddft_ctx::F_DZ);
+      xa.in[0][3] = F(m, hetddft_ctx::F_IX);
+      xa.in[0][4] = F(m, hetddft_ctx::F_IY);
+      xa.in[0][5] = F(m, hetddft_ctx::F_IZ);
+      jobs.push_back({0, m->fs.kkalfs, 1.0, F(m, hetddft_ctx::F_UACONV)});
       nin = 1;
     }
     const int d[3] = {hetddft_ctx::F_DX, hetddft_ctx::F_DY, hetddft_ctx::F_DZ};
     const int o[3] = {hetddft_ctx::F_FCX, hetddft_ctx::F_FCY, hetddft_ctx::F_FCZ};
     for (int k = 0; k < 3; ++k) {
       xa.op[nin] = XF_COPY;
-      xa.in[nin][0] = F(d[k]);
-      jobs.push_back({nin, c->fs.kvq, -1.0, F(o[k])});
+      xa.in[nin][0] = F(m, d[k]);
+      jobs.push_back({nin, m->fs.kvq, -1.0, F(m, o[k])});
       ++nin;
     }
-    RET(conv_batch<T>(c, c->fs, xa, nin, jobs));
-  }
+    return nin;
+  }));
   // ---- forceimp gather (potenimp.f90:52-71), on the side stream beside
   // the pot4 assembly below: the two read disjoint fields and write aimp and
-  // pot4 respectively; st waits for the gather after the assembly.
+  // pot4 respectively
   if (c->p.n_par > 0) {
     RET(fork_side(c));
-    S2LAUNCH((k_force_q<T>), dim3(region_blocks()), dim3(kBlock), 0, gather_args(c), (const int*)c->reg,
-             c->rimp, (const T*)cur, (const T*)psix, (const T*)F(hetddft_ctx::F_HPSIX), c->fgather);
+    ForceQTab<T> tq{};
+    BlendTab<T> tb{};
+    for (int i = 0; i < G.n; ++i) {
+      hetddft_ctx* m = G.m[i];
+      tq.reg[i] = m->reg;
+      tq.rp[i] = m->rimp;
+      tq.denx[i] = (const T*)m->denx;
+      tq.psix[i] = F(m, hetddft_ctx::F_PSIX);
+      tq.hpsix[i] = F(m, hetddft_ctx::F_HPSIX);
+      tq.out[i] = m->fgather;
+      tb.reg[i] = m->reg;
+      tb.rp[i] = m->rimp;
+      tb.fcx[i] = F(m, hetddft_ctx::F_FCX);
+      tb.fcy[i] = F(m, hetddft_ctx::F_FCY);
+      tb.fcz[i] = F(m, hetddft_ctx::F_FCZ);
+      tb.fq[i] = m->fgather;
+      tb.acc[i] = m->aimp;
+    }
+    S2LAUNCH((k_force_q<T>), dim3(region_blocks(), G.n), dim3(kBlock), 0, gather_args(c), tq);
     LAUNCHCHK(c);
-    S2LAUNCH((k_force_blend<T>), dim3(blocks_for(3LL * c->p.n_par)), dim3(kBlock), 0, gather_args(c),
-             (const int*)c->reg, c->rimp, (const T*)F(hetddft_ctx::F_FCX), (const T*)F(hetddft_ctx::F_FCY),
-             (const T*)F(hetddft_ctx::F_FCZ), (const double*)c->fgather, c->aimp);
+    S2LAUNCH((k_force_blend<T>), dim3(blocks_for(3LL * c->p.n_par), G.n), dim3(kBlock), 0, gather_args(c), tb);
     LAUNCHCHK(c);
   }
-  // pot4 assembly with the alpha_s divergence (term_alfa.f90:85-113, poten.f90:80-124)
+  // pot4 assembly with the alpha_s divergence (term_alfa.f90:85-113,
+  // poten.f90:80-124), every member in one launch (the members share the
+  // functional's constants)
   {
     Pot25T<T> p5{};
     p5.g = g;
@@ -998,24 +1143,42 @@ int veval(hetddft_ctx* c, bool leave_side = false) {
     with_shape(c->tile_k[HETDDFT_TUNE_POT], c->pf_k[HETDDFT_TUNE_POT], [&](auto sh, auto pf) {
       using SH = decltype(sh);
       constexpr bool PF = decltype(pf)::value;
-      PLAUNCH(ps, (k_pot_final25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_POT), dim3(SH::TX, SH::TY), 0, p5,
-              (const T*)F(hetddft_ctx::F_DEN), (const T*)F(hetddft_ctx::F_DENCG),
-              (const T*)F(hetddft_ctx::F_DELJ4), (const T*)F(hetddft_ctx::F_STO1),
-              (const T*)F(hetddft_ctx::F_UIMP), (const T*)F(hetddft_ctx::F_FALFS),
-              (const T*)F(hetddft_ctx::F_IX), (const T*)F(hetddft_ctx::F_IY),
-              (const T*)F(hetddft_ctx::F_IZ), (const T*)F(hetddft_ctx::F_UACONV), F(hetddft_ctx::F_POT4));
+      PotTab<T> t{};
+      t.nzb = (int)grid25<SH>(c, HETDDFT_TUNE_POT).z;
+      for (int i = 0; i < G.n; ++i) {
+        hetddft_ctx* m = G.m[i];
+        t.den[i] = F(m, hetddft_ctx::F_DEN);
+        t.dencg[i] = F(m, hetddft_ctx::F_DENCG);
+        t.delj4[i] = F(m, hetddft_ctx::F_DELJ4);
+        t.sto1c[i] = F(m, hetddft_ctx::F_STO1);
+        t.uimp[i] = F(m, hetddft_ctx::F_UIMP);
+        t.falfs[i] = F(m, hetddft_ctx::F_FALFS);
+        t.intx[i] = F(m, hetddft_ctx::F_IX);
+        t.inty[i] = F(m, hetddft_ctx::F_IY);
+        t.intz[i] = F(m, hetddft_ctx::F_IZ);
+        t.uaconv[i] = F(m, hetddft_ctx::F_UACONV);
+        t.pot4[i] = F(m, hetddft_ctx::F_POT4);
+      }
+      PLAUNCH(ps, (k_pot_final25<T, SH, PF>), grid25m<SH>(c, HETDDFT_TUNE_POT, G.n), dim3(SH::TX, SH::TY), 0, p5,
+              t);
       return 0;
     });
+    LAUNCHCHK(c);
   }
-  LAUNCHCHK(c);
   if (leave_side) {
-    c->side_pending = true;
+    for (int i = 0; i < G.n; ++i) G.m[i]->side_pending = true;
   } else {
     // H psi_X (and the gather, with particles) ran on the side stream
     RET(join_side(c, c->ev_force));
     RET(prof_mark(c));
   }
   return HETDDFT_OK;
+}
+
+template <class T>
+int veval(hetddft_ctx* c, bool leave_side = false) {
+  Group G = solo(c);
+  return veval<T>(G, leave_side);
 }
 
 int run_veval(hetddft_ctx* c) { return c->mixed ? veval<float>(c) : veval<double>(c); }
@@ -1040,189 +1203,287 @@ Ham25 ham_args(const hetddft_ctx* c, int k) {
 }
 
 template <class T>
-int step_rk_t(hetddft_ctx* c, double dt) {
+int step_rk_t(Group& G, double dt) {
+  hetddft_ctx* c = G.m[0];
   const double s2 = std::sqrt(2.0);
   const double arun[4] = {0.5, 1.0 - 1.0 / s2, 1.0 + 1.0 / s2, 1.0 / 6.0};
   const double brun[4] = {2.0, 1.0, 1.0, 2.0};
   const double crun[4] = {0.5, 1.0 - 1.0 / s2, 1.0 + 1.0 / s2, 0.5};
-  const Grid& g = c->g;
-  const Ham25 ha = ham_args(c, HETDDFT_TUNE_RK);
   const int n3 = 3 * c->p.n_par;
+  const Ham25 ha = ham_args(c, HETDDFT_TUNE_RK);
   for (int j = 0; j < 4; ++j) {
-    c64* out;
-    c64* h1o = nullptr;
-    double *r1o = nullptr, *v1o = nullptr, *a1o = nullptr;
-    if (j == 0) {
-      // psiold(3)=psiold(2); psiold(2)=psiold(1); psiold(1)=psi; hpsiold(2)=hpsiold(1)
-      c64* freed = c->old[2];
-      c->old[2] = c->old[1];
-      c->old[1] = c->old[0];
-      c->old[0] = c->psi;  // the current psi becomes psiold(1); new psi goes to `freed`
-      out = freed;
-      c64* hf = c->hold[1];
-      c->hold[1] = c->hold[0];
-      c->hold[0] = hf;
-      h1o = hf;
-      double* rf = c->rold[2];
-      c->rold[2] = c->rold[1];
-      c->rold[1] = c->rold[0];
-      c->rold[0] = rf;
-      r1o = rf;
-      double* vf = c->vold[2];
-      c->vold[2] = c->vold[1];
-      c->vold[1] = c->vold[0];
-      c->vold[0] = vf;
-      v1o = vf;
-      double* af = c->aold[1];
-      c->aold[1] = c->aold[0];
-      c->aold[0] = af;
-      a1o = af;
-    } else {
-      out = c->spare;
+    c64* outs[kMaxMembers] = {};
+    c64* h1os[kMaxMembers] = {};
+    double* r1os[kMaxMembers] = {};
+    double* v1os[kMaxMembers] = {};
+    double* a1os[kMaxMembers] = {};
+    for (int i = 0; i < G.n; ++i) {
+      hetddft_ctx* m = G.m[i];
+      if (j == 0) {
+        // psiold(3)=psiold(2); psiold(2)=psiold(1); psiold(1)=psi; hpsiold(2)=hpsiold(1)
+        c64* freed = m->old[2];
+        m->old[2] = m->old[1];
+        m->old[1] = m->old[0];
+        m->old[0] = m->psi;  // the current psi becomes psiold(1); new psi goes to `freed`
+        outs[i] = freed;
+        c64* hf = m->hold[1];
+        m->hold[1] = m->hold[0];
+        m->hold[0] = hf;
+        h1os[i] = hf;
+        double* rf = m->rold[2];
+        m->rold[2] = m->rold[1];
+        m->rold[1] = m->rold[0];
+        m->rold[0] = rf;
+        r1os[i] = rf;
+        double* vf = m->vold[2];
+        m->vold[2] = m->vold[1];
+        m->vold[1] = m->vold[0];
+        m->vold[0] = vf;
+        v1os[i] = vf;
+        double* af = m->aold[1];
+        m->aold[1] = m->aold[0];
+        m->aold[0] = af;
+        a1os[i] = af;
+      } else {
+        outs[i] = m->spare;
+      }
     }
     with_shape(c->tile_k[HETDDFT_TUNE_RK], c->pf_k[HETDDFT_TUNE_RK], [&](auto sh, auto pf) {
       using SH = decltype(sh);
       constexpr bool PF = decltype(pf)::value;
-      SLAUNCH((k_rk_stage25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_RK), dim3(SH::TX, SH::TY), 0, ha,
-              (const c64*)c->psi, (const T*)c->fld[hetddft_ctx::F_POT4], c->q, out,
-              (T*)c->fld[hetddft_ctx::F_DEN], h1o, arun[j], brun[j], crun[j], dt);
+      RkTab<T> t{};
+      t.nzb = (int)grid25<SH>(c, HETDDFT_TUNE_RK).z;
+      for (int i = 0; i < G.n; ++i) {
+        hetddft_ctx* m = G.m[i];
+        t.timec[i] = m->timec;
+        t.psi[i] = m->psi;
+        t.pot4[i] = (const T*)m->fld[hetddft_ctx::F_POT4];
+        t.q[i] = m->q;
+        t.out[i] = outs[i];
+        t.den[i] = (T*)m->fld[hetddft_ctx::F_DEN];
+        t.h1[i] = h1os[i];
+      }
+      SLAUNCH((k_rk_stage25<T, SH, PF>), grid25m<SH>(c, HETDDFT_TUNE_RK, G.n), dim3(SH::TX, SH::TY), 0, ha, t,
+              arun[j], brun[j], crun[j], dt);
       return 0;
     });
     LAUNCHCHK(c);
-    if (j == 0) {
-      c->psi = out;
-    } else {
-      c->spare = c->psi;
-      c->psi = out;
+    PartRkTab pt{};
+    for (int i = 0; i < G.n; ++i) {
+      hetddft_ctx* m = G.m[i];
+      if (j == 0) {
+        m->psi = outs[i];
+      } else {
+        m->spare = m->psi;
+        m->psi = outs[i];
+      }
+      pt.r[i] = m->rimp;
+      pt.v[i] = m->vimp;
+      pt.acc[i] = m->aimp;
+      pt.qr[i] = m->qr;
+      pt.qv[i] = m->qv;
+      pt.r1[i] = r1os[i];
+      pt.v1[i] = v1os[i];
+      pt.a1[i] = a1os[i];
     }
     if (n3 > 0) {
-      SLAUNCH(k_part_rk, dim3(blocks_for(n3)), dim3(kBlock), 0, n3, c->rimp, c->vimp,
-                         (const double*)c->aimp, c->qr, c->qv, r1o, v1o, a1o, arun[j], brun[j], crun[j], dt);
+      SLAUNCH(k_part_rk, dim3(blocks_for(n3), G.n), dim3(kBlock), 0, n3, pt, arun[j], brun[j], crun[j], dt);
       LAUNCHCHK(c);
     }
-    if (j <= 2) RET(veval<T>(c));
+    if (j <= 2) RET(veval<T>(G));
   }
-  for (int k = 0; k < 3; ++k) c->ioldp[k] = c->ioldr[k] = c->ioldv[k] = k;
-  c->ioldh[0] = c->iolda[0] = 0;
-  c->ioldh[1] = c->iolda[1] = 1;
+  for (int i = 0; i < G.n; ++i) {
+    hetddft_ctx* m = G.m[i];
+    for (int k = 0; k < 3; ++k) m->ioldp[k] = m->ioldr[k] = m->ioldv[k] = k;
+    m->ioldh[0] = m->iolda[0] = 0;
+    m->ioldh[1] = m->iolda[1] = 1;
+  }
   return HETDDFT_OK;
 }
 
 template <class T>
-int step_pc_t(hetddft_ctx* c, double dt) {
+int step_rk_t(hetddft_ctx* c, double dt) {
+  Group G = solo(c);
+  return step_rk_t<T>(G, dt);
+}
+
+template <class T>
+int step_pc_t(Group& G, double dt) {
+  hetddft_ctx* c = G.m[0];
   const double c112 = 112.0 / 121.0, c9 = 9.0 / 121.0, c4o3 = 4.0 / 3.0;
-  c->err_pending = false;  // the previous step's estimates were not asked for
   const Grid& g = c->g;
-  const Ham25 hp = ham_args(c, HETDDFT_TUNE_PCP);
-  const Ham25 hc = ham_args(c, HETDDFT_TUNE_PCC);
-  int* P = c->ioldp;
-  int* H = c->ioldh;
-  int* R = c->ioldr;
-  int* V = c->ioldv;
-  int* A = c->iolda;
   const int n3 = 3 * c->p.n_par;
   // errHe partial sums: one per workgroup of the corrector
   const unsigned nb = (unsigned)blocks25(c, HETDDFT_TUNE_PCC);
-  // Particle predictor (steppc.f90:64-87), then the density chain of the
-  // predicted positions on the side stream, beside the psi predictor.
   // In the lean layout the particle predictor, the chain, the force gather
   // and the particle corrector all run on st2, in that order, and st2 waits
   // for st only before each gather (the force fields of batch 3).
   const bool lean = c->lean && !c->one_stream;
-  if (n3 > 0) {
-    if (lean)
-      S2LAUNCH(k_part_predict, dim3(blocks_for(n3)), dim3(kBlock), 0, n3, c->rimp,
-               (const double*)c->vimp, (const double*)c->aimp, c->rold[R[2]], (const double*)c->vold[V[0]],
-               (const double*)c->vold[V[1]], c->vold[V[2]], (const double*)c->aold[A[0]], c->aold[A[1]],
-               c->pcr, c->pcv, c4o3 * dt, c112);
-    else
-      SLAUNCH(k_part_predict, dim3(blocks_for(n3)), dim3(kBlock), 0, n3, c->rimp,
-              (const double*)c->vimp, (const double*)c->aimp, c->rold[R[2]], (const double*)c->vold[V[0]],
-              (const double*)c->vold[V[1]], c->vold[V[2]], (const double*)c->aold[A[0]], c->aold[A[1]],
-              c->pcr, c->pcv, c4o3 * dt, c112);
-    LAUNCHCHK(c);
-  }
-  std::swap(A[0], A[1]);
-  if (!lean) RET(fork_side(c));
-  RET(issue_chain<T>(c));
-  // psi predictor + modifier (steppc.f90:36-62)
-  c64* x = c->psi;
-  c64* y = c->old[P[2]];
+  // Particle predictor (steppc.f90:64-87), then the density chain of the
+  // predicted positions on the side stream, beside the psi predictor.
   {
-  ProfScope pspred(c, HETDDFT_PROF_PCP, (double)g.n * (16.0 * 8 + 2.0 * sizeof(T)));
-  with_shape(c->tile_k[HETDDFT_TUNE_PCP], c->pf_k[HETDDFT_TUNE_PCP], [&](auto sh, auto pf) {
-    using SH = decltype(sh);
-    constexpr bool PF = decltype(pf)::value;
-    PLAUNCH(pspred, (k_pc_predict25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_PCP), dim3(SH::TX, SH::TY), 0, hp,
-            (const c64*)x,
-            (const T*)c->fld[hetddft_ctx::F_POT4], y, (const c64*)c->hold[H[0]], c->hold[H[1]], c->pc,
-            (T*)c->fld[hetddft_ctx::F_DEN], c4o3 * dt, c112);
-    return 0;
-  });
+    PartPredTab pt{};
+    for (int i = 0; i < G.n; ++i) {
+      hetddft_ctx* m = G.m[i];
+      m->err_pending = false;  // the previous step's estimates were not asked for
+      const int* R = m->ioldr;
+      const int* V = m->ioldv;
+      int* A = m->iolda;
+      pt.r[i] = m->rimp;
+      pt.v[i] = m->vimp;
+      pt.acc[i] = m->aimp;
+      pt.r3[i] = m->rold[R[2]];
+      pt.v1[i] = m->vold[V[0]];
+      pt.v2[i] = m->vold[V[1]];
+      pt.v3[i] = m->vold[V[2]];
+      pt.a1[i] = m->aold[A[0]];
+      pt.a2[i] = m->aold[A[1]];
+      pt.pcr[i] = m->pcr;
+      pt.pcv[i] = m->pcv;
+      std::swap(A[0], A[1]);
+    }
+    if (n3 > 0) {
+      if (lean)
+        S2LAUNCH(k_part_predict, dim3(blocks_for(n3), G.n), dim3(kBlock), 0, n3, pt, c4o3 * dt, c112);
+      else
+        SLAUNCH(k_part_predict, dim3(blocks_for(n3), G.n), dim3(kBlock), 0, n3, pt, c4o3 * dt, c112);
+      LAUNCHCHK(c);
+    }
+  }
+  if (!lean) RET(fork_side(c));
+  RET(issue_chain<T>(G));
+  // psi predictor + modifier (steppc.f90:36-62), every member in one launch
+  {
+    const Ham25 hp = ham_args(c, HETDDFT_TUNE_PCP);
+    ProfScope pspred(c, HETDDFT_PROF_PCP, (double)g.n * (16.0 * 8 + 2.0 * sizeof(T)));
+    with_shape(c->tile_k[HETDDFT_TUNE_PCP], c->pf_k[HETDDFT_TUNE_PCP], [&](auto sh, auto pf) {
+      using SH = decltype(sh);
+      constexpr bool PF = decltype(pf)::value;
+      PcpTab<T> t{};
+      t.nzb = (int)grid25<SH>(c, HETDDFT_TUNE_PCP).z;
+      for (int i = 0; i < G.n; ++i) {
+        hetddft_ctx* m = G.m[i];
+        const int* P = m->ioldp;
+        const int* H = m->ioldh;
+        t.timec[i] = m->timec;
+        t.psi[i] = m->psi;
+        t.pot4[i] = (const T*)m->fld[hetddft_ctx::F_POT4];
+        t.old3[i] = m->old[P[2]];
+        t.h1[i] = m->hold[H[0]];
+        t.h2[i] = m->hold[H[1]];
+        t.pc[i] = m->pc;
+        t.den[i] = (T*)m->fld[hetddft_ctx::F_DEN];
+      }
+      PLAUNCH(pspred, (k_pc_predict25<T, SH, PF>), grid25m<SH>(c, HETDDFT_TUNE_PCP, G.n), dim3(SH::TX, SH::TY),
+              0, hp, t, c4o3 * dt, c112);
+      return 0;
+    });
   }
   LAUNCHCHK(c);
-  c->old[P[2]] = x;
-  c->psi = y;
-  std::swap(H[0], H[1]);
-  RET(veval<T>(c, lean));
+  for (int i = 0; i < G.n; ++i) {
+    hetddft_ctx* m = G.m[i];
+    int* P = m->ioldp;
+    int* H = m->ioldh;
+    c64* x = m->psi;
+    m->psi = m->old[P[2]];
+    m->old[P[2]] = x;
+    std::swap(H[0], H[1]);
+  }
+  RET(veval<T>(G, lean));
   // Particle corrector (steppc.f90:122-151), then the density chain of the
   // corrected positions on the side stream beside the psi corrector; the
   // driver's potential evaluation (DFT4He3d.f90:861) consumes that chain.
-  if (n3 > 0) {
-    const unsigned pb = blocks_for(n3);
-    double* pp = c->part + nb;  // after the psi partials (disjoint from the psi corrector's)
-    if (lean)
-      S2LAUNCH(k_part_correct, dim3(pb), dim3(kBlock), 0, n3, c->rimp, c->vimp,
-               (const double*)c->aimp, (const double*)c->rold[R[2]], (const double*)c->rold[R[1]],
-               (const double*)c->vold[V[0]], (const double*)c->vold[V[1]], c->vold[V[2]],
-               (const double*)c->aold[A[0]], (const double*)c->aold[A[1]], c->pcr, c->pcv, pp, 3.0 * dt, c9);
-    else
-      SLAUNCH(k_part_correct, dim3(pb), dim3(kBlock), 0, n3, c->rimp, c->vimp,
-              (const double*)c->aimp, (const double*)c->rold[R[2]], (const double*)c->rold[R[1]],
-              (const double*)c->vold[V[0]], (const double*)c->vold[V[1]], c->vold[V[2]],
-              (const double*)c->aold[A[0]], (const double*)c->aold[A[1]], c->pcr, c->pcv, pp, 3.0 * dt, c9);
-    LAUNCHCHK(c);
-  }
   {
-    int t = R[2];
-    R[2] = R[1];
-    R[1] = R[0];
-    R[0] = t;
-    t = V[2];
-    V[2] = V[1];
-    V[1] = V[0];
-    V[0] = t;
+    PartCorrTab pt{};
+    for (int i = 0; i < G.n; ++i) {
+      hetddft_ctx* m = G.m[i];
+      int* R = m->ioldr;
+      int* V = m->ioldv;
+      const int* A = m->iolda;
+      pt.r[i] = m->rimp;
+      pt.v[i] = m->vimp;
+      pt.acc[i] = m->aimp;
+      pt.r3[i] = m->rold[R[2]];
+      pt.r2[i] = m->rold[R[1]];
+      pt.v1[i] = m->vold[V[0]];
+      pt.v2[i] = m->vold[V[1]];
+      pt.v3[i] = m->vold[V[2]];
+      pt.a1[i] = m->aold[A[0]];
+      pt.a2[i] = m->aold[A[1]];
+      pt.pcr[i] = m->pcr;
+      pt.pcv[i] = m->pcv;
+      pt.part[i] = m->part + nb;  // after the psi partials (disjoint from the psi corrector's)
+      int t = R[2];
+      R[2] = R[1];
+      R[1] = R[0];
+      R[0] = t;
+      t = V[2];
+      V[2] = V[1];
+      V[1] = V[0];
+      V[0] = t;
+    }
+    if (n3 > 0) {
+      const unsigned pb = blocks_for(n3);
+      if (lean)
+        S2LAUNCH(k_part_correct, dim3(pb, G.n), dim3(kBlock), 0, n3, pt, 3.0 * dt, c9);
+      else
+        SLAUNCH(k_part_correct, dim3(pb, G.n), dim3(kBlock), 0, n3, pt, 3.0 * dt, c9);
+      LAUNCHCHK(c);
+    }
   }
   if (!lean) RET(fork_side(c));
-  RET(issue_chain<T>(c));
-  // psi corrector (steppc.f90:97-120); the error sums stay on the device
-  // until fetch_errors asks for them
+  RET(issue_chain<T>(G));
+  // psi corrector (steppc.f90:97-120), every member in one launch; the
+  // error sums stay on the device until fetch_errors asks for them
   {
-  ProfScope pscorr(c, HETDDFT_PROF_PCC, (double)g.n * (16.0 * 10 + 2.0 * sizeof(T)));
-  with_shape(c->tile_k[HETDDFT_TUNE_PCC], c->pf_k[HETDDFT_TUNE_PCC], [&](auto sh, auto pf) {
-    using SH = decltype(sh);
-    constexpr bool PF = decltype(pf)::value;
-    PLAUNCH(pscorr, (k_pc_correct25<T, SH, PF>), grid25<SH>(c, HETDDFT_TUNE_PCC), dim3(SH::TX, SH::TY), 0, hc,
-            (const c64*)c->psi,
-            (const T*)c->fld[hetddft_ctx::F_POT4], (const c64*)c->old[P[2]], (const c64*)c->old[P[1]],
-            (const c64*)c->hold[H[0]], (const c64*)c->hold[H[1]], c->pc, c->spare,
-            (T*)c->fld[hetddft_ctx::F_DEN], c->part, 3.0 * dt, c9);
-    return 0;
-  });
+    const Ham25 hc = ham_args(c, HETDDFT_TUNE_PCC);
+    ProfScope pscorr(c, HETDDFT_PROF_PCC, (double)g.n * (16.0 * 10 + 2.0 * sizeof(T)));
+    with_shape(c->tile_k[HETDDFT_TUNE_PCC], c->pf_k[HETDDFT_TUNE_PCC], [&](auto sh, auto pf) {
+      using SH = decltype(sh);
+      constexpr bool PF = decltype(pf)::value;
+      PccTab<T> t{};
+      t.nzb = (int)grid25<SH>(c, HETDDFT_TUNE_PCC).z;
+      for (int i = 0; i < G.n; ++i) {
+        hetddft_ctx* m = G.m[i];
+        const int* P = m->ioldp;
+        const int* H = m->ioldh;
+        t.timec[i] = m->timec;
+        t.psi[i] = m->psi;
+        t.pot4[i] = (const T*)m->fld[hetddft_ctx::F_POT4];
+        t.o3[i] = m->old[P[2]];
+        t.o2[i] = m->old[P[1]];
+        t.h1[i] = m->hold[H[0]];
+        t.h2[i] = m->hold[H[1]];
+        t.pc[i] = m->pc;
+        t.out[i] = m->spare;
+        t.den[i] = (T*)m->fld[hetddft_ctx::F_DEN];
+        t.part[i] = m->part;
+      }
+      PLAUNCH(pscorr, (k_pc_correct25<T, SH, PF>), grid25m<SH>(c, HETDDFT_TUNE_PCC, G.n), dim3(SH::TX, SH::TY),
+              0, hc, t, 3.0 * dt, c9);
+      return 0;
+    });
   }
   LAUNCHCHK(c);
-  std::swap(c->psi, c->spare);
-  c->err_pending = true;
-  c->err_nb = (int)nb;
-  c->err_pb = n3 > 0 ? (int)blocks_for(n3) : 0;
-  LAUNCHCHK(c);
-  {
+  for (int i = 0; i < G.n; ++i) {
+    hetddft_ctx* m = G.m[i];
+    int* P = m->ioldp;
+    std::swap(m->psi, m->spare);
+    m->err_pending = true;
+    m->err_nb = (int)nb;
+    m->err_pb = n3 > 0 ? (int)blocks_for(n3) : 0;
     const int t = P[2];
     P[2] = P[1];
     P[1] = P[0];
     P[0] = t;
   }
   return HETDDFT_OK;
+}
+
+template <class T>
+int step_pc_t(hetddft_ctx* c, double dt) {
+  Group G = solo(c);
+  return step_pc_t<T>(G, dt);
 }
 
 // Reduces the last PC step's error partials into res (see err_pending).
@@ -1509,9 +1770,15 @@ void clear_graphs(hetddft_ctx* c) {
 }
 
 template <class T>
+int pc_iteration(Group& G, double dt) {
+  hetddft_ctx* c = G.m[0];
+  RET(step_pc_t<T>(G, dt));
+  return veval<T>(G, c->lean && !c->one_stream);
+}
+template <class T>
 int pc_iteration(hetddft_ctx* c, double dt) {
-  RET(step_pc_t<T>(c, dt));
-  return veval<T>(c, c->lean && !c->one_stream);
+  Group G = solo(c);
+  return pc_iteration<T>(G, dt);
 }
 
 // One PC step and the driver's potential evaluation.  With graphs on
@@ -1725,9 +1992,61 @@ static std::string tune_key(const hetddft_ctx* c) {
 // ---- ensemble executor -------------------------------------------------
 struct hetddft_batch {
   std::vector<hetddft_ctx*> ctx;
-  std::vector<std::vector<int>> part;  // instances of each host thread
+  // instances that can advance in shared launches (same grid, parameters and
+  // launch shapes), at most kMaxMembers each
+  std::vector<std::vector<int>> groups;
+  std::vector<std::vector<int>> part;  // groups of each host thread
   std::string err;
 };
+
+namespace {
+
+// Contexts with equal keys hold the same grid, parameters and launch shapes,
+// so one step's launches can carry all of them.
+std::string group_key(const hetddft_ctx* c) {
+  std::string k(reinterpret_cast<const char*>(&c->p), sizeof c->p);
+  int v[4 * HETDDFT_TUNE_NSTENCIL + 8];
+  int n = 0;
+  for (int i = 0; i < HETDDFT_TUNE_NSTENCIL; ++i) {
+    v[n++] = c->tile_k[i];
+    v[n++] = c->zc_k[i];
+    v[n++] = c->pf_k[i];
+  }
+  v[n++] = c->zp_per_cu;
+  v[n++] = c->mixed ? fft_z_tile_of<float>(const_cast<hetddft_ctx*>(c)) : fft_z_tile_of<double>(const_cast<hetddft_ctx*>(c));
+  v[n++] = c->xcd;
+  v[n++] = c->iron_fused ? 1 : 0;
+  v[n++] = c->lean ? 1 : 0;
+  v[n++] = c->graph_on ? 1 : 0;
+  k.append(reinterpret_cast<const char*>(v), n * sizeof(int));
+  return k;
+}
+
+// Members at the same step (step counter, whether a chain is queued) and
+// with the same step size advance as one group; each keeps its own chain
+// set and slot order.
+bool same_state(const hetddft_ctx* a, const hetddft_ctx* b) {
+  return a->started && b->started && a->iter_in_run == b->iter_in_run && a->chain_pending == b->chain_pending &&
+         !a->prof_on && !b->prof_on;
+}
+
+// One iteration of the reference loop (DFT4He3d.f90:849-863) for every
+// member, with shared launches; as hetddft_iterate with err = NULL.
+int iterate_group(Group& G, double dt) {
+  hetddft_ctx* c = G.m[0];
+  if (c->iter_in_run < 3) {
+    RET(settle(c));
+    for (int i = 0; i < G.n; ++i) G.m[i]->chain_pending = false;
+    RET(c->mixed ? step_rk_t<float>(G, dt) : step_rk_t<double>(G, dt));
+    RET(c->mixed ? veval<float>(G) : veval<double>(G));
+  } else {
+    RET(c->mixed ? pc_iteration<float>(G, dt) : pc_iteration<double>(G, dt));
+  }
+  for (int i = 0; i < G.n; ++i) G.m[i]->iter_in_run++;
+  return HETDDFT_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -2817,20 +3136,40 @@ int hetddft_batch_create(hetddft_ctx* const* ctxs, int n, int nthreads, hetddft_
   if (!ctxs || n <= 0) return HETDDFT_E_ARG;
   for (int i = 0; i < n; ++i)
     if (!ctxs[i]) return HETDDFT_E_ARG;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i; ++j)
+      if (ctxs[i] == ctxs[j]) return HETDDFT_E_ARG;
   auto* b = new (std::nothrow) hetddft_batch();
   if (!b) return HETDDFT_E_NOMEM;
   b->ctx.assign(ctxs, ctxs + n);
-  const int nt = std::max(1, std::min(n, nthreads > 0 ? nthreads : 8));
-  // largest grids first, each to the thread with the fewest grid points so far
-  std::vector<int> order(n);
-  for (int i = 0; i < n; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return ctxs[a]->g.n > ctxs[c]->g.n; });
+  // groups of equal keys, at most kMaxMembers each, in instance order
+  std::vector<std::string> keys(n);
+  for (int i = 0; i < n; ++i) keys[i] = group_key(ctxs[i]);
+  std::vector<bool> used(n, false);
+  for (int i = 0; i < n; ++i) {
+    if (used[i]) continue;
+    std::vector<int> grp{i};
+    used[i] = true;
+    for (int j = i + 1; j < n && (int)grp.size() < kMaxMembers; ++j)
+      if (!used[j] && keys[j] == keys[i]) {
+        grp.push_back(j);
+        used[j] = true;
+      }
+    b->groups.push_back(grp);
+  }
+  const int ng = (int)b->groups.size();
+  const int nt = std::max(1, std::min(ng, nthreads > 0 ? nthreads : 8));
+  // largest groups first, each to the thread with the fewest grid points so far
+  std::vector<int> order(ng);
+  for (int i = 0; i < ng; ++i) order[i] = i;
+  auto points = [&](int gi) { return (long long)b->groups[gi].size() * ctxs[b->groups[gi][0]]->g.n; };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return points(a) > points(c); });
   b->part.assign(nt, {});
   std::vector<long long> load(nt, 0);
-  for (int i : order) {
+  for (int gi : order) {
     const int t = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    b->part[t].push_back(i);
-    load[t] += ctxs[i]->g.n;
+    b->part[t].push_back(gi);
+    load[t] += points(gi);
   }
   for (auto& v : b->part) std::sort(v.begin(), v.end());
   *out = b;
@@ -2852,22 +3191,68 @@ int hetddft_batch_iterate(hetddft_batch* b, const double* dt, int steps, double*
     }
     stop = true;
   };
+  // One group of the batch: shared launches on the lead's streams when the
+  // members are in the same state with the same step, else one instance at
+  // a time.
+  auto drive_group = [&](const std::vector<int>& idx) -> bool {
+    bool shared = idx.size() > 1;
+    for (size_t k = 1; k < idx.size() && shared; ++k)
+      shared = same_state(b->ctx[idx[0]], b->ctx[idx[k]]) && dt[idx[k]] == dt[idx[0]];
+    if (!shared) return false;
+    Group G;
+    for (int i : idx) G.m[G.n++] = b->ctx[i];
+    hetddft_ctx* lead = G.m[0];
+    // the members' own streams are idle before the lead's carry their data
+    for (int k = 0; k < G.n; ++k) {
+      hetddft_ctx* m = G.m[k];
+      int rc = settle(m);
+      if (rc == HETDDFT_OK && (hipStreamSynchronize(m->st) != hipSuccess || hipStreamSynchronize(m->st2) != hipSuccess))
+        rc = fail(m, HETDDFT_E_HIP, "hipStreamSynchronize failed");
+      if (rc != HETDDFT_OK) {
+        record(idx[k], rc);
+        return true;
+      }
+    }
+    // flags of the members other than the lead follow the lead's streams
+    for (int s = 0; s < steps && !stop; ++s) {
+      const int rc = iterate_group(G, dt[idx[0]]);
+      if (rc != HETDDFT_OK) {
+        record(idx[0], rc);
+        return true;
+      }
+    }
+    int rc = settle(lead);
+    if (rc == HETDDFT_OK && hipStreamSynchronize(lead->st) != hipSuccess)
+      rc = fail(lead, HETDDFT_E_HIP, "hipStreamSynchronize failed");
+    if (rc != HETDDFT_OK) {
+      record(idx[0], rc);
+      return true;
+    }
+    for (int k = 0; k < G.n; ++k) G.m[k]->side_pending = false;
+    return true;
+  };
   auto drive = [&](const std::vector<int>& mine) {
+    std::vector<int> singles;
+    for (int gi : mine)
+      if (!drive_group(b->groups[gi]))
+        for (int i : b->groups[gi]) singles.push_back(i);
     for (int s = 0; s < steps && !stop; ++s)
-      for (int i : mine) {
+      for (int i : singles) {
         const int rc = hetddft_iterate(b->ctx[i], dt[i], nullptr);
         if (rc != HETDDFT_OK) {
           record(i, rc);
           return;
         }
       }
-    for (int i : mine) {
-      const int rc = hetddft_sync(b->ctx[i]);
-      if (rc != HETDDFT_OK) {
-        record(i, rc);
-        return;
+    // every instance's last estimates are reduced and checked here
+    for (int gi : mine)
+      for (int i : b->groups[gi]) {
+        const int rc = hetddft_sync(b->ctx[i]);
+        if (rc != HETDDFT_OK) {
+          record(i, rc);
+          return;
+        }
       }
-    }
   };
   std::vector<std::thread> th;
   th.reserve(b->part.size());

@@ -63,16 +63,21 @@ __device__ inline c64 vload<c64, c64>(const c64* f, int i) {
 struct Col {
   int x, y, tx, ty, tid;
   int bx, by, bz;  // tile of this workgroup (x tile, y tile, z chunk)
+  int m;           // member (instance) of a launch that carries several
+  int blk;         // workgroup index within the member
   bool in;
 };
+
 
 // Tile of this workgroup.  Workgroups are dispatched round-robin over the 8
 // XCDs, each with its own L2.  With xcd set, XCD k takes a contiguous range
 // of tiles (x tile fastest, then y tile, then z chunk), so the tiles that
 // share x/y halos and z-queue planes read them through the same L2 at about
 // the same time.  The mapping is a bijection for any block count.
+// A launch that carries several members has nzb z chunks per member in
+// gridDim.z (member-major): logical workgroup l of member m = l / (gx gy nzb).
 template <class SH>
-__device__ inline Col col_of(const Grid& g, int xcd) {
+__device__ inline Col col_of(const Grid& g, int xcd, int nzb) {
   Col c;
   c.tx = threadIdx.x;
   c.ty = threadIdx.y;
@@ -86,6 +91,10 @@ __device__ inline Col col_of(const Grid& g, int xcd) {
     const int q = nb / NXCD, r = nb % NXCD, k = p % NXCD;
     l = k * q + min(k, r) + p / NXCD;
   }
+  const int per = gx * gy * nzb;
+  c.m = l / per;
+  l -= c.m * per;
+  c.blk = l;
   c.bx = l % gx;
   c.by = (l / gx) % gy;
   c.bz = l / (gx * gy);
@@ -258,15 +267,14 @@ __device__ inline c64 ham25(const Ham25& a, const c64* tile, const ZQueue<c64, c
 // z is computed.  Without PF each plane's loads are issued in its own
 // iteration, which needs about 40 fewer VGPRs (one more resident wave per
 // SIMD) and gives the same results.
-template <class R, class SH, bool PF = true>
-__global__ void __launch_bounds__(SH::N) k_pc_predict25(Ham25 a, const c64* __restrict__ psi,
-                                                             const R* __restrict__ pot4, c64* old3,
-                                                             const c64* __restrict__ h1, c64* h2,
-                                                             c64* __restrict__ pc, R* __restrict__ den,
-                                                             double c4o3dt, double c112) {
+template <class R, class SH, bool PF>
+__device__ __forceinline__ void pc_predict25(const Ham25& a, const Col& c, const c64* __restrict__ psi,
+                                             const R* __restrict__ pot4, c64* old3,
+                                             const c64* __restrict__ h1, c64* h2,
+                                             c64* __restrict__ pc, R* __restrict__ den,
+                                             double c4o3dt, double c112) {
   __shared__ c64 tile[SH::LX * SH::LY];
   const Grid& g = a.g;
-  const Col c = col_of<SH>(g, a.xcd);
   const int z0 = c.bz * a.zc;
   const int z1 = min(z0 + a.zc, g.nz);
   const int xc = min(c.x, g.nx - 1), yc = min(c.y, g.ny - 1);
@@ -316,20 +324,19 @@ __global__ void __launch_bounds__(SH::N) k_pc_predict25(Ham25 a, const c64* __re
 
 // Hamming corrector + final value + errHe partial sums (steppc.f90:97-112),
 // software-pipelined like the predictor.
-template <class R, class SH, bool PF = true>
-__global__ void __launch_bounds__(SH::N) k_pc_correct25(Ham25 a, const c64* __restrict__ psi,
-                                                             const R* __restrict__ pot4,
-                                                             const c64* __restrict__ o3p,
-                                                             const c64* __restrict__ o2p,
-                                                             const c64* __restrict__ h1p,
-                                                             const c64* __restrict__ h2p,
-                                                             c64* __restrict__ pc, c64* __restrict__ psi_out,
-                                                             R* __restrict__ den, double* __restrict__ part,
-                                                             double dt3, double c9) {
+template <class R, class SH, bool PF>
+__device__ __forceinline__ void pc_correct25(const Ham25& a, const Col& c, const c64* __restrict__ psi,
+                                             const R* __restrict__ pot4,
+                                             const c64* __restrict__ o3p,
+                                             const c64* __restrict__ o2p,
+                                             const c64* __restrict__ h1p,
+                                             const c64* __restrict__ h2p,
+                                             c64* __restrict__ pc, c64* __restrict__ psi_out,
+                                             R* __restrict__ den, double* __restrict__ part,
+                                             double dt3, double c9) {
   __shared__ c64 tile[SH::LX * SH::LY];
   __shared__ double sh[SH::N / 64];
   const Grid& g = a.g;
-  const Col c = col_of<SH>(g, a.xcd);
   const int z0 = c.bz * a.zc;
   const int z1 = min(z0 + a.zc, g.nz);
   const int xc = min(c.x, g.nx - 1), yc = min(c.y, g.ny - 1);
@@ -379,19 +386,18 @@ __global__ void __launch_bounds__(SH::N) k_pc_correct25(Ham25 a, const c64* __re
     q.shift();
   }
   const double s = block_sum25<SH>(e, sh, c.tid);
-  if (c.tid == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+  if (c.tid == 0) part[c.blk] = s;
 }
 
 // RK-Gill stage (steprk.f90:56-74), software-pipelined like the predictor.
-template <class R, class SH, bool PF = true>
-__global__ void __launch_bounds__(SH::N) k_rk_stage25(Ham25 a, const c64* __restrict__ psi,
-                                                           const R* __restrict__ pot4, c64* __restrict__ qacc,
-                                                           c64* __restrict__ psi_out, R* __restrict__ den,
-                                                           c64* __restrict__ h1_out, double ca, double cb,
-                                                           double cc, double dt) {
+template <class R, class SH, bool PF>
+__device__ __forceinline__ void rk_stage25(const Ham25& a, const Col& c, const c64* __restrict__ psi,
+                                           const R* __restrict__ pot4, c64* __restrict__ qacc,
+                                           c64* __restrict__ psi_out, R* __restrict__ den,
+                                           c64* __restrict__ h1_out, double ca, double cb,
+                                           double cc, double dt) {
   __shared__ c64 tile[SH::LX * SH::LY];
   const Grid& g = a.g;
-  const Col c = col_of<SH>(g, a.xcd);
   const int z0 = c.bz * a.zc;
   const int z1 = min(z0 + a.zc, g.nz);
   const int xc = min(c.x, g.nx - 1), yc = min(c.y, g.ny - 1);
@@ -440,14 +446,13 @@ __global__ void __launch_bounds__(SH::N) k_rk_stage25(Ham25 a, const c64* __rest
 // software-pipelined like the predictor.  The arithmetic type is the field
 // type: FP64 in the FP64 path, FP32 (weights rounded to FP32) on the FP32
 // potential side of the mixed path.
-template <class R, class SH, bool PF = true>
-__global__ void __launch_bounds__(SH::N) k_grad3_25(Grid g, int zc, int xcd, const R* __restrict__ f,
-                                                         Stencil7<R, false> wx, Stencil7<R, false> wy,
-                                                         Stencil7<R, false> wz,
-                                                         R* __restrict__ fx, R* __restrict__ fy,
-                                                         R* __restrict__ fz) {
+template <class R, class SH, bool PF>
+__device__ __forceinline__ void grad3_25(const Grid& g, int zc, const Col& c, const R* __restrict__ f,
+                                         const Stencil7<R, false>& wx, const Stencil7<R, false>& wy,
+                                         const Stencil7<R, false>& wz,
+                                         R* __restrict__ fx, R* __restrict__ fy,
+                                         R* __restrict__ fz) {
   __shared__ R tile[SH::LX * SH::LY];
-  const Col c = col_of<SH>(g, xcd);
   const int z0 = c.bz * zc;
   const int z1 = min(z0 + zc, g.nz);
   ZQueue<R, R> q;
@@ -518,24 +523,23 @@ __device__ inline float penalty(float dn, float C, float beta, float den_m) {
   return C * (sg + beta * dn * (sg * (2.0f - sg)));
 }
 
-template <class R, class SH, bool PF = true>
-__global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25T<R> a, const R* __restrict__ den,
-                                                            const R* __restrict__ dencg,
-                                                            const R* __restrict__ delj4,
-                                                            const R* __restrict__ sto1c,
-                                                            const R* __restrict__ uimp,
-                                                            const R* __restrict__ falfs,
-                                                            const R* __restrict__ intx,
-                                                            const R* __restrict__ inty,
-                                                            const R* __restrict__ intz,
-                                                            const R* __restrict__ uaconv, R* __restrict__ pot4) {
+template <class R, class SH, bool PF>
+__device__ __forceinline__ void pot_final25(const Pot25T<R>& a, const Col& c, const R* __restrict__ den,
+                                            const R* __restrict__ dencg,
+                                            const R* __restrict__ delj4,
+                                            const R* __restrict__ sto1c,
+                                            const R* __restrict__ uimp,
+                                            const R* __restrict__ falfs,
+                                            const R* __restrict__ intx,
+                                            const R* __restrict__ inty,
+                                            const R* __restrict__ intz,
+                                            const R* __restrict__ uaconv, R* __restrict__ pot4) {
   using A = R;
   __shared__ A tx_tile[SH::TY * SH::LX];    // g_x, rows of this tile with x halo
   __shared__ A ty_tile[SH::LY * SH::TX];    // g_y, columns of this tile with y halo
   constexpr int NX = (SH::TY * SH::LX + SH::N - 1) / SH::N;
   constexpr int NY = (SH::LY * SH::TX + SH::N - 1) / SH::N;
   const Grid& g = a.g;
-  const Col c = col_of<SH>(g, a.xcd);
   const int z0 = c.bz * a.zc;
   const int z1 = min(z0 + a.zc, g.nz);
   const int xc = min(c.x, g.nx - 1), yc = min(c.y, g.ny - 1);
@@ -642,6 +646,117 @@ __global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25T<R> a, const R* __r
       for (int k = 0; k < 12; ++k) q[k] = q[k + 1];
     }
   }
+}
+
+// ---- launches carrying one or several members -----------------------------
+// Per-member pointers of each stencil kernel; nzb = z chunks per member.
+// Every member computes exactly what a launch of its own computes.
+template <class R>
+struct PcpTab {
+  int nzb;
+  const c64* timec[kMaxMembers];
+  const c64* psi[kMaxMembers];
+  const R* pot4[kMaxMembers];
+  c64* old3[kMaxMembers];
+  const c64* h1[kMaxMembers];
+  c64* h2[kMaxMembers];
+  c64* pc[kMaxMembers];
+  R* den[kMaxMembers];
+};
+template <class R>
+struct PccTab {
+  int nzb;
+  const c64* timec[kMaxMembers];
+  const c64* psi[kMaxMembers];
+  const R* pot4[kMaxMembers];
+  const c64* o3[kMaxMembers];
+  const c64* o2[kMaxMembers];
+  const c64* h1[kMaxMembers];
+  const c64* h2[kMaxMembers];
+  c64* pc[kMaxMembers];
+  c64* out[kMaxMembers];
+  R* den[kMaxMembers];
+  double* part[kMaxMembers];
+};
+template <class R>
+struct RkTab {
+  int nzb;
+  const c64* timec[kMaxMembers];
+  const c64* psi[kMaxMembers];
+  const R* pot4[kMaxMembers];
+  c64* q[kMaxMembers];
+  c64* out[kMaxMembers];
+  R* den[kMaxMembers];
+  c64* h1[kMaxMembers];
+};
+template <class R>
+struct GradTab {
+  int nzb;
+  const R* f[kMaxMembers];
+  R* fx[kMaxMembers];
+  R* fy[kMaxMembers];
+  R* fz[kMaxMembers];
+};
+template <class R>
+struct PotTab {
+  int nzb;
+  const R* den[kMaxMembers];
+  const R* dencg[kMaxMembers];
+  const R* delj4[kMaxMembers];
+  const R* sto1c[kMaxMembers];
+  const R* uimp[kMaxMembers];
+  const R* falfs[kMaxMembers];
+  const R* intx[kMaxMembers];
+  const R* inty[kMaxMembers];
+  const R* intz[kMaxMembers];
+  const R* uaconv[kMaxMembers];
+  R* pot4[kMaxMembers];
+};
+
+template <class R, class SH, bool PF = true>
+__global__ void __launch_bounds__(SH::N) k_pc_predict25(Ham25 a, PcpTab<R> t, double c4o3dt, double c112) {
+  const Col c = col_of<SH>(a.g, a.xcd, t.nzb);
+  const int m = c.m;
+  Ham25 am = a;
+  am.timec_z = t.timec[m];
+  pc_predict25<R, SH, PF>(am, c, t.psi[m], t.pot4[m], t.old3[m], t.h1[m], t.h2[m], t.pc[m], t.den[m], c4o3dt,
+                          c112);
+}
+
+template <class R, class SH, bool PF = true>
+__global__ void __launch_bounds__(SH::N) k_pc_correct25(Ham25 a, PccTab<R> t, double dt3, double c9) {
+  const Col c = col_of<SH>(a.g, a.xcd, t.nzb);
+  const int m = c.m;
+  Ham25 am = a;
+  am.timec_z = t.timec[m];
+  pc_correct25<R, SH, PF>(am, c, t.psi[m], t.pot4[m], t.o3[m], t.o2[m], t.h1[m], t.h2[m], t.pc[m], t.out[m],
+                          t.den[m], t.part[m], dt3, c9);
+}
+
+template <class R, class SH, bool PF = true>
+__global__ void __launch_bounds__(SH::N) k_rk_stage25(Ham25 a, RkTab<R> t, double ca, double cb, double cc,
+                                                           double dt) {
+  const Col c = col_of<SH>(a.g, a.xcd, t.nzb);
+  const int m = c.m;
+  Ham25 am = a;
+  am.timec_z = t.timec[m];
+  rk_stage25<R, SH, PF>(am, c, t.psi[m], t.pot4[m], t.q[m], t.out[m], t.den[m], t.h1[m], ca, cb, cc, dt);
+}
+
+template <class R, class SH, bool PF = true>
+__global__ void __launch_bounds__(SH::N) k_grad3_25(Grid g, int zc, int xcd, GradTab<R> t, Stencil7<R, false> wx,
+                                                         Stencil7<R, false> wy, Stencil7<R, false> wz) {
+  const Col c = col_of<SH>(g, xcd, t.nzb);
+  const int m = c.m;
+  grad3_25<R, SH, PF>(g, zc, c, t.f[m], wx, wy, wz, t.fx[m], t.fy[m], t.fz[m]);
+}
+
+template <class R, class SH, bool PF = true>
+__global__ void __launch_bounds__(SH::N) k_pot_final25(Pot25T<R> a, PotTab<R> t) {
+  const Col c = col_of<SH>(a.g, a.xcd, t.nzb);
+  const int m = c.m;
+  pot_final25<R, SH, PF>(a, c, t.den[m], t.dencg[m], t.delj4[m], t.sto1c[m], t.uimp[m], t.falfs[m], t.intx[m],
+                         t.inty[m], t.intz[m], t.uaconv[m], t.pot4[m]);
 }
 
 }  // namespace hetd

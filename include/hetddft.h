@@ -341,7 +341,14 @@ int hetddft_set_tuning(hetddft_ctx* ctx, const hetddft_tuning* t);
  * hetddft_iterate calls on every instance with deltat[i] for instance i: each
  * thread submits one iteration of each of its instances in turn, with no
  * synchronisation between iterations, and the call returns when every
- * instance has finished.  err (may be NULL) receives 3 doubles per instance,
+ * instance has finished.  Instances with equal hetddft_params, launch
+ * shapes (hetddft_set_tuning) and stream options form groups of up to 8 at
+ * hetddft_batch_create; a group whose members are at the same iteration
+ * and get the same deltat advances in shared launches (one launch per
+ * kernel carries every member, on the first member's streams), otherwise
+ * its members advance one by one.  Either way every instance's results are
+ * bit-identical to those of hetddft_iterate on that instance alone.
+ * err (may be NULL) receives 3 doubles per instance,
  * the PC error estimates of its last iteration; they are reduced and checked
  * for every instance in any case, so a diverged instance fails the call with
  * HETDDFT_E_NONFINITE.  On a failure the other

@@ -93,3 +93,38 @@ def test_batch_executor_reports_the_failing_instance(hip):
     b.close()
     for g in insts:
         g.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_shares_launches_across_instances_of_one_grid(hip, precision):
+    """Instances of one grid and parameters advance in shared launches (one
+    launch per kernel for all of them); each ends bit-identical to its solo
+    run, also when their particle-chain buffer sets are out of phase (one
+    instance re-ran its potential evaluation) and their droplets differ."""
+    import inputs
+    base = inputs.WORKLOADS["t32"]
+    wls = [inputs.Workload(f"g{i}", base.n, base.h, 12 + 4 * i, base.n_par, 0.5, v0=0.5 * i,
+                           vdir=(0.0, 0.6, 0.8), phase_noise=0.01, seed=60 + i) for i in range(3)]
+    print(f"shared launches across 3 instances of 32^3, precision={precision}", flush=True)
+    solo = []
+    for i, wl in enumerate(wls):
+        g = _make(hip, wl, precision)
+        if i == 1:
+            g.potential()
+        for _ in range(9):
+            g.iterate(wl.deltat, want_errors=False)
+        solo.append((g.get_state(), g.last_errors(), g.energy()))
+        g.close()
+    insts = [_make(hip, wl, precision) for wl in wls]
+    insts[1].potential()
+    b = hip.Batch(insts, [wl.deltat for wl in wls], nthreads=1)
+    b.iterate(2)
+    err = b.iterate(7, want_errors=True)
+    for g, (ref, ref_err, ref_e), got_err in zip(insts, solo, err):
+        for x, y in zip(g.get_state(), ref):
+            assert np.array_equal(x, y)
+        assert np.array_equal(np.asarray(ref_err), got_err)
+        assert g.energy() == ref_e
+    b.close()
+    for g in insts:
+        g.close()
