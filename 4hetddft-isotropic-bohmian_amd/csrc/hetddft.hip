@@ -115,6 +115,9 @@ struct FftKernelsT {
   void (*xyf)(XfArgs<T>, Grid, const cplx<T>*, const cplx<T>*, const cplx<T>*) = nullptr;
   void (*xyi)(XiArgs<T, T>, Grid, const cplx<T>*, const cplx<T>*, const cplx<T>*) = nullptr;
   size_t lds_xy = 0;
+  // persistent form of the 128 x 128 FP64 inverse plane kernel (one
+  // workgroup per CU), used when a launch holds at least two planes per CU
+  void (*xyip)(XiArgs<T, T>, Grid, int, const cplx<T>*) = nullptr;
 };
 using FftKernels64 = FftKernelsT<double>;
 using FftKernels32 = FftKernelsT<float>;
@@ -492,6 +495,7 @@ void pick_xy(FftKernels<T>& k) {
     k.xyf = k_fft_xyf128<T, T>;
     k.xyi = k_fft_xyi128<T, T>;
     k.lds_xy = kPlane128Lds * sizeof(cplx<T>);
+    if constexpr (sizeof(T) == 8) k.xyip = k_fft_xyi128p<T, T>;
   } else if constexpr (b <= 160 * 1024) {
     k.xyf = k_fft_xyf<T, T, MH, NY>;
     k.xyi = k_fft_xyi<T, T, MH, NY>;
@@ -573,6 +577,7 @@ int fft_alloc_tables(hetddft_ctx* c, FftSet& fs) {
     HIPCHK(c, hipFuncSetAttribute((const void*)k.xyf, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
     HIPCHK(c, hipFuncSetAttribute((const void*)k.xyi, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   }
+  if (k.xyip) HIPCHK(c, hipFuncSetAttribute((const void*)k.xyip, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   if (k.zp) HIPCHK(c, hipFuncSetAttribute((const void*)k.zp, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   for (int v = 0; v < k.nzv; ++v)
     HIPCHK(c, hipFuncSetAttribute((const void*)k.zpv[v], hipFuncAttributeMaxDynamicSharedMemorySize, lim));
@@ -716,8 +721,12 @@ int conv_batch_in(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin, c
   LAUNCHCHK(c);
   if (K.xyi) {
     ProfScope ps(c, HETDDFT_PROF_XYI, nout * (Sb + Rb));
-    PLAUNCH(ps, K.xyi, dim3(g.nz, nout), dim3(kPlaneThreads), K.lds_xy,
-        xi, g, (const cplx<T>*)fs.tw_xh, (const cplx<T>*)fs.twh, (const cplx<T>*)fs.tw_y);
+    const int items = g.nz * nout;
+    if (K.xyip && items >= 2 * c->cus)
+      PLAUNCH(ps, K.xyip, dim3(c->cus), dim3(kPlaneThreads), K.lds_xy, xi, g, items, (const cplx<T>*)fs.tw_y);
+    else
+      PLAUNCH(ps, K.xyi, dim3(g.nz, nout), dim3(kPlaneThreads), K.lds_xy,
+          xi, g, (const cplx<T>*)fs.tw_xh, (const cplx<T>*)fs.twh, (const cplx<T>*)fs.tw_y);
     LAUNCHCHK(c);
   } else {
     {
