@@ -10,6 +10,7 @@
 #include "common.h"
 #include "fft_kernels.h"
 #include "field_kernels.h"
+#include "init_kernels.h"
 #include "kinit.h"
 #include "kinit_device.h"
 #include "stencil25.h"
@@ -2879,6 +2880,73 @@ int hetddft_sample_particles(hetddft_ctx* c, const double* denx, const double lo
   if (e != hipSuccess) return fail(c, HETDDFT_E_HIP, "sample_particles: %s", hipGetErrorString(e));
   if (failed) return fail(c, HETDDFT_E_STATE, "sample_particles: a particle was not accepted in %lld attempts",
                           max_attempts);
+  return HETDDFT_OK;
+}
+
+int hetddft_init_droplet(hetddft_ctx* c, const hetddft_droplet* d) {
+  if (!c || !d) return HETDDFT_E_ARG;
+  if (!(d->n_he > 0.0) || !(d->rho0 > 0.0) || !(d->r0 > 0.0) || !(d->a > 0.0) || !(d->ripple >= 0.0) ||
+      !(d->phase_noise >= 0.0) || !(d->cavity >= 0.0) || (d->cavity > 0.0 && !(d->cavity_w > 0.0)))
+    return fail(c, HETDDFT_E_ARG, "init_droplet: invalid droplet parameters");
+  const Grid& g = c->g;
+  DropArgs a{};
+  a.g = g;
+  // x(ix) = -xmax + hx (ix - 1), as the grid arrays of the reference
+  a.x0 = -c->p.xmax;
+  a.y0 = -c->p.ymax;
+  a.z0 = -c->p.zmax;
+  a.hx = c->hx;
+  a.hy = c->hy;
+  a.hz = c->hz;
+  a.cx = d->centre[0];
+  a.cy = d->centre[1];
+  a.cz = d->centre[2];
+  a.rho0 = d->rho0;
+  a.R = d->r0 * std::cbrt(d->n_he);
+  a.a = d->a;
+  a.rb = d->cavity;
+  a.w = d->cavity_w;
+  a.ripple = d->ripple;
+  for (int k = 0; k < 24; ++k) a.clm[k] = d->clm[k];
+  a.kx = d->lamda[0];
+  a.ky = d->lamda[1];
+  a.kz = d->lamda[2];
+  a.sphi = d->phase_noise;
+  a.n_he = d->n_he;
+  a.dxyz = c->dxyz;
+  a.seed = d->seed;
+  RET(settle(c));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->chain_pending = false;
+  double* tmp = nullptr;  // kDropBlocks partials, then max and sum
+  if (hipMalloc((void**)&tmp, (kDropBlocks + 2) * sizeof(double)) != hipSuccess)
+    return fail(c, HETDDFT_E_NOMEM, "init_droplet: hipMalloc failed");
+  double* smax = tmp + kDropBlocks;
+  double* sum = smax + 1;
+  if (a.ripple > 0.0) {
+    hipLaunchKernelGGL(k_drop_surfmax, dim3(kDropBlocks), dim3(kBlock), 0, c->st, a, tmp);
+    hipLaunchKernelGGL(k_drop_maxfin, dim3(1), dim3(kBlock), 0, c->st, (const double*)tmp, kDropBlocks, smax);
+  }
+  hipLaunchKernelGGL(k_drop_sum, dim3(kDropBlocks), dim3(kBlock), 0, c->st, a, (const double*)smax, tmp);
+  hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(kBlock), 0, c->st, (const double*)tmp, kDropBlocks, 1, sum);
+  hipLaunchKernelGGL(k_drop_psi, dim3(blocks_for(g.n)), dim3(kBlock), 0, c->st, a, (const double*)smax,
+                     (const double*)sum, (c64*)c->psi);
+  const int np = c->p.n_par;
+  if (np > 0) {
+    const double ktops = 7.63822291;  // A/ps -> A*K (DFT4He3d.f90:600-602)
+    const double sg = d->sigma > 0.0 ? d->sigma : 0.0;
+    if (d->sigma > 0.0)
+      hipLaunchKernelGGL(k_drop_cloud, dim3(blocks_for(np)), dim3(kBlock), 0, c->st, np, a.cx, a.cy, a.cz, sg,
+                         d->v0[0] * ktops, d->v0[1] * ktops, d->v0[2] * ktops, a.seed, c->rimp, c->vimp);
+    else
+      hipLaunchKernelGGL(k_drop_velocity, dim3(blocks_for(np)), dim3(kBlock), 0, c->st, np, d->v0[0] * ktops,
+                         d->v0[1] * ktops, d->v0[2] * ktops, c->vimp);
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  hipFree(tmp);
+  if (e != hipSuccess) return fail(c, HETDDFT_E_HIP, "init_droplet: %s", hipGetErrorString(e));
+  c->started = false;
   return HETDDFT_OK;
 }
 

@@ -87,6 +87,29 @@ class Morse(ctypes.Structure):
                 ("rc", ctypes.c_double), ("umax", ctypes.c_double), ("nq", ctypes.c_int)]
 
 
+class Droplet(ctypes.Structure):
+    """Mirror of ``hetddft_droplet`` (fresh start built on the device)."""
+    _fields_ = [("n_he", ctypes.c_double), ("rho0", ctypes.c_double), ("r0", ctypes.c_double),
+                ("a", ctypes.c_double), ("cavity", ctypes.c_double), ("cavity_w", ctypes.c_double),
+                ("ripple", ctypes.c_double), ("clm", ctypes.c_double * 24), ("centre", ctypes.c_double * 3),
+                ("lamda", ctypes.c_double * 3), ("phase_noise", ctypes.c_double), ("sigma", ctypes.c_double),
+                ("v0", ctypes.c_double * 3), ("seed", ctypes.c_ulonglong)]
+
+
+def droplet(n_he, seed, rho0=0.0218, r0=2.22, a=0.7, cavity=0.0, cavity_w=0.3, ripple=0.0, clm=None,
+            centre=(0.0, 0.0, 0.0), lamda=(0.0, 0.0, 0.0), phase_noise=0.0, sigma=0.0, v0=(0.0, 0.0, 0.0)) -> Droplet:
+    """A ``hetddft_droplet`` with the Woods-Saxon defaults of the workloads."""
+    d = Droplet()
+    d.n_he, d.rho0, d.r0, d.a = n_he, rho0, r0, a
+    d.cavity, d.cavity_w, d.ripple = cavity, cavity_w, ripple
+    for k, c in enumerate(clm if clm is not None else ()):
+        d.clm[k] = c
+    for k in range(3):
+        d.centre[k], d.lamda[k], d.v0[k] = centre[k], lamda[k], v0[k]
+    d.phase_noise, d.sigma, d.seed = phase_noise, sigma, int(seed)
+    return d
+
+
 TUNE_CLASSES = {"grad_den": 0, "pot4_assembly": 1, "pc_predict": 2, "pc_correct": 3, "rk_stage": 4}
 TILES = {0: (64, 4), 1: (32, 8), 2: (32, 16), 3: (64, 8)}
 
@@ -155,6 +178,7 @@ def load_library():
         "hetddft_checkpoint_size": ([C, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
         "hetddft_sample_particles": ([C, _VP, _VP, _VP, ctypes.c_ulonglong, ctypes.c_longlong, _VP], ctypes.c_int),
         "hetddft_init_kernels": ([C, ctypes.POINTER(Morse)], ctypes.c_int),
+        "hetddft_init_droplet": ([C, ctypes.POINTER(Droplet)], ctypes.c_int),
         "hetddft_get_kernels": ([C, _VP, _VP, _VP, _VP], ctypes.c_int),
         "hetddft_checkpoint_save": ([C, _VP], ctypes.c_int),
         "hetddft_checkpoint_load": ([C, _VP], ctypes.c_int),
@@ -355,6 +379,11 @@ class Instance:
         self._check(self.lib.hetddft_sample_particles(self._c, _ptr(d), _ptr(lo), _ptr(hi), int(seed),
                                                       int(max_attempts), _ptr(out)), "sample_particles")
         return out
+
+    def init_droplet(self, d: Droplet):
+        """Fresh start built on the device (hetddft_init_droplet); replaces the
+        state like set_state, so start() follows."""
+        self._check(self.lib.hetddft_init_droplet(self._c, ctypes.byref(d)), "init_droplet")
 
     def checkpoint(self):
         """Full-history checkpoint image (bytes) for bit-exact resume."""

@@ -105,6 +105,24 @@ def _ylm_real(l, m, ct, phi):
     return P * (np.cos(m * phi) if m >= 0 else np.sin(-m * phi))
 
 
+def ripple_coeffs(wl: Workload):
+    """The 24 ripple coefficients c_lm (l = 1..4, m = -l..l in that order)."""
+    g = _rng(wl.seed + 1000)
+    return [float(g.standard_normal()) for l in range(1, 5) for m in range(-l, l + 1)]
+
+
+def droplet_spec(wl: Workload) -> dict:
+    """The workload's droplet as keyword arguments of hetddft.droplet (the
+    device-built start): the density of woods_saxon, phases and particles
+    from the Philox streams of hetddft_init_droplet (same distributions as
+    initial_state, other values)."""
+    d = np.asarray(wl.vdir, dtype=np.float64)
+    d = d / np.linalg.norm(d)
+    return dict(n_he=float(wl.n_he), seed=wl.seed, cavity=wl.cavity, cavity_w=0.3, ripple=wl.ripple,
+                clm=ripple_coeffs(wl) if wl.ripple > 0.0 else None, phase_noise=wl.phase_noise,
+                sigma=wl.sigma, v0=tuple(float(wl.v0 * d[c]) for c in range(3)))
+
+
 def woods_saxon(wl: Workload, shape=None):
     n = wl.n
     x = -wl.xmax + wl.h * np.arange(n)
@@ -112,13 +130,13 @@ def woods_saxon(wl: Workload, shape=None):
     r = np.sqrt(x3 * x3 + y3 * y3 + z3 * z3)
     R = 2.22 * wl.n_he ** (1.0 / 3.0)
     if wl.ripple > 0.0:
-        g = _rng(wl.seed + 1000)
+        coeffs = iter(ripple_coeffs(wl))
         ct = np.divide(z3, r, out=np.zeros_like(r), where=r > 0)
         phi = np.arctan2(y3, x3)
         surf = np.zeros_like(r)
         for l in range(1, 5):
             for m in range(-l, l + 1):
-                surf += g.standard_normal() * _ylm_real(l, m, ct, phi)
+                surf += next(coeffs) * _ylm_real(l, m, ct, phi)
         surf /= max(np.abs(surf).max(), 1e-300)
         R = R + wl.ripple * surf
     rho = 0.0218 / (1.0 + np.exp((r - R) / 0.7))

@@ -243,6 +243,49 @@ int hetddft_get_kernels(hetddft_ctx* ctx, double* fvlj4, double* wcgk, double* k
 int hetddft_sample_particles(hetddft_ctx* ctx, const double* denx, const double lo[3], const double hi[3],
                              unsigned long long seed, long long max_attempts, double* rimp);
 
+/* A fresh start built on the device: psi, rimp and vimp of a Woods-Saxon
+ * droplet and a Gaussian test-particle cloud, as the reference's mode-0 start
+ * (DFT4He3d.f90:585-605) builds them from a density file: psi = sqrt(den)
+ * exp(i (x lamdax + y lamday + z lamdaz)), every particle with velocity
+ * v0 * Ktops (A/ps to A*K, 7.63822291).  Here den is the closed form
+ *   rho0 / (1 + exp((r - R)/a)),  r = |x - centre|,
+ *   R = r0 n_he^(1/3) + ripple * s / max|s| over the grid,
+ *   s = sum over l = 1..4, m = -l..l of clm[k] P_l^|m|(cos theta) T_m(phi)
+ *       (k runs over (l, m) in that order; P with the Condon-Shortley
+ *       phase; T_m = cos(m phi) for m >= 0, sin(|m| phi) for m < 0),
+ * divided by 1 + exp(-(r - cavity)/cavity_w) when cavity > 0, and rescaled
+ * so that sum(den) dxyz = n_he, on the grid x(ix) = -xmax + hx (ix - 1).
+ * phase_noise > 0 adds sigma_phi * g to the phase at every point and
+ * sigma > 0 places particle p at centre + sigma (g0, g1, g2); g are standard
+ * normals from Philox4x64-10 (the generator of hetddft_sample_particles) by
+ * Box-Muller: for point i, key (seed, 1) and counter (i, 0, 0, 0); for
+ * particle p, key (seed, 2) and counter (p, 0, 0, 0), words 0-1 giving g0 and
+ * g1, words 2-3 g2; g = sqrt(-2 ln u) cos(2 pi v) (sin for g1) with
+ * u = ((w0 >> 11) + 1) / 2^53 and v = (w1 >> 11) / 2^53.  sigma <= 0 leaves
+ * rimp as it is (for hetddft_sample_particles); vimp is set either way.
+ * Replaces the state like hetddft_set_state (hetddft_start must follow).
+ * No host-side field is formed or copied, so an ensemble's instances are set
+ * up without host work per grid point.  HETDDFT_E_ARG for n_he, rho0, r0 or
+ * a not positive, negative ripple, phase_noise or cavity, or cavity > 0 with
+ * cavity_w not positive. */
+typedef struct hetddft_droplet {
+  double n_he;           /* He atoms */
+  double rho0;           /* 0.0218 A^-3 */
+  double r0;             /* 2.22 A */
+  double a;              /* surface width, 0.7 A */
+  double cavity;         /* cavity radius (A); 0: none */
+  double cavity_w;       /* cavity edge width (A) */
+  double ripple;         /* surface ripple amplitude (A); 0: none */
+  double clm[24];        /* ripple coefficients */
+  double centre[3];      /* droplet and cloud centre (A) */
+  double lamda[3];       /* plane-wave boost (xlamdax, xlamday, xlamdaz; A^-1) */
+  double phase_noise;    /* sigma_phi (rad) */
+  double sigma;          /* particle cloud width (A) */
+  double v0[3];          /* particle velocity (A/ps) */
+  unsigned long long seed;
+} hetddft_droplet;
+int hetddft_init_droplet(hetddft_ctx* ctx, const hetddft_droplet* d);
+
 /* Full-history checkpoint for bit-exact resume.  No counterpart in the
  * reference, whose restarts read psi and the particles back (readenc.f90:96-114,
  * mode 2) and begin again with three RK steps; this one keeps everything the

@@ -25,6 +25,10 @@ How each part is pinned (tests/test_oracle_*.py):
   * k-space kernels: closed forms (wcgk, kalfs), quadrature of the cored
     Lennard-Jones Fourier integral (fvlj4), analytic Fourier transform of
     the cored Morse potential (Vq).
+  * droplet start (droplet_state): the vectorised Philox against the scalar
+    one (itself equal to numpy's Philox bit for bit); the Woods-Saxon
+    density with ripple against the workload generator's scipy-based one;
+    the Box-Muller normals' moments; sum(|psi|^2) dxyz = N.
   * composite trajectory (Veval ordering inside the PC step): parity
     unpinned against a reference run -- the reference needs FFTW/MKL, which
     this image lacks, so it cannot be built here; it is pinned only through
@@ -701,6 +705,128 @@ def sorteo(prm: Params, denx, n_par, lo, hi, seed, max_attempts=1 << 20):
         else:
             raise RuntimeError(f"particle {i} not accepted in {max_attempts} attempts")
     return out, tries
+
+
+# --------------------------------------------------------------------------
+# Fresh start of a droplet (mode 0, DFT4He3d.f90:585-605) from a closed-form
+# density: the layout and random streams of hetddft_init_droplet
+# --------------------------------------------------------------------------
+_U32 = np.uint64(0xFFFFFFFF)
+
+
+def _mulhilo64(a, m):
+    """High and low 64-bit words of a * m for a uint64 array a and a 64-bit
+    constant m (schoolbook product of 32-bit halves)."""
+    a = a.astype(np.uint64)
+    m = np.uint64(m)
+    al, ah = a & _U32, a >> np.uint64(32)
+    ml, mh = m & _U32, m >> np.uint64(32)
+    ll, hl, lh, hh = al * ml, ah * ml, al * mh, ah * mh
+    cross = (ll >> np.uint64(32)) + (hl & _U32) + lh
+    hi = hh + (hl >> np.uint64(32)) + (cross >> np.uint64(32))
+    return hi, a * m
+
+
+def philox4x64_many(c0, key, rounds=10):
+    """philox4x64 of counters (c0[i], 0, 0, 0) under one key, vectorised over
+    i; returns the four words as uint64 arrays."""
+    with np.errstate(over="ignore"):
+        c = [np.asarray(c0, dtype=np.uint64), np.zeros(len(c0), np.uint64), np.zeros(len(c0), np.uint64),
+             np.zeros(len(c0), np.uint64)]
+        k0, k1 = np.uint64(int(key[0]) & _M64), np.uint64(int(key[1]) & _M64)
+        for r in range(rounds):
+            if r > 0:
+                k0 = np.uint64((int(k0) + _PHILOX_W[0]) & _M64)
+                k1 = np.uint64((int(k1) + _PHILOX_W[1]) & _M64)
+            h0, l0 = _mulhilo64(c[0], _PHILOX_M[0])
+            h1, l1 = _mulhilo64(c[2], _PHILOX_M[1])
+            c = [h1 ^ c[1] ^ k0, l1, h0 ^ c[3] ^ k1, l0]
+    return c
+
+
+def _u53_open(w):
+    """((w >> 11) + 1) / 2^53 in (0, 1]."""
+    return ((w >> np.uint64(11)) + np.uint64(1)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def _u53_vec(w):
+    return (w >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def assoc_legendre(l, m, x):
+    """P_l^m(x) with the Condon-Shortley phase, 0 <= m <= l <= 4, from the
+    Rodrigues form (-1)^m (1-x^2)^(m/2) d^m/dx^m P_l(x) with the Legendre
+    polynomials written out (independent of scipy's recurrence)."""
+    legendre = {0: [1.0], 1: [0.0, 1.0], 2: [-0.5, 0.0, 1.5], 3: [0.0, -1.5, 0.0, 2.5],
+                4: [0.375, 0.0, -3.75, 0.0, 4.375]}
+    coef = np.polynomial.polynomial.polyder(np.array(legendre[l]), m) if m else np.array(legendre[l])
+    d = np.polynomial.polynomial.polyval(x, coef)
+    s = np.sqrt(np.maximum(1.0 - x * x, 0.0))
+    return (-1.0) ** m * s ** m * d
+
+
+def droplet_density(prm: Params, n_he, rho0=0.0218, r0=2.22, a=0.7, cavity=0.0, cavity_w=0.3, ripple=0.0,
+                    clm=None, centre=(0.0, 0.0, 0.0)):
+    """Woods-Saxon density of hetddft_init_droplet on the grid x(ix) = -xmax
+    + hx (ix-1): rho0/(1+exp((r-R)/a)), R = r0 N^(1/3) + ripple s/max|s|,
+    s = sum_{l<=4, |m|<=l} c_lm P_l^|m|(cos theta) (cos m phi | sin |m| phi),
+    divided by 1+exp(-(r-cavity)/cavity_w), rescaled to sum*dxyz = N."""
+    x = prm.axis_grid("x") - centre[0]
+    y = prm.axis_grid("y") - centre[1]
+    z = prm.axis_grid("z") - centre[2]
+    z3, y3, x3 = np.meshgrid(z, y, x, indexing="ij")
+    r = np.sqrt(x3 * x3 + y3 * y3 + z3 * z3)
+    R = r0 * n_he ** (1.0 / 3.0)
+    if ripple > 0.0:
+        ct = np.divide(z3, r, out=np.zeros_like(r), where=r > 0)
+        phi = np.arctan2(y3, x3)
+        surf = np.zeros_like(r)
+        k = 0
+        for l in range(1, 5):
+            for m in range(-l, l + 1):
+                t = np.cos(m * phi) if m >= 0 else np.sin(-m * phi)
+                surf = surf + clm[k] * (assoc_legendre(l, abs(m), ct) * t)
+                k += 1
+        surf = surf / max(np.abs(surf).max(), 1e-300)
+        R = R + ripple * surf
+    rho = rho0 / (1.0 + np.exp((r - R) / a))
+    if cavity > 0.0:
+        rho = rho / (1.0 + np.exp(-(r - cavity) / cavity_w))
+    return rho * (n_he / (rho.sum() * prm.dxyz))
+
+
+def droplet_state(prm: Params, n_he, seed, lamda=(0.0, 0.0, 0.0), phase_noise=0.0, sigma=0.0,
+                  v0=(0.0, 0.0, 0.0), centre=(0.0, 0.0, 0.0), **dens):
+    """psi, rimp, vimp of hetddft_init_droplet: psi = sqrt(rho) exp(i th),
+    th = x lamdax + y lamday + z lamdaz (DFT4He3d.f90:587-596) plus
+    phase_noise * g(point); particle p at centre + sigma (g0, g1, g2) with
+    velocity v0 * Ktops (DFT4He3d.f90:600-605).  g: Box-Muller normals of
+    Philox4x64-10 words, key (seed, 1) and counter (i, 0, 0, 0) for point i,
+    key (seed, 2) and counter (p, 0, 0, 0) for particle p.  rimp is None when
+    sigma <= 0 (positions left to sorteo)."""
+    rho = droplet_density(prm, n_he, centre=centre, **dens)
+    x, y, z = prm.axis_grid("x"), prm.axis_grid("y"), prm.axis_grid("z")
+    th = (x[None, None, :] * lamda[0] + y[None, :, None] * lamda[1]) + z[:, None, None] * lamda[2]
+    th = np.broadcast_to(th, rho.shape).copy()
+    if phase_noise > 0.0:
+        w = philox4x64_many(np.arange(rho.size, dtype=np.uint64), (seed, 1))
+        g = np.sqrt(-2.0 * np.log(_u53_open(w[0]))) * np.cos(6.283185307179586 * _u53_vec(w[1]))
+        th = th + phase_noise * g.reshape(rho.shape)
+    amp = np.sqrt(rho)
+    psi = amp * np.cos(th) + 1j * (amp * np.sin(th))
+    n = prm.n_par
+    rimp = None
+    if sigma > 0.0:
+        w = philox4x64_many(np.arange(n, dtype=np.uint64), (seed, 2))
+        r01 = np.sqrt(-2.0 * np.log(_u53_open(w[0])))
+        t01 = 6.283185307179586 * _u53_vec(w[1])
+        r23 = np.sqrt(-2.0 * np.log(_u53_open(w[2])))
+        t23 = 6.283185307179586 * _u53_vec(w[3])
+        rimp = np.stack([centre[0] + sigma * (r01 * np.cos(t01)), centre[1] + sigma * (r01 * np.sin(t01)),
+                         centre[2] + sigma * (r23 * np.cos(t23))])
+    ktops = 7.63822291
+    vimp = np.stack([np.full(n, v0[c] * ktops) for c in range(3)])
+    return psi, rimp, vimp
 
 
 def diagnostics(prm: Params, psi, pot4, denx, vimp, aimp):
