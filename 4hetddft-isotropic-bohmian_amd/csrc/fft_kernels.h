@@ -497,6 +497,15 @@ __device__ inline void bfly(cplx<T>* v) {
 // LDS accesses of a wavefront walk columns (stride N+1: no bank conflicts).
 // Persistent over items like k_fft_zpipe; the next item's stage-A rows are
 // loaded while the current item runs B, C and D.
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup p on XCD p % 8),
+// and each XCD has its own L2.  xcd_order(p, nb) is p's place in an order
+// that gives every XCD one contiguous range of [0, nb).
+__device__ inline int xcd_order(int p, int nb) {
+  constexpr int NXCD = 8;
+  const int q = nb / NXCD, r = nb % NXCD, k = p % NXCD;
+  return k * q + min(k, r) + p / NXCD;
+}
+
 template <class T, int N, int TY, int R1, int R2>
 __host__ __device__ constexpr size_t zreg_bytes() {
   return (size_t)(TY * (N + 1) + N) * sizeof(cplx<T>);
@@ -516,20 +525,27 @@ __global__ void __launch_bounds__(kThreads) k_fft_zreg(ZArgs<T> a, int njobs, Gr
   cplx<T>* tw = tile + TY * ls;
   const int tid = threadIdx.x;
   const int ntile = (g.ny + TY - 1) / TY;
-  const int per_job = ntile * g.mx;
-  const int total = per_job * njobs;
+  const int ncol = ntile * g.mx;  // column tiles (kx, ky tile)
+  const int total = ncol * njobs;
   const long long pstride = (long long)g.mx * g.ny;
   for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
 
+  // Items (column tile, job) in xcd_order, the jobs of a column tile next to
+  // each other: each XCD walks its own range of column tiles, and the jobs
+  // that share an input spectrum or a k-space kernel read it under one L2
+  // at about the same time (+1.1% FP64 and +0.6% mixed PC steps/s at 128^3;
+  // the same order without the XCD grouping, or the XCD grouping of the x+y
+  // plane kernels, measured slower).
   struct Item {
     int jb, kx, ky0;
   };
   auto decode = [&](int w) {
+    const int l = xcd_order(w, total);
     Item it;
-    it.jb = w / per_job;
-    const int r = w - it.jb * per_job;
-    it.kx = r / ntile;
-    it.ky0 = (r - it.kx * ntile) * TY;
+    const int col = l / njobs;
+    it.jb = l - col * njobs;
+    it.kx = col / ntile;
+    it.ky0 = (col - it.kx * ntile) * TY;
     return it;
   };
   // stage-A rows of an item: butterfly b = l + TY j, elements kz = j + R2 t
