@@ -47,6 +47,7 @@ program he4_tddft_gpu
   type(hetddft_params) :: p
   type(hetddft_morse) :: morse
   type(hetddft_tuning) :: tuning
+  type(hetddft_droplet) :: droplet
   type(c_ptr) :: ctx
   complex(c_double_complex), allocatable :: psi(:,:,:)
   real(c_double), allocatable :: rimp(:,:), vimp(:,:), aimp(:,:)
@@ -105,8 +106,8 @@ program he4_tddft_gpu
                      'hetddft_make_kernels')
 
   if (dry_run) then
-    write(*, '("hetddft_params bytes ",i0,", hetddft_morse bytes ",i0,", hetddft_tuning bytes ",i0)') &
-        c_sizeof(p), c_sizeof(morse), c_sizeof(tuning)
+    write(*, '("hetddft_params bytes ",i0,", hetddft_morse bytes ",i0,", hetddft_tuning bytes ",i0, &
+              & ", hetddft_droplet bytes ",i0)') c_sizeof(p), c_sizeof(morse), c_sizeof(tuning), c_sizeof(droplet)
     open(newunit=u, file=trim(fileout), access='stream', form='unformatted', status='replace')
     write(u) fvlj4, wcgk, kalfs, vq
     close(u)

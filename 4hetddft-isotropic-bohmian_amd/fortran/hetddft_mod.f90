@@ -61,6 +61,15 @@ module hetddft_mod
     real(c_double) :: ms_per_step
   end type hetddft_tuning
 
+  ! Mirrors struct hetddft_droplet (fresh start built on the device; the
+  ! seed is a 64-bit unsigned value in C, passed through c_long_long).
+  type, bind(C), public :: hetddft_droplet
+    real(c_double) :: n_he, rho0, r0, a, cavity, cavity_w, ripple
+    real(c_double) :: clm(24), centre(3), lamda(3)
+    real(c_double) :: phase_noise, sigma, v0(3)
+    integer(c_long_long) :: seed
+  end type hetddft_droplet
+
   public :: hetddft_default_params, hetddft_create, hetddft_destroy, hetddft_last_error, &
             hetddft_make_kernels, hetddft_set_kernels, hetddft_set_state, hetddft_start, &
             hetddft_potential, hetddft_step_rk, hetddft_step_pc, hetddft_last_errors, &
@@ -68,7 +77,7 @@ module hetddft_mod
             hetddft_get_field, hetddft_convolve, hetddft_profile, hetddft_profile_read, &
             hetddft_sync, hetddft_device_bytes, hetddft_graph_info, hetddft_diagnostics, &
             hetddft_current, hetddft_checkpoint_size, hetddft_checkpoint_save, hetddft_checkpoint_load, &
-            hetddft_sample_particles, hetddft_init_kernels, hetddft_get_kernels, &
+            hetddft_sample_particles, hetddft_init_droplet, hetddft_init_kernels, hetddft_get_kernels, &
             hetddft_autotune, hetddft_get_tuning, hetddft_set_tuning, &
             hetddft_batch_create, hetddft_batch_iterate, hetddft_batch_last_error, hetddft_batch_destroy
   public :: hetddft_message, hetddft_check, gpu_steprk, gpu_steppc, gpu_veval, gpu_energy
@@ -235,6 +244,12 @@ module hetddft_mod
       real(c_double), intent(in) :: denx(*), lo(3), hi(3)
       integer(c_long_long), value :: seed, max_attempts
       real(c_double), intent(out) :: rimp(*)
+    end function
+
+    integer(c_int) function hetddft_init_droplet(ctx, d) bind(C, name="hetddft_init_droplet")
+      import :: c_int, c_ptr, hetddft_droplet
+      type(c_ptr), value :: ctx
+      type(hetddft_droplet), intent(in) :: d
     end function
 
     integer(c_int) function hetddft_checkpoint_size(ctx, bytes) bind(C, name="hetddft_checkpoint_size")

@@ -58,7 +58,8 @@ def test_fortran_struct_layout_and_kernels(tmp_path):
     res = subprocess.run([str(exe), "in.nml"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stderr
     sizes = [int(t) for t in res.stdout.replace(",", " ").split() if t.isdigit()]
-    assert sizes[:3] == [ctypes.sizeof(hetddft.Params), ctypes.sizeof(hetddft.Morse), ctypes.sizeof(hetddft.Tuning)]
+    assert sizes[:4] == [ctypes.sizeof(hetddft.Params), ctypes.sizeof(hetddft.Morse), ctypes.sizeof(hetddft.Tuning),
+                         ctypes.sizeof(hetddft.Droplet)]
     p = inputs.params_for(wl)
     k = hetddft.make_kernels(p, hetddft.Morse(20.0, 5.0, 0.8, 2.0, 0.0, 1500))
     raw = np.fromfile(tmp_path / "k.bin", dtype=np.float64).reshape((4,) + p.hshape)
