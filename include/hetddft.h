@@ -68,6 +68,7 @@ int hetddft_default_params(hetddft_params* p);
  * fft.f90:5-76). */
 int hetddft_create(const hetddft_params* p, hetddft_ctx** out);
 
+/* Frees the context; NULL is a no-op (HETDDFT_OK). */
 int hetddft_destroy(hetddft_ctx* ctx);
 
 /* Message of the last failure on ctx; with ctx == NULL, the message of the
@@ -402,6 +403,7 @@ typedef struct hetddft_batch hetddft_batch;
 int hetddft_batch_create(hetddft_ctx* const* ctxs, int n, int nthreads, hetddft_batch** out);
 int hetddft_batch_iterate(hetddft_batch* b, const double* deltat, int steps, double* err, int* failed);
 const char* hetddft_batch_last_error(const hetddft_batch* b);
+/* Frees the batch (not its contexts); NULL is a no-op (HETDDFT_OK). */
 int hetddft_batch_destroy(hetddft_batch* b);
 
 #ifdef __cplusplus
