@@ -128,12 +128,9 @@ struct GatherCell {
   bool ok;
 };
 
-__device__ inline GatherCell gather_cell(const BoxArgs& b, const double* __restrict__ rp, int i) {
+// from the scaled offsets a = (r - x1)/h + 1 of the three components
+__device__ inline GatherCell gather_cell_at(const BoxArgs& b, double ax, double ay, double az) {
   GatherCell q;
-  const int np = b.npar;
-  const double ax = (rp[i] - b.xmin) / b.hx + 1;
-  const double ay = (rp[np + i] - b.ymin) / b.hy + 1;
-  const double az = (rp[2 * np + i] - b.zmin) / b.hz + 1;
   // 1 <= int(a) <= n-1  <=>  1 <= a < n; the comparisons are false for a
   // non-finite position, which is then outside (and never converted)
   q.ok = ax >= 1.0 && ax < (double)b.g.nx && ay >= 1.0 && ay < (double)b.g.ny && az >= 1.0 &&
@@ -145,6 +142,12 @@ __device__ inline GatherCell gather_cell(const BoxArgs& b, const double* __restr
   q.s = ay - q.iy;
   q.t = az - q.iz;
   return q;
+}
+
+__device__ inline GatherCell gather_cell(const BoxArgs& b, const double* __restrict__ rp, int i) {
+  const int np = b.npar;
+  return gather_cell_at(b, (rp[i] - b.xmin) / b.hx + 1, (rp[np + i] - b.ymin) / b.hy + 1,
+                        (rp[2 * np + i] - b.zmin) / b.hz + 1);
 }
 
 // reg layout: [0..5] bbox of occupied cells (imin,jmin,kmin,imax,jmax,kmax);
@@ -228,10 +231,17 @@ __device__ __forceinline__ void hist_body(const BoxArgs& b, const double* __rest
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int ix = 0, iy = 0, iz = 0;
   bool in = false;
+  // (r - x1)/h once per component: the NGP index takes int(1.5 + .), the
+  // gather cell int(. + 1) (the same values as two separate divisions, and
+  // few enough registers for the kernel to start beside the psi kernels)
+  double ux = 0.0, uy = 0.0, uz = 0.0;
   if (i < b.npar) {
-    const double fx = 1.5 + (r[i] - b.xmin) / b.hx;
-    const double fy = 1.5 + (r[b.npar + i] - b.ymin) / b.hy;
-    const double fz = 1.5 + (r[2 * b.npar + i] - b.zmin) / b.hz;
+    ux = (r[i] - b.xmin) / b.hx;
+    uy = (r[b.npar + i] - b.ymin) / b.hy;
+    uz = (r[2 * b.npar + i] - b.zmin) / b.hz;
+    const double fx = 1.5 + ux;
+    const double fy = 1.5 + uy;
+    const double fz = 1.5 + uz;
     // 1 <= int(f) <= n  <=>  1 <= f < n+1; false for a non-finite position,
     // which is dropped like one outside the box (and never converted)
     in = fx >= 1.0 && fx < b.g.nx + 1.0 && fy >= 1.0 && fy < b.g.ny + 1.0 && fz >= 1.0 && fz < b.g.nz + 1.0;
@@ -248,7 +258,7 @@ __device__ __forceinline__ void hist_body(const BoxArgs& b, const double* __rest
   // the gather cell's corners are (ix-1 .. ix) 0-based in each direction
   GatherCell q{};
   q.ok = false;
-  if (i < b.npar) q = gather_cell(b, r, i);
+  if (i < b.npar) q = gather_cell_at(b, ux + 1, uy + 1, uz + 1);
   int v[kBoxPart] = {in ? ix - 1 : 0x7fffffff, in ? iy - 1 : 0x7fffffff, in ? iz - 1 : 0x7fffffff,
                      in ? ix - 1 : -1, in ? iy - 1 : -1, in ? iz - 1 : -1,
                      q.ok ? q.ix - 1 : 0x7fffffff, q.ok ? q.iy - 1 : 0x7fffffff, q.ok ? q.iz - 1 : 0x7fffffff,
@@ -620,22 +630,35 @@ __device__ __forceinline__ void force_blend_body(const GatherArgs& a, const int*
 
 // ---- particle integrators (steprk.f90:78-105, steppc.f90:64-87,122-151) --
 // All particle arrays are (N_par,3) column-major: entry c*N_par + i.
+// Element i of every array is addressed as the array's (uniform) base plus
+// one 32-bit byte offset 8i, so the loads and stores take the base from
+// scalar registers and one VGPR of address serves all of them (3 N_par
+// doubles stay below 4 GiB: hetddft_create rejects N_par > kMaxNpar).  The kernels then
+// keep to 32 VGPRs and start beside the psi kernels (see kSideVgprs).
+constexpr int kMaxNpar = 1 << 27;  // 3 * 8 * kMaxNpar bytes < 4 GiB
+template <class P>
+__device__ __forceinline__ P& el(P* base, unsigned off) {
+  using B = typename std::conditional<std::is_const<P>::value, const char, char>::type;
+  return *reinterpret_cast<P*>(reinterpret_cast<B*>(base) + off);
+}
+
 __device__ __forceinline__ void part_rk_body(int n3, double* __restrict__ r, double* __restrict__ v,
                           const double* __restrict__ acc, double* __restrict__ qr, double* __restrict__ qv,
                           double* __restrict__ r1_out, double* __restrict__ v1_out,
                           double* __restrict__ a1_out, double ca, double cb, double cc, double dt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n3) return;
-  const double vi = v[i], ai = acc[i];
-  double st = ca * (vi - cb * qr[i]);
-  qr[i] = (qr[i] + 3.0 * st) - cc * vi;
-  if (r1_out) r1_out[i] = r[i];
-  r[i] = r[i] + dt * st;
-  st = ca * (ai - cb * qv[i]);
-  qv[i] = (qv[i] + 3.0 * st) - cc * ai;
-  if (a1_out) a1_out[i] = ai;
-  if (v1_out) v1_out[i] = vi;
-  v[i] = vi + dt * st;
+  const unsigned o = 8u * (unsigned)i;
+  const double vi = el(v, o), ai = el(acc, o);
+  double st = ca * (vi - cb * el(qr, o));
+  el(qr, o) = (el(qr, o) + 3.0 * st) - cc * vi;
+  if (r1_out) el(r1_out, o) = el(r, o);
+  el(r, o) = el(r, o) + dt * st;
+  st = ca * (ai - cb * el(qv, o));
+  el(qv, o) = (el(qv, o) + 3.0 * st) - cc * ai;
+  if (a1_out) el(a1_out, o) = ai;
+  if (v1_out) el(v1_out, o) = vi;
+  el(v, o) = vi + dt * st;
 }
 
 __device__ __forceinline__ void part_predict_body(int n3, double* __restrict__ r, const double* __restrict__ v,
@@ -645,14 +668,15 @@ __device__ __forceinline__ void part_predict_body(int n3, double* __restrict__ r
                                double c4o3dt, double c112) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n3) return;
-  double st = r3[i] + c4o3dt * ((2.0 * v[i] - v1[i]) + 2.0 * v2[i]);
-  r3[i] = r[i];
-  r[i] = st - c112 * pcr[i];
-  pcr[i] = st;
-  st = v3[i] + c4o3dt * ((2.0 * acc[i] - a1[i]) + 2.0 * a2[i]);
-  v3[i] = st - c112 * pcv[i];
-  pcv[i] = st;
-  a2[i] = acc[i];
+  const unsigned o = 8u * (unsigned)i;
+  double st = el(r3, o) + c4o3dt * ((2.0 * el(v, o) - el(v1, o)) + 2.0 * el(v2, o));
+  el(r3, o) = el(r, o);
+  el(r, o) = st - c112 * el(pcr, o);
+  el(pcr, o) = st;
+  st = el(v3, o) + c4o3dt * ((2.0 * el(acc, o) - el(a1, o)) + 2.0 * el(a2, o));
+  el(v3, o) = st - c112 * el(pcv, o);
+  el(pcv, o) = st;
+  el(a2, o) = el(acc, o);
 }
 
 __device__ __forceinline__ void part_correct_body(int n3, double* __restrict__ r, double* __restrict__ v,
@@ -666,16 +690,17 @@ __device__ __forceinline__ void part_correct_body(int n3, double* __restrict__ r
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double er = 0.0, ev = 0.0;
   if (i < n3) {
-    double st = 0.125 * ((9.0 * r3[i] - r2[i]) + dt3 * ((v3[i] + 2.0 * v[i]) - v1[i]));
-    double p = pcr[i] - st;
-    pcr[i] = p;
-    r[i] = st + c9 * p;
+    const unsigned o = 8u * (unsigned)i;
+    double st = 0.125 * ((9.0 * el(r3, o) - el(r2, o)) + dt3 * ((el(v3, o) + 2.0 * el(v, o)) - el(v1, o)));
+    double p = el(pcr, o) - st;
+    el(pcr, o) = p;
+    el(r, o) = st + c9 * p;
     er = fabs(c9 * p);
-    st = 0.125 * ((9.0 * v[i] - v2[i]) + dt3 * ((acc[i] + 2.0 * a1[i]) - a2[i]));
-    p = pcv[i] - st;
-    pcv[i] = p;
-    v3[i] = v[i];
-    v[i] = st + c9 * p;
+    st = 0.125 * ((9.0 * el(v, o) - el(v2, o)) + dt3 * ((el(acc, o) + 2.0 * el(a1, o)) - el(a2, o)));
+    p = el(pcv, o) - st;
+    el(pcv, o) = p;
+    el(v3, o) = el(v, o);
+    el(v, o) = st + c9 * p;
     ev = fabs(c9 * p);
   }
   double s = block_sum(er, sh);
@@ -962,6 +987,10 @@ __global__ void __launch_bounds__(kBlock) k_sorteo(SortArgs a, int np, const dou
 }
 
 // ---- particle-side launches carrying several members -----------------------
+// The histogram and the integrators run on the side stream beside the psi
+// kernels, which leave as few as 32 VGPRs per SIMD lane free (4 waves of
+// 120 in the widest tuned shapes); a side wave of at most 32 VGPRs starts at
+// once instead of queueing until the psi launch drains.
 // blockIdx.y is the member; each member's work uses blockIdx.x and gridDim.x
 // exactly as a launch of its own.  All members share the box and step.
 struct HistTab {
@@ -1045,7 +1074,7 @@ struct PartRkTab {
   double* v1[kMaxMembers];
   double* a1[kMaxMembers];
 };
-__global__ void k_part_rk(int n3, PartRkTab t, double ca, double cb, double cc, double dt) {
+__global__ void __launch_bounds__(kBlock) k_part_rk(int n3, PartRkTab t, double ca, double cb, double cc, double dt) {
   const int m = blockIdx.y;
   part_rk_body(n3, t.r[m], t.v[m], t.acc[m], t.qr[m], t.qv[m], t.r1[m], t.v1[m], t.a1[m], ca, cb, cc, dt);
 }
@@ -1063,7 +1092,7 @@ struct PartPredTab {
   double* pcr[kMaxMembers];
   double* pcv[kMaxMembers];
 };
-__global__ void k_part_predict(int n3, PartPredTab t, double c4o3dt, double c112) {
+__global__ void __launch_bounds__(kBlock) k_part_predict(int n3, PartPredTab t, double c4o3dt, double c112) {
   const int m = blockIdx.y;
   part_predict_body(n3, t.r[m], t.v[m], t.acc[m], t.r3[m], t.v1[m], t.v2[m], t.v3[m], t.a1[m], t.a2[m], t.pcr[m],
                     t.pcv[m], c4o3dt, c112);
@@ -1084,7 +1113,7 @@ struct PartCorrTab {
   double* pcv[kMaxMembers];
   double* part[kMaxMembers];
 };
-__global__ void k_part_correct(int n3, PartCorrTab t, double dt3, double c9) {
+__global__ void __launch_bounds__(kBlock) k_part_correct(int n3, PartCorrTab t, double dt3, double c9) {
   const int m = blockIdx.y;
   part_correct_body(n3, t.r[m], t.v[m], t.acc[m], t.r3[m], t.r2[m], t.v1[m], t.v2[m], t.v3[m], t.a1[m], t.a2[m],
                     t.pcr[m], t.pcv[m], t.part[m], dt3, c9);

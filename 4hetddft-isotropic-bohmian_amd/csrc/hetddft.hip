@@ -2128,7 +2128,8 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
     fail(c, HETDDFT_E_ARG, "grid %dx%dx%d not supported (tstgrid list, nx even)", p->nx, p->ny, p->nz);
     return bad(HETDDFT_E_ARG);
   }
-  if (p->n_par < 0 || p->ironx < 0 || p->mimpur <= 0 || !(p->xmax > 0) || !(p->ymax > 0) || !(p->zmax > 0) ||
+  // particle kernels address 3 N_par doubles with 32-bit byte offsets
+  if (p->n_par < 0 || p->n_par > kMaxNpar || p->ironx < 0 || p->mimpur <= 0 || !(p->xmax > 0) || !(p->ymax > 0) || !(p->zmax > 0) ||
       (p->precision != HETDDFT_FP64 && p->precision != HETDDFT_MIXED) || p->tzsurf == 0.0) {
     fail(c, HETDDFT_E_ARG, "invalid parameters");
     return bad(HETDDFT_E_ARG);

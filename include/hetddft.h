@@ -44,7 +44,7 @@ typedef struct hetddft_ctx hetddft_ctx;
 typedef struct hetddft_params {
   int nx, ny, nz;            /* grid; each in the tstgrid list (tstgrid.f90:8-12), nx even */
   double xmax, ymax, zmax;   /* box is [-xmax, xmax) etc., hx = 2 xmax/nx */
-  int n_par;                 /* number of Bohmian test particles */
+  int n_par;                 /* number of Bohmian test particles, 0 .. 2^27 */
   double mimpur;             /* impurity mass in u */
   int ironx;                 /* smoothing passes of the particle histogram */
   int core_otc;              /* 1: core4='OTC' (alpha_s term on), 0: 'OT ' */

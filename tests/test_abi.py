@@ -20,7 +20,7 @@ def test_default_params_are_orsay_trento(hip):
     assert (p.eps4, p.sigma4, p.core_otc, p.ironx) == (10.22, 2.556, 1, 3)
 
 
-@pytest.mark.parametrize("bad", [dict(nx=7), dict(nx=14), dict(ny=257), dict(n_par=-1),
+@pytest.mark.parametrize("bad", [dict(nx=7), dict(nx=14), dict(ny=257), dict(n_par=-1), dict(n_par=2**27 + 1),
                                  dict(precision=3), dict(mimpur=0.0)])
 def test_create_rejects_bad_parameters(hip, bad):
     p = hip.default_params(**bad)
