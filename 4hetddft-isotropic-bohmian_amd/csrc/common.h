@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstddef>
+#include <type_traits>
 
 namespace hetd {
 
@@ -42,6 +43,18 @@ template <class T>
 __host__ __device__ inline cplx<T> conj(cplx<T> a) { return mk(a.x, -a.y); }
 template <class T>
 __host__ __device__ inline cplx<T> scale(cplx<T> a, T s) { return mk(a.x * s, a.y * s); }
+
+// Element at byte offset off from base.  With base the same for a whole
+// wavefront (a kernel argument plus uniform terms, held in scalar
+// registers) and off a 32-bit per-lane offset, the access compiles to one
+// global load or store of the scalar-base + vector-offset form: no 64-bit
+// address arithmetic per access, and one VGPR of address for every array
+// indexed by the same offset.  Callers keep off below 4 GiB.
+template <class P>
+__device__ __forceinline__ P& el(P* base, unsigned off) {
+  using B = typename std::conditional<std::is_const<P>::value, const char, char>::type;
+  return *reinterpret_cast<P*>(reinterpret_cast<B*>(base) + off);
+}
 
 // Maximum FFT plan depth (radix stages) and supported line length.
 constexpr int kMaxStages = 8;

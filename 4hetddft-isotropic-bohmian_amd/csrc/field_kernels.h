@@ -636,11 +636,6 @@ __device__ __forceinline__ void force_blend_body(const GatherArgs& a, const int*
 // doubles stay below 4 GiB: hetddft_create rejects N_par > kMaxNpar).  The kernels then
 // keep to 32 VGPRs and start beside the psi kernels (see kSideVgprs).
 constexpr int kMaxNpar = 1 << 27;  // 3 * 8 * kMaxNpar bytes < 4 GiB
-template <class P>
-__device__ __forceinline__ P& el(P* base, unsigned off) {
-  using B = typename std::conditional<std::is_const<P>::value, const char, char>::type;
-  return *reinterpret_cast<P*>(reinterpret_cast<B*>(base) + off);
-}
 
 __device__ __forceinline__ void part_rk_body(int n3, double* __restrict__ r, double* __restrict__ v,
                           const double* __restrict__ acc, double* __restrict__ qr, double* __restrict__ qv,

@@ -1470,7 +1470,7 @@ int step_pc_t(Group& G, double dt) {
         t.part[i] = m->part;
       }
       PLAUNCH(pscorr, (k_pc_correct25<T, SH, PF>), grid25m<SH>(c, HETDDFT_TUNE_PCC, G.n), dim3(SH::TX, SH::TY),
-              0, hc, t, 3.0 * dt, c9);
+              0, hc, t, 3.0 * dt, c9, c112);
       return 0;
     });
   }

@@ -267,11 +267,15 @@ __device__ inline c64 ham25(const Ham25& a, const c64* tile, const ZQueue<c64, c
 // z is computed.  Without PF each plane's loads are issued in its own
 // iteration, which needs about 40 fewer VGPRs (one more resident wave per
 // SIMD) and gives the same results.
+// The reference stores the predicted value in pc (pc = Sto1c, :55); here pc
+// keeps the previous corrector's value and the corrector forms Sto1c again
+// from the modified psi it reads, Sto1c = psi + c112 pc (the modifier
+// inverted, equal up to one rounding): one 16 B store per point fewer.
 template <class R, class SH, bool PF>
 __device__ __forceinline__ void pc_predict25(const Ham25& a, const Col& c, const c64* __restrict__ psi,
                                              const R* __restrict__ pot4, c64* old3,
                                              const c64* __restrict__ h1, c64* h2,
-                                             c64* __restrict__ pc, R* __restrict__ den,
+                                             const c64* __restrict__ pc, R* __restrict__ den,
                                              double c4o3dt, double c112) {
   __shared__ c64 tile[SH::LX * SH::LY];
   const Grid& g = a.g;
@@ -313,7 +317,6 @@ __device__ __forceinline__ void pc_predict25(const Ham25& a, const Col& c, const
                         o3.y + c4o3dt * ((2.0 * s4.y - a1.y) + 2.0 * a2.y));
       const c64 np = mk(s1.x - c112 * pci.x, s1.y - c112 * pci.y);
       old3[i] = np;
-      pc[i] = s1;
       h2[i] = s4;
       den[i] = (R)(np.x * np.x + np.y * np.y);
     }
@@ -323,7 +326,8 @@ __device__ __forceinline__ void pc_predict25(const Ham25& a, const Col& c, const
 }
 
 // Hamming corrector + final value + errHe partial sums (steppc.f90:97-112),
-// software-pipelined like the predictor.
+// software-pipelined like the predictor.  pc holds the previous corrector's
+// value; the predicted value is psi + c112 pc (see pc_predict25).
 template <class R, class SH, bool PF>
 __device__ __forceinline__ void pc_correct25(const Ham25& a, const Col& c, const c64* __restrict__ psi,
                                              const R* __restrict__ pot4,
@@ -333,7 +337,7 @@ __device__ __forceinline__ void pc_correct25(const Ham25& a, const Col& c, const
                                              const c64* __restrict__ h2p,
                                              c64* __restrict__ pc, c64* __restrict__ psi_out,
                                              R* __restrict__ den, double* __restrict__ part,
-                                             double dt3, double c9) {
+                                             double dt3, double c9, double c112) {
   __shared__ c64 tile[SH::LX * SH::LY];
   __shared__ double sh[SH::N / 64];
   const Grid& g = a.g;
@@ -375,7 +379,9 @@ __device__ __forceinline__ void pc_correct25(const Ham25& a, const Col& c, const
       const c64 s4 = ham25<SH, std::is_same<R, float>::value>(a, tile, q, c, pv, z);
       const c64 s5 = mk(0.125 * ((9.0 * v3.x - v2.x) + dt3 * ((s4.x + 2.0 * a1.x) - a2.x)),
                         0.125 * ((9.0 * v3.y - v2.y) + dt3 * ((s4.y + 2.0 * a1.y) - a2.y)));
-      const c64 npc = mk(pci.x - s5.x, pci.y - s5.y);
+      const c64 ps = q.centre();
+      const c64 s1 = mk(ps.x + c112 * pci.x, ps.y + c112 * pci.y);
+      const c64 npc = mk(s1.x - s5.x, s1.y - s5.y);
       pc[i] = npc;
       const c64 np = mk(s5.x + c9 * npc.x, s5.y + c9 * npc.y);
       psi_out[i] = np;
@@ -660,7 +666,7 @@ struct PcpTab {
   c64* old3[kMaxMembers];
   const c64* h1[kMaxMembers];
   c64* h2[kMaxMembers];
-  c64* pc[kMaxMembers];
+  const c64* pc[kMaxMembers];
   R* den[kMaxMembers];
 };
 template <class R>
@@ -724,13 +730,14 @@ __global__ void __launch_bounds__(SH::N) k_pc_predict25(Ham25 a, PcpTab<R> t, do
 }
 
 template <class R, class SH, bool PF = true>
-__global__ void __launch_bounds__(SH::N) k_pc_correct25(Ham25 a, PccTab<R> t, double dt3, double c9) {
+__global__ void __launch_bounds__(SH::N) k_pc_correct25(Ham25 a, PccTab<R> t, double dt3, double c9,
+                                                         double c112) {
   const Col c = col_of<SH>(a.g, a.xcd, t.nzb);
   const int m = c.m;
   Ham25 am = a;
   am.timec_z = t.timec[m];
   pc_correct25<R, SH, PF>(am, c, t.psi[m], t.pot4[m], t.o3[m], t.o2[m], t.h1[m], t.h2[m], t.pc[m], t.out[m],
-                          t.den[m], t.part[m], dt3, c9);
+                          t.den[m], t.part[m], dt3, c9, c112);
 }
 
 template <class R, class SH, bool PF = true>
