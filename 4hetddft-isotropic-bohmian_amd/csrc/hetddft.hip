@@ -1365,7 +1365,9 @@ int step_pc_t(Group& G, double dt) {
   // psi predictor + modifier (steppc.f90:36-62), every member in one launch
   {
     const Ham25 hp = ham_args(c, HETDDFT_TUNE_PCP);
-    ProfScope pspred(c, HETDDFT_PROF_PCP, (double)g.n * (16.0 * 8 + 2.0 * sizeof(T)));
+    // reads psi, psi_old(3), two H psi slots, pc and pot4; writes psi (into
+    // slot 3), one H psi slot and den
+    ProfScope pspred(c, HETDDFT_PROF_PCP, (double)g.n * (16.0 * 7 + 2.0 * sizeof(T)));
     with_shape(c->tile_k[HETDDFT_TUNE_PCP], c->pf_k[HETDDFT_TUNE_PCP], [&](auto sh, auto pf) {
       using SH = decltype(sh);
       constexpr bool PF = decltype(pf)::value;
@@ -1447,7 +1449,9 @@ int step_pc_t(Group& G, double dt) {
   // error sums stay on the device until fetch_errors asks for them
   {
     const Ham25 hc = ham_args(c, HETDDFT_TUNE_PCC);
-    ProfScope pscorr(c, HETDDFT_PROF_PCC, (double)g.n * (16.0 * 10 + 2.0 * sizeof(T)));
+    // reads psi, two psi_old slots, two H psi slots, pc and pot4; writes pc,
+    // psi and den
+    ProfScope pscorr(c, HETDDFT_PROF_PCC, (double)g.n * (16.0 * 8 + 2.0 * sizeof(T)));
     with_shape(c->tile_k[HETDDFT_TUNE_PCC], c->pf_k[HETDDFT_TUNE_PCC], [&](auto sh, auto pf) {
       using SH = decltype(sh);
       constexpr bool PF = decltype(pf)::value;
