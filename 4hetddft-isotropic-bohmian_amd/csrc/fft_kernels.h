@@ -1393,6 +1393,27 @@ __device__ inline int p128_ye(int c, int j, int m1) {
 // stage-B lane s of a row holds k1 = 1 7 2 6 3 5 0 4 (partner k1 of 8-k1 at s ^ 1)
 __device__ inline int p128_k1(int s) { return (0x40536271 >> (4 * s)) & 7; }
 
+// The same maps as a lane's base XOR a compile-time constant.  Each map
+// packs fields of 3 bits that do not overlap, so XOR with a constant that
+// touches only the fields indexed by the loop variable moves exactly that
+// index:
+//   p128_xe(yl, s, o)          = (yl*64 + 8 (s^yl) + s)   ^ o
+//   p128_xe(yl, j, k1)         = (72 yl + k1)              ^ 9 j
+//   p128_at(y, k1 + 8 k2)      = (64 y + (k1 ^ sw(y)))     ^ 8 k2
+//   p128_ye(c, q, m)           = (128 c + 8 (q^2c) + (rot(q) ^ 4 (q>>3))) ^ m
+//   p128_ye(c, i0 + 2 i1, m1)  = (128 c + 8 (i0^2c) + (m1 ^ 4 i0))        ^ 17 i1
+// with sw(y) the row swizzle of p128_at.  Byte offsets (element index times
+// the element size, a power of two) keep the form, and the wavefront's
+// exchange area (512 elements from w*512) adds bits above the XOR, so
+// every such access costs one XOR besides the LDS instruction.  Read rows
+// q + 16 t of a column (q < 16) and write rows m1 + 8 r + 64 i0 (m1 < 8) of
+// a column take two bases and immediate offsets.
+__device__ inline int p128_sw(int y) { return (y & 15) ^ ((y & 1) << 3) ^ ((y >> 6) * 12); }
+template <class T>
+__device__ __forceinline__ cplx<T>& lds_at(cplx<T>* base, unsigned byte_off) {
+  return *reinterpret_cast<cplx<T>*>(reinterpret_cast<char*>(base) + byte_off);
+}
+
 // Producer values z[t] = (f[2n], f[2n+1]), n = s + 8t, of one row of 128.
 template <class T, class R>
 __device__ inline void p128_load_row(const XfArgs<R>& a, int f, long long row0, int s, cplx<T>* z) {
@@ -1473,13 +1494,18 @@ __device__ __forceinline__ void xyf128_plane(cplx<T>* P, const cplx<T>* twx, con
   // ---- x: thread (row y, lane s of the row)
   const int yl = lane >> 3, s = lane & 7, y = 8 * w + yl;
   const int k1 = p128_k1(s);
+  constexpr unsigned E = sizeof(cplx<T>);
   bfly<T, false, 8>(v);  // v[o] = Y[j = s][o]
-  cplx<T>* xe = P + w * 512;
+  {
+    // p128_xe(yl, s, o) and p128_xe(yl, j, k1) in the wavefront's area
+    const unsigned bs = (unsigned)(w * 512 + yl * 64 + 8 * (s ^ yl) + s) * E;
+    const unsigned br = (unsigned)(w * 512 + 72 * yl + k1) * E;
 #pragma unroll
-  for (int o = 0; o < 8; ++o) xe[p128_xe(yl, s, o)] = v[o];
-  wave_lds_sync();
+    for (int o = 0; o < 8; ++o) lds_at(P, bs ^ (o * E)) = v[o];
+    wave_lds_sync();
 #pragma unroll
-  for (int j = 0; j < 8; ++j) v[j] = xe[p128_xe(yl, j, k1)];
+    for (int j = 0; j < 8; ++j) v[j] = lds_at(P, br ^ (9 * j * E));
+  }
 #pragma unroll
   for (int j = 1; j < 8; ++j) v[j] = tmul(v[j], twx[2 * j * k1]);
   bfly<T, false, 8>(v);  // v[k2] = X[k1 + 8 k2]
@@ -1501,26 +1527,39 @@ __device__ __forceinline__ void xyf128_plane(cplx<T>* P, const cplx<T>* twx, con
       // k = 0: DC + i Nyquist, both real
       if (k2 == 0 && k1 == 0) zk[0] = mk(za.x + za.y, za.x - za.y);
     }
+    // p128_at(y, k1 + 8 k2)
+    const unsigned bp = (unsigned)(y * 64 + (k1 ^ p128_sw(y))) * E;
 #pragma unroll
-    for (int k2 = 0; k2 < 8; ++k2) P[p128_at(y, k1 + 8 * k2)] = zk[k2];
+    for (int k2 = 0; k2 < 8; ++k2) lds_at(P, bp ^ (8 * k2 * E)) = zk[k2];
   }
   __syncthreads();
   mid();
   // ---- y: thread (column kx = 4w + c, lane q of the column)
   const int c = lane >> 4, q = lane & 15, kx = 4 * w + c;
+  {
+    // p128_at(q + 16 t, kx): rows t >= 4 (y >> 6 = 1) flip the swizzle by 12
+    const unsigned b0 = (unsigned)(q * 64 + (kx ^ p128_sw(q))) * E;
+    const unsigned b1 = (unsigned)(q * 64 + (kx ^ p128_sw(q) ^ 12)) * E;
 #pragma unroll
-  for (int t = 0; t < 8; ++t) v[t] = P[p128_at(q + 16 * t, kx)];
+    for (int t = 0; t < 8; ++t) v[t] = lds_at(P, (t < 4 ? b0 : b1) + 1024 * t * E);
+  }
   __syncthreads();  // every plane read precedes the column exchanges below
   bfly<T, false, 8>(v);  // v[m1]
 #pragma unroll
   for (int m = 1; m < 8; ++m) v[m] = tmul(v[m], tws[q * m]);
   cplx<T>* ye = P + w * 512;
-#pragma unroll
-  for (int m = 0; m < 8; ++m) ye[p128_ye(c, q, m)] = v[m];
-  wave_lds_sync();
   const int m1 = q >> 1, i0 = q & 1;
+  {
+    // p128_ye(c, q, m) and p128_ye(c, i0 + 2 i1, m1) in the wavefront's area
+    const int rot = ((q & 1) << 2) | ((q >> 1) & 3);
+    const unsigned bs = (unsigned)(w * 512 + 128 * c + 8 * (q ^ (2 * c)) + (rot ^ ((q >> 3) << 2))) * E;
+    const unsigned br = (unsigned)(w * 512 + 128 * c + 8 * (i0 ^ (2 * c)) + (m1 ^ (i0 << 2))) * E;
 #pragma unroll
-  for (int i1 = 0; i1 < 8; ++i1) v[i1] = ye[p128_ye(c, i0 + 2 * i1, m1)];
+    for (int m = 0; m < 8; ++m) lds_at(P, bs ^ (m * E)) = v[m];
+    wave_lds_sync();
+#pragma unroll
+    for (int i1 = 0; i1 < 8; ++i1) v[i1] = lds_at(P, br ^ (17 * i1 * E));
+  }
   bfly<T, false, 8>(v);  // B_i0[r]
 #pragma unroll
   for (int r = 0; r < 8; ++r) v[r] = tmul(v[r], tws[8 * r * i0]);
@@ -1594,17 +1633,23 @@ __device__ __forceinline__ void xyi128_plane(const XiArgs<T, R>& a, int f, cplx<
   constexpr int NX = 128;
   const int w = tid >> 6, lane = tid & 63;
   const int c = lane >> 4, q = lane & 15, kx = 4 * w + c;
+  constexpr unsigned E = sizeof(cplx<T>);
   bfly<T, true, 8>(v);
 #pragma unroll
   for (int m = 1; m < 8; ++m) v[m] = tmulc(v[m], twy[q * m]);
-  cplx<T>* ex = P + w * 512;
-#pragma unroll
-  for (int m = 0; m < 8; ++m) ex[p128_ye(c, q, m)] = v[m];
-  wave_lds_sync();
   {
     const int m1 = q >> 1, i0 = q & 1;
+    {
+      // p128_ye(c, q, m) and p128_ye(c, i0 + 2 i1, m1) in the wavefront's area
+      const int rot = ((q & 1) << 2) | ((q >> 1) & 3);
+      const unsigned bs = (unsigned)(w * 512 + 128 * c + 8 * (q ^ (2 * c)) + (rot ^ ((q >> 3) << 2))) * E;
+      const unsigned br = (unsigned)(w * 512 + 128 * c + 8 * (i0 ^ (2 * c)) + (m1 ^ (i0 << 2))) * E;
 #pragma unroll
-    for (int i1 = 0; i1 < 8; ++i1) v[i1] = ex[p128_ye(c, i0 + 2 * i1, m1)];
+      for (int m = 0; m < 8; ++m) lds_at(P, bs ^ (m * E)) = v[m];
+      wave_lds_sync();
+#pragma unroll
+      for (int i1 = 0; i1 < 8; ++i1) v[i1] = lds_at(P, br ^ (17 * i1 * E));
+    }
     bfly<T, true, 8>(v);
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = tmulc(v[r], twy[8 * r * i0]);
@@ -1616,16 +1661,26 @@ __device__ __forceinline__ void xyi128_plane(const XiArgs<T, R>& a, int f, cplx<
       v[r] = mk(fmad(v[r].x, sg, o.x), fmad(v[r].y, sg, o.y));
     }
     __syncthreads();  // every wavefront's column exchange is read before the plane is written
+    {
+      // p128_at(m1 + 8 r + 64 i0, kx): odd r flip the swizzle by 8
+      const int swl = m1 ^ ((m1 & 1) << 3) ^ (12 * i0);
+      const unsigned be = (unsigned)(m1 * 64 + 4096 * i0 + (kx ^ swl)) * E;
+      const unsigned bo = (unsigned)(m1 * 64 + 4096 * i0 + (kx ^ swl ^ 8)) * E;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) P[p128_at(m1 + 8 * r + 64 * i0, kx)] = v[r];
+      for (int r = 0; r < 8; ++r) lds_at(P, ((r & 1) ? bo : be) + 512 * r * E) = v[r];
+    }
   }
   __syncthreads();
   mid();
   // ---- x: thread (row y, lane s), stage-A lanes hold k1 = p128_k1(s)
   const int yl = lane >> 3, s = lane & 7, y = 8 * w + yl;
   const int k1 = p128_k1(s);
+  {
+    // p128_at(y, k1 + 8 k2)
+    const unsigned bp = (unsigned)(y * 64 + (k1 ^ p128_sw(y))) * E;
 #pragma unroll
-  for (int k2 = 0; k2 < 8; ++k2) v[k2] = P[p128_at(y, k1 + 8 * k2)];
+    for (int k2 = 0; k2 < 8; ++k2) v[k2] = lds_at(P, bp ^ (8 * k2 * E));
+  }
   {
     // Hermitian merge X[k] = s + i d, s = Z[k] + conj Z[64-k],
     // d = (Z[k] - conj Z[64-k]) conj(W_128^k); k = 0 merges DC and Nyquist
@@ -1652,12 +1707,16 @@ __device__ __forceinline__ void xyi128_plane(const XiArgs<T, R>& a, int f, cplx<
   bfly<T, true, 8>(v);  // v[j] = G[k1][j]
 #pragma unroll
   for (int j = 1; j < 8; ++j) v[j] = tmulc(v[j], tws[2 * j * k1]);
-  cplx<T>* xe = P + w * 512;
+  {
+    // p128_xe(yl, j, k1) and p128_xe(yl, s, o) in the wavefront's area
+    const unsigned bs = (unsigned)(w * 512 + 72 * yl + k1) * E;
+    const unsigned br = (unsigned)(w * 512 + yl * 64 + 8 * (s ^ yl) + s) * E;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) xe[p128_xe(yl, j, k1)] = v[j];
-  wave_lds_sync();
+    for (int j = 0; j < 8; ++j) lds_at(P, bs ^ (9 * j * E)) = v[j];
+    wave_lds_sync();
 #pragma unroll
-  for (int o = 0; o < 8; ++o) v[o] = xe[p128_xe(yl, s, o)];
+    for (int o = 0; o < 8; ++o) v[o] = lds_at(P, br ^ (o * E));
+  }
   bfly<T, true, 8>(v);  // v[t] = z[s + 8t] = (x[2n], x[2n+1])
   cplx<R>* dst = reinterpret_cast<cplx<R>*>(dst0 + (long long)y * NX);
   const T renor = (T)a.renor;
