@@ -1339,7 +1339,20 @@ __global__ void __launch_bounds__(kPlaneThreads) k_fft_xyi(XiArgs<T, R> a, Grid 
 // XYI runs the same steps in reverse with conjugate twiddles.  Column 0
 // carries DC + i Nyquist as in k_fft_xyf; its unpack is a fourth, small
 // exchange inside wavefront 0.  tw[m] = exp(-2 pi i m/128), m < 128.
-constexpr int kPlane128Lds = (128 * 64 + 128);  // plane + staged twiddles, in complex values
+// Twiddles of the 128 x 128 plane kernels: tw[m] = W^m = exp(-2 pi i m/128),
+// m < 128, and the products the radix stages take, regrouped so that a
+// lane reads its twiddles of one stage at a lane base plus immediate
+// offsets (no index arithmetic), neighbouring lanes reading neighbouring
+// entries (no LDS bank conflicts):
+//   TQ[16 m + q] = W^(q m)    (y transform, first stage; q < 16, m < 8)
+//   TX[8 j + k1] = W^(2 j k1) (x transform, second stage; k1, j < 8)
+//   TR[2 r + i0] = W^(8 r i0) (y transform, radix-2 step; i0 < 2, r < 8)
+// The device table of the regrouped ones holds TQ, TX, TR in that order;
+// the workgroup stages tw and the regrouped table together in LDS.
+constexpr int kP128Regrouped = 128 + 64 + 16;
+constexpr int kP128GQ = 0, kP128GX = 128, kP128GR = 192;               // in the device table
+constexpr int kP128TQ = 128, kP128TX = 128 + 64 + 64, kP128TR = 128 + 192;  // in the LDS copy
+constexpr int kPlane128Lds = (128 * 64 + 128 + kP128Regrouped);  // plane + staged twiddles, in complex values
 // FP32 planes take 66.5 KB of LDS: two workgroups per CU when each thread
 // keeps to 64 VGPRs (8 wavefronts per SIMD)
 template <class T>
@@ -1483,12 +1496,13 @@ __device__ __forceinline__ int opaque_int(int x) {
 
 // r2c in x and y of one 128 x 128 z plane from the producer values v of row
 // y = 8w + lane/8 (complex s + 8t, s = lane%8), written to out (the plane's
-// half spectrum).  twx: twiddles of the x pass and the Hermitian split
-// (global or staged); tws: the staged copy in LDS.  mid() runs after the
-// barrier that completes the plane in LDS.
+// half spectrum).  twx: tw for the Hermitian split and txr: TX for the x
+// pass (global or staged); tws: the staged copies in LDS (tw, then TQ, TX,
+// TR at kP128TQ, kP128TX, kP128TR).  mid() runs after the barrier that
+// completes the plane in LDS.
 template <class T, class Mid>
-__device__ __forceinline__ void xyf128_plane(cplx<T>* P, const cplx<T>* twx, const cplx<T>* tws, int tid,
-                                             cplx<T>* v, cplx<T>* out, Mid mid) {
+__device__ __forceinline__ void xyf128_plane(cplx<T>* P, const cplx<T>* twx, const cplx<T>* txr,
+                                             const cplx<T>* tws, int tid, cplx<T>* v, cplx<T>* out, Mid mid) {
   constexpr int NY = 128, MH = 64;
   const int w = tid >> 6, lane = tid & 63;
   // ---- x: thread (row y, lane s of the row)
@@ -1507,7 +1521,7 @@ __device__ __forceinline__ void xyf128_plane(cplx<T>* P, const cplx<T>* twx, con
     for (int j = 0; j < 8; ++j) v[j] = lds_at(P, br ^ (9 * j * E));
   }
 #pragma unroll
-  for (int j = 1; j < 8; ++j) v[j] = tmul(v[j], twx[2 * j * k1]);
+  for (int j = 1; j < 8; ++j) v[j] = tmul(v[j], txr[8 * j + k1]);
   bfly<T, false, 8>(v);  // v[k2] = X[k1 + 8 k2]
   // Hermitian split Z[k] = E + W_128^k O with X[64 - k] from the partner lane
   {
@@ -1546,7 +1560,7 @@ __device__ __forceinline__ void xyf128_plane(cplx<T>* P, const cplx<T>* twx, con
   __syncthreads();  // every plane read precedes the column exchanges below
   bfly<T, false, 8>(v);  // v[m1]
 #pragma unroll
-  for (int m = 1; m < 8; ++m) v[m] = tmul(v[m], tws[q * m]);
+  for (int m = 1; m < 8; ++m) v[m] = tmul(v[m], tws[kP128TQ + 16 * m + q]);
   cplx<T>* ye = P + w * 512;
   const int m1 = q >> 1, i0 = q & 1;
   {
@@ -1562,7 +1576,7 @@ __device__ __forceinline__ void xyf128_plane(cplx<T>* P, const cplx<T>* twx, con
   }
   bfly<T, false, 8>(v);  // B_i0[r]
 #pragma unroll
-  for (int r = 0; r < 8; ++r) v[r] = tmul(v[r], tws[8 * r * i0]);
+  for (int r = 0; r < 8; ++r) v[r] = tmul(v[r], tws[kP128TR + 2 * r + i0]);
   // ky = m1 + 8 r + 64 i0: v + o (i0 = 0) or o - v (i0 = 1), as one fma
   // with the exact factor +-1
   {
@@ -1598,12 +1612,21 @@ __device__ __forceinline__ void xyf128_plane(cplx<T>* P, const cplx<T>* twx, con
   }
 }
 
+// Stages tw (128 values) and the regrouped table twp (TQ, TX, TR) in LDS.
+template <class T>
+__device__ __forceinline__ void p128_stage_twiddles(cplx<T>* tws, const cplx<T>* __restrict__ tw,
+                                                    const cplx<T>* __restrict__ twp, int tid) {
+  if (tid < 128) tws[tid] = tw[tid];
+  else if (tid < 128 + kP128Regrouped) tws[tid] = twp[tid - 128];
+}
+
 // r2c in x and y of one 128 x 128 z plane.  grid = (nz, nfields), 1024 threads.
-// FP32 reads every twiddle from the LDS copy (13-15% faster than global
-// reads at 128^3); FP64 reads the x-pass twiddles from global memory, which
-// spares it a barrier.
+// twp: the regrouped twiddles (TQ, TX, TR), tw: W^m, m < 128.  FP32 reads
+// every twiddle from the LDS copy (13-15% faster than global reads at
+// 128^3); FP64 reads the x-pass twiddles from global memory, which spares it
+// a barrier.
 template <class T, class R>
-__global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyf128(XfArgs<R> a, Grid g, const cplx<T>* __restrict__ twx,
+__global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyf128(XfArgs<R> a, Grid g, const cplx<T>* __restrict__ twp,
                                                      const cplx<T>* __restrict__ twh,
                                                      const cplx<T>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
@@ -1614,12 +1637,12 @@ __global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyf128(XfArgs<R
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   const int iz = blockIdx.x, f = blockIdx.y;
   const int y = 8 * w + (lane >> 3), s = lane & 7;
-  if (tid < 128) tws[tid] = tw[tid];
+  p128_stage_twiddles<T>(tws, tw, twp, tid);
   cplx<T> v[8];
   p128_load_row<T, R>(a, f, ((long long)iz * NY + y) * NX, s, v);
   if constexpr (kStaged) __syncthreads();  // twiddles staged
   cplx<T>* out = reinterpret_cast<cplx<T>*>(a.out[f]) + (long long)iz * M * NY;
-  xyf128_plane<T>(P, kStaged ? tws : tw, tws, tid, v, out, [] {});
+  xyf128_plane<T>(P, kStaged ? tws : tw, kStaged ? tws + kP128TX : twp + kP128GX, tws, tid, v, out, [] {});
 }
 
 // c2r in y and x of one 128 x 128 z plane from the spectrum columns v
@@ -1628,15 +1651,16 @@ __global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyf128(XfArgs<R
 // rows of dst (the plane's first row).  mid() runs after the barrier that
 // completes the plane in LDS.
 template <class T, class R, class Mid>
-__device__ __forceinline__ void xyi128_plane(const XiArgs<T, R>& a, int f, cplx<T>* P, const cplx<T>* twy,
-                                             const cplx<T>* tws, int tid, cplx<T>* v, R* dst0, Mid mid) {
+__device__ __forceinline__ void xyi128_plane(const XiArgs<T, R>& a, int f, cplx<T>* P, const cplx<T>* tyq,
+                                             const cplx<T>* tyr, const cplx<T>* tws, int tid, cplx<T>* v, R* dst0,
+                                             Mid mid) {
   constexpr int NX = 128;
   const int w = tid >> 6, lane = tid & 63;
   const int c = lane >> 4, q = lane & 15, kx = 4 * w + c;
   constexpr unsigned E = sizeof(cplx<T>);
   bfly<T, true, 8>(v);
 #pragma unroll
-  for (int m = 1; m < 8; ++m) v[m] = tmulc(v[m], twy[q * m]);
+  for (int m = 1; m < 8; ++m) v[m] = tmulc(v[m], tyq[16 * m + q]);
   {
     const int m1 = q >> 1, i0 = q & 1;
     {
@@ -1652,7 +1676,7 @@ __device__ __forceinline__ void xyi128_plane(const XiArgs<T, R>& a, int f, cplx<
     }
     bfly<T, true, 8>(v);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = tmulc(v[r], twy[8 * r * i0]);
+    for (int r = 0; r < 8; ++r) v[r] = tmulc(v[r], tyr[2 * r + i0]);
     // v + o (i0 = 0) or o - v (i0 = 1), as one fma with the exact factor +-1
     const T sg = i0 ? T(-1) : T(1);
 #pragma unroll
@@ -1706,7 +1730,7 @@ __device__ __forceinline__ void xyi128_plane(const XiArgs<T, R>& a, int f, cplx<
   }
   bfly<T, true, 8>(v);  // v[j] = G[k1][j]
 #pragma unroll
-  for (int j = 1; j < 8; ++j) v[j] = tmulc(v[j], tws[2 * j * k1]);
+  for (int j = 1; j < 8; ++j) v[j] = tmulc(v[j], tws[kP128TX + 8 * j + k1]);
   {
     // p128_xe(yl, j, k1) and p128_xe(yl, s, o) in the wavefront's area
     const unsigned bs = (unsigned)(w * 512 + 72 * yl + k1) * E;
@@ -1756,8 +1780,10 @@ __device__ __forceinline__ void p128_pack_col0(const cplx<T>* in, int tid, cplx<
 }
 
 // c2r in y and x of one 128 x 128 z plane, times renor, with the XI epilogues.
+// twp: the regrouped twiddles (TQ, TX, TR), tw: W^m, m < 128; the y pass
+// reads TQ and TR from global memory (it runs before any barrier).
 template <class T, class R>
-__global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyi128(XiArgs<T, R> a, Grid g, const cplx<T>* __restrict__ twx,
+__global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyi128(XiArgs<T, R> a, Grid g, const cplx<T>* __restrict__ twp,
                                                      const cplx<T>* __restrict__ twh,
                                                      const cplx<T>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
@@ -1770,8 +1796,9 @@ __global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyi128(XiArgs<T
   cplx<T> v[8];
   p128_columns<T>(in, tid, v);
   p128_pack_col0<T>(in, tid, v);
-  if (tid < 128) tws[tid] = tw[tid];
-  xyi128_plane<T, R>(a, f, P, tw, tws, tid, v, a.out[f] + (long long)iz * NY * NX, [] {});
+  p128_stage_twiddles<T>(tws, tw, twp, tid);
+  xyi128_plane<T, R>(a, f, P, twp + kP128GQ, twp + kP128GR, tws, tid, v, a.out[f] + (long long)iz * NY * NX,
+                     [] {});
 }
 
 // Persistent form of k_fft_xyi128 for a plane that fills the LDS (FP64: 133
@@ -1784,13 +1811,14 @@ __global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyi128(XiArgs<T
 // grid = (<= items), 1024 threads.
 template <class T, class R>
 __global__ void __launch_bounds__(1024, 4) k_fft_xyi128p(XiArgs<T, R> a, Grid g, int nitems,
-                                                         const cplx<T>* __restrict__ tw) {
+                                                         const cplx<T>* __restrict__ tw,
+                                                         const cplx<T>* __restrict__ twp) {
   extern __shared__ unsigned char smem_raw[];
   constexpr int NX = 128, NY = 128, M = 65;
   cplx<T>* P = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* tws = P + NY * 64;
   const int tid0 = threadIdx.x;
-  if (tid0 < 128) tws[tid0] = tw[tid0];
+  p128_stage_twiddles<T>(tws, tw, twp, tid0);
   auto spec_of = [&](int item) {
     return reinterpret_cast<const cplx<T>*>(a.in[item / g.nz]) + (long long)(item % g.nz) * M * NY;
   };
@@ -1806,7 +1834,7 @@ __global__ void __launch_bounds__(1024, 4) k_fft_xyi128p(XiArgs<T, R> a, Grid g,
     for (int t = 0; t < 8; ++t) v[t] = nxt[t];
     p128_pack_col0<T>(spec_of(it), tid, v);
     const int nit = it + gridDim.x;
-    xyi128_plane<T, R>(a, f, P, tws, tws, tid, v, a.out[f] + (long long)iz * NY * NX, [&] {
+    xyi128_plane<T, R>(a, f, P, tws + kP128TQ, tws + kP128TR, tws, tid, v, a.out[f] + (long long)iz * NY * NX, [&] {
       if (nit < nitems) p128_columns<T>(spec_of(nit), tid, nxt);
     });
     __syncthreads();  // the exchange areas are read before the next item writes them

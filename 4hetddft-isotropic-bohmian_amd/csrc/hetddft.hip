@@ -118,7 +118,7 @@ struct FftKernelsT {
   size_t lds_xy = 0;
   // persistent form of the 128 x 128 FP64 inverse plane kernel (one
   // workgroup per CU), used when a launch holds at least two planes per CU
-  void (*xyip)(XiArgs<T, T>, Grid, int, const cplx<T>*) = nullptr;
+  void (*xyip)(XiArgs<T, T>, Grid, int, const cplx<T>*, const cplx<T>*) = nullptr;
 };
 using FftKernels64 = FftKernelsT<double>;
 using FftKernels32 = FftKernelsT<float>;
@@ -131,6 +131,7 @@ struct FftSet {
   void* twh = nullptr;    // exp(-2 pi i k/nx), k = 0..nx/2
   void* tw_y = nullptr;
   void* tw_z = nullptr;
+  void* tw_p128 = nullptr;  // regrouped twiddles of the 128 x 128 plane kernels (TQ, TX, TR)
   void* spec[kSpectra] = {};
   void* kfvlj4 = nullptr;
   void* kwcgk = nullptr;
@@ -419,6 +420,29 @@ int upload_twiddles(hetddft_ctx* c, void** dst, int n, int count, int denom) {
   return HETDDFT_OK;
 }
 
+// Regrouped twiddles of the 128 x 128 plane kernels (fft_kernels.h):
+// TQ[16m+q] = W^(qm), TX[8j+k1] = W^(2jk1), TR[2r+i0] = W^(8ri0), W^k =
+// exp(-2 pi i k/128) formed as in upload_twiddles (the same values as its
+// table).
+template <class T>
+int upload_twiddles_p128(hetddft_ctx* c, void** dst) {
+  std::vector<cplx<T>> h(kP128Regrouped);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  auto w = [&](int k) {
+    const long double a = -2.0L * pi * (long double)k / 128.0L;
+    return mk((T)cosl(a), (T)sinl(a));
+  };
+  for (int q = 0; q < 16; ++q)
+    for (int m = 0; m < 8; ++m) h[kP128GQ + 16 * m + q] = w(q * m);
+  for (int k1 = 0; k1 < 8; ++k1)
+    for (int j = 0; j < 8; ++j) h[kP128GX + 8 * j + k1] = w(2 * j * k1);
+  for (int i0 = 0; i0 < 2; ++i0)
+    for (int r = 0; r < 8; ++r) h[kP128GR + 2 * r + i0] = w(8 * r * i0);
+  RET(dalloc(c, dst, sizeof(cplx<T>) * kP128Regrouped));
+  HIPCHK(c, hipMemcpy(*dst, h.data(), sizeof(cplx<T>) * kP128Regrouped, hipMemcpyHostToDevice));
+  return HETDDFT_OK;
+}
+
 inline unsigned blocks_for(long long n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
 
 // ---- FFT kernel selection: compile-time lengths for the common grid sizes,
@@ -561,6 +585,7 @@ int fft_alloc_tables(hetddft_ctx* c, FftSet& fs) {
   RET(upload_twiddles<T>(c, &fs.twh, c->g.nx, m + 1, c->g.nx));
   RET(upload_twiddles<T>(c, &fs.tw_y, c->g.ny, c->g.ny, c->g.ny));
   RET(upload_twiddles<T>(c, &fs.tw_z, c->g.nz, c->g.nz, c->g.nz));
+  if (m == 64 && c->g.ny == 128) RET(upload_twiddles_p128<T>(c, &fs.tw_p128));
   FftKernels<T>& k = fft_of<T>(c);
   k = select_fft<T>(m, c->g.ny, c->g.nz);
   fs.lds_x = k.lds_x;
@@ -666,8 +691,10 @@ int conv_batch_in(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin, c
   const double in_bytes = (double)fields.size() * Rb;
   if (K.xyf) {
     ProfScope ps(c, HETDDFT_PROF_XYF, in_bytes + nin * Sb);
+    // the 128 x 128 plane kernels take the regrouped twiddles in the first slot
     PLAUNCH(ps, K.xyf, dim3(g.nz, nin), dim3(kPlaneThreads), K.lds_xy,
-        xa, g, (const cplx<T>*)fs.tw_xh, (const cplx<T>*)fs.twh, (const cplx<T>*)fs.tw_y);
+        xa, g, (const cplx<T>*)(fs.tw_p128 ? fs.tw_p128 : fs.tw_xh), (const cplx<T>*)fs.twh,
+        (const cplx<T>*)fs.tw_y);
     LAUNCHCHK(c);
   } else {
     {
@@ -724,10 +751,12 @@ int conv_batch_in(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin, c
     ProfScope ps(c, HETDDFT_PROF_XYI, nout * (Sb + Rb));
     const int items = g.nz * nout;
     if (K.xyip && items >= 2 * c->cus)
-      PLAUNCH(ps, K.xyip, dim3(c->cus), dim3(kPlaneThreads), K.lds_xy, xi, g, items, (const cplx<T>*)fs.tw_y);
+      PLAUNCH(ps, K.xyip, dim3(c->cus), dim3(kPlaneThreads), K.lds_xy, xi, g, items, (const cplx<T>*)fs.tw_y,
+              (const cplx<T>*)fs.tw_p128);
     else
       PLAUNCH(ps, K.xyi, dim3(g.nz, nout), dim3(kPlaneThreads), K.lds_xy,
-          xi, g, (const cplx<T>*)fs.tw_xh, (const cplx<T>*)fs.twh, (const cplx<T>*)fs.tw_y);
+          xi, g, (const cplx<T>*)(fs.tw_p128 ? fs.tw_p128 : fs.tw_xh), (const cplx<T>*)fs.twh,
+          (const cplx<T>*)fs.tw_y);
     LAUNCHCHK(c);
   } else {
     {
