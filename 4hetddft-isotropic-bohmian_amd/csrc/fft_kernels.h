@@ -33,6 +33,41 @@ constexpr int kThreads = 256;
 constexpr int kRowsX = 16;   // rows (consecutive y) per x-pass workgroup
 constexpr int kLinesY = 16;  // contiguous lines per y-pass workgroup
 
+// Output planes of a convolution job.  A job whose output is read only inside
+// a box (the force convolutions, which the trilinear gather reads at the
+// particles' cell corners) carries a device pointer to that box, (x0, y0, z0,
+// x1, y1, z1) inclusive and 0-based, empty when z1 < z0; the inverse passes
+// then produce its z planes z0..z1 only.  After the z pass the spectra are
+// indexed by real-space z and the x+y transforms of a plane use that plane
+// alone, so the planes produced hold the bits of the full transform.  A null
+// box: every plane.  The box is read on the device, where the particle chain
+// wrote it, so the host needs no copy of it.
+struct ZRange {
+  int lo, hi;
+};
+__device__ inline ZRange z_range(const int* box, int nz) {
+  if (!box) return {0, nz - 1};
+  return {box[2], box[5]};
+}
+// The bytes a pass moves for the planes of its boxed jobs are counted on the
+// device (the host does not know the boxes), into *pbytes when profiling is
+// on (non-null): the first thread of the launch adds plane_bytes per plane
+// of every boxed job, once.
+__device__ inline bool first_thread() {
+  return threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0;
+}
+template <class A>
+__device__ inline void count_boxed_planes(const A& a, int njobs, int nz, long long plane_bytes) {
+  if (!a.pbytes || !first_thread()) return;
+  long long b = 0;
+  for (int j = 0; j < njobs; ++j)
+    if (a.zbox[j]) {
+      const ZRange r = z_range(a.zbox[j], nz);
+      b += (long long)max(r.hi - r.lo + 1, 0) * plane_bytes;
+    }
+  if (b > 0) atomicAdd(a.pbytes, (unsigned long long)b);
+}
+
 // Consecutive ky per z-pass workgroup: 256-byte rows up to nz = 128, half
 // that for longer (or runtime-length) lines to bound the LDS tile.
 template <class T, int N>
@@ -239,8 +274,18 @@ struct XiArgs {
   R* out[kMaxBatch];
   int epi[kMaxBatch];
   double ec[kMaxBatch];
+  const int* zbox[kMaxBatch];  // output planes (z_range), null: all
+  unsigned long long* pbytes;  // bytes of the boxed jobs (profiling), or null
   double renor;
 };
+
+// Whether job f needs plane iz.
+template <class T, class R>
+__device__ inline bool xi_plane_needed(const XiArgs<T, R>& a, int f, int iz, int nz) {
+  if (!a.zbox[f]) return true;
+  const ZRange r = z_range(a.zbox[f], nz);
+  return iz >= r.lo && iz <= r.hi;
+}
 
 template <class T, class R, int MH>
 __global__ void __launch_bounds__(kThreads) k_fft_xi(XiArgs<T, R> a, Grid g, FftPlan pl,
@@ -258,6 +303,8 @@ __global__ void __launch_bounds__(kThreads) k_fft_xi(XiArgs<T, R> a, Grid g, Fft
   const int y0 = blockIdx.x * ROWS;
   const int iz = blockIdx.y;
   const int nrow = (g.ny - y0) < ROWS ? (g.ny - y0) : ROWS;
+  count_boxed_planes(a, gridDim.z, g.nz, (long long)g.ny * (M * sizeof(cplx<T>) + nx * sizeof(R)));
+  if (!xi_plane_needed(a, f, iz, g.nz)) return;
   const cplx<T>* in = reinterpret_cast<const cplx<T>*>(a.in[f]) + (long long)iz * g.mx * g.ny + y0;
   for (int e = threadIdx.x; e < ROWS * M; e += kThreads) {
     const int kx = e / ROWS, r = e - kx * ROWS;
@@ -304,7 +351,19 @@ __global__ void __launch_bounds__(kThreads) k_fft_xi(XiArgs<T, R> a, Grid g, Fft
 template <class T>
 struct YArgs {
   void* buf[kMaxBatch];
+  const int* zbox[kMaxBatch];  // inverse pass: output planes (z_range), null: all
+  unsigned long long* pbytes;  // bytes of the boxed jobs (profiling), or null
 };
+
+// Inverse y pass: whether a workgroup's lines q0 .. q0+nl-1 (line q lies in
+// plane q / mx) reach the planes its job needs.
+template <class T>
+__device__ inline bool y_lines_needed(const YArgs<T>& a, int f, const Grid& g, int q0, int nl, int n) {
+  count_boxed_planes(a, gridDim.z, g.nz, 2LL * g.mx * n * (long long)sizeof(cplx<T>));
+  if (!a.zbox[f]) return true;
+  const ZRange r = z_range(a.zbox[f], g.nz);
+  return q0 / g.mx <= r.hi && (q0 + nl - 1) / g.mx >= r.lo;
+}
 
 template <class T, bool INV, int N>
 __global__ void __launch_bounds__(kThreads) k_fft_y(YArgs<T> a, Grid g, FftPlan pl, const cplx<T>* __restrict__ tw) {
@@ -317,6 +376,7 @@ __global__ void __launch_bounds__(kThreads) k_fft_y(YArgs<T> a, Grid g, FftPlan 
   const int nlines = g.mx * g.nz;
   const int q0 = blockIdx.x * L;
   const int nl = (nlines - q0) < L ? (nlines - q0) : L;
+  if (INV && !y_lines_needed(a, blockIdx.z, g, q0, nl, n)) return;
   cplx<T>* base = reinterpret_cast<cplx<T>*>(a.buf[blockIdx.z]) + (long long)q0 * n;
   for (int e = threadIdx.x; e < L * n; e += kThreads) {
     const int l = e / n, j = e - l * n;
@@ -340,6 +400,8 @@ struct ZArgs {
   const T* kern[kMaxJobs];
   T ksign[kMaxJobs];
   void* out[kMaxJobs];
+  const int* zbox[kMaxJobs];   // output planes (z_range), null: all
+  unsigned long long* pbytes;  // bytes written for the boxed jobs (profiling), or null
 };
 
 template <class T, int N, int TY>
@@ -378,9 +440,13 @@ __global__ void __launch_bounds__(kThreads) k_fft_zconv(ZArgs<T> a, Grid g, FftP
   cplx<T>* other = (F == b0) ? b1 : b0;
   const cplx<T>* Y = fft_tile<T, true, N, TY>(F, other, ls, pl, tw);
   cplx<T>* dst = reinterpret_cast<cplx<T>*>(a.out[jb]) + off;
+  // a boxed job: its planes only
+  const ZRange zr = z_range(a.zbox[jb], n);
+  const unsigned span = (unsigned)(zr.hi - zr.lo);
+  count_boxed_planes(a, gridDim.z, n, pstride * (long long)sizeof(cplx<T>));
   for (int e = threadIdx.x; e < TY * n; e += kThreads) {
     const int j = e / TY, l = e - j * TY;
-    if (l < nty) dst[(long long)j * pstride + l] = Y[l * ls + j];
+    if (l < nty && (unsigned)(j - zr.lo) <= span) dst[(long long)j * pstride + l] = Y[l * ls + j];
   }
 }
 
@@ -436,6 +502,7 @@ __global__ void __launch_bounds__(kThreads) k_fft_zpipe(ZArgs<T> a, int njobs, G
     }
   };
 
+  count_boxed_planes(a, njobs, N, pstride * (long long)sizeof(cplx<T>));
   int w = blockIdx.x;
   if (w < total) fetch(w);
   for (; w < total; w += gridDim.x) {
@@ -463,12 +530,25 @@ __global__ void __launch_bounds__(kThreads) k_fft_zpipe(ZArgs<T> a, int njobs, G
     }
     __syncthreads();
     stages_ip<T, true, N, 1, N, TY>(tile, ls, tw);
+    const int* zb = a.zbox[jb];
     if (ky < g.ny) {
       cplx<T>* dst = reinterpret_cast<cplx<T>*>(a.out[jb]) + (long long)kx * g.ny + ky;
+      if (!zb) {
 #pragma unroll
-      for (int q = 0; q < PER; ++q)
-        if (FULL || j0 + q * RSTEP < N) dst[(long long)(j0 + q * RSTEP) * pstride] = tile[l * ls + j0 + q * RSTEP];
+        for (int q = 0; q < PER; ++q)
+          if (FULL || j0 + q * RSTEP < N) dst[(long long)(j0 + q * RSTEP) * pstride] = tile[l * ls + j0 + q * RSTEP];
+      } else {
+        // a boxed job: its planes only
+        const ZRange zr = z_range(zb, N);
+        const unsigned span = (unsigned)(zr.hi - zr.lo);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          const int kz = j0 + q * RSTEP;
+          if ((FULL || kz < N) && (unsigned)(kz - zr.lo) <= span) dst[(long long)kz * pstride] = tile[l * ls + kz];
+        }
+      }
     }
+
     __syncthreads();
   }
 }
@@ -565,6 +645,7 @@ __global__ void __launch_bounds__(kThreads) k_fft_zreg(ZArgs<T> a, int njobs, Gr
     }
   };
 
+  count_boxed_planes(a, njobs, N, pstride * (long long)sizeof(cplx<T>));
   int w = blockIdx.x;
   if (w < total) load_a(decode(w));
   __syncthreads();  // twiddles
@@ -625,8 +706,9 @@ __global__ void __launch_bounds__(kThreads) k_fft_zreg(ZArgs<T> a, int njobs, Gr
       }
     }
     __syncthreads();
-    // D: twiddles + inverse radix R1 -> HBM
+    // D: twiddles + inverse radix R1 -> HBM (a boxed job: its planes only)
     cplx<T>* dst = reinterpret_cast<cplx<T>*>(a.out[it.jb]) + (long long)it.kx * g.ny + it.ky0;
+    const int* zb = a.zbox[it.jb];
 #pragma unroll
     for (int q = 0; q < PA; ++q) {
       const int b = tid + q * kThreads;
@@ -639,11 +721,21 @@ __global__ void __launch_bounds__(kThreads) k_fft_zreg(ZArgs<T> a, int njobs, Gr
         for (int t = 1; t < R1; ++t) v[t] = tmulc(v[t], tw[t * j]);
         bfly<T, true, R1>(v);
         if (it.ky0 + l < g.ny) {
+          if (!zb) {
 #pragma unroll
-          for (int t = 0; t < R1; ++t) dst[(long long)(j + R2 * t) * pstride + l] = v[t];
+            for (int t = 0; t < R1; ++t) dst[(long long)(j + R2 * t) * pstride + l] = v[t];
+          } else {
+            const ZRange zr = z_range(zb, N);
+            const unsigned span = (unsigned)(zr.hi - zr.lo);
+            const int jl = j - zr.lo;
+#pragma unroll
+            for (int t = 0; t < R1; ++t)
+              if ((unsigned)(jl + R2 * t) <= span) dst[(long long)(j + R2 * t) * pstride + l] = v[t];
+          }
         }
       }
     }
+
     __syncthreads();  // stage-D reads done before the next item's stage-A writes
   }
 }
@@ -680,9 +772,10 @@ __global__ void __launch_bounds__(kThreads) k_fft_yreg(YArgs<T> a, Grid g, FftPl
   cplx<T>* tw = tile + TL * LS;
   const int tid = threadIdx.x;
   (void)pl;
-  for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
   const int nlines = g.mx * g.nz;
   const int q0 = blockIdx.x * TL;
+  if (INV && !y_lines_needed(a, blockIdx.z, g, q0, min(TL, nlines - q0), N)) return;
+  for (int i = tid; i < N; i += kThreads) tw[i] = twg[i];
   cplx<T>* base = reinterpret_cast<cplx<T>*>(a.buf[blockIdx.z]) + (long long)q0 * N;
   // A: loads of every line of the workgroup, radix R1, to LDS
   cplx<T> v[PA][R1];
@@ -919,6 +1012,8 @@ __global__ void __launch_bounds__(kThreads) k_fft_xireg(XiArgs<T, R> a, Grid g, 
   const int y0 = blockIdx.x * ROWS;
   const int iz = blockIdx.y;
   const int nrow = (g.ny - y0) < ROWS ? (g.ny - y0) : ROWS;
+  count_boxed_planes(a, gridDim.z, g.nz, (long long)g.ny * (M * sizeof(cplx<T>) + NX * sizeof(R)));
+  if (!xi_plane_needed(a, f, iz, g.nz)) return;
   const cplx<T>* in = reinterpret_cast<const cplx<T>*>(a.in[f]) + (long long)iz * g.mx * g.ny + y0;
   {
     // transposed read: every load of the thread is issued before the first
@@ -1251,6 +1346,8 @@ __global__ void __launch_bounds__(kPlaneThreads) k_fft_xyi(XiArgs<T, R> a, Grid 
   const T* Pr = reinterpret_cast<const T*>(smem_raw);
   const int iz = blockIdx.x;
   const int f = blockIdx.y;
+  count_boxed_planes(a, gridDim.y, g.nz, (long long)M * NY * sizeof(cplx<T>) + (long long)NY * NX * sizeof(R));
+  if (!xi_plane_needed(a, f, iz, g.nz)) return;
   const cplx<T>* in = reinterpret_cast<const cplx<T>*>(a.in[f]) + (long long)iz * M * NY;
   // Columns kx = 0 and MH have Hermitian symmetry in ky (their inverse y
   // transforms are real): they travel packed as col0 + i col_MH in column 0,
@@ -1792,6 +1889,8 @@ __global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyi128(XiArgs<T
   cplx<T>* tws = P + NY * 64;
   const int tid = threadIdx.x;
   const int iz = blockIdx.x, f = blockIdx.y;
+  count_boxed_planes(a, gridDim.y, g.nz, (long long)M * NY * sizeof(cplx<T>) + (long long)NY * NX * sizeof(R));
+  if (!xi_plane_needed(a, f, iz, g.nz)) return;
   const cplx<T>* in = reinterpret_cast<const cplx<T>*>(a.in[f]) + (long long)iz * M * NY;
   cplx<T> v[8];
   p128_columns<T>(in, tid, v);
@@ -1803,14 +1902,14 @@ __global__ void __launch_bounds__(1024, kPlane128Waves<T>) k_fft_xyi128(XiArgs<T
 
 // Persistent form of k_fft_xyi128 for a plane that fills the LDS (FP64: 133
 // KB, one workgroup per CU, so a workgroup's loads, transforms and stores
-// would follow one another).  Workgroups walk the items (plane iz, field f),
-// item = blockIdx.x + k gridDim.x, iz = item % nz, f = item / nz, and load
-// the next item's spectrum columns once the current plane is in LDS, so the
-// loads are in flight during its x pass and stores (28 instead of 30 us per
-// 4-field launch at 128^3).  Same arithmetic as k_fft_xyi128.
-// grid = (<= items), 1024 threads.
+// would follow one another).  The items are the planes each job needs
+// (z_range), in job order; workgroups walk them, item = blockIdx.x + k
+// gridDim.x, and load the next item's spectrum columns once the current plane
+// is in LDS, so the loads are in flight during its x pass and stores (28
+// instead of 30 us per 4-field launch at 128^3).  Same arithmetic as
+// k_fft_xyi128.  grid = (a few per CU), 1024 threads.
 template <class T, class R>
-__global__ void __launch_bounds__(1024, 4) k_fft_xyi128p(XiArgs<T, R> a, Grid g, int nitems,
+__global__ void __launch_bounds__(1024, 4) k_fft_xyi128p(XiArgs<T, R> a, Grid g, int njobs,
                                                          const cplx<T>* __restrict__ tw,
                                                          const cplx<T>* __restrict__ twp) {
   extern __shared__ unsigned char smem_raw[];
@@ -1819,20 +1918,49 @@ __global__ void __launch_bounds__(1024, 4) k_fft_xyi128p(XiArgs<T, R> a, Grid g,
   cplx<T>* tws = P + NY * 64;
   const int tid0 = threadIdx.x;
   p128_stage_twiddles<T>(tws, tw, twp, tid0);
-  auto spec_of = [&](int item) {
-    return reinterpret_cast<const cplx<T>*>(a.in[item / g.nz]) + (long long)(item % g.nz) * M * NY;
+  int lo[kMaxBatch], cnt[kMaxBatch];
+  int nitems = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxBatch; ++j) {
+    const ZRange r = j < njobs ? z_range(a.zbox[j], g.nz) : ZRange{0, -1};
+    lo[j] = r.lo;
+    cnt[j] = max(r.hi - r.lo + 1, 0);
+    nitems += cnt[j];
+  }
+  // item -> (job f, plane iz)
+  auto decode = [&](int item, int& f, int& iz) {
+    f = 0;
+    iz = 0;
+    bool found = false;
+#pragma unroll
+    for (int j = 0; j < kMaxBatch; ++j) {
+      if (!found && item < cnt[j]) {
+        f = j;
+        iz = lo[j] + item;
+        found = true;
+      } else if (!found) {
+        item -= cnt[j];
+      }
+    }
   };
+  auto spec_of = [&](int item) {
+    int f, iz;
+    decode(item, f, iz);
+    return reinterpret_cast<const cplx<T>*>(a.in[f]) + (long long)iz * M * NY;
+  };
+  count_boxed_planes(a, njobs, g.nz, (long long)M * NY * sizeof(cplx<T>) + (long long)NY * NX * sizeof(R));
   cplx<T> nxt[8];
   int it = blockIdx.x;
   if (it < nitems) p128_columns<T>(spec_of(it), tid0, nxt);
   __syncthreads();  // twiddles staged
   for (; it < nitems; it += gridDim.x) {
     const int tid = opaque_int(tid0);
-    const int iz = it % g.nz, f = it / g.nz;
+    int f, iz;
+    decode(it, f, iz);
     cplx<T> v[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) v[t] = nxt[t];
-    p128_pack_col0<T>(spec_of(it), tid, v);
+    p128_pack_col0<T>(reinterpret_cast<const cplx<T>*>(a.in[f]) + (long long)iz * M * NY, tid, v);
     const int nit = it + gridDim.x;
     xyi128_plane<T, R>(a, f, P, tws + kP128TQ, tws + kP128TR, tws, tid, v, a.out[f] + (long long)iz * NY * NX, [&] {
       if (nit < nitems) p128_columns<T>(spec_of(nit), tid, nxt);

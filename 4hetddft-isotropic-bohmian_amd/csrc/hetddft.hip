@@ -251,6 +251,7 @@ struct hetddft_ctx {
   // which advances the slots exactly as the captured step does.
   bool graph_on = false;
   bool graph_one_stream = false;  // capture with the particle chain on st
+  bool capturing = false;         // st is being captured into a graph
   bool one_stream = false;        // set while such a capture runs
   bool dry = false;
   std::map<std::string, hipGraphExec_t> graphs;
@@ -267,6 +268,9 @@ struct hetddft_ctx {
   double prof_ms[HETDDFT_PROF_N] = {};
   long long prof_n[HETDDFT_PROF_N] = {};
   double prof_bytes[HETDDFT_PROF_N] = {};
+  // bytes per class that kernels count on the device (transform planes of
+  // jobs whose extent only the device knows)
+  unsigned long long* prof_dev = nullptr;
   FftKernels64 fft64;
   FftKernels32 fft32;
 };
@@ -332,6 +336,21 @@ int prof_mark(hetddft_ctx* c) {
       hipExtLaunchKernelGGL(kern, grid, block, lds, c->st, (ps).a, (ps).b, 0u, __VA_ARGS__);   \
     else                                                                                       \
       hipLaunchKernelGGL(kern, grid, block, lds, c->st, __VA_ARGS__);                          \
+  } while (0)
+
+// A launch on st after which `done` (an event, or null) is complete.  The
+// launch signals it itself, as its stop event, so no marker packet sits on st
+// between this kernel and the next; with profiling on (the launch carries the
+// profiling event) or under graph capture, done is recorded after the launch.
+#define PLAUNCH_DONE(ps, done, kern, grid, block, lds, ...)                                    \
+  do {                                                                                         \
+    if (c->dry) break;                                                                         \
+    if ((done) && !(ps).b && !c->capturing) {                                                  \
+      hipExtLaunchKernelGGL(kern, grid, block, lds, c->st, nullptr, (done), 0u, __VA_ARGS__);  \
+    } else {                                                                                   \
+      PLAUNCH(ps, kern, grid, block, lds, __VA_ARGS__);                                        \
+      if (done) HIPCHK(c, hipEventRecord((done), c->st));                                      \
+    }                                                                                          \
   } while (0)
 
 // A launch on the side stream.
@@ -619,6 +638,7 @@ struct ConvJob {
   void* out;        // real field T*
   int epi = XI_STORE;
   double ec = 0.0;
+  const int* zbox = nullptr;  // device box whose z planes the output needs (z_range), null: all
 };
 
 // Five passes over HBM (see fft_kernels.h): inputs i < nin are formed by
@@ -648,7 +668,7 @@ bool batch_buffers(const FftSet& fs, int nin, const std::vector<ConvJob<T>>& job
 
 template <class T>
 int conv_batch_in(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin, const std::vector<ConvJob<T>>& jobs,
-                  void* const* spec_in, void* const* spec_out);
+                  void* const* spec_in, void* const* spec_out, hipEvent_t done = nullptr);
 
 template <class T>
 int conv_batch(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin,
@@ -662,10 +682,11 @@ int conv_batch(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin,
 // One launch per pass for all inputs and jobs given: input i's spectrum in
 // spec_in[i], job k's in spec_out[k] (its input's buffer when in place).
 // fs supplies the twiddle tables.  The jobs of several instances of one grid
-// go into one batch this way.
+// go into one batch this way.  done (or null): an event that completes with
+// the batch's last launch.
 template <class T>
 int conv_batch_in(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin, const std::vector<ConvJob<T>>& jobs,
-                  void* const* spec_in, void* const* spec_out) {
+                  void* const* spec_in, void* const* spec_out, hipEvent_t done) {
   const Grid& g = c->g;
   const int nout = (int)jobs.size();
   if (nin < 1 || nout < 1 || nin > kMaxBatch || nout > kMaxBatch || nout > kMaxJobs)
@@ -713,6 +734,13 @@ int conv_batch_in(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin, c
   XiArgs<T, T> xi{};
   YArgs<T> yb{};
   xi.renor = c->renor;
+  // jobs with a z box: the device counts what the passes move for them
+  // (profiling only); the host counts the rest
+  int nfull = 0;
+  auto pbytes = [&](int cls) { return c->prof_on ? c->prof_dev + cls : (unsigned long long*)nullptr; };
+  za.pbytes = pbytes(HETDDFT_PROF_Z);
+  xi.pbytes = pbytes(K.xyi ? HETDDFT_PROF_XYI : HETDDFT_PROF_XI);
+  yb.pbytes = pbytes(HETDDFT_PROF_YI);
   for (int k = 0; k < nout; ++k) {
     const ConvJob<T>& j = jobs[k];
     za.in[k] = spec_in[j.input];
@@ -720,11 +748,15 @@ int conv_batch_in(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin, c
     za.ksign[k] = (T)j.sign;
     void* os = spec_out[k];
     za.out[k] = os;
+    za.zbox[k] = j.zbox;
     yb.buf[k] = os;
+    yb.zbox[k] = j.zbox;
     xi.in[k] = os;
     xi.out[k] = (T*)j.out;
     xi.epi[k] = j.epi;
     xi.ec[k] = j.ec;
+    xi.zbox[k] = j.zbox;
+    if (!j.zbox) ++nfull;
   }
   // z pass: each distinct input spectrum and k-space kernel read once, each
   // job's output spectrum written once
@@ -734,7 +766,7 @@ int conv_batch_in(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin, c
     if (std::find(kerns.begin(), kerns.end(), j.kern) == kerns.end()) kerns.push_back(j.kern);
     if (std::find(zin.begin(), zin.end(), j.input) == zin.end()) zin.push_back(j.input);
   }
-  const double z_bytes = ((double)zin.size() + nout) * Sb + (double)kerns.size() * Kb;
+  const double z_bytes = ((double)zin.size() + nfull) * Sb + (double)kerns.size() * Kb;
   if (K.zp) {
     // persistent grid: a few workgroups per CU, each walking the work items
     const long long items = (long long)((g.ny + K.typ - 1) / K.typ) * g.mx * nout;
@@ -748,26 +780,26 @@ int conv_batch_in(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin, c
   }
   LAUNCHCHK(c);
   if (K.xyi) {
-    ProfScope ps(c, HETDDFT_PROF_XYI, nout * (Sb + Rb));
+    ProfScope ps(c, HETDDFT_PROF_XYI, nfull * (Sb + Rb));
     const int items = g.nz * nout;
     if (K.xyip && items >= 2 * c->cus)
-      PLAUNCH(ps, K.xyip, dim3(c->cus), dim3(kPlaneThreads), K.lds_xy, xi, g, items, (const cplx<T>*)fs.tw_y,
-              (const cplx<T>*)fs.tw_p128);
+      PLAUNCH_DONE(ps, done, K.xyip, dim3(c->cus), dim3(kPlaneThreads), K.lds_xy, xi, g, nout,
+                   (const cplx<T>*)fs.tw_y, (const cplx<T>*)fs.tw_p128);
     else
-      PLAUNCH(ps, K.xyi, dim3(g.nz, nout), dim3(kPlaneThreads), K.lds_xy,
-          xi, g, (const cplx<T>*)(fs.tw_p128 ? fs.tw_p128 : fs.tw_xh), (const cplx<T>*)fs.twh,
-          (const cplx<T>*)fs.tw_y);
+      PLAUNCH_DONE(ps, done, K.xyi, dim3(g.nz, nout), dim3(kPlaneThreads), K.lds_xy, xi, g,
+                   (const cplx<T>*)(fs.tw_p128 ? fs.tw_p128 : fs.tw_xh), (const cplx<T>*)fs.twh,
+                   (const cplx<T>*)fs.tw_y);
     LAUNCHCHK(c);
   } else {
     {
-      ProfScope ps(c, HETDDFT_PROF_YI, nout * 2.0 * Sb);
+      ProfScope ps(c, HETDDFT_PROF_YI, nfull * 2.0 * Sb);
       PLAUNCH(ps, K.yi, dim3(ylines, 1, nout), dim3(kThreads), fs.lds_y, yb, g, c->pl_y, (const cplx<T>*)fs.tw_y);
     }
     LAUNCHCHK(c);
-    ProfScope ps(c, HETDDFT_PROF_XI, nout * (Sb + Rb));
+    ProfScope ps(c, HETDDFT_PROF_XI, nfull * (Sb + Rb));
     const dim3 gxi((unsigned)((g.ny + kRowsX - 1) / kRowsX), (unsigned)g.nz, (unsigned)nout);
-    PLAUNCH(ps, K.xi, gxi, dim3(kThreads), fs.lds_x, xi, g, c->pl_xh, (const cplx<T>*)fs.tw_xh,
-                                                   (const cplx<T>*)fs.twh);
+    PLAUNCH_DONE(ps, done, K.xi, gxi, dim3(kThreads), fs.lds_x, xi, g, c->pl_xh, (const cplx<T>*)fs.tw_xh,
+                 (const cplx<T>*)fs.twh);
     LAUNCHCHK(c);
   }
   return HETDDFT_OK;
@@ -1034,7 +1066,7 @@ int veval(Group& G, bool leave_side = false) {
   // Each batch: the members' inputs and jobs in one launch per pass, as
   // many members per launch as the argument arrays hold (kMaxBatch inputs and
   // jobs: larger kernel arguments cost every launch several microseconds).
-  auto run_batch = [&](auto&& build) -> int {
+  auto run_batch = [&](auto&& build, hipEvent_t done = nullptr) -> int {
     XfArgs<T> xa{};
     int nin = 0;
     std::vector<ConvJob<T>> jobs;
@@ -1066,7 +1098,7 @@ int veval(Group& G, bool leave_side = false) {
       }
       nin += nm;
     }
-    return conv_batch_in<T>(c, c->fs, xa, nin, jobs, sin.data(), sout.data());
+    return conv_batch_in<T>(c, c->fs, xa, nin, jobs, sin.data(), sout.data(), done);
   };
   // batch 1: denx*Vq -> uimp (potenimp.f90:15-17); den*{wcgk, fvlj4, kalfs}
   //          -> dencg, delj4, falfs = 1 - denalf/den0s (poten.f90:27-50,94-99;
@@ -1106,7 +1138,10 @@ int veval(Group& G, bool leave_side = false) {
     return nin;
   }));
   // batch 3: grad(den).int -> ualphas convolution (term_alfa.f90:65-80);
-  //          -Vq*FFT(d_c den) -> force convolutions (potenimp.f90:39-50)
+  //          -Vq*FFT(d_c den) -> force convolutions (potenimp.f90:39-50).
+  // Its last launch completes ev_fork, which the force gather waits for.
+  const bool gather = c->p.n_par > 0;
+  const bool fork_on_launch = gather && !c->dry && !c->one_stream;
   RET(run_batch([&](hetddft_ctx* m, XfArgs<T>& xa, std::vector<ConvJob<T>>& jobs) {
     int nin = 0;
     if (otc) {
@@ -1120,21 +1155,27 @@ int veval(Group& G, bool leave_side = false) {
       jobs.push_back({0, m->fs.kkalfs, 1.0, F(m, hetddft_ctx::F_UACONV)});
       nin = 1;
     }
+    // The force fields enter only the trilinear gather (interpola3,
+    // potenimp.f90:71), which reads them at the particles' cell corners: the
+    // inverse passes produce the z planes of the gather box (reg, written by
+    // this evaluation's particle chain) and leave the others as they were.
     const int d[3] = {hetddft_ctx::F_DX, hetddft_ctx::F_DY, hetddft_ctx::F_DZ};
     const int o[3] = {hetddft_ctx::F_FCX, hetddft_ctx::F_FCY, hetddft_ctx::F_FCZ};
     for (int k = 0; k < 3; ++k) {
       xa.op[nin] = XF_COPY;
       xa.in[nin][0] = F(m, d[k]);
-      jobs.push_back({nin, m->fs.kvq, -1.0, F(m, o[k])});
+      ConvJob<T> j{nin, m->fs.kvq, -1.0, F(m, o[k])};
+      if (m->p.n_par > 0) j.zbox = m->reg + kRegGather;
+      jobs.push_back(j);
       ++nin;
     }
     return nin;
-  }));
+  }, fork_on_launch ? c->ev_fork : nullptr));
   // ---- forceimp gather (potenimp.f90:52-71), on the side stream beside
   // the pot4 assembly below: the two read disjoint fields and write aimp and
   // pot4 respectively
-  if (c->p.n_par > 0) {
-    RET(fork_side(c));
+  if (gather) {
+    if (fork_on_launch) HIPCHK(c, hipStreamWaitEvent(c->st2, c->ev_fork, 0));
     ForceQTab<T> tq{};
     BlendTab<T> tb{};
     for (int i = 0; i < G.n; ++i) {
@@ -1857,7 +1898,9 @@ int iterate_pc(hetddft_ctx* c, double dt) {
   bool ok = hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal) == hipSuccess;
   if (ok) {
     c->one_stream = c->graph_one_stream;
+    c->capturing = true;
     const int rc = direct();
+    c->capturing = false;
     c->one_stream = false;
     const hipError_t e = hipStreamEndCapture(c->st, &gr);
     ok = rc == HETDDFT_OK && e == hipSuccess && gr != nullptr &&
@@ -2287,6 +2330,7 @@ int hetddft_create(const hetddft_params* p, hetddft_ctx** out) {
   c->part_len = (long long)kDiagSlots * blocks_for(g.n) + 2LL * blocks_for((long long)np3) + 2048;
   want.push_back({(void**)&c->part, (size_t)c->part_len * 8});
   want.push_back({(void**)&c->res, 64 * 8});
+  want.push_back({(void**)&c->prof_dev, HETDDFT_PROF_N * sizeof(unsigned long long)});
   want.push_back({(void**)&c->timec, (size_t)g.nz * 16});
   for (auto& w : want) {
     rc = A(w.first, w.second);
@@ -3078,6 +3122,7 @@ int hetddft_profile(hetddft_ctx* c, int enable) {
     c->prof_n[k] = 0;
     c->prof_bytes[k] = 0.0;
   }
+  HIPCHK(c, hipMemset(c->prof_dev, 0, HETDDFT_PROF_N * sizeof(unsigned long long)));
   c->prof_on = enable != 0;
   return HETDDFT_OK;
 }
@@ -3102,6 +3147,12 @@ int hetddft_profile_read(hetddft_ctx* c, int cls, double* ms_total, long long* l
     c->prof_pool.push_back(r.b);
   }
   c->prof_pending.clear();
+  {
+    unsigned long long dev[HETDDFT_PROF_N];
+    HIPCHK(c, hipMemcpy(dev, c->prof_dev, sizeof dev, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemset(c->prof_dev, 0, sizeof dev));
+    for (int k = 0; k < HETDDFT_PROF_N; ++k) c->prof_bytes[k] += (double)dev[k];
+  }
   if (ms_total) *ms_total = c->prof_ms[cls];
   if (launches) *launches = c->prof_n[cls];
   if (bytes_total) *bytes_total = c->prof_bytes[cls];

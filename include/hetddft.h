@@ -142,7 +142,12 @@ int hetddft_energy(hetddft_ctx* ctx, double out[12]);
 /* Downloads psi, rimp, vimp, aimp (any pointer may be NULL). */
 int hetddft_get_state(hetddft_ctx* ctx, double* psi, double* rimp, double* vimp, double* aimp);
 
-/* Downloads one real field (as double) by id. */
+/* Downloads one real field (as double) by id.  The force convolutions
+ * FXCONV..FZCONV (-V*d_c den, potenimp.f90:39-50) are read only by the
+ * trilinear gather at the particles' cell corners, so with n_par > 0 a
+ * potential evaluation writes only their z planes that hold such corners
+ * (their values there are those of the full transform); the other planes
+ * keep what they held before. */
 #define HETDDFT_F_DEN 0
 #define HETDDFT_F_POT4 1
 #define HETDDFT_F_UIMP 2

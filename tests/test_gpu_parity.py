@@ -179,6 +179,32 @@ def test_force_gather_any_cloud(hip, cloud):
     g.close()
 
 
+def test_force_fields_in_gather_box(hip):
+    """The force convolutions -V*d_c den (potenimp.f90:39-50) are read only
+    by the trilinear gather at the particles' cell corners (interpola3,
+    potenimp.f90:71-116), so a potential evaluation produces their z planes
+    that hold such corners: there they equal the oracle's full transforms,
+    and the other planes keep the zeros of a fresh context."""
+    from oracle import hetddft_oracle as orc
+    wl, o, g = _setup(hip, "t32")
+    print("force fields in the gather box", flush=True)
+    o.start()
+    g.start()
+    p = o.p
+    r = o.rimp
+    a = [(r[0] + p.xmax) / p.hx + 1, (r[1] + p.ymax) / p.hy + 1, (r[2] + p.zmax) / p.hz + 1]
+    ok = ((a[0] >= 1) & (a[0] < p.nx) & (a[1] >= 1) & (a[1] < p.ny) & (a[2] >= 1) & (a[2] < p.nz))
+    iz = np.trunc(a[2][ok]).astype(np.int64)
+    z0, z1 = int(iz.min()) - 1, int(iz.max())   # corners iz-1 and iz, 0-based
+    assert 0 < z0 and z1 < p.nz - 1
+    for name, d in (("fxconv", o.dxden), ("fyconv", o.dyden), ("fzconv", o.dzden)):
+        full = orc.c2r(-(orc.r2c(d) * o.vq), p.shape, p.renor)
+        f = g.get_field(name)
+        assert rel_max(f[z0:z1 + 1], full[z0:z1 + 1]) < TOL, name
+        assert np.all(f[:z0] == 0.0) and np.all(f[z1 + 1:] == 0.0), name
+    g.close()
+
+
 @pytest.mark.parametrize("ironx", [0, 1, 2, 3, 4])
 def test_ironing_passes(hip, ironx):
     """ironx smoothing passes of the particle histogram (ironing.f90): the
