@@ -56,6 +56,27 @@ __device__ __forceinline__ P& el(P* base, unsigned off) {
   return *reinterpret_cast<P*>(reinterpret_cast<B*>(base) + off);
 }
 
+// Streaming access for values a kernel reads or writes once: the load or
+// store carries the non-temporal hint, so the L2 keeps those lines at low
+// priority and they do not push out lines that neighbouring workgroups read
+// again (the halo rows of the stencil kernels).
+__device__ __forceinline__ double ld_stream(const double* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float ld_stream(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ c64 ld_stream(const c64* p) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const d2 v = __builtin_nontemporal_load(reinterpret_cast<const d2*>(p));
+  return mk(v.x, v.y);
+}
+__device__ __forceinline__ void st_stream(double* p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_stream(float* p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_stream(c64* p, c64 v) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  d2 w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<d2*>(p));
+}
+
 // Maximum FFT plan depth (radix stages) and supported line length.
 constexpr int kMaxStages = 8;
 constexpr int kMaxN = 256;

@@ -51,6 +51,17 @@ __device__ inline c64 madd(c64 acc, double w, c64 v) {
   else return vmadd(acc, w, v);
 }
 
+// Store of a real field that the next kernels read.  An FP64 field streams
+// (st_stream: it would not stay in the caches beside the halo rows of the
+// stencil kernels anyway); an FP32 field of the mixed path is stored plainly,
+// so that the next kernels find much of it in the caches (streaming the FP32
+// gradient and pot4 cost 1% of the mixed PC step at 128^3).
+template <class R>
+__device__ __forceinline__ void st_field(R* p, R v) {
+  if constexpr (sizeof(R) == 8) st_stream(p, v);
+  else *p = v;
+}
+
 template <class V, class R>
 __device__ inline V vload(const R* f, int i) {
   return (V)f[i];
@@ -291,11 +302,11 @@ __device__ __forceinline__ void pc_predict25(const Ham25& a, const Col& c, const
     qn = q.next(psi, g, c, z);
     plane_load<SH>(pr, psi, g, c, z);
     const int j = gidx(g, xc, yc, z);
-    pvn = (double)pot4[j];
-    o3n = old3[j];
-    a1n = h1[j];
-    a2n = h2[j];
-    pcn = pc[j];
+    pvn = (double)ld_stream(pot4 + j);
+    o3n = ld_stream(old3 + j);
+    a1n = ld_stream(h1 + j);
+    a2n = ld_stream(h2 + j);
+    pcn = ld_stream(pc + j);
   };
   if constexpr (PF) {
     if (z0 < z1) prefetch(z0);
@@ -316,9 +327,9 @@ __device__ __forceinline__ void pc_predict25(const Ham25& a, const Col& c, const
       const c64 s1 = mk(o3.x + c4o3dt * ((2.0 * s4.x - a1.x) + 2.0 * a2.x),
                         o3.y + c4o3dt * ((2.0 * s4.y - a1.y) + 2.0 * a2.y));
       const c64 np = mk(s1.x - c112 * pci.x, s1.y - c112 * pci.y);
-      old3[i] = np;
-      h2[i] = s4;
-      den[i] = (R)(np.x * np.x + np.y * np.y);
+      st_stream(old3 + i, np);
+      st_stream(h2 + i, s4);
+      st_field(den + i, (R)(np.x * np.x + np.y * np.y));
     }
     __syncthreads();
     q.shift();
@@ -353,12 +364,12 @@ __device__ __forceinline__ void pc_correct25(const Ham25& a, const Col& c, const
     qn = q.next(psi, g, c, z);
     plane_load<SH>(pr, psi, g, c, z);
     const int j = gidx(g, xc, yc, z);
-    pvn = (double)pot4[j];
-    v3n = o3p[j];
-    v2n = o2p[j];
-    a1n = h1p[j];
-    a2n = h2p[j];
-    pcn = pc[j];
+    pvn = (double)ld_stream(pot4 + j);
+    v3n = ld_stream(o3p + j);
+    v2n = ld_stream(o2p + j);
+    a1n = ld_stream(h1p + j);
+    a2n = ld_stream(h2p + j);
+    pcn = ld_stream(pc + j);
   };
   if constexpr (PF) {
     if (z0 < z1) prefetch(z0);
@@ -382,10 +393,10 @@ __device__ __forceinline__ void pc_correct25(const Ham25& a, const Col& c, const
       const c64 ps = q.centre();
       const c64 s1 = mk(ps.x + c112 * pci.x, ps.y + c112 * pci.y);
       const c64 npc = mk(s1.x - s5.x, s1.y - s5.y);
-      pc[i] = npc;
+      st_stream(pc + i, npc);
       const c64 np = mk(s5.x + c9 * npc.x, s5.y + c9 * npc.y);
-      psi_out[i] = np;
-      den[i] = (R)(np.x * np.x + np.y * np.y);
+      st_stream(psi_out + i, np);
+      st_field(den + i, (R)(np.x * np.x + np.y * np.y));
       e += hypot(npc.x, npc.y);
     }
     __syncthreads();
@@ -416,8 +427,8 @@ __device__ __forceinline__ void rk_stage25(const Ham25& a, const Col& c, const c
     qn = q.next(psi, g, c, z);
     plane_load<SH>(pr, psi, g, c, z);
     const int j = gidx(g, xc, yc, z);
-    pvn = (double)pot4[j];
-    qin = qacc[j];
+    pvn = (double)ld_stream(pot4 + j);
+    qin = ld_stream(qacc + j);
   };
   if constexpr (PF) {
     if (z0 < z1) prefetch(z0);
@@ -436,12 +447,12 @@ __device__ __forceinline__ void rk_stage25(const Ham25& a, const Col& c, const c
       const int i = gidx(g, c.x, c.y, z);
       const c64 s4 = ham25<SH, std::is_same<R, float>::value>(a, tile, q, c, pv, z);
       const c64 s1 = mk(ca * (s4.x - cb * qi.x), ca * (s4.y - cb * qi.y));
-      qacc[i] = mk((qi.x + 3.0 * s1.x) - cc * s4.x, (qi.y + 3.0 * s1.y) - cc * s4.y);
-      if (h1_out) h1_out[i] = s4;
+      st_stream(qacc + i, mk((qi.x + 3.0 * s1.x) - cc * s4.x, (qi.y + 3.0 * s1.y) - cc * s4.y));
+      if (h1_out) st_stream(h1_out + i, s4);
       const c64 ps = q.centre();
       const c64 np = mk(ps.x + dt * s1.x, ps.y + dt * s1.y);
-      psi_out[i] = np;
-      den[i] = (R)(np.x * np.x + np.y * np.y);
+      st_stream(psi_out + i, np);
+      st_field(den + i, (R)(np.x * np.x + np.y * np.y));
     }
     __syncthreads();
     q.shift();
@@ -483,9 +494,9 @@ __device__ __forceinline__ void grad3_25(const Grid& g, int zc, const Col& c, co
     if (c.in) {
       const int i = gidx(g, c.x, c.y, z);
       constexpr bool FU = std::is_same<R, float>::value;
-      fx[i] = (R)tile_dx<SH, FU>(tile, c, wx);
-      fy[i] = (R)tile_dy<SH, FU>(tile, c, wy);
-      fz[i] = (R)q.template dz<FU>(wz);
+      st_field(fx + i, (R)tile_dx<SH, FU>(tile, c, wx));
+      st_field(fy + i, (R)tile_dy<SH, FU>(tile, c, wy));
+      st_field(fz + i, (R)q.template dz<FU>(wz));
     }
     __syncthreads();
     q.shift();
@@ -593,14 +604,14 @@ __device__ __forceinline__ void pot_final25(const Pot25T<R>& a, const Col& c, co
         fyn[k] = falfs[j];
         iyn[k] = inty[j];
       }
-      uan = uaconv[gidx(g, xc, yc, z)];
+      uan = ld_stream(uaconv + gidx(g, xc, yc, z));
     }
     const int i = gidx(g, xc, yc, z);
-    dnn = den[i];
-    dcgn = dencg[i];
-    dljn = delj4[i];
-    s1n = sto1c[i];
-    uin = uimp[i];
+    dnn = ld_stream(den + i);
+    dcgn = ld_stream(dencg + i);
+    dljn = ld_stream(delj4 + i);
+    s1n = ld_stream(sto1c + i);
+    uin = ld_stream(uimp + i);
   };
   if constexpr (PF) {
     if (z0 < z1) prefetch(z0);
@@ -644,7 +655,7 @@ __device__ __forceinline__ void pot_final25(const Pot25T<R>& a, const Col& c, co
         const A ual = (((ua + s4) + s5) + s6) * a.h2o2m4 * a.alphas;
         p4 = p4 + ual;
       }
-      pot4[i] = (R)((p4 + ui) + penalty(dn, a.C, a.beta, a.den_m));
+      st_field(pot4 + i, (R)((p4 + ui) + penalty(dn, a.C, a.beta, a.den_m)));
     }
     __syncthreads();
     if (a.otc) {
