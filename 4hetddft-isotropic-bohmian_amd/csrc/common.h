@@ -67,6 +67,11 @@ __device__ __forceinline__ c64 ld_stream(const c64* p) {
   const d2 v = __builtin_nontemporal_load(reinterpret_cast<const d2*>(p));
   return mk(v.x, v.y);
 }
+__device__ __forceinline__ c32 ld_stream(const c32* p) {
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  const f2 v = __builtin_nontemporal_load(reinterpret_cast<const f2*>(p));
+  return mk(v.x, v.y);
+}
 __device__ __forceinline__ void st_stream(double* p, double v) { __builtin_nontemporal_store(v, p); }
 __device__ __forceinline__ void st_stream(float* p, float v) { __builtin_nontemporal_store(v, p); }
 __device__ __forceinline__ void st_stream(c64* p, c64 v) {

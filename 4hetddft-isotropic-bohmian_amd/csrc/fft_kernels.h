@@ -1860,7 +1860,7 @@ __device__ __forceinline__ void p128_columns(const cplx<T>* in, int tid, cplx<T>
   constexpr int NY = 128, MH = 64;
   const int lane = tid & 63, kx = 4 * (tid >> 6) + (lane >> 4), q = lane & 15;
 #pragma unroll
-  for (int t = 0; t < 8; ++t) v[t] = in[kx * NY + q + 16 * t];
+  for (int t = 0; t < 8; ++t) v[t] = ld_stream(in + kx * NY + q + 16 * t);
 }
 template <class T>
 __device__ __forceinline__ void p128_pack_col0(const cplx<T>* in, int tid, cplx<T>* v) {
@@ -1870,7 +1870,7 @@ __device__ __forceinline__ void p128_pack_col0(const cplx<T>* in, int tid, cplx<
     // columns 0 and MH (real inverse transforms) travel packed as col0 + i col_MH
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-      const cplx<T> u = in[MH * NY + q + 16 * t];
+      const cplx<T> u = ld_stream(in + MH * NY + q + 16 * t);
       v[t] = mk(v[t].x - u.y, v[t].y + u.x);
     }
   }
