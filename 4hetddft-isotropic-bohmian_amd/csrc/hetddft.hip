@@ -3280,9 +3280,14 @@ int hetddft_autotune(hetddft_ctx* c, double dt, int steps, hetddft_tuning* out) 
         }
       RET(apply_tuning(c, &best));
     }
-    // z transform: tile width x persistent workgroups per CU
+    // z transform: tile width x persistent workgroups per CU; the three
+    // fastest by launch time, then by whole PC step as above
     if (best.fft_z_tile != 0) {
-      double zbest = 1e300;
+      struct ZCand {
+        double ms;
+        int w, o;
+      };
+      std::vector<ZCand> zc;
       for (int w : {8, 16, 32})
         for (int o = 1; o <= 4; ++o) {
           hetddft_tuning t = best;
@@ -3291,12 +3296,24 @@ int hetddft_autotune(hetddft_ctx* c, double dt, int steps, hetddft_tuning* out) 
           if (apply_tuning(c, &t) != HETDDFT_OK) continue;
           double ms[HETDDFT_PROF_N];
           RET(time_classes(c, dt, steps, ms));
-          if (ms[HETDDFT_PROF_Z] > 0.0 && ms[HETDDFT_PROF_Z] < zbest) {
-            zbest = ms[HETDDFT_PROF_Z];
-            best.fft_z_tile = w;
-            best.fft_z_wgs = o;
-          }
+          if (ms[HETDDFT_PROF_Z] > 0.0) zc.push_back({ms[HETDDFT_PROF_Z], w, o});
         }
+      std::sort(zc.begin(), zc.end(), [](const ZCand& a, const ZCand& b) { return a.ms < b.ms; });
+      double cur = 1e300;
+      hetddft_tuning keep = best;
+      for (size_t r = 0; r < zc.size() && r < 3; ++r) {
+        hetddft_tuning t = best;
+        t.fft_z_tile = zc[r].w;
+        t.fft_z_wgs = zc[r].o;
+        RET(apply_tuning(c, &t));
+        double w = 0.0;
+        RET(time_steps(c, dt, 2 * steps, &w));
+        if (w < cur) {
+          cur = w;
+          keep = t;
+        }
+      }
+      best = keep;
       RET(apply_tuning(c, &best));
     }
     // wall time of the chosen shapes
