@@ -245,6 +245,14 @@ struct hetddft_ctx {
   double* jcur = nullptr;      // current density (3 fields, FP64), allocated by the first hetddft_current
   double* hres = nullptr;      // pinned host copy
   double last_err[3] = {0, 0, 0};
+  // Particle means reduced together with the PC error estimates, so that a
+  // host that asks for both every step (DFT4He3d.f90:855-878) waits for the
+  // GPU once per step: means_wanted records that the last fetch was
+  // followed by a particle_means call, means_ready that means_cache holds
+  // the means of the current particle state (any step or state change
+  // clears it).
+  bool means_wanted = false, means_ready = false;
+  double means_cache[9] = {};
   // PC steps replayed from hipGraphs (hetddft_iterate): one graph per host
   // state of the rotating slots (two in steady state), keyed by that state.
   // In a dry pass the step's host logic runs without launching anything,
@@ -1025,6 +1033,7 @@ template <class T>
 int veval(Group& G, bool leave_side = false) {
   hetddft_ctx* c = G.m[0];
   const Grid& g = c->g;
+  for (int i = 0; i < G.n; ++i) G.m[i]->means_ready = false;
   auto F = [&](hetddft_ctx* m, int id) { return (T*)m->fld[id]; };
   const bool otc = c->p.core_otc != 0;
   if (!c->chain_pending) {
@@ -1285,6 +1294,7 @@ Ham25 ham_args(const hetddft_ctx* c, int k) {
 template <class T>
 int step_rk_t(Group& G, double dt) {
   hetddft_ctx* c = G.m[0];
+  for (int i = 0; i < G.n; ++i) G.m[i]->means_ready = false;
   const double s2 = std::sqrt(2.0);
   const double arun[4] = {0.5, 1.0 - 1.0 / s2, 1.0 + 1.0 / s2, 1.0 / 6.0};
   const double brun[4] = {2.0, 1.0, 1.0, 2.0};
@@ -1390,6 +1400,7 @@ int step_rk_t(hetddft_ctx* c, double dt) {
 template <class T>
 int step_pc_t(Group& G, double dt) {
   hetddft_ctx* c = G.m[0];
+  for (int i = 0; i < G.n; ++i) G.m[i]->means_ready = false;
   const double c112 = 112.0 / 121.0, c9 = 9.0 / 121.0, c4o3 = 4.0 / 3.0;
   const Grid& g = c->g;
   const int n3 = 3 * c->p.n_par;
@@ -1584,11 +1595,47 @@ int flush_errors(hetddft_ctx* c) {
   return HETDDFT_OK;
 }
 
+// The nine particle means (r, v, a per axis) into res[16..24], on st.
+int launch_means(hetddft_ctx* c) {
+  const int np = c->p.n_par;
+  const unsigned nb = std::min(blocks_for(np), 64u);
+  hipLaunchKernelGGL(k_part_sums, dim3(nb, 9), dim3(kBlock), 0, c->st, np, (const double*)c->rimp,
+                     (const double*)c->vimp, (const double*)c->aimp, c->part);
+  LAUNCHCHK(c);
+  hipLaunchKernelGGL(k_finish_sums, dim3(9), dim3(kBlock), 0, c->st, (const double*)c->part, (int)nb, 9,
+                     c->res + 16);
+  LAUNCHCHK(c);
+  return HETDDFT_OK;
+}
+
+// The means in the reference's units from the sums in hres[16..24]
+// (r_cm.f90-style averages; velocities and accelerations back to A/ps).
+void means_from_host(const hetddft_ctx* c, double out[9]) {
+  const int np = c->p.n_par;
+  const double pstok = 1.0 / 7.63822291;
+  for (int k = 0; k < 9; ++k) {
+    double v = c->hres[16 + k] / np;
+    if (k >= 3 && k < 6) v = v * pstok;
+    if (k >= 6) v = v * pstok * pstok;
+    out[k] = v;
+  }
+}
+
 int fetch_errors(hetddft_ctx* c, double err[3]) {
   RET(settle(c));
   RET(flush_errors(c));
-  HIPCHK(c, hipMemcpyAsync(c->hres, c->res, 3 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  // the means too when the last fetch was followed by a particle_means call
+  // (one wait for the GPU instead of two); the error partials are reduced
+  // before the means reuse the partials buffer (same stream)
+  const bool means = c->means_wanted && c->p.n_par > 0;
+  c->means_wanted = false;
+  if (means) RET(launch_means(c));
+  HIPCHK(c, hipMemcpyAsync(c->hres, c->res, (means ? 25 : 3) * sizeof(double), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
+  if (means) {
+    means_from_host(c, c->means_cache);
+    c->means_ready = true;
+  }
   const double c9 = 9.0 / 121.0;
   const double e0 = c9 * c->hres[0] / (double)c->g.n;
   const double e1 = c->hres[1] * 0.3333333333 / (double)c->p.n_par;
@@ -2613,6 +2660,7 @@ int hetddft_set_kernels(hetddft_ctx* c, const double* fvlj4, const double* wcgk,
 
 int hetddft_set_state(hetddft_ctx* c, const double* psi, const double* rimp, const double* vimp) {
   if (!c) return HETDDFT_E_ARG;
+  c->means_ready = false;
   RET(settle(c));
   HIPCHK(c, hipStreamSynchronize(c->st));
   c->chain_pending = false;
@@ -2716,6 +2764,11 @@ int hetddft_iterate(hetddft_ctx* c, double dt, double err[3]) {
 
 int hetddft_particle_means(hetddft_ctx* c, double out[9]) {
   if (!c || !out) return HETDDFT_E_ARG;
+  c->means_wanted = true;
+  if (c->means_ready) {
+    std::memcpy(out, c->means_cache, sizeof c->means_cache);
+    return HETDDFT_OK;
+  }
   RET(settle(c));
   RET(flush_errors(c));
   const int np = c->p.n_par;
@@ -2723,22 +2776,12 @@ int hetddft_particle_means(hetddft_ctx* c, double out[9]) {
     std::memset(out, 0, 9 * sizeof(double));
     return HETDDFT_OK;
   }
-  const unsigned nb = std::min(blocks_for(np), 64u);
-  hipLaunchKernelGGL(k_part_sums, dim3(nb, 9), dim3(kBlock), 0, c->st, np, (const double*)c->rimp,
-                     (const double*)c->vimp, (const double*)c->aimp, c->part);
-  LAUNCHCHK(c);
-  hipLaunchKernelGGL(k_finish_sums, dim3(9), dim3(kBlock), 0, c->st, (const double*)c->part, (int)nb, 9,
-                     c->res + 16);
-  LAUNCHCHK(c);
+  RET(launch_means(c));
   HIPCHK(c, hipMemcpyAsync(c->hres + 16, c->res + 16, 9 * sizeof(double), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
-  const double pstok = 1.0 / 7.63822291;
-  for (int k = 0; k < 9; ++k) {
-    double v = c->hres[16 + k] / np;
-    if (k >= 3 && k < 6) v = v * pstok;
-    if (k >= 6) v = v * pstok * pstok;
-    out[k] = v;
-  }
+  means_from_host(c, c->means_cache);
+  c->means_ready = true;
+  std::memcpy(out, c->means_cache, sizeof c->means_cache);
   return HETDDFT_OK;
 }
 
@@ -2853,6 +2896,7 @@ int hetddft_checkpoint_save(hetddft_ctx* c, void* buf) {
 
 int hetddft_checkpoint_load(hetddft_ctx* c, const void* buf) {
   if (!c || !buf) return HETDDFT_E_ARG;
+  c->means_ready = false;
   if (!c->kernels_set) return fail(c, HETDDFT_E_STATE, "hetddft_set_kernels not called");
   CkptHeader h;
   std::memcpy(&h, buf, sizeof h);
@@ -2965,6 +3009,7 @@ int hetddft_sample_particles(hetddft_ctx* c, const double* denx, const double lo
 
 int hetddft_init_droplet(hetddft_ctx* c, const hetddft_droplet* d) {
   if (!c || !d) return HETDDFT_E_ARG;
+  c->means_ready = false;
   if (!(d->n_he > 0.0) || !(d->rho0 > 0.0) || !(d->r0 > 0.0) || !(d->a > 0.0) || !(d->ripple >= 0.0) ||
       !(d->phase_noise >= 0.0) || !(d->cavity >= 0.0) || (d->cavity > 0.0 && !(d->cavity_w > 0.0)))
     return fail(c, HETDDFT_E_ARG, "init_droplet: invalid droplet parameters");

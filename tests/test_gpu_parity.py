@@ -283,3 +283,43 @@ def test_error_estimates_after_other_calls(hip):
             assert np.array_equal(np.asarray(b.last_errors()), np.asarray(eb))
     a.close()
     b.close()
+
+
+def test_particle_means_with_error_fetch(hip):
+    """A host that fetches the PC error estimates and the particle means every
+    step (DFT4He3d.f90:855-878) gets the means reduced in the same GPU round
+    trip as the estimates; they equal the means of a twin instance that asks
+    only for the means, and a state change in between (set_state + start, a
+    checkpoint load) is never answered from the earlier reduction."""
+    import inputs
+    wl = inputs.WORKLOADS["t24"]
+    p = inputs.params_for(wl)
+    k = hip.make_kernels(p)
+    psi, r, v = inputs.initial_state(wl)
+    a, b = hip.Instance(p), hip.Instance(p)
+    for g in (a, b):
+        g.set_kernels(k)
+        g.set_state(psi, r, v)
+        g.start()
+    print("particle means with error fetch", flush=True)
+    for it in range(3 + 4):
+        a.iterate(wl.deltat, want_errors=True)
+        b.iterate(wl.deltat, want_errors=False)
+        ma, mb = a.particle_means(), b.particle_means()
+        assert np.array_equal(ma, mb), (it, ma, mb)
+    blob = a.checkpoint()
+    r2 = r + 0.25
+    a.set_state(psi, r2, v)
+    a.start()
+    c = hip.Instance(p)
+    c.set_kernels(k)
+    c.set_state(psi, r2, v)
+    c.start()
+    assert np.array_equal(a.particle_means(), c.particle_means())
+    a.iterate(wl.deltat, want_errors=True)
+    a.particle_means()
+    a.restore(blob)
+    assert np.array_equal(a.particle_means(), b.particle_means())
+    for g in (a, b, c):
+        g.close()
+
