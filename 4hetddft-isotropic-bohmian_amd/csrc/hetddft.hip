@@ -1412,67 +1412,85 @@ int step_pc_t(Group& G, double dt) {
   const bool lean = c->lean && !c->one_stream;
   // Particle predictor (steppc.f90:64-87), then the density chain of the
   // predicted positions on the side stream, beside the psi predictor.
-  {
-    PartPredTab pt{};
-    for (int i = 0; i < G.n; ++i) {
-      hetddft_ctx* m = G.m[i];
-      m->err_pending = false;  // the previous step's estimates were not asked for
-      const int* R = m->ioldr;
-      const int* V = m->ioldv;
-      int* A = m->iolda;
-      pt.r[i] = m->rimp;
-      pt.v[i] = m->vimp;
-      pt.acc[i] = m->aimp;
-      pt.r3[i] = m->rold[R[2]];
-      pt.v1[i] = m->vold[V[0]];
-      pt.v2[i] = m->vold[V[1]];
-      pt.v3[i] = m->vold[V[2]];
-      pt.a1[i] = m->aold[A[0]];
-      pt.a2[i] = m->aold[A[1]];
-      pt.pcr[i] = m->pcr;
-      pt.pcv[i] = m->pcv;
-      std::swap(A[0], A[1]);
-    }
-    if (n3 > 0) {
-      if (lean)
-        S2LAUNCH(k_part_predict, dim3(blocks_for(n3), G.n), dim3(kBlock), 0, n3, pt, c4o3 * dt, c112);
-      else
-        SLAUNCH(k_part_predict, dim3(blocks_for(n3), G.n), dim3(kBlock), 0, n3, pt, c4o3 * dt, c112);
-      LAUNCHCHK(c);
-    }
-  }
-  if (!lean) RET(fork_side(c));
-  RET(issue_chain<T>(G));
-  // psi predictor + modifier (steppc.f90:36-62), every member in one launch
-  {
-    const Ham25 hp = ham_args(c, HETDDFT_TUNE_PCP);
-    // reads psi, psi_old(3), two H psi slots, pc and pot4; writes psi (into
-    // slot 3), one H psi slot and den
-    ProfScope pspred(c, HETDDFT_PROF_PCP, (double)g.n * (16.0 * 7 + 2.0 * sizeof(T)));
-    with_shape(c->tile_k[HETDDFT_TUNE_PCP], c->pf_k[HETDDFT_TUNE_PCP], [&](auto sh, auto pf) {
-      using SH = decltype(sh);
-      constexpr bool PF = decltype(pf)::value;
-      PcpTab<T> t{};
-      t.nzb = (int)grid25<SH>(c, HETDDFT_TUNE_PCP).z;
+  auto particle_side = [&]() -> int {
+    {
+      PartPredTab pt{};
       for (int i = 0; i < G.n; ++i) {
         hetddft_ctx* m = G.m[i];
-        const int* P = m->ioldp;
-        const int* H = m->ioldh;
-        t.timec[i] = m->timec;
-        t.psi[i] = m->psi;
-        t.pot4[i] = (const T*)m->fld[hetddft_ctx::F_POT4];
-        t.old3[i] = m->old[P[2]];
-        t.h1[i] = m->hold[H[0]];
-        t.h2[i] = m->hold[H[1]];
-        t.pc[i] = m->pc;
-        t.den[i] = (T*)m->fld[hetddft_ctx::F_DEN];
+        m->err_pending = false;  // the previous step's estimates were not asked for
+        const int* R = m->ioldr;
+        const int* V = m->ioldv;
+        int* A = m->iolda;
+        pt.r[i] = m->rimp;
+        pt.v[i] = m->vimp;
+        pt.acc[i] = m->aimp;
+        pt.r3[i] = m->rold[R[2]];
+        pt.v1[i] = m->vold[V[0]];
+        pt.v2[i] = m->vold[V[1]];
+        pt.v3[i] = m->vold[V[2]];
+        pt.a1[i] = m->aold[A[0]];
+        pt.a2[i] = m->aold[A[1]];
+        pt.pcr[i] = m->pcr;
+        pt.pcv[i] = m->pcv;
+        std::swap(A[0], A[1]);
       }
-      PLAUNCH(pspred, (k_pc_predict25<T, SH, PF>), grid25m<SH>(c, HETDDFT_TUNE_PCP, G.n), dim3(SH::TX, SH::TY),
-              0, hp, t, c4o3 * dt, c112);
-      return 0;
-    });
+      if (n3 > 0) {
+        if (lean)
+          S2LAUNCH(k_part_predict, dim3(blocks_for(n3), G.n), dim3(kBlock), 0, n3, pt, c4o3 * dt, c112);
+        else
+          SLAUNCH(k_part_predict, dim3(blocks_for(n3), G.n), dim3(kBlock), 0, n3, pt, c4o3 * dt, c112);
+        LAUNCHCHK(c);
+      }
+    }
+    if (!lean) RET(fork_side(c));
+    RET(issue_chain<T>(G));
+    return HETDDFT_OK;
+  };
+  auto psi_predictor = [&]() -> int {
+    // psi predictor + modifier (steppc.f90:36-62), every member in one launch
+    {
+      const Ham25 hp = ham_args(c, HETDDFT_TUNE_PCP);
+      // reads psi, psi_old(3), two H psi slots, pc and pot4; writes psi (into
+      // slot 3), one H psi slot and den
+      ProfScope pspred(c, HETDDFT_PROF_PCP, (double)g.n * (16.0 * 7 + 2.0 * sizeof(T)));
+      with_shape(c->tile_k[HETDDFT_TUNE_PCP], c->pf_k[HETDDFT_TUNE_PCP], [&](auto sh, auto pf) {
+        using SH = decltype(sh);
+        constexpr bool PF = decltype(pf)::value;
+        PcpTab<T> t{};
+        t.nzb = (int)grid25<SH>(c, HETDDFT_TUNE_PCP).z;
+        for (int i = 0; i < G.n; ++i) {
+          hetddft_ctx* m = G.m[i];
+          const int* P = m->ioldp;
+          const int* H = m->ioldh;
+          t.timec[i] = m->timec;
+          t.psi[i] = m->psi;
+          t.pot4[i] = (const T*)m->fld[hetddft_ctx::F_POT4];
+          t.old3[i] = m->old[P[2]];
+          t.h1[i] = m->hold[H[0]];
+          t.h2[i] = m->hold[H[1]];
+          t.pc[i] = m->pc;
+          t.den[i] = (T*)m->fld[hetddft_ctx::F_DEN];
+        }
+        PLAUNCH(pspred, (k_pc_predict25<T, SH, PF>), grid25m<SH>(c, HETDDFT_TUNE_PCP, G.n), dim3(SH::TX, SH::TY),
+                0, hp, t, c4o3 * dt, c112);
+        return 0;
+      });
+    }
+    LAUNCHCHK(c);
+    return HETDDFT_OK;
+  };
+  // In the lean layout the streams do not wait for each other here, so the
+  // psi predictor goes first: after a host synchronisation (a host that
+  // fetches the error estimates every step) the compute stream then has its
+  // longest kernel queued while the side stream's launches are still being
+  // issued.  The fork of the other layout must follow the particle predictor.
+  if (lean) {
+    RET(psi_predictor());
+    RET(particle_side());
+  } else {
+    RET(particle_side());
+    RET(psi_predictor());
   }
-  LAUNCHCHK(c);
   for (int i = 0; i < G.n; ++i) {
     hetddft_ctx* m = G.m[i];
     int* P = m->ioldp;
