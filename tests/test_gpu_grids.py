@@ -17,8 +17,11 @@ TOL = 1e-10
 
 # (nx, ny, nz): x rows of 256 (half length 128), y lines of 256, z lines of 256,
 # and 128 x 128 planes (the register-resident whole-plane kernels of the
-# 128^3 workloads) with every producer and epilogue of the potential
-GRIDS = [(256, 24, 32), (32, 256, 24), (24, 32, 256), (128, 128, 24)]
+# 128^3 workloads) with every producer and epilogue of the potential; and
+# lengths with no compile-time kernel (x half length 18, y 40, z 30: the
+# run-time radix plans of every pass, including the force convolutions'
+# plane selection in the run-time z and inverse passes)
+GRIDS = [(256, 24, 32), (32, 256, 24), (24, 32, 256), (128, 128, 24), (36, 40, 30)]
 
 
 def _state(nx, ny, nz, h, n_par, seed):
