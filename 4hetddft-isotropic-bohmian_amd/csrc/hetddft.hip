@@ -1626,8 +1626,9 @@ int launch_means(hetddft_ctx* c) {
   return HETDDFT_OK;
 }
 
-// The means in the reference's units from the sums in hres[16..24]
-// (r_cm.f90-style averages; velocities and accelerations back to A/ps).
+// The means written by the reference's driver (DFT4He3d.f90:864-878) from
+// the sums in hres[16..24]: positions in A, velocities in A/ps,
+// accelerations in A/ps^2.
 void means_from_host(const hetddft_ctx* c, double out[9]) {
   const int np = c->p.n_par;
   const double pstok = 1.0 / 7.63822291;
