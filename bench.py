@@ -173,12 +173,13 @@ def run_ensemble(wls, precision, steps, warmup, threads=8, tune=True, executor="
         b.iterate(steps)
         t1 = time.perf_counter()
         b.close()
-        energies = [g.energy()["etot"] for g in insts]
+        full = [g.energy() for g in insts]
+        energies = [e["etot"] for e in full]
         dev = sum(g.device_bytes for g in insts)
         for g in insts:
             g.close()
         return dict(seconds=t1 - t0, instance_steps_per_s=len(insts) * steps / (t1 - t0), energies=energies,
-                    device_bytes=dev, tune_seconds=tune_s)
+                    terms=[_term_scale(e) for e in full], device_bytes=dev, tune_seconds=tune_s)
     for _ in range(warmup):
         for g, dt in zip(insts, dts):
             g.iterate(dt, want_errors=False)
@@ -206,12 +207,20 @@ def run_ensemble(wls, precision, steps, warmup, threads=8, tune=True, executor="
     t1 = time.perf_counter()
     if errors:
         raise errors[0]
-    energies = [g.energy()["etot"] for g in insts]
+    full = [g.energy() for g in insts]
+    energies = [e["etot"] for e in full]
     dev = sum(g.device_bytes for g in insts)
     for g in insts:
         g.close()
     return dict(seconds=t1 - t0, instance_steps_per_s=len(insts) * steps / (t1 - t0), energies=energies,
-                device_bytes=dev, tune_seconds=tune_s)
+                terms=[_term_scale(e) for e in full], device_bytes=dev, tune_seconds=tune_s)
+
+
+def _term_scale(e):
+    """Sum of the magnitudes of the energy terms whose sum is etot
+    (energy.f90:62-124): the scale against which a difference of totals that
+    nearly cancel is measured."""
+    return sum(abs(e[k]) for k in ("ekin4", "elj4", "ealphas", "esolid", "ecor4", "ekinx", "ekinqx", "ehex"))
 
 
 def ensemble_main(args):
@@ -223,6 +232,7 @@ def ensemble_main(args):
     rel = max(rels)
     worst = rels.index(rel)
     abs_err = max(abs(a - b) for a, b in zip(r64["energies"], r32["energies"]))
+    rel_terms = max(abs(a - b) / s for a, b, s in zip(r64["energies"], r32["energies"], r64["terms"]))
     sizes = {}
     for wl in wls:
         sizes[f"{wl.n}^3"] = sizes.get(f"{wl.n}^3", 0) + 1
@@ -242,6 +252,8 @@ def ensemble_main(args):
         "mixed_max_rel_energy_err_instance": {"name": wls[worst].name, "n_he": wls[worst].n_he,
                                               "etot_fp64": r64["energies"][worst]},
         "mixed_max_abs_energy_err_K": abs_err,
+        # the same differences relative to the sum of the terms' magnitudes
+        "mixed_max_energy_err_vs_terms": rel_terms,
         "model_gb_per_instance_step_mixed": mb / 1e9,
         "model_hbm_gbs": mb * r32["instance_steps_per_s"] / 1e9,
         "device_bytes_fp64": r64["device_bytes"], "device_bytes_mixed": r32["device_bytes"],
