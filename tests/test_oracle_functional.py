@@ -335,3 +335,19 @@ def test_denpart_normalisation_is_the_fraction_inside():
     inside = prm.n_par - 50
     assert abs(float(np.sum(o.denx)) * prm.dxyz - inside / prm.n_par) < 1e-14
     assert (o.denx >= 0.0).all()
+
+
+def test_energy_terms_sum_to_total():
+    """bench.py measures the ensemble's mixed-vs-FP64 energy differences
+    against the sum of the magnitudes of the terms whose sum is etot
+    (energy.f90:62-124): those terms add up to etot, and the scale is not
+    smaller than |etot|."""
+    import bench
+    prm, o = _uniform()
+    o.rimp = np.array([[0.3], [0.1], [-0.2]])
+    o.vimp = np.array([[1.0], [0.5], [2.0]])
+    e = o.energy()
+    keys = ("ekin4", "elj4", "ealphas", "esolid", "ecor4", "ekinx", "ekinqx", "ehex")
+    assert abs(sum(e[k] for k in keys) - e["etot"]) <= 1e-12 * bench._term_scale(e)
+    assert bench._term_scale(e) >= abs(e["etot"])
+    assert bench._term_scale(e) == sum(abs(e[k]) for k in keys)
