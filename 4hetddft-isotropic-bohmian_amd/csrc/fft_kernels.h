@@ -1261,16 +1261,17 @@ __device__ inline void load_plane_real(const XfArgs<R>& a, int f, long long src0
   }
 }
 
-// r2c in x and y of one z plane.  grid = (nz, nfields), kPlaneThreads.
-// LDS P[ky][kx], row length M = MH+1 (real rows hold 2*MH values first).
-template <class T, class R, int MH, int NY>
-__global__ void __launch_bounds__(kPlaneThreads) k_fft_xyf(XfArgs<R> a, Grid g, const cplx<T>* __restrict__ twx,
-                                                           const cplx<T>* __restrict__ twh,
-                                                           const cplx<T>* __restrict__ twy) {
+// r2c in x and y of one z plane.  grid = (nz, nfields), THR threads (at
+// least NY).  LDS P[ky][kx], row length M = MH+1 (real rows hold 2*MH values
+// first).
+template <class T, class R, int MH, int NY, int THR>
+__global__ void __launch_bounds__(THR) k_fft_xyf(XfArgs<R> a, Grid g, const cplx<T>* __restrict__ twx,
+                                                 const cplx<T>* __restrict__ twh,
+                                                 const cplx<T>* __restrict__ twy) {
+  static_assert(THR >= NY, "the column-0 unpack takes one thread per ky");
   extern __shared__ unsigned char smem_raw[];
   constexpr int M = MH + 1;
   constexpr int NX = 2 * MH;
-  constexpr int THR = kPlaneThreads;
   cplx<T>* P = reinterpret_cast<cplx<T>*>(smem_raw);
   T* Pr = reinterpret_cast<T*>(smem_raw);
   const int iz = blockIdx.x;
@@ -1333,15 +1334,16 @@ __global__ void __launch_bounds__(kPlaneThreads) k_fft_xyf(XfArgs<R> a, Grid g, 
   }
 }
 
-// c2r in y and x of one z plane, times renor, with the XI epilogues.
-template <class T, class R, int MH, int NY>
-__global__ void __launch_bounds__(kPlaneThreads) k_fft_xyi(XiArgs<T, R> a, Grid g, const cplx<T>* __restrict__ twx,
-                                                           const cplx<T>* __restrict__ twh,
-                                                           const cplx<T>* __restrict__ twy) {
+// c2r in y and x of one z plane, times renor, with the XI epilogues.  THR
+// threads, at least NY (the threads of kx = 0 also read column MH).
+template <class T, class R, int MH, int NY, int THR>
+__global__ void __launch_bounds__(THR) k_fft_xyi(XiArgs<T, R> a, Grid g, const cplx<T>* __restrict__ twx,
+                                                 const cplx<T>* __restrict__ twh,
+                                                 const cplx<T>* __restrict__ twy) {
+  static_assert(THR >= NY, "the first NY elements must be the threads' first elements");
   extern __shared__ unsigned char smem_raw[];
   constexpr int M = MH + 1;
   constexpr int NX = 2 * MH;
-  constexpr int THR = kPlaneThreads;
   cplx<T>* P = reinterpret_cast<cplx<T>*>(smem_raw);
   const T* Pr = reinterpret_cast<const T*>(smem_raw);
   const int iz = blockIdx.x;

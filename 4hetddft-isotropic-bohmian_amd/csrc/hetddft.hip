@@ -116,6 +116,10 @@ struct FftKernelsT {
   void (*xyf)(XfArgs<T>, Grid, const cplx<T>*, const cplx<T>*, const cplx<T>*) = nullptr;
   void (*xyi)(XiArgs<T, T>, Grid, const cplx<T>*, const cplx<T>*, const cplx<T>*) = nullptr;
   size_t lds_xy = 0;
+  // the same generic plane kernels in 512-thread workgroups, for launches of
+  // at least two planes per CU (null for the 128 x 128 kernels)
+  void (*xyf_h)(XfArgs<T>, Grid, const cplx<T>*, const cplx<T>*, const cplx<T>*) = nullptr;
+  void (*xyi_h)(XiArgs<T, T>, Grid, const cplx<T>*, const cplx<T>*, const cplx<T>*) = nullptr;
   // persistent form of the 128 x 128 FP64 inverse plane kernel (one
   // workgroup per CU), used when a launch holds at least two planes per CU
   void (*xyip)(XiArgs<T, T>, Grid, int, const cplx<T>*, const cplx<T>*) = nullptr;
@@ -540,6 +544,14 @@ void pick_z(FftKernels<T>& k, int n) {
   }
 }
 
+// Generic plane kernels run in 1024-thread workgroups while a launch gives a
+// CU fewer than two planes (shorter per-plane latency), and in 512-thread
+// workgroups from two planes per CU on, so that one CU holds more planes at
+// different phases of their load, transform and store.  Both forms compute
+// every value with the same operations in the same order.
+constexpr int kPlaneThreadsHalf = 512;
+inline bool plane_half(int items, int cus) { return items >= 2 * cus; }
+
 template <class T, int MH, int NY>
 void pick_xy(FftKernels<T>& k) {
   constexpr size_t b = plane_bytes<T>(MH, NY);
@@ -549,8 +561,10 @@ void pick_xy(FftKernels<T>& k) {
     k.lds_xy = kPlane128Lds * sizeof(cplx<T>);
     if constexpr (sizeof(T) == 8) k.xyip = k_fft_xyi128p<T, T>;
   } else if constexpr (b <= 160 * 1024) {
-    k.xyf = k_fft_xyf<T, T, MH, NY>;
-    k.xyi = k_fft_xyi<T, T, MH, NY>;
+    k.xyf = k_fft_xyf<T, T, MH, NY, kPlaneThreads>;
+    k.xyi = k_fft_xyi<T, T, MH, NY, kPlaneThreads>;
+    k.xyf_h = k_fft_xyf<T, T, MH, NY, kPlaneThreadsHalf>;
+    k.xyi_h = k_fft_xyi<T, T, MH, NY, kPlaneThreadsHalf>;
     k.lds_xy = b;
   }
 }
@@ -629,6 +643,10 @@ int fft_alloc_tables(hetddft_ctx* c, FftSet& fs) {
   if (k.xyf) {
     HIPCHK(c, hipFuncSetAttribute((const void*)k.xyf, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
     HIPCHK(c, hipFuncSetAttribute((const void*)k.xyi, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    if (k.xyf_h) {
+      HIPCHK(c, hipFuncSetAttribute((const void*)k.xyf_h, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+      HIPCHK(c, hipFuncSetAttribute((const void*)k.xyi_h, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    }
   }
   if (k.xyip) HIPCHK(c, hipFuncSetAttribute((const void*)k.xyip, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   if (k.zp) HIPCHK(c, hipFuncSetAttribute((const void*)k.zp, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
@@ -721,7 +739,8 @@ int conv_batch_in(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin, c
   if (K.xyf) {
     ProfScope ps(c, HETDDFT_PROF_XYF, in_bytes + nin * Sb);
     // the 128 x 128 plane kernels take the regrouped twiddles in the first slot
-    PLAUNCH(ps, K.xyf, dim3(g.nz, nin), dim3(kPlaneThreads), K.lds_xy,
+    const bool half = K.xyf_h && plane_half(g.nz * nin, c->cus);
+    PLAUNCH(ps, half ? K.xyf_h : K.xyf, dim3(g.nz, nin), dim3(half ? kPlaneThreadsHalf : kPlaneThreads), K.lds_xy,
         xa, g, (const cplx<T>*)(fs.tw_p128 ? fs.tw_p128 : fs.tw_xh), (const cplx<T>*)fs.twh,
         (const cplx<T>*)fs.tw_y);
     LAUNCHCHK(c);
@@ -793,6 +812,9 @@ int conv_batch_in(hetddft_ctx* c, FftSet& fs, const XfArgs<T>& xf_in, int nin, c
     if (K.xyip && items >= 2 * c->cus)
       PLAUNCH_DONE(ps, done, K.xyip, dim3(c->cus), dim3(kPlaneThreads), K.lds_xy, xi, g, nout,
                    (const cplx<T>*)fs.tw_y, (const cplx<T>*)fs.tw_p128);
+    else if (K.xyi_h && plane_half(items, c->cus))
+      PLAUNCH_DONE(ps, done, K.xyi_h, dim3(g.nz, nout), dim3(kPlaneThreadsHalf), K.lds_xy, xi, g,
+                   (const cplx<T>*)fs.tw_xh, (const cplx<T>*)fs.twh, (const cplx<T>*)fs.tw_y);
     else
       PLAUNCH_DONE(ps, done, K.xyi, dim3(g.nz, nout), dim3(kPlaneThreads), K.lds_xy, xi, g,
                    (const cplx<T>*)(fs.tw_p128 ? fs.tw_p128 : fs.tw_xh), (const cplx<T>*)fs.twh,
