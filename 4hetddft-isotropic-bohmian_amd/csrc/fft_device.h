@@ -148,6 +148,67 @@ __device__ inline void bfly16(cplx<T>* v) {
   }
 }
 
+// cos and sin of 2 pi m / r for the composite radices r = 6, 10, 12.
+__host__ __device__ constexpr double root_cos(int r, int m) {
+  constexpr double c30 = 0.86602540378443864676372317075294;  // cos(pi/6)
+  constexpr double c36 = 0.80901699437494742410229341718282;  // cos(pi/5)
+  constexpr double c72 = 0.30901699437494742410229341718282;  // cos(2pi/5)
+  constexpr double t12[12] = {1, c30, 0.5, 0, -0.5, -c30, -1, -c30, -0.5, 0, 0.5, c30};
+  constexpr double t10[10] = {1, c36, c72, -c72, -c36, -1, -c36, -c72, c72, c36};
+  return r == 6 ? t12[(2 * m) % 12] : r == 12 ? t12[m % 12] : t10[m % 10];
+}
+__host__ __device__ constexpr double root_sin(int r, int m) {
+  constexpr double c30 = 0.86602540378443864676372317075294;  // sin(pi/3)
+  constexpr double s36 = 0.58778525229247312916870595463907;  // sin(pi/5)
+  constexpr double s72 = 0.95105651629515357211643933337938;  // sin(2pi/5)
+  constexpr double t12[12] = {0, 0.5, c30, 1, c30, 0.5, 0, -0.5, -c30, -1, -c30, -0.5};
+  constexpr double t10[10] = {0, s36, s72, s72, s36, 0, -s36, -s72, -s72, -s36};
+  return r == 6 ? t12[(2 * m) % 12] : r == 12 ? t12[m % 12] : t10[m % 10];
+}
+static_assert(root_cos(10, 3) < 0 && root_sin(12, 4) > 0.86 && root_cos(6, 2) == -0.5, "root tables");
+
+// DFT of length A in registers (A = 2, 3, 4, 5, 8), natural order.
+template <class T, bool INV, int A>
+__device__ inline void small_dft(cplx<T>* v) {
+  static_assert(A == 2 || A == 3 || A == 4 || A == 5 || A == 8, "small DFT length");
+  if constexpr (A == 2) bfly2<T, INV>(v);
+  if constexpr (A == 3) bfly3<T, INV>(v);
+  if constexpr (A == 4) bfly4<T, INV>(v);
+  if constexpr (A == 5) bfly5<T, INV>(v);
+  if constexpr (A == 8) bfly8<T, INV>(v);
+}
+
+// Radix A*B in registers, as bfly16 is 4 x 4: length-A DFTs over n1
+// (elements B n1 + n2), twiddles W_{AB}^(n2 k1), length-B DFTs over n2;
+// X[k1 + A k2] in natural order.  W = exp(-2 pi i/AB), conjugate for INV.
+template <class T, bool INV, int A, int B>
+__device__ inline void bfly_ab(cplx<T>* v) {
+  constexpr int R = A * B;
+  cplx<T> y[B][A];
+#pragma unroll
+  for (int n2 = 0; n2 < B; ++n2) {
+#pragma unroll
+    for (int n1 = 0; n1 < A; ++n1) y[n2][n1] = v[B * n1 + n2];
+    small_dft<T, INV, A>(y[n2]);
+  }
+#pragma unroll
+  for (int n2 = 1; n2 < B; ++n2)
+#pragma unroll
+    for (int k1 = 1; k1 < A; ++k1) {
+      const T wc = T(root_cos(R, n2 * k1)), ws = T(root_sin(R, n2 * k1));
+      y[n2][k1] = tmul(y[n2][k1], mk(wc, INV ? ws : -ws));
+    }
+#pragma unroll
+  for (int k1 = 0; k1 < A; ++k1) {
+    cplx<T> z[B];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) z[n2] = y[n2][k1];
+    small_dft<T, INV, B>(z);
+#pragma unroll
+    for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = z[k2];
+  }
+}
+
 // One Stockham stage of radix R over L lines of length n (line stride ls in
 // LDS).  tw is the length-n table exp(-2 pi i m/n).
 template <class T, bool INV, int R>

@@ -553,15 +553,20 @@ __global__ void __launch_bounds__(kThreads) k_fft_zpipe(ZArgs<T> a, int njobs, G
   }
 }
 
-// Radix-R butterfly on v[0..R) (R = 8 or 16).
+// Radix-R butterfly on v[0..R) (R = 6, 8, 10, 12 or 16).
 template <class T, bool INV, int R>
 __device__ inline void bfly(cplx<T>* v) {
+  static_assert(R == 6 || R == 8 || R == 10 || R == 12 || R == 16, "register radix");
+  if constexpr (R == 6) bfly_ab<T, INV, 2, 3>(v);
   if constexpr (R == 8) bfly8<T, INV>(v);
+  if constexpr (R == 10) bfly_ab<T, INV, 2, 5>(v);
+  if constexpr (R == 12) bfly_ab<T, INV, 4, 3>(v);
   if constexpr (R == 16) bfly16<T, INV>(v);
 }
 
 // Z pass with the transforms held in registers between two LDS exchanges,
-// for lengths N = R1 * R2 (R1, R2 in {8, 16}).  Per work item (TY consecutive
+// for lengths N = R1 * R2 (R1 in {8, 16}; R2 in {8, 16}, or 6, 10, 12 for
+// N = 48, 80, 96).  Per work item (TY consecutive
 // ky of one kx, one job) the four Stockham stages run as:
 //   A  forward radix-R1 stage straight from HBM (rows kz = j + R2 t), written
 //      to LDS in Stockham order (j R1 + t);
@@ -594,7 +599,8 @@ __host__ __device__ constexpr size_t zreg_bytes() {
 template <class T, int N, int TY, int R1, int R2>
 __global__ void __launch_bounds__(kThreads) k_fft_zreg(ZArgs<T> a, int njobs, Grid g,
                                                        const cplx<T>* __restrict__ twg) {
-  static_assert(R1 * R2 == N && (R1 == 8 || R1 == 16) && (R2 == 8 || R2 == 16), "N = R1 * R2");
+  static_assert(R1 * R2 == N && (R1 == 8 || R1 == 16) && (R2 == 6 || R2 == 8 || R2 == 10 || R2 == 12 || R2 == 16),
+                "N = R1 * R2");
   extern __shared__ unsigned char smem_raw[];
   constexpr int ls = N + 1;
   constexpr int NA = TY * R2;  // stage A / D butterflies

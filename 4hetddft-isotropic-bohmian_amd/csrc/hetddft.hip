@@ -518,7 +518,7 @@ void pick_z(FftKernels<T>& k, int n) {
   k.zc = k_fft_zconv<T, N, TY>;
   k.tyz = TY;
   k.lds_z = (N > 0 ? 1 : 2) * tile_bytes<T>(k.tyz, n);
-  if constexpr (N == 64 || N == 128 || N == 256) {
+  if constexpr (N == 48 || N == 64 || N == 80 || N == 96 || N == 128 || N == 256) {
     // register-resident z pass (two LDS exchanges per transform pair), at
     // ky-tile widths 8, 16 and 32 for the autotuner; 32 by default
     constexpr int R1 = N == 256 ? 16 : 8;
