@@ -1160,8 +1160,7 @@ __device__ inline void stage_g(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
       if constexpr (R == 3) bfly3<T, INV>(v[q]);
       if constexpr (R == 4) bfly4<T, INV>(v[q]);
       if constexpr (R == 5) bfly5<T, INV>(v[q]);
-      if constexpr (R == 8) bfly8<T, INV>(v[q]);
-      if constexpr (R == 16) bfly16<T, INV>(v[q]);
+      if constexpr (R == 6 || R == 8 || R == 10 || R == 12 || R == 16) bfly<T, INV, R>(v[q]);
       dst[q] = l * LS + ((j / NS) * NS * R + k) * ES;
     }
   }
@@ -1176,10 +1175,18 @@ __device__ inline void stage_g(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
   __syncthreads();
 }
 
+// Radix of the next plane-kernel stage: a remaining length of 6, 10 or 12
+// is one register butterfly (one LDS pass instead of two).
+__host__ __device__ constexpr int pick_radix_g(int m, int n, int L, int thr) {
+  return (m == 6 || m == 10 || m == 12) ? m : pick_radix_for(m, n, L, thr);
+}
+
 template <class T, bool INV, int N, int NS, int REM, int L, int THR, int LS, int ES>
 __device__ inline void stages_g(cplx<T>* buf, const cplx<T>* __restrict__ tw) {
   if constexpr (REM > 1) {
-    constexpr int R = pick_radix_for(REM, N, L, THR);
+    constexpr int R = pick_radix_g(REM, N, L, THR);
+    static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 6 || R == 8 || R == 10 || R == 12 || R == 16,
+                  "plane-kernel radix");
     stage_g<T, INV, R, N, NS, L, THR, LS, ES>(buf, tw);
     stages_g<T, INV, N, NS * R, REM / R, L, THR, LS, ES>(buf, tw);
   }
