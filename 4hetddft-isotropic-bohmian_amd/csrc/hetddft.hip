@@ -2906,8 +2906,9 @@ int hetddft_checkpoint_size(hetddft_ctx* c, long long* bytes) {
   return HETDDFT_OK;
 }
 
-int hetddft_checkpoint_save(hetddft_ctx* c, void* buf) {
-  if (!c || !buf) return HETDDFT_E_ARG;
+// Header to hdr (host memory), arrays to data: host memory (kind D2H) or
+// device memory (kind D2D, the autotuner's snapshot).
+static int checkpoint_save_to(hetddft_ctx* c, void* hdr, void* data, hipMemcpyKind kind) {
   if (!c->started) return fail(c, HETDDFT_E_STATE, "hetddft_start not called");
   RET(settle(c));
   HIPCHK(c, hipStreamSynchronize(c->st));
@@ -2925,22 +2926,28 @@ int hetddft_checkpoint_save(hetddft_ctx* c, void* buf) {
   std::copy(c->ioldr, c->ioldr + 3, h.ioldr);
   std::copy(c->ioldv, c->ioldv + 3, h.ioldv);
   std::copy(c->iolda, c->iolda + 2, h.iolda);
-  unsigned char* o = static_cast<unsigned char*>(buf);
-  std::memcpy(o, &h, sizeof h);
-  o += sizeof h;
+  std::memcpy(hdr, &h, sizeof h);
+  unsigned char* o = static_cast<unsigned char*>(data);
   for (const CkptPart& q : ckpt_parts(c)) {
-    HIPCHK(c, hipMemcpy(o, q.dev, q.bytes, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpyAsync(o, q.dev, q.bytes, kind, c->st));
     o += q.bytes;
   }
+  HIPCHK(c, hipStreamSynchronize(c->st));
   return HETDDFT_OK;
 }
 
-int hetddft_checkpoint_load(hetddft_ctx* c, const void* buf) {
+int hetddft_checkpoint_save(hetddft_ctx* c, void* buf) {
   if (!c || !buf) return HETDDFT_E_ARG;
+  return checkpoint_save_to(c, buf, static_cast<unsigned char*>(buf) + sizeof(CkptHeader), hipMemcpyDeviceToHost);
+}
+
+// Header from hdr (host memory), arrays from data (host: kind H2D; device:
+// kind D2D).
+static int checkpoint_load_from(hetddft_ctx* c, const void* hdr, const void* data, hipMemcpyKind kind) {
   c->means_ready = false;
   if (!c->kernels_set) return fail(c, HETDDFT_E_STATE, "hetddft_set_kernels not called");
   CkptHeader h;
-  std::memcpy(&h, buf, sizeof h);
+  std::memcpy(&h, hdr, sizeof h);
   if (std::memcmp(h.magic, "HETDCKP1", 8) != 0) return fail(c, HETDDFT_E_ARG, "checkpoint: bad magic");
   if (h.nx != c->p.nx || h.ny != c->p.ny || h.nz != c->p.nz || h.n_par != c->p.n_par ||
       h.precision != c->p.precision)
@@ -2959,11 +2966,12 @@ int hetddft_checkpoint_load(hetddft_ctx* c, const void* buf) {
     return fail(c, HETDDFT_E_ARG, "checkpoint: corrupt slot order");
   RET(settle(c));
   HIPCHK(c, hipStreamSynchronize(c->st));
-  const unsigned char* o = static_cast<const unsigned char*>(buf) + sizeof h;
+  const unsigned char* o = static_cast<const unsigned char*>(data);
   for (const CkptPart& q : ckpt_parts(c)) {
-    HIPCHK(c, hipMemcpy(q.dev, o, q.bytes, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpyAsync(q.dev, o, q.bytes, kind, c->st));
     o += q.bytes;
   }
+  HIPCHK(c, hipStreamSynchronize(c->st));
   std::copy(h.ioldp, h.ioldp + 3, c->ioldp);
   std::copy(h.ioldh, h.ioldh + 2, c->ioldh);
   std::copy(h.ioldr, h.ioldr + 3, c->ioldr);
@@ -2990,6 +2998,11 @@ int hetddft_checkpoint_load(hetddft_ctx* c, const void* buf) {
   HIPCHK(c, hipStreamSynchronize(c->st));
   c->started = true;
   return HETDDFT_OK;
+}
+
+int hetddft_checkpoint_load(hetddft_ctx* c, const void* buf) {
+  if (!c || !buf) return HETDDFT_E_ARG;
+  return checkpoint_load_from(c, buf, static_cast<const unsigned char*>(buf) + sizeof(CkptHeader), hipMemcpyHostToDevice);
 }
 
 int hetddft_sample_particles(hetddft_ctx* c, const double* denx, const double lo[3], const double hi[3],
@@ -3293,10 +3306,28 @@ int hetddft_autotune(hetddft_ctx* c, double dt, int steps, hetddft_tuning* out) 
       return HETDDFT_OK;
     }
   }
+  // Snapshot of the state, restored at the end.  In device memory when it
+  // fits, so that the restore neither waits on a host copy nor leaves the
+  // compute units idle for it; in host memory otherwise.
   long long sz = 0;
   RET(hetddft_checkpoint_size(c, &sz));
-  std::vector<unsigned char> img((size_t)sz);
-  RET(hetddft_checkpoint_save(c, img.data()));
+  CkptHeader snap_hdr;
+  std::vector<unsigned char> img;
+  void* dimg = nullptr;
+  const size_t data_bytes = (size_t)sz - sizeof(CkptHeader);
+  if (hipMalloc(&dimg, data_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    dimg = nullptr;
+    img.resize(data_bytes);
+  }
+  {
+    const int rs = checkpoint_save_to(c, &snap_hdr, dimg ? dimg : img.data(),
+                                      dimg ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost);
+    if (rs != HETDDFT_OK) {
+      if (dimg) (void)hipFree(dimg);
+      return rs;
+    }
+  }
   const bool graphs = c->graph_on;
   c->graph_on = false;
   hetddft_tuning best;
@@ -3408,7 +3439,9 @@ int hetddft_autotune(hetddft_ctx* c, double dt, int steps, hetddft_tuning* out) 
   rc = run();
   c->prof_on = false;
   c->graph_on = graphs;
-  const int rc2 = hetddft_checkpoint_load(c, img.data());
+  const int rc2 = checkpoint_load_from(c, &snap_hdr, dimg ? dimg : img.data(),
+                                       dimg ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice);
+  if (dimg) (void)hipFree(dimg);
   if (rc != HETDDFT_OK) return rc;
   if (rc2 != HETDDFT_OK) return rc2;
   {
